@@ -1,0 +1,240 @@
+/*
+ * scdat_amd.h - C ABI of the MI355X (gfx950) PIC-MCC cycle.
+ *
+ * Plain C types only: opaque handles, POD descriptors, caller-owned host
+ * buffers copied in and out, int status returns (0 ok, < 0 error, message via
+ * scdat_last_error).  No exception crosses this boundary.  One context owns
+ * all device memory of one simulation and runs on one HIP stream; contexts
+ * share nothing, so several may coexist in one process.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the SCDAT source tree).
+ */
+#ifndef SCDAT_AMD_H
+#define SCDAT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SCDAT_OK 0
+#define SCDAT_ERR_ARG (-1)     /* invalid argument                          */
+#define SCDAT_ERR_STATE (-2)   /* call out of order (e.g. step before mesh) */
+#define SCDAT_ERR_DEVICE (-3)  /* HIP runtime error / no device             */
+#define SCDAT_ERR_NOMEM (-4)   /* allocation failed / capacity exceeded     */
+#define SCDAT_ERR_SOLVER (-5)  /* Poisson PCG failed; previous phi kept     */
+
+/* ---- boundary groups ----------------------------------------------------
+ * Every boundary face of the mesh carries a group id.  The box mesher uses
+ * 0..5 for the outer walls in the order of PICCycle::BoundaryFace
+ * (XMin, XMax, YMin, YMax, ZMin, ZMax; Tools/PICcore/include/PICCycle.h:36-44)
+ * and 6..11 for the faces of the conductor box (-x,+x,-y,+y,-z,+z).          */
+#define SCDAT_MAX_GROUPS 16
+#define SCDAT_BC_ABSORB 0     /* AbsorbingBoundaryCondition, p = 1
+                                 (Tools/Boundary/src/BoundaryConditions.cpp:360-392) */
+#define SCDAT_BC_PERIODIC 1   /* PeriodicBoundaryCondition (:68-130)                 */
+#define SCDAT_BC_REFLECT 2    /* ReflectiveBoundaryCondition (specular)              */
+#define SCDAT_BC_CONDUCTOR 3  /* absorbed onto the floating conductor: its charge
+                                 feeds the floating-potential solve                  */
+
+/* node kinds */
+#define SCDAT_NODE_FREE 0
+#define SCDAT_NODE_DIRICHLET 1
+#define SCDAT_NODE_CONDUCTOR 2
+
+typedef struct scdat_ctx scdat_ctx;
+
+/* ---- structured mesh generation (host only, no device needed) ----------
+ * Replaces GeometryMeshGenerator::generateBoxMesh
+ * (Tools/Mesh/src/MeshParsing.cpp:1208-1261): same lexicographic node order,
+ * but a conforming 6-tet (Kuhn) split, an optional conductor box cut out of
+ * the domain (its faces snapped to grid lines) and boundary-face groups.   */
+typedef struct {
+  double lo[3], hi[3];   /* domain box (m)                                  */
+  int32_t n[3];          /* hex cells per axis                              */
+  int32_t conductor;     /* 1: remove [c_lo, c_hi]; its surface is floating */
+  double c_lo[3], c_hi[3];
+} scdat_box_spec;
+
+int scdat_box_mesh_size(const scdat_box_spec* spec, int64_t* n_nodes, int64_t* n_tets);
+/* xyz[3*n_nodes], tets[4*n_tets], node_kind[n_nodes], face_group[4*n_tets]
+ * (face k is opposite local vertex k; -1 = interior face).               */
+int scdat_box_mesh_build(const scdat_box_spec* spec, double* xyz, int32_t* tets,
+                         int32_t* node_kind, int32_t* face_group);
+
+/* Analytic locate on the box mesh built from `spec` (cell = -1 outside the
+ * mesh or inside the conductor).  Replaces the O(N_elem) scan of
+ * CrossTetrahedron::locateParticle (PICParticleAlgorithms.cpp:853-891) for
+ * structured meshes.                                                       */
+int scdat_box_locate(const scdat_box_spec* spec, int64_t n, const double* x, int32_t* cell);
+
+/* Philox4x32-10 counter-based generator (Salmon et al., SC'11). */
+void scdat_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Synthetic Maxwellian plasma on the box mesh: positions uniform in the
+ * domain minus the conductor, velocities N(drift, vth^2) per component, cells
+ * located analytically.  Particle p's state depends only on (seed, stream, p).
+ * Replaces the loading loop of Main/main_PICcore.cpp:195-229.              */
+int scdat_load_maxwellian(const scdat_box_spec* spec, int64_t n, uint64_t seed, uint32_t stream,
+                          double vth, const double* drift, double* x, double* v, int32_t* cell);
+
+/* ---- context configuration ---------------------------------------------
+ * Mirrors PICCycle::Parameters (Tools/PICcore/include/PICCycle.h:46-58) plus
+ * the precision, Poisson and sorting controls of the device path.          */
+typedef struct {
+  int32_t precision;        /* 0 mixed: FP32 particles/SpMV, FP64 refinement;
+                               1 FP64 everywhere                                  */
+  double dt;                /* time_step (s)                                      */
+  int32_t adaptive_substep; /* adaptive sub-stepping of CrossTetrahedron
+                               (Tools/Particle/src/PICParticleAlgorithms.cpp:750-801) */
+  double floor_ratio;       /* sub-step floor as a fraction of the remaining time */
+  int32_t max_substeps;     /* sub-step cap per particle step                     */
+  int32_t poisson_operator; /* 0 P1 FEM (MultiBoundaryElectricFieldSolver.cpp:1704-1759),
+                               1 graph Laplacian (PoissonSolver.cpp:276-327)      */
+  double permittivity;      /* F/m                                                */
+  int32_t load_sign;        /* +1 physical load (+rho), -1 reference sign         */
+  double pcg_tol;           /* relative residual target ||r|| <= tol ||b||        */
+  int32_t pcg_max_iter;     /* total inner iterations per solve                   */
+  double inner_tol;         /* mixed mode: FP32 inner PCG relative tolerance      */
+  int32_t sort_interval;    /* steps between cell sorts (0 = never)               */
+  double bfield[3];         /* uniform magnetic field (T); reference B = 0        */
+} scdat_config;
+
+void scdat_config_default(scdat_config* cfg);
+
+/* Replaces PICCycle::PICCycle(const Parameters&) (PICCycle.h:60-61). */
+int scdat_create(const scdat_config* cfg, scdat_ctx** out);
+void scdat_destroy(scdat_ctx* ctx);
+const char* scdat_last_error(const scdat_ctx* ctx);
+/* device name, CU count; returns SCDAT_ERR_DEVICE if no GPU */
+int scdat_device_info(char* name, int32_t name_len, int32_t* n_cu);
+
+/* Replaces PICCycle::setMesh(nodes, elements) (PICCycle.h:63-64) and
+ * PoissonSolver(VolMeshPtr) (Tools/FieldSolver/include/PoissonSolver.h:26-40):
+ * uploads the mesh, builds tet maps, face neighbours, the Poisson operator and
+ * the nodal-gradient operator.  dirichlet may be NULL (all zero);
+ * face_group may be NULL (all boundary faces in group 0).                   */
+int scdat_set_mesh(scdat_ctx* ctx, int64_t n_nodes, const double* xyz, int64_t n_tets,
+                   const int32_t* tets, const int32_t* node_kind, const double* dirichlet,
+                   const int32_t* face_group);
+
+/* Replaces PICCycle::setBoundaryCondition(BoundaryFace, BoundaryConditionPtr)
+ * (PICCycle.h:67).  For SCDAT_BC_PERIODIC, `partner_group` is the opposite
+ * group and shift the translation from this group's faces to the partner's.
+ * Must be called before scdat_initialize.                                   */
+int scdat_set_boundary(scdat_ctx* ctx, int32_t group, int32_t type, int32_t partner_group,
+                       const double* shift);
+
+/* Species (ParticleManager::createElectron/createIon semantics,
+ * Tools/Particle/include/ParticleManager.h:821-926): charge q (C), mass m (kg),
+ * macro weight w, device capacity in particles.                             */
+int scdat_add_species(scdat_ctx* ctx, double q, double m, double w, int64_t capacity,
+                      int32_t* species_id);
+
+/* Bulk upload / download of one species (replaces iteration over
+ * ParticleContainer, ParticleManager.h:55-156).  cell[i] = containing tet;
+ * pass cell = NULL to locate on the device by a walk from a seed tet.
+ * Removed particles are compacted out on download.                          */
+int scdat_set_particles(scdat_ctx* ctx, int32_t species, int64_t n, const double* x,
+                        const double* v, const int32_t* cell);
+int scdat_get_particle_count(scdat_ctx* ctx, int32_t species, int64_t* n_alive,
+                             int64_t* n_slots);
+/* Copies n_slots entries in device (cell-sorted) order; removed particles
+ * have cell = -1; id[] is the upload index (or creation order) of each
+ * particle.  Any output pointer may be NULL.                                */
+int scdat_get_particles(scdat_ctx* ctx, int32_t species, double* x, double* v, int32_t* cell,
+                        int32_t* id);
+
+/* Initial conductor charge (C) of the floating conductor. */
+int scdat_set_conductor_charge(scdat_ctx* ctx, double q);
+
+/* Replaces PICCycle::initialize (PICCycle.cpp:255-309) and computes the
+ * initial field: deposit -> Poisson -> E.                                   */
+int scdat_initialize(scdat_ctx* ctx);
+
+/* Replaces PICCycle::executeTimeStep (PICCycle.cpp:311-416), n times:
+ * push + gather + walk + deposit with the previous E, boundary events,
+ * Poisson (with floating conductor), nodal E.                               */
+int scdat_step(scdat_ctx* ctx, int32_t n);
+
+/* Individual stages, for callers that drive the cycle themselves and tests.
+ * scdat_push: one push+deposit pass (rho cleared first);
+ * scdat_deposit: deposit current particles without moving them;
+ * scdat_solve: PoissonSolver::solve (PoissonSolver.cpp:387-401) + nodal E
+ * (:474-529) from the current rho.                                          */
+int scdat_push(scdat_ctx* ctx);
+int scdat_deposit(scdat_ctx* ctx);
+int scdat_solve(scdat_ctx* ctx);
+int scdat_compute_efield(scdat_ctx* ctx);
+int scdat_sort(scdat_ctx* ctx);
+
+/* Inject a nodal field (V/m, 3 per node) / potential (V per node); the
+ * latter mirrors PICCycle::setPotential (PICCycle.h:90-105) and recomputes E. */
+int scdat_set_efield(scdat_ctx* ctx, const double* E);
+int scdat_set_potential(scdat_ctx* ctx, const double* phi);
+int scdat_set_rho(scdat_ctx* ctx, const double* rho);
+
+/* Replaces PICCycle::getChargeDensity / getPotential / getElectricField
+ * (PICCycle.h:73-84).  Any pointer may be NULL.  rho is deposited charge (C)
+ * per node.                                                                 */
+int scdat_get_fields(scdat_ctx* ctx, double* rho, double* phi, double* E);
+/* Fixed-point charge: rho_fx[i] * unit = rho[i] (C). */
+int scdat_get_rho_fixed(scdat_ctx* ctx, int64_t* rho_fx, double* unit);
+/* FP32 tet maps the device uses: 12 floats per tet (3x3 inverse Jacobian
+ * row-major, origin xyz) and the per-species integer charge c_s (rho_fx
+ * units per particle per unit barycentric weight 2^-24).                    */
+int scdat_get_tet_maps(scdat_ctx* ctx, float* maps);
+int scdat_get_species_charge_fixed(scdat_ctx* ctx, int32_t species, int64_t* c_s);
+
+typedef struct {
+  int64_t completed_steps;
+  double simulation_time;
+  double total_charge;         /* sum of deposited rho (C)                  */
+  double total_energy;         /* sum 1/2 m w v^2 over alive particles (J) */
+  int64_t active_particles;
+  double surface_deposited_charge; /* absorbed on all groups since start (C) */
+  double conductor_charge;     /* floating conductor charge (C)             */
+  double floating_potential;   /* V                                          */
+  int32_t pcg_iterations;      /* last solve, inner iterations              */
+  int32_t pcg_outer;           /* last solve, refinement passes             */
+  double pcg_rel_residual;     /* last solve, FP64 ||b - A x|| / ||b||      */
+  int32_t pcg_converged;
+  int64_t lost_particles;      /* walk limit exceeded                        */
+} scdat_stats;
+
+/* Replaces PICCycle::getStatistics (PICCycle.h:135-149). */
+int scdat_get_stats(scdat_ctx* ctx, scdat_stats* st);
+
+typedef struct {
+  double absorbed_electron_charge_c;
+  double absorbed_ion_charge_c;
+  double emitted_electron_charge_c;
+  double incident_energy_j;
+  int64_t absorbed_electron_particles;
+  int64_t absorbed_ion_particles;
+  int64_t emitted_electron_particles;
+} scdat_ledger_entry;
+
+/* Replaces PICCycle::getSurfaceCurrentLedger (PICCycle.h:166-169); one entry
+ * per boundary group (n_groups <= SCDAT_MAX_GROUPS).                        */
+int scdat_get_ledger(scdat_ctx* ctx, scdat_ledger_entry* out, int32_t n_groups);
+
+/* Per-stage device time of the last scdat_step call, milliseconds:
+ * [push, rhs, pcg, field, sort].                                            */
+int scdat_get_stage_times(scdat_ctx* ctx, double* ms5);
+
+/* Average device time (ms) of the fused push kernel over the last
+ * scdat_step call, measured with HIP events on the context's stream, and the
+ * number of launches it averages.                                           */
+int scdat_get_push_kernel_time(scdat_ctx* ctx, double* avg_ms, int64_t* launches);
+
+/* Synchronise the context's stream. */
+int scdat_sync(scdat_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCDAT_AMD_H */

@@ -1,0 +1,388 @@
+"""MI355X-native PIC-MCC cycle of SCDAT: Python binding over the C ABI.
+
+The compute path is the gfx950 library ``libscdat_amd.so`` built in-tree
+(``python -m scdat_amd.build``).  This module is a thin ctypes layer over
+``include/scdat_amd.h``; it never computes physics itself.  Importing it
+without the built library raises immediately: there is no CPU fallback.
+
+The class :class:`PICCycle` mirrors the reference orchestrator
+``SCDAT::PICcore::PICCycle`` (Tools/PICcore/include/PICCycle.h:33-185):
+``setMesh`` / ``setBoundaryCondition`` / ``initialize`` / ``executeTimeStep`` /
+``getChargeDensity`` / ``getPotential`` / ``getElectricField`` /
+``getStatistics`` / ``getSurfaceCurrentLedger``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libscdat_amd.so")
+
+# physical constants, Tools/Basic/include/Constants.h:46-53
+ELEMENTARY_CHARGE = 1.602176634e-19
+ELECTRON_MASS = 9.1093837015e-31
+PROTON_MASS = 1.67262192369e-27
+ATOMIC_MASS_UNIT = 1.66053906660e-27
+VACUUM_PERMITTIVITY = 8.8541878128e-12
+BOLTZMANN = 1.380649e-23
+
+OK, ERR_ARG, ERR_STATE, ERR_DEVICE, ERR_NOMEM, ERR_SOLVER = 0, -1, -2, -3, -4, -5
+BC_ABSORB, BC_PERIODIC, BC_REFLECT, BC_CONDUCTOR = 0, 1, 2, 3
+NODE_FREE, NODE_DIRICHLET, NODE_CONDUCTOR = 0, 1, 2
+MAX_GROUPS = 16
+
+
+class ScdatError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"scdat error {code}: {msg}")
+        self.code = code
+
+
+class BoxSpec(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("n", C.c_int32 * 3),
+                ("conductor", C.c_int32), ("c_lo", C.c_double * 3), ("c_hi", C.c_double * 3)]
+
+
+class Config(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("dt", C.c_double), ("adaptive_substep", C.c_int32),
+                ("floor_ratio", C.c_double), ("max_substeps", C.c_int32),
+                ("poisson_operator", C.c_int32), ("permittivity", C.c_double),
+                ("load_sign", C.c_int32), ("pcg_tol", C.c_double), ("pcg_max_iter", C.c_int32),
+                ("inner_tol", C.c_double), ("sort_interval", C.c_int32),
+                ("bfield", C.c_double * 3)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("completed_steps", C.c_int64), ("simulation_time", C.c_double),
+                ("total_charge", C.c_double), ("total_energy", C.c_double),
+                ("active_particles", C.c_int64), ("surface_deposited_charge", C.c_double),
+                ("conductor_charge", C.c_double), ("floating_potential", C.c_double),
+                ("pcg_iterations", C.c_int32), ("pcg_outer", C.c_int32),
+                ("pcg_rel_residual", C.c_double), ("pcg_converged", C.c_int32),
+                ("lost_particles", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LedgerEntry(C.Structure):
+    _fields_ = [("absorbed_electron_charge_c", C.c_double), ("absorbed_ion_charge_c", C.c_double),
+                ("emitted_electron_charge_c", C.c_double), ("incident_energy_j", C.c_double),
+                ("absorbed_electron_particles", C.c_int64), ("absorbed_ion_particles", C.c_int64),
+                ("emitted_electron_particles", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+
+# exported symbol -> (restype, argtypes); the test-suite checks that this
+# table, the header and the library agree.
+SIGNATURES = {
+    "scdat_config_default": (None, [C.POINTER(Config)]),
+    "scdat_box_mesh_size": (C.c_int, [C.POINTER(BoxSpec), _I64, _I64]),
+    "scdat_box_mesh_build": (C.c_int, [C.POINTER(BoxSpec), _D, _I32, _I32, _I32]),
+    "scdat_box_locate": (C.c_int, [C.POINTER(BoxSpec), C.c_int64, _D, _I32]),
+    "scdat_philox4x32": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "scdat_load_maxwellian": (C.c_int, [C.POINTER(BoxSpec), C.c_int64, C.c_uint64, C.c_uint32,
+                                        C.c_double, _D, _D, _D, _I32]),
+    "scdat_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),
+    "scdat_destroy": (None, [_P]),
+    "scdat_last_error": (C.c_char_p, [_P]),
+    "scdat_device_info": (C.c_int, [C.c_char_p, C.c_int32, _I32]),
+    "scdat_set_mesh": (C.c_int, [_P, C.c_int64, _D, C.c_int64, _I32, _I32, _D, _I32]),
+    "scdat_set_boundary": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _D]),
+    "scdat_add_species": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_int64, _I32]),
+    "scdat_set_particles": (C.c_int, [_P, C.c_int32, C.c_int64, _D, _D, _I32]),
+    "scdat_get_particle_count": (C.c_int, [_P, C.c_int32, _I64, _I64]),
+    "scdat_get_particles": (C.c_int, [_P, C.c_int32, _D, _D, _I32, _I32]),
+    "scdat_set_conductor_charge": (C.c_int, [_P, C.c_double]),
+    "scdat_initialize": (C.c_int, [_P]),
+    "scdat_step": (C.c_int, [_P, C.c_int32]),
+    "scdat_push": (C.c_int, [_P]),
+    "scdat_deposit": (C.c_int, [_P]),
+    "scdat_solve": (C.c_int, [_P]),
+    "scdat_compute_efield": (C.c_int, [_P]),
+    "scdat_sort": (C.c_int, [_P]),
+    "scdat_set_efield": (C.c_int, [_P, _D]),
+    "scdat_set_potential": (C.c_int, [_P, _D]),
+    "scdat_set_rho": (C.c_int, [_P, _D]),
+    "scdat_get_fields": (C.c_int, [_P, _D, _D, _D]),
+    "scdat_get_rho_fixed": (C.c_int, [_P, _I64, _D]),
+    "scdat_get_tet_maps": (C.c_int, [_P, _F]),
+    "scdat_get_species_charge_fixed": (C.c_int, [_P, C.c_int32, _I64]),
+    "scdat_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "scdat_get_ledger": (C.c_int, [_P, C.POINTER(LedgerEntry), C.c_int32]),
+    "scdat_get_stage_times": (C.c_int, [_P, _D]),
+    "scdat_get_push_kernel_time": (C.c_int, [_P, _D, _I64]),
+    "scdat_sync": (C.c_int, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the gfx950 library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m scdat_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    ncu = C.c_int32(0)
+    rc = lib().scdat_device_info(name, 256, C.byref(ncu))
+    if rc != OK:
+        raise ScdatError(rc, "no HIP device")
+    return name.value.decode(), ncu.value
+
+
+# ---------------------------------------------------------------------------
+# mesh generation (host only)
+# ---------------------------------------------------------------------------
+def box_spec(n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), conductor=None):
+    s = BoxSpec()
+    nn = (n, n, n) if np.isscalar(n) else tuple(n)
+    for r in range(3):
+        s.lo[r], s.hi[r], s.n[r] = lo[r], hi[r], nn[r]
+    if conductor is not None:
+        s.conductor = 1
+        for r in range(3):
+            s.c_lo[r], s.c_hi[r] = conductor[0][r], conductor[1][r]
+    return s
+
+
+def box_mesh(n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), conductor=None):
+    """Structured 6-tet box mesh (optionally with a conductor box removed).
+
+    Returns dict(xyz[nn,3], tets[nt,4], kind[nn], group[nt,4]).
+    """
+    s = box_spec(n, lo, hi, conductor)
+    nn, nt = C.c_int64(), C.c_int64()
+    rc = lib().scdat_box_mesh_size(C.byref(s), C.byref(nn), C.byref(nt))
+    if rc != OK:
+        raise ScdatError(rc, "invalid box spec")
+    xyz = np.zeros((nn.value, 3))
+    tets = np.zeros((nt.value, 4), np.int32)
+    kind = np.zeros(nn.value, np.int32)
+    grp = np.zeros((nt.value, 4), np.int32)
+    rc = lib().scdat_box_mesh_build(C.byref(s), _ptr(xyz, C.c_double), _ptr(tets, C.c_int32),
+                                    _ptr(kind, C.c_int32), _ptr(grp, C.c_int32))
+    if rc != OK:
+        raise ScdatError(rc, "box mesh build failed")
+    return {"xyz": xyz, "tets": tets, "kind": kind, "group": grp}
+
+
+def default_config(**kw):
+    c = Config()
+    lib().scdat_config_default(C.byref(c))
+    for k, v in kw.items():
+        if k == "bfield":
+            for r in range(3):
+                c.bfield[r] = v[r]
+        else:
+            setattr(c, k, v)
+    return c
+
+
+# ---------------------------------------------------------------------------
+# context
+# ---------------------------------------------------------------------------
+class Context:
+    """One device simulation (owns all device memory; one HIP stream)."""
+
+    def __init__(self, config=None, **kw):
+        self.cfg = config if config is not None else default_config(**kw)
+        h = _P()
+        rc = lib().scdat_create(C.byref(self.cfg), C.byref(h))
+        if rc != OK:
+            raise ScdatError(rc, "scdat_create failed (no device or bad config)")
+        self.h = h
+        self.n_nodes = 0
+        self.n_tets = 0
+        self.species = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().scdat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != OK:
+            msg = lib().scdat_last_error(self.h)
+            raise ScdatError(rc, msg.decode() if msg else "")
+        return rc
+
+    # -- setup --------------------------------------------------------------
+    def set_mesh(self, xyz, tets, kind, dirichlet=None, group=None):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        tets = np.ascontiguousarray(tets, np.int32)
+        kind = np.ascontiguousarray(kind, np.int32)
+        d = None if dirichlet is None else np.ascontiguousarray(dirichlet, np.float64)
+        g = None if group is None else np.ascontiguousarray(group, np.int32)
+        self._chk(lib().scdat_set_mesh(self.h, len(kind), _ptr(xyz, C.c_double), len(tets),
+                                       _ptr(tets, C.c_int32), _ptr(kind, C.c_int32),
+                                       _ptr(d, C.c_double), _ptr(g, C.c_int32)))
+        self.n_nodes, self.n_tets = len(kind), len(tets)
+
+    def set_boundary(self, group, bc_type, partner=-1, shift=None):
+        sh = None if shift is None else np.ascontiguousarray(shift, np.float64)
+        self._chk(lib().scdat_set_boundary(self.h, group, bc_type, partner, _ptr(sh, C.c_double)))
+
+    def add_species(self, q, m, w, capacity):
+        sid = C.c_int32()
+        self._chk(lib().scdat_add_species(self.h, q, m, w, int(capacity), C.byref(sid)))
+        self.species.append((q, m, w))
+        return sid.value
+
+    def set_particles(self, species, x, v, cell=None):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        c = None if cell is None else np.ascontiguousarray(cell, np.int32)
+        self._chk(lib().scdat_set_particles(self.h, species, len(x), _ptr(x, C.c_double),
+                                            _ptr(v, C.c_double), _ptr(c, C.c_int32)))
+
+    def particle_count(self, species):
+        a, s = C.c_int64(), C.c_int64()
+        self._chk(lib().scdat_get_particle_count(self.h, species, C.byref(a), C.byref(s)))
+        return a.value, s.value
+
+    def get_particles(self, species):
+        """Returns (x[n,3], v[n,3], cell[n], id[n]) in device order."""
+        _, n = self.particle_count(species)
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        c = np.zeros(n, np.int32)
+        i = np.zeros(n, np.int32)
+        self._chk(lib().scdat_get_particles(self.h, species, _ptr(x, C.c_double), _ptr(v, C.c_double),
+                                            _ptr(c, C.c_int32), _ptr(i, C.c_int32)))
+        return x, v, c, i
+
+    def get_particles_by_id(self, species, n_ids):
+        """Particles scattered to upload order; removed ones get cell -1, NaN state."""
+        x, v, c, i = self.get_particles(species)
+        X = np.full((n_ids, 3), np.nan)
+        V = np.full((n_ids, 3), np.nan)
+        Cc = np.full(n_ids, -1, np.int32)
+        m = (i >= 0) & (i < n_ids)
+        X[i[m]] = x[m]
+        V[i[m]] = v[m]
+        Cc[i[m]] = c[m]
+        return X, V, Cc
+
+    def set_conductor_charge(self, q):
+        self._chk(lib().scdat_set_conductor_charge(self.h, q))
+
+    # -- cycle --------------------------------------------------------------
+    def initialize(self):
+        self._chk(lib().scdat_initialize(self.h))
+
+    def step(self, n=1):
+        self._chk(lib().scdat_step(self.h, n))
+
+    def push(self):
+        self._chk(lib().scdat_push(self.h))
+
+    def deposit(self):
+        self._chk(lib().scdat_deposit(self.h))
+
+    def solve(self):
+        self._chk(lib().scdat_solve(self.h))
+
+    def compute_efield(self):
+        self._chk(lib().scdat_compute_efield(self.h))
+
+    def sort(self):
+        self._chk(lib().scdat_sort(self.h))
+
+    def set_efield(self, E):
+        E = np.ascontiguousarray(E, np.float64).reshape(-1, 3)
+        self._chk(lib().scdat_set_efield(self.h, _ptr(E, C.c_double)))
+
+    def set_potential(self, phi):
+        phi = np.ascontiguousarray(phi, np.float64)
+        self._chk(lib().scdat_set_potential(self.h, _ptr(phi, C.c_double)))
+
+    def set_rho(self, rho):
+        rho = np.ascontiguousarray(rho, np.float64)
+        self._chk(lib().scdat_set_rho(self.h, _ptr(rho, C.c_double)))
+
+    def fields(self):
+        n = self.n_nodes
+        rho, phi, E = np.zeros(n), np.zeros(n), np.zeros((n, 3))
+        self._chk(lib().scdat_get_fields(self.h, _ptr(rho, C.c_double), _ptr(phi, C.c_double),
+                                         _ptr(E, C.c_double)))
+        return rho, phi, E
+
+    def rho_fixed(self):
+        r = np.zeros(self.n_nodes, np.int64)
+        u = C.c_double()
+        self._chk(lib().scdat_get_rho_fixed(self.h, _ptr(r, C.c_int64), C.byref(u)))
+        return r, u.value
+
+    def tet_maps(self):
+        m = np.zeros((self.n_tets, 12), np.float32)
+        self._chk(lib().scdat_get_tet_maps(self.h, _ptr(m, C.c_float)))
+        return m
+
+    def species_charge_fixed(self, s):
+        v = C.c_int64()
+        self._chk(lib().scdat_get_species_charge_fixed(self.h, s, C.byref(v)))
+        return v.value
+
+    def stats(self):
+        st = Stats()
+        self._chk(lib().scdat_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def ledger(self, n_groups=12):
+        arr = (LedgerEntry * n_groups)()
+        self._chk(lib().scdat_get_ledger(self.h, arr, n_groups))
+        return [e.as_dict() for e in arr]
+
+    def stage_times(self):
+        a = np.zeros(5)
+        self._chk(lib().scdat_get_stage_times(self.h, _ptr(a, C.c_double)))
+        return dict(zip(["push", "rhs", "pcg", "field", "sort"], a.tolist()))
+
+    def push_kernel_time(self):
+        t, n = C.c_double(), C.c_int64()
+        self._chk(lib().scdat_get_push_kernel_time(self.h, C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def sync(self):
+        self._chk(lib().scdat_sync(self.h))
+
+
+from .loading import maxwellian_box, philox4x32  # noqa: E402,F401
+from .piccycle import PICCycle  # noqa: E402,F401

@@ -1,0 +1,1493 @@
+// C-ABI implementation: context, mesh setup, particle I/O and the step
+// orchestration of the PIC-MCC cycle on one gfx950 GPU.
+//
+// Step order follows PICCycle::executeTimeStep
+// (Tools/PICcore/src/PICCycle.cpp:311-416): push + gather + walk + deposit
+// with the previous step's E, Poisson with the floating conductor, nodal E.
+// Everything stays resident on the device; the host only polls the PCG
+// convergence flag between batches of iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/scdat_amd.h"
+#include "device_common.hpp"
+#include "field.hpp"
+#include "mesh_host.hpp"
+#include "push.hpp"
+#include "sort.hpp"
+
+namespace scdat {
+
+constexpr double kElem = 1.602176634e-19;  // Tools/Basic/include/Constants.h:46
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      throw Error(SCDAT_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Owning device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) b = 16;
+    if (hipMalloc(&p, b) != hipSuccess) {
+      p = nullptr;
+      throw Error(SCDAT_ERR_NOMEM, "hipMalloc failed (" + std::to_string(b) + " bytes)");
+    }
+    bytes = b;
+  }
+  template <typename U>
+  U* as() const { return static_cast<U*>(p); }
+  template <typename U>
+  void upload(const std::vector<U>& v, hipStream_t s) {
+    alloc(sizeof(U) * v.size());
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), sizeof(U) * v.size(), hipMemcpyHostToDevice, s));
+  }
+};
+
+// Host-side copy of the mesh and everything derived from it.
+struct HostMesh {
+  int64_t nn = 0, nt = 0;
+  std::vector<double> xyz, dval;
+  std::vector<int32_t> tets, kind, group, nb, partner;
+  std::vector<double> A, X0, vol;  // per tet, FP64
+  int W = 0;
+  std::vector<int32_t> col;        // slot-major W*nn
+  std::vector<double> kfull, aval, gx, gy, gz, lift, b1, dinv, dscale;
+  std::vector<int32_t> kc_idx;
+  std::vector<double> kc_val;
+  bool has_cond = false;
+};
+
+static bool solve3(double M[3][3], double r[3], double out[3]) {
+  for (int p = 0; p < 3; ++p) {
+    int best = p;
+    for (int row = p + 1; row < 3; ++row)
+      if (std::abs(M[row][p]) > std::abs(M[best][p])) best = row;
+    if (std::abs(M[best][p]) < 1e-300) return false;
+    if (best != p) {
+      for (int c = 0; c < 3; ++c) std::swap(M[best][c], M[p][c]);
+      std::swap(r[best], r[p]);
+    }
+    for (int row = p + 1; row < 3; ++row) {
+      const double f = M[row][p] / M[p][p];
+      for (int c = p; c < 3; ++c) M[row][c] -= f * M[p][c];
+      r[row] -= f * r[p];
+    }
+  }
+  for (int row = 2; row >= 0; --row) {
+    double s = r[row];
+    for (int c = row + 1; c < 3; ++c) s -= M[row][c] * out[c];
+    out[row] = s / M[row][row];
+  }
+  return true;
+}
+
+// Builds tet maps, ELL structure, operator, elimination, conductor vectors
+// and the least-squares gradient weights.
+static void prepare_mesh(HostMesh& h, int op, double eps) {
+  const int64_t nn = h.nn, nt = h.nt;
+  face_neighbors(nt, h.tets.data(), h.nb);
+  for (int64_t f = 0; f < 4 * nt; ++f)
+    if (h.nb[f] >= 0) h.group[f] = -1;
+    else if (h.group[f] < 0 || h.group[f] >= kMaxGroups) h.group[f] = 0;
+  h.partner.assign(4 * nt, -1);
+  h.A.assign(9 * nt, 0);
+  h.X0.assign(3 * nt, 0);
+  h.vol.assign(nt, 0);
+  for (int64_t t = 0; t < nt; ++t) {
+    const double* p[4];
+    for (int k = 0; k < 4; ++k) p[k] = &h.xyz[3 * (size_t)h.tets[4 * t + k]];
+    double J[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) J[r][c] = p[c + 1][r] - p[0][r];
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    if (!(std::abs(det) > 0)) throw Error(SCDAT_ERR_ARG, "degenerate tetrahedron " + std::to_string(t));
+    h.vol[t] = std::abs(det) / 6.0;
+    double* a = &h.A[9 * t];
+    const double id = 1.0 / det;
+    a[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+    a[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    a[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    a[3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+    a[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    a[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    a[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+    a[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    a[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    for (int r = 0; r < 3; ++r) h.X0[3 * t + r] = p[0][r];
+  }
+  // ELL structure from node adjacency (self first).
+  std::vector<int64_t> rp;
+  std::vector<int32_t> ci;
+  node_adjacency(nn, nt, h.tets.data(), rp, ci);
+  int W = 0;
+  for (int64_t i = 0; i < nn; ++i) W = std::max<int>(W, (int)(rp[i + 1] - rp[i]));
+  h.W = W;
+  h.col.assign((size_t)W * nn, 0);
+  for (int64_t i = 0; i < nn; ++i)
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = rp[i] + s;
+      h.col[(size_t)s * nn + i] = k < rp[i + 1] ? ci[k] : (int32_t)i;
+    }
+  auto slot = [&](int64_t i, int32_t j) -> int64_t {
+    const int64_t len = rp[i + 1] - rp[i];
+    for (int s = 0; s < len; ++s)
+      if (ci[rp[i] + s] == j) return (int64_t)s * nn + i;
+    return -1;
+  };
+  h.kfull.assign((size_t)W * nn, 0.0);
+  for (int64_t t = 0; t < nt; ++t) {
+    const int32_t* nd = &h.tets[4 * t];
+    const double V = h.vol[t];
+    if (op == 0) {
+      // P1 stiffness, MultiBoundaryElectricFieldSolver.cpp:1704-1759,1870-1894.
+      const double* a = &h.A[9 * t];
+      double g[4][3];
+      for (int r = 0; r < 3; ++r) {
+        g[1][r] = a[r];
+        g[2][r] = a[3 + r];
+        g[3][r] = a[6 + r];
+        g[0][r] = -(g[1][r] + g[2][r] + g[3][r]);
+      }
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+          h.kfull[slot(nd[i], nd[j])] +=
+              eps * V * (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2]);
+    } else {
+      // Graph Laplacian, PoissonSolver.cpp:276-327.
+      for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j) {
+          const double* pi = &h.xyz[3 * (size_t)nd[i]];
+          const double* pj = &h.xyz[3 * (size_t)nd[j]];
+          double d2 = 0;
+          for (int r = 0; r < 3; ++r) d2 += (pi[r] - pj[r]) * (pi[r] - pj[r]);
+          const double d = std::max(std::sqrt(d2), 1e-12);
+          const double w = eps * (V / 6.0) / (d * d);
+          h.kfull[slot(nd[i], nd[i])] += w;
+          h.kfull[slot(nd[j], nd[j])] += w;
+          h.kfull[slot(nd[i], nd[j])] -= w;
+          h.kfull[slot(nd[j], nd[i])] -= w;
+        }
+    }
+  }
+  // Symmetric elimination of Dirichlet and conductor nodes.
+  h.aval = h.kfull;
+  h.lift.assign(nn, 0.0);
+  h.b1.assign(nn, 0.0);
+  h.dinv.assign(nn, 1.0);
+  h.dscale.assign(nn, 1.0);
+  h.has_cond = false;
+  for (int64_t i = 0; i < nn; ++i) {
+    if (h.kind[i] != SCDAT_NODE_FREE) {
+      // Constrained row: K_ii x_i = K_ii g_i keeps the row on the operator's
+      // own scale, so a relative residual test weighs all rows alike.
+      const double kii = h.kfull[i];
+      if (!(kii > 0)) throw Error(SCDAT_ERR_ARG, "operator diagonal not positive at node " + std::to_string(i));
+      for (int s = 0; s < W; ++s) h.aval[(size_t)s * nn + i] = 0.0;
+      h.aval[i] = kii;  // slot 0 is self
+      h.dscale[i] = kii;
+      h.dinv[i] = 1.0 / kii;
+      if (h.kind[i] == SCDAT_NODE_CONDUCTOR) {
+        h.b1[i] = kii;
+        h.has_cond = true;
+      }
+      continue;
+    }
+    const int64_t len = rp[i + 1] - rp[i];
+    for (int s = 1; s < len; ++s) {
+      const size_t o = (size_t)s * nn + i;
+      const int32_t j = h.col[o];
+      if (h.kind[j] == SCDAT_NODE_DIRICHLET) {
+        h.lift[i] -= h.kfull[o] * h.dval[j];
+        h.aval[o] = 0.0;
+      } else if (h.kind[j] == SCDAT_NODE_CONDUCTOR) {
+        h.b1[i] -= h.kfull[o];
+        h.aval[o] = 0.0;
+      }
+    }
+    const double d = h.aval[i];
+    if (!(d > 0)) throw Error(SCDAT_ERR_ARG, "operator diagonal not positive at node " + std::to_string(i));
+    h.dinv[i] = 1.0 / d;
+  }
+  // kc = sum of the unconstrained conductor rows of K.
+  h.kc_idx.clear();
+  h.kc_val.clear();
+  if (h.has_cond) {
+    std::vector<double> kc(nn, 0.0);
+    std::vector<char> touched(nn, 0);
+    for (int64_t i = 0; i < nn; ++i) {
+      if (h.kind[i] != SCDAT_NODE_CONDUCTOR) continue;
+      const int64_t len = rp[i + 1] - rp[i];
+      for (int s = 0; s < len; ++s) {
+        const size_t o = (size_t)s * nn + i;
+        kc[h.col[o]] += h.kfull[o];
+        touched[h.col[o]] = 1;
+      }
+    }
+    for (int64_t j = 0; j < nn; ++j)
+      if (touched[j]) {
+        h.kc_idx.push_back((int32_t)j);
+        h.kc_val.push_back(kc[j]);
+      }
+  }
+  // Least-squares gradient weights, PoissonSolver.cpp:474-529,749-787.
+  h.gx.assign((size_t)W * nn, 0.0);
+  h.gy.assign((size_t)W * nn, 0.0);
+  h.gz.assign((size_t)W * nn, 0.0);
+  for (int64_t i = 0; i < nn; ++i) {
+    const double* xi = &h.xyz[3 * (size_t)i];
+    const int64_t len = rp[i + 1] - rp[i];
+    double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int s = 1; s < len; ++s) {
+      const double* xj = &h.xyz[3 * (size_t)h.col[(size_t)s * nn + i]];
+      const double d[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) M[r][c] += d[r] * d[c];
+    }
+    for (int s = 1; s < len; ++s) {
+      const size_t o = (size_t)s * nn + i;
+      const double* xj = &h.xyz[3 * (size_t)h.col[o]];
+      double d[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+      double Mc[3][3];
+      std::memcpy(Mc, M, sizeof(M));
+      double w[3] = {0, 0, 0};
+      if (!solve3(Mc, d, w)) w[0] = w[1] = w[2] = 0;
+      h.gx[o] = w[0];
+      h.gy[o] = w[1];
+      h.gz[o] = w[2];
+    }
+  }
+}
+
+// Host bucket grid for locating particles uploaded without a cell.
+static void host_locate(const HostMesh& h, int64_t n, const double* x, std::vector<int32_t>& cell) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t i = 0; i < h.nn; ++i)
+    for (int r = 0; r < 3; ++r) {
+      lo[r] = std::min(lo[r], h.xyz[3 * i + r]);
+      hi[r] = std::max(hi[r], h.xyz[3 * i + r]);
+    }
+  const int64_t nb = std::max<int64_t>(1, (int64_t)std::cbrt((double)h.nt / 4.0));
+  const int64_t B = std::min<int64_t>(nb, 256);
+  auto bidx = [&](double v, int r) {
+    int64_t b = (int64_t)((v - lo[r]) / (hi[r] - lo[r] + 1e-300) * B);
+    return std::min<int64_t>(std::max<int64_t>(b, 0), B - 1);
+  };
+  std::vector<std::vector<int32_t>> bucket((size_t)(B * B * B));
+  for (int64_t t = 0; t < h.nt; ++t) {
+    int64_t bl[3] = {B, B, B}, bh[3] = {-1, -1, -1};
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) {
+        const int64_t b = bidx(h.xyz[3 * (size_t)h.tets[4 * t + k] + r], r);
+        bl[r] = std::min(bl[r], b);
+        bh[r] = std::max(bh[r], b);
+      }
+    for (int64_t a = bl[0]; a <= bh[0]; ++a)
+      for (int64_t b = bl[1]; b <= bh[1]; ++b)
+        for (int64_t c = bl[2]; c <= bh[2]; ++c) bucket[(a * B + b) * B + c].push_back((int32_t)t);
+  }
+  cell.assign(n, -1);
+  for (int64_t i = 0; i < n; ++i) {
+    const double* p = &x[3 * i];
+    const auto& bk = bucket[(bidx(p[0], 0) * B + bidx(p[1], 1)) * B + bidx(p[2], 2)];
+    double best = -1e300;
+    int32_t bt = -1;
+    for (int32_t t : bk) {
+      const double* a = &h.A[9 * (size_t)t];
+      const double* o = &h.X0[3 * (size_t)t];
+      const double d0 = p[0] - o[0], d1 = p[1] - o[1], d2 = p[2] - o[2];
+      const double l1 = a[0] * d0 + a[1] * d1 + a[2] * d2;
+      const double l2 = a[3] * d0 + a[4] * d1 + a[5] * d2;
+      const double l3 = a[6] * d0 + a[7] * d1 + a[8] * d2;
+      const double mn = std::min(std::min(1.0 - l1 - l2 - l3, l1), std::min(l2, l3));
+      if (mn > best) {
+        best = mn;
+        bt = t;
+      }
+    }
+    if (best >= -1e-9) cell[i] = bt;
+  }
+}
+
+struct SpeciesState {
+  double q = 0, m = 0, w = 0;
+  int64_t cap = 0, n = 0;  // n = slots in use
+  int64_t cs = 0;
+  int cur = 0;
+  DevBuf buf[2][8];        // x,y,z,vx,vy,vz,cell,id
+  DevBuf ke_partial, alive_partial;
+  int32_t next_id = 0;
+};
+
+struct Engine {
+  virtual ~Engine() = default;
+};
+
+}  // namespace scdat
+
+using namespace scdat;
+
+struct scdat_ctx {
+  scdat_config cfg{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  bool has_mesh = false, initialized = false;
+  HostMesh h;
+  std::array<int, kMaxGroups> gtype{};
+  std::array<int, kMaxGroups> gpartner{};
+  std::array<std::array<double, 3>, kMaxGroups> gshift{};
+  // device mesh
+  DevBuf d_tets, d_nb, d_groups, d_partner;
+  // ELL operator
+  DevBuf d_col, d_v32, d_v64, d_dinv32, d_dinv64, d_g[3];
+  DevBuf d_kind, d_lift, d_dval, d_dscale, d_phi1, d_kc_idx, d_kc_val, d_b64, d_x64, d_r64, d_partial, d_sc;
+  DevBuf d_inner[6];
+  // fields
+  DevBuf d_E, d_phiT, d_phi64, d_rho[2];
+  int rho_target = 0, rho_last = 0;
+  bool rho_clean[2] = {false, false};
+  // diagnostics
+  DevBuf d_ledger_count, d_ledger_energy, d_lost;
+  // sort scratch
+  DevBuf d_count, d_offset, d_cursor, d_tiles, d_total;
+  std::deque<SpeciesState> sp;
+  double unit = 0.0, w_ref = 0.0, q_init = 0.0, ccap = 0.0;
+  int64_t steps = 0;
+  double sim_time = 0.0;
+  PcgScalars* h_sc = nullptr;  // pinned mirror
+  int last_iters = 0, last_outer = 0, last_conv = 0;
+  double last_rel = 0.0;
+  double stage_ms[5] = {0, 0, 0, 0, 0};
+  double push_ms_sum = 0.0;
+  int64_t push_launches = 0;
+  std::vector<hipEvent_t> ev_pool;
+  bool is_fp64() const { return cfg.precision == 1; }
+};
+
+namespace {
+
+int fail(scdat_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+template <typename F>
+int guarded(scdat_ctx* c, F&& fn) {
+  if (!c) return SCDAT_ERR_ARG;
+  try {
+    fn();
+    return SCDAT_OK;
+  } catch (const Error& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    c->err = "host allocation failed";
+    return SCDAT_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return SCDAT_ERR_STATE;
+  }
+}
+
+size_t tsize(const scdat_ctx* c) { return c->is_fp64() ? 8 : 4; }
+
+template <typename T>
+void upload_mesh_device(scdat_ctx* c) {
+  HostMesh& h = c->h;
+  hipStream_t s = c->stream;
+  std::vector<TetMap<T>> tm((size_t)h.nt);
+  for (int64_t t = 0; t < h.nt; ++t) {
+    TetMap<T>& m = tm[t];
+    std::memset(&m, 0, sizeof(m));
+    for (int k = 0; k < 9; ++k) m.a[k] = (T)h.A[9 * t + k];
+    for (int k = 0; k < 3; ++k) m.x0[k] = (T)h.X0[3 * t + k];
+    for (int k = 0; k < 4; ++k) m.node[k] = h.tets[4 * t + k];
+  }
+  c->d_tets.upload(tm, s);
+  c->d_nb.upload(h.nb, s);
+  c->d_groups.upload(h.group, s);
+  c->d_partner.upload(h.partner, s);
+  c->d_col.upload(h.col, s);
+  std::vector<float> f32(h.aval.size());
+  for (size_t i = 0; i < f32.size(); ++i) f32[i] = (float)h.aval[i];
+  c->d_v32.upload(f32, s);
+  c->d_v64.upload(h.aval, s);
+  std::vector<float> di32(h.dinv.size());
+  for (size_t i = 0; i < di32.size(); ++i) di32[i] = (float)h.dinv[i];
+  c->d_dinv32.upload(di32, s);
+  c->d_dinv64.upload(h.dinv, s);
+  const std::vector<double>* g[3] = {&h.gx, &h.gy, &h.gz};
+  for (int r = 0; r < 3; ++r) {
+    if (sizeof(T) == 4) {
+      std::vector<float> v(g[r]->size());
+      for (size_t i = 0; i < v.size(); ++i) v[i] = (float)(*g[r])[i];
+      c->d_g[r].upload(v, s);
+    } else {
+      c->d_g[r].upload(*g[r], s);
+    }
+  }
+  c->d_kind.upload(h.kind, s);
+  c->d_lift.upload(h.lift, s);
+  c->d_dval.upload(h.dval, s);
+  c->d_dscale.upload(h.dscale, s);
+  c->d_kc_idx.upload(h.kc_idx, s);
+  c->d_kc_val.upload(h.kc_val, s);
+  const size_t nn = (size_t)h.nn;
+  c->d_b64.alloc(8 * nn);
+  c->d_x64.alloc(8 * nn);
+  c->d_r64.alloc(8 * nn);
+  c->d_phi1.alloc(8 * nn);
+  HIPCHK(hipMemsetAsync(c->d_x64.p, 0, 8 * nn, s));
+  HIPCHK(hipMemsetAsync(c->d_phi1.p, 0, 8 * nn, s));
+  const size_t grid = (nn + kBlock - 1) / kBlock;
+  c->d_partial.alloc(8 * 2 * grid + 64);
+  c->d_sc.alloc(sizeof(PcgScalars));
+  HIPCHK(hipMemsetAsync(c->d_sc.p, 0, sizeof(PcgScalars), s));
+  for (auto& b : c->d_inner) b.alloc(sizeof(T) * nn);
+  c->d_E.alloc(sizeof(Vec4<T>) * nn);
+  HIPCHK(hipMemsetAsync(c->d_E.p, 0, sizeof(Vec4<T>) * nn, s));
+  c->d_phiT.alloc(sizeof(T) * nn);
+  c->d_phi64.alloc(8 * nn);
+  std::vector<double> phi0(nn, 0.0);
+  for (size_t i = 0; i < nn; ++i)
+    if (h.kind[i] == SCDAT_NODE_DIRICHLET) phi0[i] = h.dval[i];
+  HIPCHK(hipMemcpyAsync(c->d_phi64.p, phi0.data(), 8 * nn, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_x64.p, phi0.data(), 8 * nn, hipMemcpyHostToDevice, s));
+  for (int k = 0; k < 2; ++k) {
+    c->d_rho[k].alloc(8 * nn);
+    HIPCHK(hipMemsetAsync(c->d_rho[k].p, 0, 8 * nn, s));
+    c->rho_clean[k] = true;
+  }
+  c->d_ledger_count.alloc(8 * kMaxGroups * kMaxSpecies);
+  c->d_ledger_energy.alloc(8 * kMaxGroups * kMaxSpecies);
+  c->d_lost.alloc(8);
+  HIPCHK(hipMemsetAsync(c->d_ledger_count.p, 0, 8 * kMaxGroups * kMaxSpecies, s));
+  HIPCHK(hipMemsetAsync(c->d_ledger_energy.p, 0, 8 * kMaxGroups * kMaxSpecies, s));
+  HIPCHK(hipMemsetAsync(c->d_lost.p, 0, 8, s));
+  c->d_count.alloc(4 * (size_t)h.nt);
+  c->d_offset.alloc(4 * (size_t)h.nt);
+  c->d_cursor.alloc(4 * (size_t)h.nt);
+  c->d_tiles.alloc(4 * sort_tiles(h.nt) + 16);
+  c->d_total.alloc(16);
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+EllDev ell(scdat_ctx* c) {
+  EllDev A{};
+  A.n = c->h.nn;
+  A.w = c->h.W;
+  A.col = c->d_col.as<int32_t>();
+  A.v32 = c->d_v32.as<float>();
+  A.v64 = c->d_v64.as<double>();
+  A.dinv32 = c->d_dinv32.as<float>();
+  A.dinv64 = c->d_dinv64.as<double>();
+  if (c->is_fp64()) {
+    A.gx64 = c->d_g[0].as<double>();
+    A.gy64 = c->d_g[1].as<double>();
+    A.gz64 = c->d_g[2].as<double>();
+  } else {
+    A.gx32 = c->d_g[0].as<float>();
+    A.gy32 = c->d_g[1].as<float>();
+    A.gz32 = c->d_g[2].as<float>();
+  }
+  return A;
+}
+
+FieldBufs fbufs(scdat_ctx* c) {
+  FieldBufs f{};
+  f.kind = c->d_kind.as<int32_t>();
+  f.lift = c->d_lift.as<double>();
+  f.dval = c->d_dval.as<double>();
+  f.dscale = c->d_dscale.as<double>();
+  f.phi1 = c->h.has_cond ? c->d_phi1.as<double>() : nullptr;
+  f.kc_idx = c->d_kc_idx.as<int32_t>();
+  f.kc_val = c->d_kc_val.as<double>();
+  f.n_kc = (int64_t)c->h.kc_idx.size();
+  f.b64 = c->d_b64.as<double>();
+  f.x64 = c->d_x64.as<double>();
+  f.r64 = c->d_r64.as<double>();
+  f.partial = c->d_partial.as<double>();
+  f.sc = c->d_sc.as<PcgScalars>();
+  return f;
+}
+
+template <typename T>
+InnerBufs<T> ibufs(scdat_ctx* c) {
+  InnerBufs<T> b;
+  b.r = c->d_inner[0].as<T>();
+  b.z = c->d_inner[1].as<T>();
+  b.d = c->d_inner[2].as<T>();
+  b.p0 = c->d_inner[3].as<T>();
+  b.p1 = c->d_inner[4].as<T>();
+  b.q = c->d_inner[5].as<T>();
+  return b;
+}
+
+void read_scalars(scdat_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->h_sc, c->d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// PCG driver: b64 and the scalars' bnorm2 are already set (k_rhs or host).
+// Iterations are launched in growing batches; kernels after convergence
+// return immediately, the host polls the done flags between batches.
+template <typename T>
+void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, int max_outer) {
+  const EllDev A = ell(c);
+  const InnerBufs<T> ib = ibufs<T>(c);
+  hipStream_t s = c->stream;
+  // scalar controls
+  {
+    PcgScalars ctl;
+    HIPCHK(hipMemcpyAsync(&ctl, f.sc, sizeof(ctl), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ctl.tol = tol;
+    ctl.inner_tol = inner_tol;
+    ctl.max_it = c->cfg.pcg_max_iter;
+    ctl.max_outer = max_outer;
+    HIPCHK(hipMemcpyAsync(f.sc, &ctl, sizeof(ctl), hipMemcpyHostToDevice, s));
+  }
+  for (int outer = 0; outer <= max_outer; ++outer) {
+    launch_resid64(A, f, s);
+    read_scalars(c);
+    if (c->h_sc->outer_done) break;
+    launch_inner_init<T>(A, f, ib, s);
+    int it = 0, batch = 8;
+    for (;;) {
+      for (int k = 0; k < batch; ++k, ++it) launch_inner_iter<T>(A, f, ib, it & 1, s);
+      read_scalars(c);
+      if (c->h_sc->inner_done) break;
+      batch = std::min(batch * 2, 64);
+    }
+    launch_apply<T>(A, f, ib, s);
+  }
+  read_scalars(c);
+  c->last_iters = c->h_sc->total_it;
+  c->last_outer = c->h_sc->outer_it;
+  const double bn = std::sqrt(c->h_sc->bnorm2);
+  c->last_rel = bn > 0 ? c->h_sc->rnorm / bn : c->h_sc->rnorm;
+  c->last_conv = c->h_sc->failed ? 0 : 1;
+}
+
+template <typename T>
+void solve_phi1(scdat_ctx* c) {
+  if (!c->h.has_cond) return;
+  FieldBufs f = fbufs(c);
+  f.x64 = c->d_phi1.as<double>();
+  HIPCHK(hipMemcpyAsync(c->d_b64.p, c->h.b1.data(), 8 * (size_t)c->h.nn, hipMemcpyHostToDevice, c->stream));
+  PcgScalars init{};
+  double bb = 0;
+  for (double v : c->h.b1) bb += v * v;
+  init.bnorm2 = bb;
+  HIPCHK(hipMemcpyAsync(f.sc, &init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  pcg_drive<T>(c, f, 1e-13, c->is_fp64() ? 0.0 : 1e-5, 12);
+  std::vector<double> phi1((size_t)c->h.nn);
+  HIPCHK(hipMemcpyAsync(phi1.data(), c->d_phi1.p, 8 * phi1.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double cap = 0;
+  for (size_t k = 0; k < c->h.kc_idx.size(); ++k) cap += c->h.kc_val[k] * phi1[c->h.kc_idx[k]];
+  c->ccap = cap;
+  if (!(cap > 0)) throw Error(SCDAT_ERR_SOLVER, "conductor capacitance not positive");
+  PcgScalars z{};
+  HIPCHK(hipMemcpyAsync(f.sc, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+}
+
+FloatArgs float_args(scdat_ctx* c) {
+  FloatArgs fa{};
+  fa.q_init = c->q_init;
+  fa.ledger_count = c->d_ledger_count.as<unsigned long long>();
+  fa.emit_count = nullptr;
+  for (size_t s = 0; s < c->sp.size() && s < (size_t)kMaxSpecies; ++s) fa.qw[s] = c->sp[s].q * c->sp[s].w;
+  fa.cond_mask = 0;
+  for (int g = 0; g < kMaxGroups; ++g)
+    if (c->gtype[g] == SCDAT_BC_CONDUCTOR) fa.cond_mask |= 1u << g;
+  return fa;
+}
+
+template <typename T>
+void do_solve(scdat_ctx* c, int rho_idx) {
+  const EllDev A = ell(c);
+  FieldBufs f = fbufs(c);
+  launch_rhs(A, f, c->d_rho[rho_idx].as<unsigned long long>(), c->unit, c->cfg.load_sign, c->stream);
+  const bool fp64 = c->is_fp64();
+  pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12);
+  if (c->h.has_cond) {
+    HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_float(f, float_args(c), c->stream);
+  }
+}
+
+template <typename T>
+void do_field(scdat_ctx* c, unsigned long long* rho_clear) {
+  launch_phi_efield<T>(ell(c), fbufs(c), c->d_phiT.as<T>(), c->d_phi64.as<double>(),
+                       c->d_E.as<Vec4<T>>(), rho_clear, c->stream);
+}
+
+template <typename T>
+PushArgs<T> push_args(scdat_ctx* c, int si) {
+  SpeciesState& S = c->sp[si];
+  PushArgs<T> a{};
+  a.tets = c->d_tets.as<TetMap<T>>();
+  a.nb = c->d_nb.as<int4>();
+  a.groups = c->d_groups.as<int4>();
+  a.partner = c->d_partner.as<int4>();
+  a.E = c->d_E.as<Vec4<T>>();
+  a.rho = c->d_rho[c->rho_target].as<unsigned long long>();
+  for (int g = 0; g < kMaxGroups; ++g) {
+    a.gtype[g] = c->gtype[g];
+    for (int r = 0; r < 3; ++r) a.gshift[g][r] = (T)c->gshift[g][r];
+  }
+  a.x = S.buf[S.cur][0].as<T>();
+  a.y = S.buf[S.cur][1].as<T>();
+  a.z = S.buf[S.cur][2].as<T>();
+  a.vx = S.buf[S.cur][3].as<T>();
+  a.vy = S.buf[S.cur][4].as<T>();
+  a.vz = S.buf[S.cur][5].as<T>();
+  a.cell = S.buf[S.cur][6].as<int32_t>();
+  a.n = S.n;
+  a.species = si;
+  a.qm = (T)(S.q / S.m);
+  a.cs = S.cs;
+  a.mass = S.m;
+  a.weight = S.w;
+  a.energy_weight = std::max(1.0, S.w);
+  a.dt = (T)c->cfg.dt;
+  a.adaptive = c->cfg.adaptive_substep;
+  a.floor_ratio = (T)c->cfg.floor_ratio;
+  a.max_sub = c->cfg.max_substeps;
+  a.max_walk = 256;
+  a.tol = sizeof(T) == 4 ? (T)2e-6 : (T)1e-10;
+  for (int r = 0; r < 3; ++r) a.B[r] = (T)c->cfg.bfield[r];
+  const double bm = std::sqrt(c->cfg.bfield[0] * c->cfg.bfield[0] + c->cfg.bfield[1] * c->cfg.bfield[1] +
+                              c->cfg.bfield[2] * c->cfg.bfield[2]);
+  a.bmag = (T)bm;
+  a.has_b = bm > 1e-20 ? 1 : 0;
+  a.ledger_count = c->d_ledger_count.as<unsigned long long>();
+  a.ledger_energy = c->d_ledger_energy.as<double>();
+  a.lost = c->d_lost.as<unsigned long long>();
+  a.ke_partial = S.ke_partial.as<double>();
+  a.alive_partial = S.alive_partial.as<int32_t>();
+  return a;
+}
+
+template <typename T>
+void zero_rho(scdat_ctx* c, int k) {
+  if (!c->rho_clean[k]) {
+    HIPCHK(hipMemsetAsync(c->d_rho[k].p, 0, 8 * (size_t)c->h.nn, c->stream));
+    c->rho_clean[k] = true;
+  }
+}
+
+hipEvent_t take_event(scdat_ctx* c, size_t idx) {
+  while (c->ev_pool.size() <= idx) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[idx];
+}
+
+// One push of all species into rho_target; events bracket each launch.
+template <typename T>
+void do_push(scdat_ctx* c, std::vector<std::pair<size_t, size_t>>* ev, size_t* evn) {
+  zero_rho<T>(c, c->rho_target);
+  for (size_t si = 0; si < c->sp.size(); ++si) {
+    if (c->sp[si].n == 0) continue;
+    const PushArgs<T> a = push_args<T>(c, (int)si);
+    if (ev) {
+      const size_t e0 = (*evn)++, e1 = (*evn)++;
+      HIPCHK(hipEventRecord(take_event(c, e0), c->stream));
+      launch_push<T>(a, c->stream);
+      HIPCHK(hipEventRecord(take_event(c, e1), c->stream));
+      ev->push_back({e0, e1});
+    } else {
+      launch_push<T>(a, c->stream);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  c->rho_clean[c->rho_target] = false;
+  c->rho_last = c->rho_target;
+  c->rho_target = 1 - c->rho_target;
+}
+
+template <typename T>
+void do_sort_species(scdat_ctx* c, int si) {
+  SpeciesState& S = c->sp[si];
+  if (S.n == 0) return;
+  SortArgs<T> a{};
+  const int s0 = S.cur, s1 = 1 - S.cur;
+  a.src = {S.buf[s0][0].as<T>(), S.buf[s0][1].as<T>(), S.buf[s0][2].as<T>(), S.buf[s0][3].as<T>(),
+           S.buf[s0][4].as<T>(), S.buf[s0][5].as<T>(), S.buf[s0][6].as<int32_t>(),
+           S.buf[s0][7].as<int32_t>()};
+  a.dst = {S.buf[s1][0].as<T>(), S.buf[s1][1].as<T>(), S.buf[s1][2].as<T>(), S.buf[s1][3].as<T>(),
+           S.buf[s1][4].as<T>(), S.buf[s1][5].as<T>(), S.buf[s1][6].as<int32_t>(),
+           S.buf[s1][7].as<int32_t>()};
+  a.n = S.n;
+  a.ncell = c->h.nt;
+  a.count = c->d_count.as<uint32_t>();
+  a.offset = c->d_offset.as<uint32_t>();
+  a.cursor = c->d_cursor.as<uint32_t>();
+  a.tiles = c->d_tiles.as<uint32_t>();
+  a.total = c->d_total.as<uint32_t>();
+  launch_sort<T>(a, c->stream);
+  HIPCHK(hipGetLastError());
+  uint32_t tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, a.total, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  S.cur = s1;
+  S.n = tot;
+}
+
+template <typename T>
+void do_sort(scdat_ctx* c) {
+  for (size_t si = 0; si < c->sp.size(); ++si) do_sort_species<T>(c, (int)si);
+}
+
+template <typename T>
+void do_initialize(scdat_ctx* c) {
+  // Periodic partners.
+  HostMesh& h = c->h;
+  h.partner.assign(4 * (size_t)h.nt, -1);
+  for (int g = 0; g < kMaxGroups; ++g) {
+    if (c->gtype[g] != SCDAT_BC_PERIODIC) continue;
+    const int pg = c->gpartner[g];
+    if (pg < 0 || pg >= kMaxGroups || c->gtype[pg] != SCDAT_BC_PERIODIC)
+      throw Error(SCDAT_ERR_ARG, "periodic group " + std::to_string(g) + " lacks a periodic partner");
+    if (pg < g) continue;
+    const int m = periodic_partners(h.nn, h.xyz.data(), h.nt, h.tets.data(), h.group.data(), g, pg,
+                                    c->gshift[g].data(), h.partner);
+    if (m == 0) throw Error(SCDAT_ERR_ARG, "periodic groups do not match under the given shift");
+  }
+  c->d_partner.upload(h.partner, c->stream);
+  solve_phi1<T>(c);
+  // Initial deposit -> Poisson -> E (PICCycle::initialize + first field).
+  zero_rho<T>(c, 0);
+  c->rho_target = 0;
+  for (size_t si = 0; si < c->sp.size(); ++si) {
+    if (c->sp[si].n == 0) continue;
+    launch_deposit<T>(push_args<T>(c, (int)si), c->stream);
+  }
+  c->rho_clean[0] = false;
+  c->rho_last = 0;
+  c->rho_target = 1;
+  do_solve<T>(c, 0);
+  zero_rho<T>(c, 1);
+  do_field<T>(c, nullptr);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->initialized = true;
+}
+
+template <typename T>
+void do_step(scdat_ctx* c, int n) {
+  std::vector<std::pair<size_t, size_t>> pev;
+  size_t evn = 0;
+  struct StageEv { size_t a, b, cc, d, e; };
+  std::vector<StageEv> sev;
+  c->push_ms_sum = 0;
+  c->push_launches = 0;
+  for (int k = 0; k < n; ++k) {
+    StageEv se;
+    se.a = evn++;
+    HIPCHK(hipEventRecord(take_event(c, se.a), c->stream));
+    do_push<T>(c, &pev, &evn);
+    se.b = evn++;
+    HIPCHK(hipEventRecord(take_event(c, se.b), c->stream));
+    do_solve<T>(c, c->rho_last);
+    se.cc = evn++;
+    HIPCHK(hipEventRecord(take_event(c, se.cc), c->stream));
+    // E for the next push; clears the buffer the next push deposits into.
+    do_field<T>(c, c->d_rho[c->rho_target].as<unsigned long long>());
+    c->rho_clean[c->rho_target] = true;
+    se.d = evn++;
+    HIPCHK(hipEventRecord(take_event(c, se.d), c->stream));
+    c->steps++;
+    c->sim_time += c->cfg.dt;
+    if (c->cfg.sort_interval > 0 && c->steps % c->cfg.sort_interval == 0) do_sort<T>(c);
+    se.e = evn++;
+    HIPCHK(hipEventRecord(take_event(c, se.e), c->stream));
+    sev.push_back(se);
+    if (!c->last_conv) {
+      // PICCycle keeps the previous potential when the solver fails
+      // (PICCycle.cpp:731-767); here the field is still updated from the
+      // best iterate and the failure is reported in the statistics.
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 5; ++k) c->stage_ms[k] = 0;
+  for (auto& se : sev) {
+    float t;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[se.a], c->ev_pool[se.b]));
+    c->stage_ms[0] += t;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[se.b], c->ev_pool[se.cc]));
+    c->stage_ms[2] += t;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[se.cc], c->ev_pool[se.d]));
+    c->stage_ms[3] += t;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[se.d], c->ev_pool[se.e]));
+    c->stage_ms[4] += t;
+  }
+  for (auto& pe : pev) {
+    float t;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[pe.first], c->ev_pool[pe.second]));
+    c->push_ms_sum += t;
+    c->push_launches++;
+  }
+}
+
+template <typename T>
+void set_particles_t(scdat_ctx* c, int si, int64_t n, const double* x, const double* v,
+                     const int32_t* cell) {
+  SpeciesState& S = c->sp[si];
+  if (n > S.cap) throw Error(SCDAT_ERR_NOMEM, "particle count exceeds species capacity");
+  std::vector<int32_t> cl;
+  if (cell) cl.assign(cell, cell + n);
+  else host_locate(c->h, n, x, cl);
+  for (int64_t i = 0; i < n; ++i)
+    if (cl[i] < -1 || cl[i] >= c->h.nt) throw Error(SCDAT_ERR_ARG, "cell index out of range");
+  std::vector<T> f[6];
+  for (int r = 0; r < 6; ++r) f[r].resize(n);
+  for (int64_t i = 0; i < n; ++i)
+    for (int r = 0; r < 3; ++r) {
+      f[r][i] = (T)x[3 * i + r];
+      f[3 + r][i] = (T)v[3 * i + r];
+    }
+  std::vector<int32_t> id(n);
+  for (int64_t i = 0; i < n; ++i) id[i] = (int32_t)i;
+  S.cur = 0;
+  for (int r = 0; r < 6; ++r)
+    if (n) HIPCHK(hipMemcpyAsync(S.buf[0][r].p, f[r].data(), sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+  if (n) HIPCHK(hipMemcpyAsync(S.buf[0][6].p, cl.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+  if (n) HIPCHK(hipMemcpyAsync(S.buf[0][7].p, id.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+  S.n = n;
+  S.next_id = (int32_t)n;
+  do_sort_species<T>(c, si);
+}
+
+template <typename T>
+void get_particles_t(scdat_ctx* c, int si, double* x, double* v, int32_t* cell, int32_t* id) {
+  SpeciesState& S = c->sp[si];
+  const int64_t n = S.n;
+  if (n == 0) return;
+  std::vector<T> f[6];
+  for (int r = 0; r < 6; ++r) {
+    f[r].resize(n);
+    HIPCHK(hipMemcpyAsync(f[r].data(), S.buf[S.cur][r].p, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  std::vector<int32_t> cl(n), ids(n);
+  HIPCHK(hipMemcpyAsync(cl.data(), S.buf[S.cur][6].p, 4 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(ids.data(), S.buf[S.cur][7].p, 4 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < n; ++i)
+    for (int r = 0; r < 3; ++r) {
+      if (x) x[3 * i + r] = (double)f[r][i];
+      if (v) v[3 * i + r] = (double)f[3 + r][i];
+    }
+  if (cell) std::memcpy(cell, cl.data(), 4 * n);
+  if (id) std::memcpy(id, ids.data(), 4 * n);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void scdat_config_default(scdat_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->precision = 0;
+  c->dt = 1e-12;
+  c->adaptive_substep = 1;
+  c->floor_ratio = 1e-3;
+  c->max_substeps = 64;
+  c->poisson_operator = 0;
+  c->permittivity = 8.8541878128e-12;
+  c->load_sign = 1;
+  c->pcg_tol = 1e-8;
+  c->pcg_max_iter = 20000;
+  c->inner_tol = 1e-5;
+  c->sort_interval = 20;
+}
+
+int scdat_box_mesh_size(const scdat_box_spec* spec, int64_t* n_nodes, int64_t* n_tets) {
+  if (!spec || !n_nodes || !n_tets) return SCDAT_ERR_ARG;
+  BoxSpec s;
+  std::memcpy(&s, spec, sizeof(s));
+  std::vector<double> xyz;
+  std::vector<int32_t> tets, kind, group;
+  std::string err;
+  try {
+    if (!box_mesh(s, xyz, tets, kind, group, err)) return SCDAT_ERR_ARG;
+  } catch (...) {
+    return SCDAT_ERR_NOMEM;
+  }
+  *n_nodes = (int64_t)kind.size();
+  *n_tets = (int64_t)(tets.size() / 4);
+  return SCDAT_OK;
+}
+
+int scdat_box_mesh_build(const scdat_box_spec* spec, double* xyz, int32_t* tets, int32_t* node_kind,
+                         int32_t* face_group) {
+  if (!spec || !xyz || !tets || !node_kind) return SCDAT_ERR_ARG;
+  BoxSpec s;
+  std::memcpy(&s, spec, sizeof(s));
+  std::vector<double> vx;
+  std::vector<int32_t> vt, vk, vg;
+  std::string err;
+  try {
+    if (!box_mesh(s, vx, vt, vk, vg, err)) return SCDAT_ERR_ARG;
+  } catch (...) {
+    return SCDAT_ERR_NOMEM;
+  }
+  std::memcpy(xyz, vx.data(), sizeof(double) * vx.size());
+  std::memcpy(tets, vt.data(), sizeof(int32_t) * vt.size());
+  std::memcpy(node_kind, vk.data(), sizeof(int32_t) * vk.size());
+  if (face_group) std::memcpy(face_group, vg.data(), sizeof(int32_t) * vg.size());
+  return SCDAT_OK;
+}
+
+int scdat_device_info(char* name, int32_t name_len, int32_t* n_cu) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return SCDAT_ERR_DEVICE;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, 0) != hipSuccess) return SCDAT_ERR_DEVICE;
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (n_cu) *n_cu = p.multiProcessorCount;
+  return SCDAT_OK;
+}
+
+int scdat_create(const scdat_config* cfg, scdat_ctx** out) {
+  if (!out) return SCDAT_ERR_ARG;
+  *out = nullptr;
+  scdat_ctx* c = new (std::nothrow) scdat_ctx();
+  if (!c) return SCDAT_ERR_NOMEM;
+  if (cfg) c->cfg = *cfg;
+  else scdat_config_default(&c->cfg);
+  if (c->cfg.precision != 0 && c->cfg.precision != 1) {
+    delete c;
+    return SCDAT_ERR_ARG;
+  }
+  if (!(c->cfg.dt > 0) || c->cfg.max_substeps < 1 || c->cfg.pcg_max_iter < 1) {
+    delete c;
+    return SCDAT_ERR_ARG;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    delete c;
+    return SCDAT_ERR_DEVICE;
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return SCDAT_ERR_DEVICE;
+  }
+  if (hipHostMalloc((void**)&c->h_sc, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SCDAT_ERR_DEVICE;
+  }
+  for (int g = 0; g < kMaxGroups; ++g) {
+    c->gtype[g] = SCDAT_BC_ABSORB;
+    c->gpartner[g] = -1;
+  }
+  *out = c;
+  return SCDAT_OK;
+}
+
+void scdat_destroy(scdat_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->h_sc) (void)hipHostFree(c->h_sc);
+  hipStream_t s = c->stream;
+  delete c;  // DevBuf destructors free device memory
+  if (s) (void)hipStreamDestroy(s);
+}
+
+const char* scdat_last_error(const scdat_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int scdat_set_mesh(scdat_ctx* c, int64_t n_nodes, const double* xyz, int64_t n_tets,
+                   const int32_t* tets, const int32_t* node_kind, const double* dirichlet,
+                   const int32_t* face_group) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (n_nodes < 4 || n_tets < 1 || !xyz || !tets || !node_kind)
+    return fail(c, SCDAT_ERR_ARG, "set_mesh: empty mesh or null array");
+  if (n_nodes > INT32_MAX || n_tets > INT32_MAX) return fail(c, SCDAT_ERR_ARG, "set_mesh: mesh too large");
+  return guarded(c, [&] {
+    HostMesh& h = c->h;
+    h = HostMesh();
+    h.nn = n_nodes;
+    h.nt = n_tets;
+    h.xyz.assign(xyz, xyz + 3 * n_nodes);
+    h.tets.assign(tets, tets + 4 * n_tets);
+    for (int32_t v : h.tets)
+      if (v < 0 || v >= n_nodes) throw Error(SCDAT_ERR_ARG, "set_mesh: tet references a missing node");
+    h.kind.assign(node_kind, node_kind + n_nodes);
+    for (int32_t k : h.kind)
+      if (k < 0 || k > 2) throw Error(SCDAT_ERR_ARG, "set_mesh: node kind must be 0, 1 or 2");
+    h.dval.assign(n_nodes, 0.0);
+    if (dirichlet) h.dval.assign(dirichlet, dirichlet + n_nodes);
+    h.group.assign(4 * n_tets, 0);
+    if (face_group) h.group.assign(face_group, face_group + 4 * n_tets);
+    prepare_mesh(h, c->cfg.poisson_operator, c->cfg.permittivity);
+    if (c->is_fp64()) upload_mesh_device<double>(c);
+    else upload_mesh_device<float>(c);
+    c->has_mesh = true;
+    c->initialized = false;
+  });
+}
+
+int scdat_set_boundary(scdat_ctx* c, int32_t group, int32_t type, int32_t partner_group,
+                       const double* shift) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (group < 0 || group >= kMaxGroups || type < 0 || type > 3)
+    return fail(c, SCDAT_ERR_ARG, "set_boundary: bad group or type");
+  c->gtype[group] = type;
+  c->gpartner[group] = partner_group;
+  for (int r = 0; r < 3; ++r) c->gshift[group][r] = shift ? shift[r] : 0.0;
+  c->initialized = false;
+  return SCDAT_OK;
+}
+
+int scdat_add_species(scdat_ctx* c, double q, double m, double w, int64_t capacity, int32_t* id) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "add_species: set the mesh first");
+  if (!(m > 0) || !(w > 0) || capacity < 0 || c->sp.size() >= (size_t)kMaxSpecies)
+    return fail(c, SCDAT_ERR_ARG, "add_species: bad mass/weight/capacity or too many species");
+  if (capacity > INT32_MAX) return fail(c, SCDAT_ERR_ARG, "add_species: capacity above 2^31");
+  return guarded(c, [&] {
+    if (c->sp.empty()) {
+      c->w_ref = w;
+      // rho_fx unit: one elementary charge of reference weight is 2^32 units
+      // (2^8 charge resolution x 2^24 barycentric resolution).
+      c->unit = kElem * w / 4294967296.0;
+    }
+    c->sp.emplace_back();
+    SpeciesState& S = c->sp.back();
+    S.q = q;
+    S.m = m;
+    S.w = w;
+    S.cap = capacity;
+    const double csd = q * w / (kElem * c->w_ref) * 256.0;
+    S.cs = (int64_t)std::llround(csd);
+    if (std::abs(csd - (double)S.cs) > 1e-6 * std::max(1.0, std::abs(csd)))
+      throw Error(SCDAT_ERR_ARG, "add_species: q*w must be a multiple of e*w_ref/256");
+    const size_t ts = tsize(c);
+    for (int b = 0; b < 2; ++b) {
+      for (int r = 0; r < 6; ++r) S.buf[b][r].alloc(ts * (size_t)capacity);
+      S.buf[b][6].alloc(4 * (size_t)capacity);
+      S.buf[b][7].alloc(4 * (size_t)capacity);
+    }
+    const size_t nb = (size_t)((capacity + kBlock - 1) / kBlock) + 1;
+    S.ke_partial.alloc(8 * nb);
+    S.alive_partial.alloc(4 * nb);
+    HIPCHK(hipMemsetAsync(S.ke_partial.p, 0, 8 * nb, c->stream));
+    HIPCHK(hipMemsetAsync(S.alive_partial.p, 0, 4 * nb, c->stream));
+    if (id) *id = (int32_t)c->sp.size() - 1;
+  });
+}
+
+int scdat_set_particles(scdat_ctx* c, int32_t species, int64_t n, const double* x, const double* v,
+                        const int32_t* cell) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (species < 0 || species >= (int)c->sp.size()) return fail(c, SCDAT_ERR_ARG, "set_particles: bad species");
+  if (n < 0 || (n > 0 && (!x || !v))) return fail(c, SCDAT_ERR_ARG, "set_particles: bad arrays");
+  return guarded(c, [&] {
+    if (c->is_fp64()) set_particles_t<double>(c, species, n, x, v, cell);
+    else set_particles_t<float>(c, species, n, x, v, cell);
+  });
+}
+
+int scdat_get_particle_count(scdat_ctx* c, int32_t species, int64_t* n_alive, int64_t* n_slots) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (species < 0 || species >= (int)c->sp.size()) return fail(c, SCDAT_ERR_ARG, "bad species");
+  return guarded(c, [&] {
+    SpeciesState& S = c->sp[species];
+    if (n_slots) *n_slots = S.n;
+    if (n_alive) {
+      std::vector<int32_t> cl(S.n);
+      if (S.n) {
+        HIPCHK(hipMemcpyAsync(cl.data(), S.buf[S.cur][6].p, 4 * S.n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      }
+      int64_t a = 0;
+      for (int32_t v : cl) a += v >= 0;
+      *n_alive = a;
+    }
+  });
+}
+
+int scdat_get_particles(scdat_ctx* c, int32_t species, double* x, double* v, int32_t* cell,
+                        int32_t* id) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (species < 0 || species >= (int)c->sp.size()) return fail(c, SCDAT_ERR_ARG, "bad species");
+  return guarded(c, [&] {
+    if (c->is_fp64()) get_particles_t<double>(c, species, x, v, cell, id);
+    else get_particles_t<float>(c, species, x, v, cell, id);
+  });
+}
+
+int scdat_set_conductor_charge(scdat_ctx* c, double q) {
+  if (!c) return SCDAT_ERR_ARG;
+  c->q_init = q;
+  return SCDAT_OK;
+}
+
+int scdat_initialize(scdat_ctx* c) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "initialize: no mesh");
+  return guarded(c, [&] {
+    if (c->is_fp64()) do_initialize<double>(c);
+    else do_initialize<float>(c);
+  });
+}
+
+int scdat_step(scdat_ctx* c, int32_t n) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->initialized) return fail(c, SCDAT_ERR_STATE, "step: call scdat_initialize first");
+  if (n < 0) return fail(c, SCDAT_ERR_ARG, "step: negative count");
+  const int rc = guarded(c, [&] {
+    if (c->is_fp64()) do_step<double>(c, n);
+    else do_step<float>(c, n);
+  });
+  if (rc == SCDAT_OK && !c->last_conv) {
+    c->err = "Poisson PCG did not reach the tolerance";
+    return SCDAT_ERR_SOLVER;
+  }
+  return rc;
+}
+
+int scdat_push(scdat_ctx* c) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->initialized) return fail(c, SCDAT_ERR_STATE, "push: call scdat_initialize first");
+  return guarded(c, [&] {
+    if (c->is_fp64()) do_push<double>(c, nullptr, nullptr);
+    else do_push<float>(c, nullptr, nullptr);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int scdat_deposit(scdat_ctx* c) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "deposit: no mesh");
+  return guarded(c, [&] {
+    const int k = c->rho_target;
+    HIPCHK(hipMemsetAsync(c->d_rho[k].p, 0, 8 * (size_t)c->h.nn, c->stream));
+    for (size_t si = 0; si < c->sp.size(); ++si) {
+      if (c->sp[si].n == 0) continue;
+      if (c->is_fp64()) launch_deposit<double>(push_args<double>(c, (int)si), c->stream);
+      else launch_deposit<float>(push_args<float>(c, (int)si), c->stream);
+    }
+    c->rho_clean[k] = false;
+    c->rho_last = k;
+    c->rho_target = 1 - k;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int scdat_solve(scdat_ctx* c) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "solve: no mesh");
+  const int rc = guarded(c, [&] {
+    if (c->h.has_cond && !(c->ccap > 0)) {
+      if (c->is_fp64()) solve_phi1<double>(c);
+      else solve_phi1<float>(c);
+    }
+    if (c->is_fp64()) {
+      do_solve<double>(c, c->rho_last);
+      do_field<double>(c, nullptr);
+    } else {
+      do_solve<float>(c, c->rho_last);
+      do_field<float>(c, nullptr);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+  if (rc == SCDAT_OK && !c->last_conv) {
+    c->err = "Poisson PCG did not reach the tolerance";
+    return SCDAT_ERR_SOLVER;
+  }
+  return rc;
+}
+
+int scdat_compute_efield(scdat_ctx* c) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "compute_efield: no mesh");
+  return guarded(c, [&] {
+    const size_t nn = (size_t)c->h.nn;
+    std::vector<double> p(nn);
+    HIPCHK(hipMemcpyAsync(p.data(), c->d_phi64.p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->is_fp64()) {
+      HIPCHK(hipMemcpyAsync(c->d_phiT.p, p.data(), 8 * nn, hipMemcpyHostToDevice, c->stream));
+      launch_efield_only<double>(ell(c), c->d_phiT.as<double>(), c->d_E.as<Vec4<double>>(), c->stream);
+    } else {
+      std::vector<float> pf(nn);
+      for (size_t i = 0; i < nn; ++i) pf[i] = (float)p[i];
+      HIPCHK(hipMemcpyAsync(c->d_phiT.p, pf.data(), 4 * nn, hipMemcpyHostToDevice, c->stream));
+      launch_efield_only<float>(ell(c), c->d_phiT.as<float>(), c->d_E.as<Vec4<float>>(), c->stream);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int scdat_sort(scdat_ctx* c) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "sort: no mesh");
+  return guarded(c, [&] {
+    if (c->is_fp64()) do_sort<double>(c);
+    else do_sort<float>(c);
+  });
+}
+
+int scdat_set_efield(scdat_ctx* c, const double* E) {
+  if (!c || !E) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_efield: no mesh");
+  return guarded(c, [&] {
+    const size_t nn = (size_t)c->h.nn;
+    if (c->is_fp64()) {
+      std::vector<Vec4<double>> e(nn);
+      for (size_t i = 0; i < nn; ++i) e[i] = {E[3 * i], E[3 * i + 1], E[3 * i + 2], 0.0};
+      HIPCHK(hipMemcpyAsync(c->d_E.p, e.data(), sizeof(Vec4<double>) * nn, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+      std::vector<Vec4<float>> e(nn);
+      for (size_t i = 0; i < nn; ++i) e[i] = {(float)E[3 * i], (float)E[3 * i + 1], (float)E[3 * i + 2], 0.f};
+      HIPCHK(hipMemcpyAsync(c->d_E.p, e.data(), sizeof(Vec4<float>) * nn, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
+int scdat_set_potential(scdat_ctx* c, const double* phi) {
+  if (!c || !phi) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_potential: no mesh");
+  const int rc = guarded(c, [&] {
+    const size_t nn = (size_t)c->h.nn;
+    HIPCHK(hipMemcpyAsync(c->d_phi64.p, phi, 8 * nn, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+  return rc != SCDAT_OK ? rc : scdat_compute_efield(c);
+}
+
+int scdat_set_rho(scdat_ctx* c, const double* rho) {
+  if (!c || !rho) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_rho: no mesh");
+  return guarded(c, [&] {
+    const size_t nn = (size_t)c->h.nn;
+    std::vector<long long> fx(nn);
+    for (size_t i = 0; i < nn; ++i) fx[i] = std::llround(rho[i] / c->unit);
+    const int k = c->rho_target;
+    HIPCHK(hipMemcpyAsync(c->d_rho[k].p, fx.data(), 8 * nn, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->rho_clean[k] = false;
+    c->rho_last = k;
+    c->rho_target = 1 - k;
+  });
+}
+
+int scdat_get_fields(scdat_ctx* c, double* rho, double* phi, double* E) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "get_fields: no mesh");
+  return guarded(c, [&] {
+    const size_t nn = (size_t)c->h.nn;
+    if (rho) {
+      std::vector<long long> fx(nn);
+      HIPCHK(hipMemcpyAsync(fx.data(), c->d_rho[c->rho_last].p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (size_t i = 0; i < nn; ++i) rho[i] = (double)fx[i] * c->unit;
+    }
+    if (phi) {
+      HIPCHK(hipMemcpyAsync(phi, c->d_phi64.p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (E) {
+      if (c->is_fp64()) {
+        std::vector<Vec4<double>> e(nn);
+        HIPCHK(hipMemcpyAsync(e.data(), c->d_E.p, sizeof(Vec4<double>) * nn, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < nn; ++i) {
+          E[3 * i] = e[i].x;
+          E[3 * i + 1] = e[i].y;
+          E[3 * i + 2] = e[i].z;
+        }
+      } else {
+        std::vector<Vec4<float>> e(nn);
+        HIPCHK(hipMemcpyAsync(e.data(), c->d_E.p, sizeof(Vec4<float>) * nn, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < nn; ++i) {
+          E[3 * i] = e[i].x;
+          E[3 * i + 1] = e[i].y;
+          E[3 * i + 2] = e[i].z;
+        }
+      }
+    }
+  });
+}
+
+int scdat_get_rho_fixed(scdat_ctx* c, int64_t* rho_fx, double* unit) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "get_rho_fixed: no mesh");
+  return guarded(c, [&] {
+    if (unit) *unit = c->unit;
+    if (rho_fx) {
+      HIPCHK(hipMemcpyAsync(rho_fx, c->d_rho[c->rho_last].p, 8 * (size_t)c->h.nn, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
+int scdat_get_tet_maps(scdat_ctx* c, float* maps) {
+  if (!c || !maps) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "get_tet_maps: no mesh");
+  const HostMesh& h = c->h;
+  for (int64_t t = 0; t < h.nt; ++t) {
+    for (int k = 0; k < 9; ++k) maps[12 * t + k] = (float)h.A[9 * t + k];
+    for (int k = 0; k < 3; ++k) maps[12 * t + 9 + k] = (float)h.X0[3 * t + k];
+  }
+  return SCDAT_OK;
+}
+
+int scdat_get_species_charge_fixed(scdat_ctx* c, int32_t species, int64_t* cs) {
+  if (!c || !cs) return SCDAT_ERR_ARG;
+  if (species < 0 || species >= (int)c->sp.size()) return fail(c, SCDAT_ERR_ARG, "bad species");
+  *cs = c->sp[species].cs;
+  return SCDAT_OK;
+}
+
+int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
+  if (!c || !st) return SCDAT_ERR_ARG;
+  return guarded(c, [&] {
+    std::memset(st, 0, sizeof(*st));
+    st->completed_steps = c->steps;
+    st->simulation_time = c->sim_time;
+    double ke = 0;
+    int64_t alive = 0;
+    for (auto& S : c->sp) {
+      const size_t nb = (size_t)((S.n + kBlock - 1) / kBlock);
+      if (!nb) continue;
+      std::vector<double> k(nb);
+      std::vector<int32_t> a(nb);
+      HIPCHK(hipMemcpyAsync(k.data(), S.ke_partial.p, 8 * nb, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(a.data(), S.alive_partial.p, 4 * nb, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (size_t b = 0; b < nb; ++b) {
+        ke += k[b];
+        alive += a[b];
+      }
+    }
+    st->total_energy = ke;
+    st->active_particles = alive;
+    if (c->has_mesh) {
+      const size_t nn = (size_t)c->h.nn;
+      std::vector<long long> fx(nn);
+      HIPCHK(hipMemcpyAsync(fx.data(), c->d_rho[c->rho_last].p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
+      std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
+      HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
+      unsigned long long lost = 0;
+      HIPCHK(hipMemcpyAsync(&lost, c->d_lost.p, 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(c->h_sc, c->d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      long double tq = 0;
+      for (long long v : fx) tq += (long double)v;
+      st->total_charge = (double)(tq * (long double)c->unit);
+      double surf = 0, qc = c->q_init;
+      for (int g = 0; g < kMaxGroups; ++g)
+        for (size_t s = 0; s < c->sp.size(); ++s) {
+          const double q = (double)cnt[g * kMaxSpecies + s] * c->sp[s].q * c->sp[s].w;
+          surf += q;
+          if (c->gtype[g] == SCDAT_BC_CONDUCTOR) qc += q;
+        }
+      st->surface_deposited_charge = surf;
+      st->conductor_charge = qc;
+      st->floating_potential = c->h.has_cond ? c->h_sc->vfloat : 0.0;
+      st->lost_particles = (int64_t)lost;
+    }
+    st->pcg_iterations = c->last_iters;
+    st->pcg_outer = c->last_outer;
+    st->pcg_rel_residual = c->last_rel;
+    st->pcg_converged = c->last_conv;
+  });
+}
+
+int scdat_get_ledger(scdat_ctx* c, scdat_ledger_entry* out, int32_t n_groups) {
+  if (!c || !out || n_groups < 0 || n_groups > kMaxGroups) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "get_ledger: no mesh");
+  return guarded(c, [&] {
+    std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
+    std::vector<double> en(kMaxGroups * kMaxSpecies);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(en.data(), c->d_ledger_energy.p, 8 * en.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int g = 0; g < n_groups; ++g) {
+      scdat_ledger_entry& L = out[g];
+      std::memset(&L, 0, sizeof(L));
+      for (size_t s = 0; s < c->sp.size(); ++s) {
+        const unsigned long long k = cnt[g * kMaxSpecies + s];
+        const double q = (double)k * c->sp[s].q * c->sp[s].w;
+        L.incident_energy_j += en[g * kMaxSpecies + s];
+        // PICCycle::recordSurfaceBoundaryInteraction (PICCycle.cpp:1295-1329):
+        // electron-like = negative charge.
+        if (c->sp[s].q < 0) {
+          L.absorbed_electron_charge_c += q;
+          L.absorbed_electron_particles += (int64_t)k;
+        } else {
+          L.absorbed_ion_charge_c += q;
+          L.absorbed_ion_particles += (int64_t)k;
+        }
+      }
+    }
+  });
+}
+
+int scdat_get_stage_times(scdat_ctx* c, double* ms5) {
+  if (!c || !ms5) return SCDAT_ERR_ARG;
+  for (int k = 0; k < 5; ++k) ms5[k] = c->stage_ms[k];
+  return SCDAT_OK;
+}
+
+int scdat_get_push_kernel_time(scdat_ctx* c, double* avg_ms, int64_t* launches) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (avg_ms) *avg_ms = c->push_launches ? c->push_ms_sum / (double)c->push_launches : 0.0;
+  if (launches) *launches = c->push_launches;
+  return SCDAT_OK;
+}
+
+int scdat_sync(scdat_ctx* c) {
+  if (!c) return SCDAT_ERR_ARG;
+  return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+}  // extern "C"

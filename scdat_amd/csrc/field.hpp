@@ -1,0 +1,94 @@
+// Launch interface of the Poisson / nodal-field kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+
+namespace scdat {
+
+// Scalars of one solve, resident on the device; the host polls `inner_done`
+// and `outer_done` between batches of iterations.
+struct PcgScalars {
+  double bnorm2, cond_load, rnorm, inv_rnorm;
+  double rz, pq, alpha, beta, rr;
+  double vfloat, qcond, ccap;
+  double tol, inner_tol, inner_tol_eff;
+  int32_t outer_done, inner_done, inner_it, outer_it, total_it, max_it, max_outer, failed;
+  unsigned int ticket[4];
+};
+
+// ELL operator over all nodes, slot-major (slot s of row i at s*n + i), self
+// in slot 0.  Constrained rows (Dirichlet, conductor) are identity rows and
+// their columns are eliminated from free rows (symmetric elimination).
+struct EllDev {
+  int64_t n;
+  int w;
+  const int32_t* col;
+  const float* v32;
+  const double* v64;
+  const float* dinv32;
+  const double* dinv64;
+  // nodal-gradient weights (LSQ), same slots, 3 components
+  const float* gx32;
+  const float* gy32;
+  const float* gz32;
+  const double* gx64;
+  const double* gy64;
+  const double* gz64;
+};
+
+struct FieldBufs {
+  const int32_t* kind;      // node kind
+  const double* lift;       // Dirichlet lifting of the load (free rows)
+  const double* dval;       // Dirichlet values
+  const double* dscale;     // diagonal of constrained rows
+  const double* phi1;       // unit-potential conductor solution
+  const int32_t* kc_idx;    // sparse vector sum of conductor rows of K
+  const double* kc_val;
+  int64_t n_kc;
+  double* b64;
+  double* x64;
+  double* r64;
+  double* partial;          // >= 2 * grid doubles
+  PcgScalars* sc;
+};
+
+template <typename T>
+struct InnerBufs {
+  T* r;
+  T* z;
+  T* d;
+  T* p0;
+  T* p1;
+  T* q;
+};
+
+void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* rho_fx,
+                double unit, int load_sign, hipStream_t s);
+void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s);
+template <typename T>
+void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s);
+template <typename T>
+void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int parity,
+                       hipStream_t s);
+template <typename T>
+void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s);
+// Conductor charge = q_init + sum over conductor groups and species of
+// absorbed count * q_s w_s; floating potential from Gauss's law.
+struct FloatArgs {
+  double q_init;
+  const unsigned long long* ledger_count;  // [kMaxGroups][kMaxSpecies]
+  const unsigned long long* emit_count;    // [kMaxGroups][kMaxSpecies] or null
+  double qw[kMaxSpecies];
+  uint32_t cond_mask;
+};
+void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s);
+template <typename T>
+void launch_phi_efield(const EllDev& A, const FieldBufs& f, T* phiT, double* phi64, Vec4<T>* E,
+                       unsigned long long* rho_clear, hipStream_t s);
+template <typename T>
+void launch_efield_only(const EllDev& A, const T* phiT, Vec4<T>* E, hipStream_t s);
+
+}  // namespace scdat

@@ -1,0 +1,53 @@
+// Launch interface of the fused particle kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+
+namespace scdat {
+
+template <typename T>
+struct PushArgs {
+  // mesh
+  const TetMap<T>* tets;
+  const int4* nb;       // face neighbours (face k opposite vertex k), <0 boundary
+  const int4* groups;   // boundary group per face
+  const int4* partner;  // periodic partner tet per face
+  const Vec4<T>* E;     // nodal E
+  unsigned long long* rho;  // int64 fixed-point nodal charge
+  int gtype[kMaxGroups];
+  T gshift[kMaxGroups][3];
+  // species SoA
+  T *x, *y, *z, *vx, *vy, *vz;
+  int32_t* cell;
+  int64_t n;
+  int species;
+  T qm;
+  int64_t cs;           // integer charge per particle per unit weight 2^-24
+  double mass, weight, energy_weight;
+  // push controls
+  T dt;
+  int adaptive;
+  T floor_ratio;
+  int max_sub;
+  int max_walk;
+  T tol;
+  T B[3];
+  T bmag;
+  int has_b;
+  // diagnostics
+  unsigned long long* ledger_count;  // [kMaxGroups][kMaxSpecies]
+  double* ledger_energy;             // [kMaxGroups][kMaxSpecies]
+  unsigned long long* lost;
+  double* ke_partial;                // per block
+  int32_t* alive_partial;            // per block
+};
+
+template <typename T>
+void launch_push(const PushArgs<T>& a, hipStream_t s);
+template <typename T>
+void launch_deposit(const PushArgs<T>& a, hipStream_t s);
+
+}  // namespace scdat

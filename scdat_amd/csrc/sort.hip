@@ -1,0 +1,135 @@
+// Counting sort of one species by cell, removing absorbed particles.
+// Replaces ParticleContainer::removeInactiveParticles
+// (Tools/Particle/src/ParticleManager.cpp:157-177) and keeps the SoA arrays in
+// tet order, which is what lets the fused push kernel share tet records and
+// reduce deposition within a wavefront.
+//   1. histogram of alive particles per cell (32-bit atomics)
+//   2. exclusive scan of the histogram (block scan + block totals + add)
+//   3. scatter of every SoA field to its slot (offset[cell] + atomic cursor)
+// The order inside one cell depends on atomic arrival; no result depends on
+// it (per-particle updates are independent, deposition is integer), and the
+// particle id travels with the particle.
+#include "device_common.hpp"
+#include "sort.hpp"
+
+namespace scdat {
+
+namespace {
+constexpr int kScanPer = 4;  // elements per thread in the block scan
+constexpr int kScanTile = kBlock * kScanPer;
+
+__global__ void __launch_bounds__(kBlock) k_zero_u32(uint32_t* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_hist(const int32_t* cell, int64_t n, uint32_t* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell[i];
+  if (c >= 0) atomicAdd(&cnt[c], 1u);
+}
+
+// Exclusive scan of one tile of 1024 counts; tile total to tot[blockIdx.x].
+__global__ void __launch_bounds__(kBlock) k_scan_tile(const uint32_t* in, uint32_t* out, int64_t n,
+                                                       uint32_t* tot) {
+  __shared__ uint32_t s[kBlock];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+  uint32_t v[kScanPer];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0u;
+    sum += v[k];
+  }
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    const uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - sum;  // exclusive prefix of this thread
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == kBlock - 1) tot[blockIdx.x] = s[kBlock - 1];
+}
+
+// Single-block exclusive scan of the tile totals (any count, serial chunks).
+__global__ void __launch_bounds__(kBlock) k_scan_totals(uint32_t* tot, int64_t nt,
+                                                         uint32_t* grand) {
+  __shared__ uint32_t s[kBlock];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < nt; base += kBlock) {
+    const int64_t i = base + threadIdx.x;
+    const uint32_t v = i < nt ? tot[i] : 0u;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+      const uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nt) tot[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == kBlock - 1) carry += s[kBlock - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_add(uint32_t* out, int64_t n, const uint32_t* tot,
+                                                      uint32_t* cursor) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t o = out[i] + tot[i / kScanTile];
+  out[i] = o;
+  cursor[i] = 0u;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_scatter(SortArgs<T> a) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int c = a.src.cell[i];
+  if (c < 0) return;
+  const uint32_t d = a.offset[c] + atomicAdd(&a.cursor[c], 1u);
+  a.dst.x[d] = a.src.x[i];
+  a.dst.y[d] = a.src.y[i];
+  a.dst.z[d] = a.src.z[i];
+  a.dst.vx[d] = a.src.vx[i];
+  a.dst.vy[d] = a.src.vy[i];
+  a.dst.vz[d] = a.src.vz[i];
+  a.dst.cell[d] = c;
+  a.dst.id[d] = a.src.id[i];
+}
+
+inline uint32_t blocks(int64_t n, int64_t per) { return (uint32_t)((n + per - 1) / per); }
+}  // namespace
+
+template <typename T>
+void launch_sort(const SortArgs<T>& a, hipStream_t s) {
+  // zero-fill by kernel (no memset nodes, so the sort can be graph-captured)
+  hipLaunchKernelGGL(k_zero_u32, dim3(blocks(a.ncell, kBlock)), dim3(kBlock), 0, s, a.count, a.ncell);
+  if (a.n > 0) hipLaunchKernelGGL(k_hist, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a.src.cell, a.n, a.count);
+  const uint32_t nt = blocks(a.ncell, kScanTile);
+  hipLaunchKernelGGL(k_scan_tile, dim3(nt), dim3(kBlock), 0, s, a.count, a.offset, a.ncell, a.tiles);
+  hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kBlock), 0, s, a.tiles, (int64_t)nt, a.total);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks(a.ncell, kBlock)), dim3(kBlock), 0, s, a.offset, a.ncell,
+                     a.tiles, a.cursor);
+  if (a.n > 0) hipLaunchKernelGGL(k_scatter<T>, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a);
+}
+
+template void launch_sort<float>(const SortArgs<float>&, hipStream_t);
+template void launch_sort<double>(const SortArgs<double>&, hipStream_t);
+
+size_t sort_tiles(int64_t ncell) { return (size_t)blocks(ncell, kScanTile); }
+
+}  // namespace scdat
