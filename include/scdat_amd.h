@@ -203,6 +203,8 @@ typedef struct {
   double pcg_rel_residual;     /* last solve, FP64 ||b - A x|| / ||b||      */
   int32_t pcg_converged;
   int64_t lost_particles;      /* walk limit exceeded                        */
+  int64_t particle_pushes;     /* cumulative particles pushed by scdat_step/
+                                  scdat_push (alive at the start of a push)  */
 } scdat_stats;
 
 /* Replaces PICCycle::getStatistics (PICCycle.h:135-149). */

@@ -349,6 +349,7 @@ struct SpeciesState {
   DevBuf buf[2][8];        // x,y,z,vx,vy,vz,cell,id
   DevBuf ke_partial, alive_partial;
   int32_t next_id = 0;
+  int64_t stat_blocks = 0;  // partials written by the last push
 };
 
 struct Engine {
@@ -369,7 +370,7 @@ struct scdat_ctx {
   std::array<int, kMaxGroups> gpartner{};
   std::array<std::array<double, 3>, kMaxGroups> gshift{};
   // device mesh
-  DevBuf d_tets, d_nb, d_groups, d_partner;
+  DevBuf d_tets, d_nb, d_groups, d_partner, d_gshift;
   // ELL operator
   DevBuf d_col, d_v32, d_v64, d_dinv32, d_dinv64, d_g[3];
   DevBuf d_kind, d_lift, d_dval, d_dscale, d_phi1, d_kc_idx, d_kc_val, d_b64, d_x64, d_r64, d_partial, d_sc;
@@ -379,7 +380,7 @@ struct scdat_ctx {
   int rho_target = 0, rho_last = 0;
   bool rho_clean[2] = {false, false};
   // diagnostics
-  DevBuf d_ledger_count, d_ledger_energy, d_lost;
+  DevBuf d_ledger_count, d_ledger_energy, d_lost, d_pushes;
   // sort scratch
   DevBuf d_count, d_offset, d_cursor, d_tiles, d_total;
   std::deque<SpeciesState> sp;
@@ -471,8 +472,7 @@ void upload_mesh_device(scdat_ctx* c) {
   c->d_phi1.alloc(8 * nn);
   HIPCHK(hipMemsetAsync(c->d_x64.p, 0, 8 * nn, s));
   HIPCHK(hipMemsetAsync(c->d_phi1.p, 0, 8 * nn, s));
-  const size_t grid = (nn + kBlock - 1) / kBlock;
-  c->d_partial.alloc(8 * 2 * grid + 64);
+  c->d_partial.alloc(8 * partial_doubles(h.nn));
   c->d_sc.alloc(sizeof(PcgScalars));
   HIPCHK(hipMemsetAsync(c->d_sc.p, 0, sizeof(PcgScalars), s));
   for (auto& b : c->d_inner) b.alloc(sizeof(T) * nn);
@@ -493,6 +493,8 @@ void upload_mesh_device(scdat_ctx* c) {
   c->d_ledger_count.alloc(8 * kMaxGroups * kMaxSpecies);
   c->d_ledger_energy.alloc(8 * kMaxGroups * kMaxSpecies);
   c->d_lost.alloc(8);
+  c->d_pushes.alloc(8);
+  HIPCHK(hipMemsetAsync(c->d_pushes.p, 0, 8, s));
   HIPCHK(hipMemsetAsync(c->d_ledger_count.p, 0, 8 * kMaxGroups * kMaxSpecies, s));
   HIPCHK(hipMemsetAsync(c->d_ledger_energy.p, 0, 8 * kMaxGroups * kMaxSpecies, s));
   HIPCHK(hipMemsetAsync(c->d_lost.p, 0, 8, s));
@@ -586,7 +588,7 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
     launch_inner_init<T>(A, f, ib, s);
     int it = 0, batch = 8;
     for (;;) {
-      for (int k = 0; k < batch; ++k, ++it) launch_inner_iter<T>(A, f, ib, it & 1, s);
+      for (int k = 0; k < batch; ++k, ++it) launch_inner_iter<T>(A, f, ib, it, s);
       read_scalars(c);
       if (c->h_sc->inner_done) break;
       batch = std::min(batch * 2, 64);
@@ -665,10 +667,13 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   a.partner = c->d_partner.as<int4>();
   a.E = c->d_E.as<Vec4<T>>();
   a.rho = c->d_rho[c->rho_target].as<unsigned long long>();
+  a.periodic_mask = 0;
+  a.reflect_mask = 0;
   for (int g = 0; g < kMaxGroups; ++g) {
-    a.gtype[g] = c->gtype[g];
-    for (int r = 0; r < 3; ++r) a.gshift[g][r] = (T)c->gshift[g][r];
+    if (c->gtype[g] == SCDAT_BC_PERIODIC) a.periodic_mask |= 1u << g;
+    if (c->gtype[g] == SCDAT_BC_REFLECT) a.reflect_mask |= 1u << g;
   }
+  a.gshift = c->d_gshift.as<T>();
   a.x = S.buf[S.cur][0].as<T>();
   a.y = S.buf[S.cur][1].as<T>();
   a.z = S.buf[S.cur][2].as<T>();
@@ -697,6 +702,7 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   a.ledger_count = c->d_ledger_count.as<unsigned long long>();
   a.ledger_energy = c->d_ledger_energy.as<double>();
   a.lost = c->d_lost.as<unsigned long long>();
+  a.pushes = c->d_pushes.as<unsigned long long>();
   a.ke_partial = S.ke_partial.as<double>();
   a.alive_partial = S.alive_partial.as<int32_t>();
   return a;
@@ -726,6 +732,7 @@ void do_push(scdat_ctx* c, std::vector<std::pair<size_t, size_t>>* ev, size_t* e
   for (size_t si = 0; si < c->sp.size(); ++si) {
     if (c->sp[si].n == 0) continue;
     const PushArgs<T> a = push_args<T>(c, (int)si);
+    c->sp[si].stat_blocks = push_blocks(a.n);
     if (ev) {
       const size_t e0 = (*evn)++, e1 = (*evn)++;
       HIPCHK(hipEventRecord(take_event(c, e0), c->stream));
@@ -791,6 +798,12 @@ void do_initialize(scdat_ctx* c) {
     if (m == 0) throw Error(SCDAT_ERR_ARG, "periodic groups do not match under the given shift");
   }
   c->d_partner.upload(h.partner, c->stream);
+  {
+    std::vector<T> sh(3 * kMaxGroups);
+    for (int g = 0; g < kMaxGroups; ++g)
+      for (int r = 0; r < 3; ++r) sh[3 * g + r] = (T)c->gshift[g][r];
+    c->d_gshift.upload(sh, c->stream);
+  }
   solve_phi1<T>(c);
   // Initial deposit -> Poisson -> E (PICCycle::initialize + first field).
   zero_rho<T>(c, 0);
@@ -1396,7 +1409,7 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
     double ke = 0;
     int64_t alive = 0;
     for (auto& S : c->sp) {
-      const size_t nb = (size_t)((S.n + kBlock - 1) / kBlock);
+      const size_t nb = (size_t)S.stat_blocks;
       if (!nb) continue;
       std::vector<double> k(nb);
       std::vector<int32_t> a(nb);
@@ -1416,8 +1429,9 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
       HIPCHK(hipMemcpyAsync(fx.data(), c->d_rho[c->rho_last].p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
       std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
       HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
-      unsigned long long lost = 0;
+      unsigned long long lost = 0, pushes = 0;
       HIPCHK(hipMemcpyAsync(&lost, c->d_lost.p, 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(&pushes, c->d_pushes.p, 8, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipMemcpyAsync(c->h_sc, c->d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       long double tq = 0;
@@ -1434,6 +1448,7 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
       st->conductor_charge = qc;
       st->floating_potential = c->h.has_cond ? c->h_sc->vfloat : 0.0;
       st->lost_particles = (int64_t)lost;
+      st->particle_pushes = (int64_t)pushes;
     }
     st->pcg_iterations = c->last_iters;
     st->pcg_outer = c->last_outer;

@@ -59,7 +59,7 @@ __host__ __device__ inline double fma_t<double>(double a, double b, double c) {
 // fixed-point deposition contract: the host checker evaluates the identical
 // expression, so the integer weights agree bit for bit.
 template <typename T>
-__device__ inline void bary(const TetMap<T>& m, const T p[3], T lam[4]) {
+__device__ __forceinline__ void bary(const TetMap<T>& m, const T p[3], T lam[4]) {
   const T d0 = p[0] - m.x0[0], d1 = p[1] - m.x0[1], d2 = p[2] - m.x0[2];
   lam[1] = fma_t(m.a[0], d0, fma_t(m.a[1], d1, m.a[2] * d2));
   lam[2] = fma_t(m.a[3], d0, fma_t(m.a[4], d1, m.a[5] * d2));
@@ -71,7 +71,7 @@ __device__ inline void bary(const TetMap<T>& m, const T p[3], T lam[4]) {
 // 2^-24, lambda_0 takes the remainder so the four weights sum to exactly 2^24
 // (charge is conserved exactly in the integer accumulator).
 template <typename T>
-__device__ inline void lambda_fixed(const T lam[4], int32_t f[4]) {
+__device__ __forceinline__ void lambda_fixed(const T lam[4], int32_t f[4]) {
   int32_t s = 0;
 #pragma unroll
   for (int k = 1; k < 4; ++k) {
@@ -86,7 +86,7 @@ __device__ inline void lambda_fixed(const T lam[4], int32_t f[4]) {
 // Boris velocity update (PICParticleAlgorithms.cpp:803-851): half E kick,
 // rotation about B when |B| > 1e-20 and |t|^2 < 100, half E kick.
 template <typename T>
-__device__ inline void boris(T v[3], const T E[3], const T B[3], bool has_b, T qm, T dt) {
+__device__ __forceinline__ void boris(T v[3], const T E[3], const T B[3], bool has_b, T qm, T dt) {
   const T h = T(0.5) * dt * qm;
   T vm[3] = {v[0] + E[0] * h, v[1] + E[1] * h, v[2] + E[2] * h};
   if (has_b) {
@@ -112,7 +112,7 @@ __device__ inline void boris(T v[3], const T E[3], const T B[3], bool has_b, T q
 // Adaptive sub-step (PICParticleAlgorithms.cpp:750-801).  v0 is the
 // start-of-step velocity, as in the reference.
 template <typename T>
-__device__ inline T adaptive_dt(const T v0[3], const T E[3], T bmag, T qm, T dt_max,
+__device__ __forceinline__ T adaptive_dt(const T v0[3], const T E[3], T bmag, T qm, T dt_max,
                                 T floor_ratio) {
   T dt = dt_max;
   const T emag = sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
@@ -160,21 +160,6 @@ __device__ inline double block_sum(double v, double* lds) {
     for (int i = 0; i < nw; ++i) s += lds[i];
   }
   return s;  // valid in thread 0
-}
-
-// Grid-wide "last block reduces" ticket: every block publishes its partial,
-// the block that arrives last sees all of them.
-__device__ inline bool last_block(unsigned int* ticket) {
-  __shared__ bool is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int t = atomicAdd(ticket, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last;
 }
 
 }  // namespace scdat

@@ -40,42 +40,42 @@ __device__ inline const double* dinv<double>(const EllDev& A) { return A.dinv64;
 
 inline uint32_t grid_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
-// Two-value block reduction published to partial[lb] and partial[g + lb];
-// returns true in the last block, whose thread 0 then holds the totals.
-__device__ inline bool reduce2(double a, double b, double* partial, unsigned int* ticket,
-                               uint32_t lb, double* ta, double* tb) {
-  __shared__ double lds[kBlock / 64];
-  const double sa = block_sum(a, lds);
-  const double sb = block_sum(b, lds);
-  const uint32_t g = gridDim.x;
-  if (threadIdx.x == 0) {
-    partial[lb] = sa;
-    partial[g + lb] = sb;
-  }
-  if (!last_block(ticket)) return false;
-  double xa = 0.0, xb = 0.0;
-  for (uint32_t j = threadIdx.x; j < g; j += blockDim.x) {
-    xa += partial[j];
-    xb += partial[g + j];
-  }
-  // fixed-order tree over the per-thread strided sums
-  const double ra = block_sum(xa, lds);
-  const double rb = block_sum(xb, lds);
-  if (threadIdx.x == 0) {
-    *ta = ra;
-    *tb = rb;
-    *ticket = 0u;
-  }
-  return true;
+// Reductions.  Every reducing kernel publishes one partial per block; the
+// totals are formed either by a one-block finalize kernel or, inside the PCG
+// iteration, by every block of the *next* kernel summing the same partials in
+// the same fixed order.  No device-scope fence or atomic is needed (kernel
+// boundaries order the memory), and every block obtains bit-identical
+// scalars, so a solve is bitwise reproducible.
+constexpr int kPB = kBlock;           // PCG block: 4 waves, two rows per thread
+constexpr int kRowsPerBlock = 2 * kPB;
+enum Slot { S_BB = 0, S_CL, S_RR, S_RZ0, S_RZ1, S_RR0, S_RR1, S_PQ, S_COUNT };
+
+__device__ inline double* slot(const FieldBufs& f, int s, int G) { return f.partial + (size_t)s * G; }
+
+// Sum of G partials, identical in every block; result broadcast to all threads.
+__device__ inline double sum_partials(const double* p, int G, double* lds) {
+  double x = 0.0;
+  for (int j = threadIdx.x; j < G; j += blockDim.x) x += p[j];
+  __shared__ double bc;
+  const double t = block_sum(x, lds);
+  if (threadIdx.x == 0) bc = t;
+  __syncthreads();
+  return bc;
 }
 
-__global__ void __launch_bounds__(kBlock) k_rhs(EllDev A, FieldBufs f,
+__device__ inline void publish(double v, double* p, double* lds) {
+  const double t = block_sum(v, lds);
+  if (threadIdx.x == 0) p[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kPB) k_rhs(EllDev A, FieldBufs f,
                                                  const unsigned long long* rho, double unit,
                                                  int load_sign) {
-  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
+  __shared__ double lds[kPB / 64];
+  const int G = gridDim.x;
   double bb = 0.0, cl = 0.0;
-  if (i < A.n) {
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
     const int k = f.kind[i];
     const double load = (double)load_sign * unit * (double)(long long)rho[i];
     double b;
@@ -83,29 +83,38 @@ __global__ void __launch_bounds__(kBlock) k_rhs(EllDev A, FieldBufs f,
     else if (k == 1) b = f.dscale[i] * f.dval[i];
     else {
       b = 0.0;
-      cl = load;
+      cl += load;
     }
     f.b64[i] = b;
-    bb = b * b;
+    bb += b * b;
   }
-  double ta, tb;
-  if (reduce2(bb, cl, f.partial, &f.sc->ticket[0], lb, &ta, &tb) && threadIdx.x == 0) {
-    f.sc->bnorm2 = ta;
-    f.sc->cond_load = tb;
-    f.sc->outer_done = 0;
-    f.sc->inner_done = 0;
-    f.sc->outer_it = 0;
-    f.sc->total_it = 0;
-    f.sc->failed = 0;
+  publish(bb, slot(f, S_BB, G), lds);
+  publish(cl, slot(f, S_CL, G), lds);
+}
+
+__global__ void __launch_bounds__(kPB) k_fin_rhs(FieldBufs f, int G) {
+  __shared__ double lds[kPB / 64];
+  const double bb = sum_partials(slot(f, S_BB, G), G, lds);
+  const double cl = sum_partials(slot(f, S_CL, G), G, lds);
+  if (threadIdx.x == 0) {
+    PcgScalars* sc = f.sc;
+    sc->bnorm2 = bb;
+    sc->cond_load = cl;
+    sc->outer_done = 0;
+    sc->inner_done = 0;
+    sc->outer_it = 0;
+    sc->total_it = 0;
+    sc->failed = 0;
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_resid64(EllDev A, FieldBufs f) {
+__global__ void __launch_bounds__(kPB) k_resid64(EllDev A, FieldBufs f) {
   if (f.sc->outer_done) return;
-  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
+  __shared__ double lds[kPB / 64];
+  const int G = gridDim.x;
   double rr = 0.0;
-  if (i < A.n) {
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
     double s = 0.0;
     for (int k = 0; k < A.w; ++k) {
       const int64_t o = (int64_t)k * A.n + i;
@@ -113,12 +122,18 @@ __global__ void __launch_bounds__(kBlock) k_resid64(EllDev A, FieldBufs f) {
     }
     const double r = f.b64[i] - s;
     f.r64[i] = r;
-    rr = r * r;
+    rr += r * r;
   }
-  double ta, tb;
-  if (reduce2(rr, 0.0, f.partial, &f.sc->ticket[1], lb, &ta, &tb) && threadIdx.x == 0) {
+  publish(rr, slot(f, S_RR, G), lds);
+}
+
+__global__ void __launch_bounds__(kPB) k_fin_resid(FieldBufs f, int G) {
+  if (f.sc->outer_done) return;
+  __shared__ double lds[kPB / 64];
+  const double rr = sum_partials(slot(f, S_RR, G), G, lds);
+  if (threadIdx.x == 0) {
     PcgScalars* sc = f.sc;
-    const double rn = sqrt(ta);
+    const double rn = sqrt(rr);
     const double bn = sqrt(sc->bnorm2);
     sc->rnorm = rn;
     sc->inv_rnorm = rn > 0.0 ? 1.0 / rn : 0.0;
@@ -137,101 +152,148 @@ __global__ void __launch_bounds__(kBlock) k_resid64(EllDev A, FieldBufs f) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_inner_init(EllDev A, FieldBufs f, InnerBufs<T> ib) {
+__global__ void __launch_bounds__(kPB) k_inner_init(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   if (f.sc->outer_done) return;
-  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
-  double rz = 0.0;
-  if (i < A.n) {
-    const T r = (T)(f.r64[i] * f.sc->inv_rnorm);
+  __shared__ double lds[kPB / 64];
+  const int G = gridDim.x;
+  const double inv = f.sc->inv_rnorm;
+  double rz = 0.0, rr = 0.0;
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
+    const T r = (T)(f.r64[i] * inv);
     const T z = dinv<T>(A)[i] * r;
     ib.r[i] = r;
     ib.z[i] = z;
     ib.d[i] = T(0);
     ib.p0[i] = T(0);
     ib.p1[i] = T(0);
-    rz = (double)r * (double)z;
+    rz += (double)r * (double)z;
+    rr += (double)r * (double)r;
   }
-  double ta, tb;
-  if (reduce2(rz, 0.0, f.partial, &f.sc->ticket[2], lb, &ta, &tb) && threadIdx.x == 0) {
-    f.sc->rz = ta;
-    f.sc->beta = 0.0;
-  }
+  publish(rz, slot(f, S_RZ0, G), lds);
+  publish(rr, slot(f, S_RR0, G), lds);
 }
 
-// q = A p_new with p_new = z + beta p_old formed on the fly; pq reduced.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_spmv(EllDev A, FieldBufs f, const T* __restrict__ z,
+// Scalars of inner iteration `it`, identical in every block: rz_it, rr_it
+// (from the previous update or the init), rz_{it-1} for beta, and the stop
+// decision.
+struct IterScalars {
+  double rz, rz_prev, rr;
+  bool done;
+};
+
+__device__ inline IterScalars iter_scalars(const FieldBufs& f, int it, int G, double* lds,
+                                           bool need_prev) {
+  IterScalars s;
+  const int cur = it & 1;
+  s.rz = sum_partials(slot(f, cur ? S_RZ1 : S_RZ0, G), G, lds);
+  s.rr = sum_partials(slot(f, cur ? S_RR1 : S_RR0, G), G, lds);
+  s.rz_prev = (need_prev && it > 0) ? sum_partials(slot(f, cur ? S_RZ0 : S_RZ1, G), G, lds) : 0.0;
+  const PcgScalars* sc = f.sc;
+  const double tol = sc->inner_tol_eff;
+  s.done = sc->outer_done || (it > 0 && s.rr <= tol * tol) || (sc->total_it + it >= sc->max_it) ||
+           !(s.rz == s.rz);
+  return s;
+}
+
+// q = A p_new with p_new = z + beta p_old formed on the fly; pq published.
+// W > 0: compile-time ELL width, fully unrolled so the 2W coalesced slot
+// loads are all in flight before the gathers; W = 0: runtime width.
+template <typename T, int W>
+__global__ void __launch_bounds__(kPB) k_spmv(EllDev A, FieldBufs f, const T* __restrict__ z,
                                                   const T* __restrict__ pold, T* __restrict__ pnew,
-                                                  T* __restrict__ q) {
-  if (f.sc->inner_done) return;
-  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
-  const T beta = (T)f.sc->beta;
+                                                  T* __restrict__ q, int it) {
+  __shared__ double lds[kPB / 64];
+  const int G = gridDim.x;
+  const IterScalars s = iter_scalars(f, it, G, lds, true);
+  if (s.done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) f.sc->inner_done = 1;
+    return;
+  }
+  const T beta = it > 0 && s.rz_prev != 0.0 ? (T)(s.rz / s.rz_prev) : T(0);
+  const T* __restrict__ v = vals<T>(A);
+  const int32_t* __restrict__ col = A.col;
+  const int64_t n = A.n;
   double pq = 0.0;
-  if (i < A.n) {
-    const T* v = vals<T>(A);
-    T s = T(0);
-    for (int k = 0; k < A.w; ++k) {
-      const int64_t o = (int64_t)k * A.n + i;
-      const int j = A.col[o];
-      s = fma_t(v[o], fma_t(beta, pold[j], z[j]), s);
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < n; i += kPB) {
+    T acc = T(0);
+    if constexpr (W > 0) {
+      int32_t cj[W];
+      T vj[W];
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        cj[k] = col[(int64_t)k * n + i];
+        vj[k] = v[(int64_t)k * n + i];
+      }
+#pragma unroll
+      for (int k = 0; k < W; ++k) acc = fma_t(vj[k], fma_t(beta, pold[cj[k]], z[cj[k]]), acc);
+    } else {
+      for (int k = 0; k < A.w; ++k) {
+        const int64_t o = (int64_t)k * n + i;
+        const int j = col[o];
+        acc = fma_t(v[o], fma_t(beta, pold[j], z[j]), acc);
+      }
     }
     const T pi = fma_t(beta, pold[i], z[i]);
     pnew[i] = pi;
-    q[i] = s;
-    pq = (double)pi * (double)s;
+    q[i] = acc;
+    pq += (double)pi * (double)acc;
   }
-  double ta, tb;
-  if (reduce2(pq, 0.0, f.partial, &f.sc->ticket[3], lb, &ta, &tb) && threadIdx.x == 0) {
-    PcgScalars* sc = f.sc;
-    if (!(fabs(ta) > 1e-300)) {
-      sc->inner_done = 1;
-      sc->alpha = 0.0;
-    } else {
-      sc->pq = ta;
-      sc->alpha = sc->rz / ta;
-    }
-  }
+  publish(pq, slot(f, S_PQ, G), lds);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_update(EllDev A, FieldBufs f, InnerBufs<T> ib,
-                                                    const T* __restrict__ pnew) {
-  if (f.sc->inner_done) return;
-  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
-  const T alpha = (T)f.sc->alpha;
+__global__ void __launch_bounds__(kPB) k_update(EllDev A, FieldBufs f, InnerBufs<T> ib,
+                                                    const T* __restrict__ pnew, int it) {
+  __shared__ double lds[kPB / 64];
+  const int G = gridDim.x;
+  // update(it) overwrites the other rz/rr parity, so it must not read it
+  const IterScalars s = iter_scalars(f, it, G, lds, false);
+  if (s.done) return;
+  const double pq = sum_partials(slot(f, S_PQ, G), G, lds);
+  if (!(fabs(pq) > 1e-300)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) f.sc->inner_done = 1;
+    return;
+  }
+  const T alpha = (T)(s.rz / pq);
   double rz = 0.0, rr = 0.0;
-  if (i < A.n) {
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
     const T d = fma_t(alpha, pnew[i], ib.d[i]);
     const T r = ib.r[i] - alpha * ib.q[i];
     const T z = dinv<T>(A)[i] * r;
     ib.d[i] = d;
     ib.r[i] = r;
     ib.z[i] = z;
-    rz = (double)r * (double)z;
-    rr = (double)r * (double)r;
+    rz += (double)r * (double)z;
+    rr += (double)r * (double)r;
   }
-  double ta, tb;
-  if (reduce2(rz, rr, f.partial, &f.sc->ticket[0], lb, &ta, &tb) && threadIdx.x == 0) {
-    PcgScalars* sc = f.sc;
-    sc->beta = sc->rz != 0.0 ? ta / sc->rz : 0.0;
-    sc->rz = ta;
-    sc->rr = tb;
-    sc->inner_it += 1;
-    sc->total_it += 1;
-    if (tb <= sc->inner_tol_eff * sc->inner_tol_eff || sc->total_it >= sc->max_it) sc->inner_done = 1;
+  const int nxt = (it + 1) & 1;
+  publish(rz, slot(f, nxt ? S_RZ1 : S_RZ0, G), lds);
+  publish(rr, slot(f, nxt ? S_RR1 : S_RR0, G), lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    f.sc->inner_it = it + 1;
+    f.sc->rz = s.rz;
+    f.sc->alpha = s.rz / pq;
   }
 }
 
+// Closes an inner solve: records its iteration count and applies
+// x += |r| d (skipped when the outer loop has already converged).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_apply(EllDev A, FieldBufs f, InnerBufs<T> ib) {
+__global__ void __launch_bounds__(kPB) k_apply(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   if (f.sc->outer_done) return;
-  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
-  if (i < A.n) f.x64[i] = fma(f.sc->rnorm, (double)ib.d[i], f.x64[i]);
-  if (blockIdx.x == 0 && threadIdx.x == 0) f.sc->outer_it += 1;
+  const double rn = f.sc->rnorm;
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB)
+    f.x64[i] = fma(rn, (double)ib.d[i], f.x64[i]);
+}
+
+__global__ void k_count_outer(FieldBufs f) {
+  if (f.sc->outer_done) return;
+  f.sc->outer_it += 1;
+  f.sc->total_it += f.sc->inner_it;
 }
 
 // Floating potential from Gauss's law on the conductor (one block).
@@ -307,33 +369,50 @@ __global__ void __launch_bounds__(kBlock) k_efield(EllDev A, const T* __restrict
 
 }  // namespace
 
+static inline int grid_rows(int64_t n) { return (int)((n + kRowsPerBlock - 1) / kRowsPerBlock); }
+
 void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* rho_fx,
                 double unit, int load_sign, hipStream_t s) {
-  hipLaunchKernelGGL(k_rhs, dim3(grid_for(A.n)), dim3(kBlock), 0, s, A, f, rho_fx, unit,
-                     load_sign);
+  const int G = grid_rows(A.n);
+  hipLaunchKernelGGL(k_rhs, dim3(G), dim3(kPB), 0, s, A, f, rho_fx, unit, load_sign);
+  hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(kPB), 0, s, f, G);
 }
 void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s) {
-  hipLaunchKernelGGL(k_resid64, dim3(grid_for(A.n)), dim3(kBlock), 0, s, A, f);
+  const int G = grid_rows(A.n);
+  hipLaunchKernelGGL(k_resid64, dim3(G), dim3(kPB), 0, s, A, f);
+  hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(kPB), 0, s, f, G);
 }
 template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_inner_init<T>, dim3(grid_for(A.n)), dim3(kBlock), 0, s, A, f, ib);
+  hipLaunchKernelGGL(k_inner_init<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
 }
 template <typename T>
-void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int parity,
+void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
                        hipStream_t s) {
-  const T* pold = parity ? ib.p1 : ib.p0;
-  T* pnew = parity ? ib.p0 : ib.p1;
-  hipLaunchKernelGGL(k_spmv<T>, dim3(grid_for(A.n)), dim3(kBlock), 0, s, A, f, ib.z, pold, pnew,
-                     ib.q);
-  hipLaunchKernelGGL(k_update<T>, dim3(grid_for(A.n)), dim3(kBlock), 0, s, A, f, ib,
-                     (const T*)pnew);
+  const T* pold = (it & 1) ? ib.p1 : ib.p0;
+  T* pnew = (it & 1) ? ib.p0 : ib.p1;
+  const int G = grid_rows(A.n);
+  const dim3 g(G), b(kPB);
+  switch (A.w) {
+#define SCDAT_W(w) \
+  case w:          \
+    hipLaunchKernelGGL((k_spmv<T, w>), g, b, 0, s, A, f, ib.z, pold, pnew, ib.q, it); \
+    break;
+    SCDAT_W(7) SCDAT_W(8) SCDAT_W(9) SCDAT_W(10) SCDAT_W(11) SCDAT_W(12) SCDAT_W(13) SCDAT_W(14)
+    SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18) SCDAT_W(19) SCDAT_W(20)
+#undef SCDAT_W
+    default:
+      hipLaunchKernelGGL((k_spmv<T, 0>), g, b, 0, s, A, f, ib.z, pold, pnew, ib.q, it);
+  }
+  hipLaunchKernelGGL(k_update<T>, dim3(G), dim3(kPB), 0, s, A, f, ib, (const T*)pnew, it);
 }
 template <typename T>
 void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s) {
-  hipLaunchKernelGGL(k_apply<T>, dim3(grid_for(A.n)), dim3(kBlock), 0, s, A, f, ib);
+  hipLaunchKernelGGL(k_apply<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
+  hipLaunchKernelGGL(k_count_outer, dim3(1), dim3(1), 0, s, f);
 }
+size_t partial_doubles(int64_t n) { return (size_t)S_COUNT * (size_t)grid_rows(n) + 16; }
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s) {
   hipLaunchKernelGGL(k_float, dim3(1), dim3(kBlock), 0, s, f, fa);
 }

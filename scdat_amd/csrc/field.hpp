@@ -70,9 +70,12 @@ void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* r
 void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s);
 template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s);
+// iteration `it` of the current inner solve (0, 1, ...)
 template <typename T>
-void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int parity,
+void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
                        hipStream_t s);
+// doubles needed by the partial-sum buffer for n rows
+size_t partial_doubles(int64_t n);
 template <typename T>
 void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s);
 // Conductor charge = q_init + sum over conductor groups and species of

@@ -19,7 +19,7 @@
 namespace scdat {
 
 template <typename T>
-__device__ inline void gather_E(const PushArgs<T>& a, const TetMap<T>& m, const T lam[4], T E[3]) {
+__device__ __forceinline__ void gather_E(const PushArgs<T>& a, const TetMap<T>& m, const T lam[4], T E[3]) {
   E[0] = E[1] = E[2] = T(0);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -34,7 +34,7 @@ __device__ inline void gather_E(const PushArgs<T>& a, const TetMap<T>& m, const 
 // cell, -1 when the particle left through an absorbing/conductor face (group
 // and hit point set), -2 when the walk limit was exceeded.
 template <typename T>
-__device__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1[3], T v[3], int& group,
+__device__ __forceinline__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1[3], T v[3], int& group,
                     T hit[3], TetMap<T>& m) {
   const T tol = a.tol;
   for (int it = 0; it < a.max_walk; ++it) {
@@ -77,28 +77,28 @@ __device__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1[3], T v[3], int
     const int g = kb == 0 ? g4.x : (kb == 1 ? g4.y : (kb == 2 ? g4.z : g4.w));
 #pragma unroll
     for (int r = 0; r < 3; ++r) hit[r] = x0[r] + tb * (x1[r] - x0[r]);
-    const int ty = (g >= 0 && g < kMaxGroups) ? a.gtype[g] : BC_ABSORB;
-    if (ty == BC_PERIODIC) {
+    const uint32_t gbit = (g >= 0 && g < kMaxGroups) ? (1u << g) : 0u;
+    if (a.periodic_mask & gbit) {
       const int4 p4 = a.partner[cur];
       const int pt = kb == 0 ? p4.x : (kb == 1 ? p4.y : (kb == 2 ? p4.z : p4.w));
       if (pt >= 0) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          x0[r] = hit[r] + a.gshift[g][r];
-          x1[r] += a.gshift[g][r];
+          const T sh = a.gshift[3 * g + r];
+          x0[r] = hit[r] + sh;
+          x1[r] += sh;
         }
         cur = pt;
         m = a.tets[cur];
         continue;
       }
-    } else if (ty == BC_REFLECT) {
+    } else if (a.reflect_mask & gbit) {
+      // inward normal of face kb = grad lambda_kb (rows of the map)
       T nrm[3];
-      if (kb == 0) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r) nrm[r] = -(m.a[r] + m.a[3 + r] + m.a[6 + r]);
-      } else {
-#pragma unroll
-        for (int r = 0; r < 3; ++r) nrm[r] = m.a[3 * (kb - 1) + r];
+      for (int r = 0; r < 3; ++r) {
+        const T g1 = m.a[r], g2 = m.a[3 + r], g3 = m.a[6 + r];
+        nrm[r] = kb == 0 ? -(g1 + g2 + g3) : (kb == 1 ? g1 : (kb == 2 ? g2 : g3));
       }
       const T inl = T(1) / sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
 #pragma unroll
@@ -119,81 +119,46 @@ __device__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1[3], T v[3], int
   return -2;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
-  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  int key = -1;
-  int32_t f[4] = {0, 0, 0, 0};
-  double ke = 0.0;
-  int alive = 0;
-  int c = (i < a.n) ? a.cell[i] : -1;
-  if (c >= 0) {
-    T x[3] = {a.x[i], a.y[i], a.z[i]};
-    T v[3] = {a.vx[i], a.vy[i], a.vz[i]};
-    const T v0[3] = {v[0], v[1], v[2]};
-    TetMap<T> m = a.tets[c];
-    T remaining = a.dt;
-    int sub = 0;
-    int dead = 0;
-    int group = -1;
-    T hit[3] = {x[0], x[1], x[2]};
-    // Sub-step loop, PICParticleAlgorithms.cpp:593-701.
-    while (remaining > T(1e-20) && sub < a.max_sub) {
-      ++sub;
-      T lam[4];
-      bary(m, x, lam);
-      T E[3];
-      gather_E(a, m, lam, E);
-      const T h = a.adaptive ? adaptive_dt(v0, E, a.bmag, a.qm, remaining, a.floor_ratio) : remaining;
-      boris(v, E, a.B, a.has_b != 0, a.qm, h);
-      T xs[3] = {x[0], x[1], x[2]};
-      T xn[3] = {fma_t(v[0], h, x[0]), fma_t(v[1], h, x[1]), fma_t(v[2], h, x[2])};
-      const int r = walk(a, c, xs, xn, v, group, hit, m);
-      if (r < 0) {
-        dead = r;
-        break;
-      }
-      c = r;
-      x[0] = xn[0];
-      x[1] = xn[1];
-      x[2] = xn[2];
-      remaining -= h;
-    }
-    if (dead) {
-      if (dead == -2) {
-        atomicAdd(a.lost, 1ull);
-      } else {
-        const int g = (group >= 0 && group < kMaxGroups) ? group : 0;
-        const double k = 0.5 * (double)a.mass *
-                         ((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]);
-        atomicAdd(&a.ledger_count[g * kMaxSpecies + a.species], 1ull);
-        atomicAdd(&a.ledger_energy[g * kMaxSpecies + a.species], k * a.energy_weight);
-      }
-      a.cell[i] = -1;
-      a.x[i] = hit[0];
-      a.y[i] = hit[1];
-      a.z[i] = hit[2];
-    } else {
-      T lam[4];
-      bary(m, x, lam);
-      lambda_fixed(lam, f);
-      key = c;
-      alive = 1;
-      ke = 0.5 * (double)a.mass * (double)a.weight *
-           ((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]);
-      a.x[i] = x[0];
-      a.y[i] = x[1];
-      a.z[i] = x[2];
-      a.cell[i] = c;
-    }
-    a.vx[i] = v[0];
-    a.vy[i] = v[1];
-    a.vz[i] = v[2];
-  }
+// LDS-tiled fixed-point deposition.
+//  1. wave level: runs of lanes with the same cell (particles are cell-sorted)
+//     are reduced by a segmented inclusive scan on the 32-bit weights;
+//  2. block level: each run's tail lane adds c_s * weight for its 4 nodes into
+//     an LDS open-addressing table keyed by node id (64-bit LDS atomics);
+//  3. after all chunks of the block, the table is flushed with one 64-bit
+//     global atomic per distinct node.
+// A node that finds no free slot within kProbe probes goes straight to the
+// global accumulator, so any particle order is correct, just slower.
+constexpr int kHash = 512;
+constexpr int kProbe = 8;
+constexpr int kChunks = 4;  // particles per thread in the push kernel
 
-  // ---- wave-segmented reduction of integer weights by cell -----------------
+struct DepositTile {
+  int32_t key[kHash];
+  unsigned long long val[kHash];
+};
+
+__device__ __forceinline__ void tile_clear(DepositTile& L) {
+  for (int j = threadIdx.x; j < kHash; j += kBlock) {
+    L.key[j] = -1;
+    L.val[j] = 0ull;
+  }
+}
+
+__device__ __forceinline__ void tile_flush(DepositTile& L, unsigned long long* rho) {
+  for (int j = threadIdx.x; j < kHash; j += kBlock) {
+    const int32_t k = L.key[j];
+    if (k >= 0) {
+      const unsigned long long v = L.val[j];
+      if (v) atomicAdd(&rho[k], v);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_runs(int key, int32_t f[4], int64_t cs,
+                                          const TetMap<T>* __restrict__ tets,
+                                          unsigned long long* rho, DepositTile& L) {
+  const int lane = threadIdx.x & 63;
   const int prev = __shfl_up(key, 1, 64);
   const bool head = lane == 0 || prev != key;
   const uint64_t heads = __ballot(head);
@@ -215,20 +180,135 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
   const int next = __shfl_down(key, 1, 64);
   const bool tail = lane == 63 || next != key;
   if (tail && key >= 0) {
-    const int4 nd = *reinterpret_cast<const int4*>(a.tets[key].node);
-    atomicAdd(&a.rho[nd.x], (unsigned long long)(a.cs * (int64_t)f[0]));
-    atomicAdd(&a.rho[nd.y], (unsigned long long)(a.cs * (int64_t)f[1]));
-    atomicAdd(&a.rho[nd.z], (unsigned long long)(a.cs * (int64_t)f[2]));
-    atomicAdd(&a.rho[nd.w], (unsigned long long)(a.cs * (int64_t)f[3]));
+    const int4 nd = *reinterpret_cast<const int4*>(tets[key].node);
+    const int32_t nodes[4] = {nd.x, nd.y, nd.z, nd.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t node = nodes[q];
+      const unsigned long long val = (unsigned long long)(cs * (int64_t)f[q]);
+      uint32_t h = ((uint32_t)node * 2654435761u) >> (32 - 9);
+      bool done = false;
+      for (int p = 0; p < kProbe && !done; ++p) {
+        const int32_t k = atomicCAS(&L.key[h], -1, node);
+        if (k == -1 || k == node) {
+          atomicAdd(&L.val[h], val);
+          done = true;
+        }
+        h = (h + 1) & (kHash - 1);
+      }
+      if (!done) atomicAdd(&rho[node], val);
+    }
   }
+}
 
-  // ---- per-block kinetic energy and alive count (deterministic partials) ----
+// One block pushes kChunks x kBlock consecutive particles.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
+  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+  __shared__ DepositTile tile;
+  __shared__ unsigned int led_n[kMaxGroups];
+  __shared__ double led_e[kMaxGroups];
+  tile_clear(tile);
+  if (threadIdx.x < kMaxGroups) {
+    led_n[threadIdx.x] = 0u;
+    led_e[threadIdx.x] = 0.0;
+  }
+  __syncthreads();
+  double ke = 0.0;
+  int alive = 0, started = 0;
+#pragma unroll 1
+  for (int ch = 0; ch < kChunks; ++ch) {
+    const int64_t i = ((int64_t)lb * kChunks + ch) * kBlock + threadIdx.x;
+    int key = -1;
+    int32_t f[4] = {0, 0, 0, 0};
+    int c = (i < a.n) ? a.cell[i] : -1;
+    if (c >= 0) {
+      ++started;
+      T x[3] = {a.x[i], a.y[i], a.z[i]};
+      T v[3] = {a.vx[i], a.vy[i], a.vz[i]};
+      const T v0[3] = {v[0], v[1], v[2]};
+      TetMap<T> m = a.tets[c];
+      T remaining = a.dt;
+      int sub = 0;
+      int dead = 0;
+      int group = -1;
+      T hit[3] = {x[0], x[1], x[2]};
+      // Sub-step loop, PICParticleAlgorithms.cpp:593-701.
+      while (remaining > T(1e-20) && sub < a.max_sub) {
+        ++sub;
+        T lam[4];
+        bary(m, x, lam);
+        T E[3];
+        gather_E(a, m, lam, E);
+        const T h =
+            a.adaptive ? adaptive_dt(v0, E, a.bmag, a.qm, remaining, a.floor_ratio) : remaining;
+        boris(v, E, a.B, a.has_b != 0, a.qm, h);
+        T xs[3] = {x[0], x[1], x[2]};
+        T xn[3] = {fma_t(v[0], h, x[0]), fma_t(v[1], h, x[1]), fma_t(v[2], h, x[2])};
+        const int r = walk(a, c, xs, xn, v, group, hit, m);
+        if (r < 0) {
+          dead = r;
+          break;
+        }
+        c = r;
+        x[0] = xn[0];
+        x[1] = xn[1];
+        x[2] = xn[2];
+        remaining -= h;
+      }
+      if (dead) {
+        if (dead == -2) {
+          atomicAdd(a.lost, 1ull);
+        } else {
+          // PICCycle::recordSurfaceBoundaryInteraction (PICCycle.cpp:1295-1329):
+          // count and incident energy KE * max(1, w), aggregated per block.
+          const int g = (group >= 0 && group < kMaxGroups) ? group : 0;
+          const double k = 0.5 * (double)a.mass *
+                           ((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]);
+          atomicAdd(&led_n[g], 1u);
+          atomicAdd(&led_e[g], k * a.energy_weight);
+        }
+        a.cell[i] = -1;
+        a.x[i] = hit[0];
+        a.y[i] = hit[1];
+        a.z[i] = hit[2];
+      } else {
+        T lam[4];
+        bary(m, x, lam);
+        lambda_fixed(lam, f);
+        key = c;
+        alive += 1;
+        ke += 0.5 * (double)a.mass * (double)a.weight *
+              ((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]);
+        a.x[i] = x[0];
+        a.y[i] = x[1];
+        a.z[i] = x[2];
+        a.cell[i] = c;
+      }
+      a.vx[i] = v[0];
+      a.vy[i] = v[1];
+      a.vz[i] = v[2];
+    }
+    tile_runs(key, f, a.cs, a.tets, a.rho, tile);
+  }
+  __syncthreads();
+  tile_flush(tile, a.rho);
+  if (threadIdx.x < kMaxGroups) {
+    const unsigned int n = led_n[threadIdx.x];
+    if (n) {
+      atomicAdd(&a.ledger_count[threadIdx.x * kMaxSpecies + a.species], (unsigned long long)n);
+      unsafeAtomicAdd(&a.ledger_energy[threadIdx.x * kMaxSpecies + a.species], led_e[threadIdx.x]);
+    }
+  }
+  // per-block kinetic energy, alive and started counts (deterministic partials)
   __shared__ double lds[kBlock / 64];
   const double ks = block_sum(ke, lds);
   const double as = block_sum((double)alive, lds);
+  const double ss = block_sum((double)started, lds);
   if (threadIdx.x == 0) {
     a.ke_partial[lb] = ks;
     a.alive_partial[lb] = (int32_t)as;
+    if (ss > 0) atomicAdd(a.pushes, (unsigned long long)ss);
   }
 }
 
@@ -236,59 +316,40 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
 template <typename T>
 __global__ void __launch_bounds__(kBlock) deposit_kernel(PushArgs<T> a) {
   const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  int key = -1;
-  int32_t f[4] = {0, 0, 0, 0};
-  const int c = (i < a.n) ? a.cell[i] : -1;
-  if (c >= 0) {
-    const TetMap<T> m = a.tets[c];
-    const T x[3] = {a.x[i], a.y[i], a.z[i]};
-    T lam[4];
-    bary(m, x, lam);
-    lambda_fixed(lam, f);
-    key = c;
-  }
-  const int prev = __shfl_up(key, 1, 64);
-  const bool head = lane == 0 || prev != key;
-  const uint64_t heads = __ballot(head);
-  const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-  const int start = 63 - __builtin_clzll(heads & le);
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int32_t o0 = __shfl_up(f[0], d, 64);
-    const int32_t o1 = __shfl_up(f[1], d, 64);
-    const int32_t o2 = __shfl_up(f[2], d, 64);
-    const int32_t o3 = __shfl_up(f[3], d, 64);
-    if (lane - d >= start) {
-      f[0] += o0;
-      f[1] += o1;
-      f[2] += o2;
-      f[3] += o3;
+  __shared__ DepositTile tile;
+  tile_clear(tile);
+  __syncthreads();
+#pragma unroll 1
+  for (int ch = 0; ch < kChunks; ++ch) {
+    const int64_t i = ((int64_t)lb * kChunks + ch) * kBlock + threadIdx.x;
+    int key = -1;
+    int32_t f[4] = {0, 0, 0, 0};
+    const int c = (i < a.n) ? a.cell[i] : -1;
+    if (c >= 0) {
+      const TetMap<T> m = a.tets[c];
+      const T x[3] = {a.x[i], a.y[i], a.z[i]};
+      T lam[4];
+      bary(m, x, lam);
+      lambda_fixed(lam, f);
+      key = c;
     }
+    tile_runs(key, f, a.cs, a.tets, a.rho, tile);
   }
-  const int next = __shfl_down(key, 1, 64);
-  const bool tail = lane == 63 || next != key;
-  if (tail && key >= 0) {
-    const int4 nd = *reinterpret_cast<const int4*>(a.tets[key].node);
-    atomicAdd(&a.rho[nd.x], (unsigned long long)(a.cs * (int64_t)f[0]));
-    atomicAdd(&a.rho[nd.y], (unsigned long long)(a.cs * (int64_t)f[1]));
-    atomicAdd(&a.rho[nd.z], (unsigned long long)(a.cs * (int64_t)f[2]));
-    atomicAdd(&a.rho[nd.w], (unsigned long long)(a.cs * (int64_t)f[3]));
-  }
+  __syncthreads();
+  tile_flush(tile, a.rho);
 }
+
+int push_blocks(int64_t n) { return (int)((n + (int64_t)kBlock * kChunks - 1) / ((int64_t)kBlock * kChunks)); }
 
 template <typename T>
 void launch_push(const PushArgs<T>& a, hipStream_t s) {
   if (a.n <= 0) return;
-  const uint32_t g = (uint32_t)((a.n + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(push_kernel<T>, dim3(g), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(push_kernel<T>, dim3(push_blocks(a.n)), dim3(kBlock), 0, s, a);
 }
 template <typename T>
 void launch_deposit(const PushArgs<T>& a, hipStream_t s) {
   if (a.n <= 0) return;
-  const uint32_t g = (uint32_t)((a.n + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(deposit_kernel<T>, dim3(g), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(deposit_kernel<T>, dim3(push_blocks(a.n)), dim3(kBlock), 0, s, a);
 }
 template void launch_push<float>(const PushArgs<float>&, hipStream_t);
 template void launch_push<double>(const PushArgs<double>&, hipStream_t);

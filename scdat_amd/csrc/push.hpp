@@ -17,8 +17,9 @@ struct PushArgs {
   const int4* partner;  // periodic partner tet per face
   const Vec4<T>* E;     // nodal E
   unsigned long long* rho;  // int64 fixed-point nodal charge
-  int gtype[kMaxGroups];
-  T gshift[kMaxGroups][3];
+  uint32_t periodic_mask;  // bit g set: group g periodic
+  uint32_t reflect_mask;   // bit g set: group g specular
+  const T* gshift;         // [kMaxGroups][3] periodic translation (device)
   // species SoA
   T *x, *y, *z, *vx, *vy, *vz;
   int32_t* cell;
@@ -41,10 +42,13 @@ struct PushArgs {
   unsigned long long* ledger_count;  // [kMaxGroups][kMaxSpecies]
   double* ledger_energy;             // [kMaxGroups][kMaxSpecies]
   unsigned long long* lost;
+  unsigned long long* pushes;        // cumulative particles pushed
   double* ke_partial;                // per block
   int32_t* alive_partial;            // per block
 };
 
+// blocks of one push/deposit launch over n particle slots (one partial each)
+int push_blocks(int64_t n);
 template <typename T>
 void launch_push(const PushArgs<T>& a, hipStream_t s);
 template <typename T>
