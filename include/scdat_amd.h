@@ -205,6 +205,7 @@ typedef struct {
   int64_t lost_particles;      /* walk limit exceeded                        */
   int64_t particle_pushes;     /* cumulative particles pushed by scdat_step/
                                   scdat_push (alive at the start of a push)  */
+  double pcg_initial_rel_residual; /* last solve, warm-start ||r0|| / ||b||  */
 } scdat_stats;
 
 /* Replaces PICCycle::getStatistics (PICCycle.h:135-149). */

@@ -60,7 +60,8 @@ class Stats(C.Structure):
                 ("conductor_charge", C.c_double), ("floating_potential", C.c_double),
                 ("pcg_iterations", C.c_int32), ("pcg_outer", C.c_int32),
                 ("pcg_rel_residual", C.c_double), ("pcg_converged", C.c_int32),
-                ("lost_particles", C.c_int64), ("particle_pushes", C.c_int64)]
+                ("lost_particles", C.c_int64), ("particle_pushes", C.c_int64),
+                ("pcg_initial_rel_residual", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

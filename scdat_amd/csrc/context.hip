@@ -389,7 +389,7 @@ struct scdat_ctx {
   double sim_time = 0.0;
   PcgScalars* h_sc = nullptr;  // pinned mirror
   int last_iters = 0, last_outer = 0, last_conv = 0;
-  double last_rel = 0.0;
+  double last_rel = 0.0, last_init_rel = 0.0;
   double stage_ms[5] = {0, 0, 0, 0, 0};
   double push_ms_sum = 0.0;
   int64_t push_launches = 0;
@@ -584,6 +584,10 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
   for (int outer = 0; outer <= max_outer; ++outer) {
     launch_resid64(A, f, s);
     read_scalars(c);
+    if (outer == 0) {
+      const double bn0 = std::sqrt(c->h_sc->bnorm2);
+      c->last_init_rel = bn0 > 0 ? c->h_sc->rnorm / bn0 : 0.0;
+    }
     if (c->h_sc->outer_done) break;
     launch_inner_init<T>(A, f, ib, s);
     int it = 0, batch = 8;
@@ -609,6 +613,12 @@ void solve_phi1(scdat_ctx* c) {
   FieldBufs f = fbufs(c);
   f.x64 = c->d_phi1.as<double>();
   HIPCHK(hipMemcpyAsync(c->d_b64.p, c->h.b1.data(), 8 * (size_t)c->h.nn, hipMemcpyHostToDevice, c->stream));
+  // start from the constrained values (1 on the conductor, 0 on the walls) so
+  // constrained rows have zero residual and only free rows iterate
+  std::vector<double> x0((size_t)c->h.nn, 0.0);
+  for (size_t i = 0; i < x0.size(); ++i)
+    if (c->h.kind[i] == SCDAT_NODE_CONDUCTOR) x0[i] = 1.0;
+  HIPCHK(hipMemcpyAsync(c->d_phi1.p, x0.data(), 8 * x0.size(), hipMemcpyHostToDevice, c->stream));
   PcgScalars init{};
   double bb = 0;
   for (double v : c->h.b1) bb += v * v;
@@ -950,7 +960,7 @@ void scdat_config_default(scdat_config* c) {
   c->load_sign = 1;
   c->pcg_tol = 1e-8;
   c->pcg_max_iter = 20000;
-  c->inner_tol = 1e-5;
+  c->inner_tol = 1e-6;
   c->sort_interval = 20;
 }
 
@@ -1454,6 +1464,7 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
     st->pcg_outer = c->last_outer;
     st->pcg_rel_residual = c->last_rel;
     st->pcg_converged = c->last_conv;
+    st->pcg_initial_rel_residual = c->last_init_rel;
   });
 }
 

@@ -197,37 +197,49 @@ __device__ inline IterScalars iter_scalars(const FieldBufs& f, int it, int G, do
 }
 
 // q = A p_new with p_new = z + beta p_old formed on the fly; pq published.
-// W > 0: compile-time ELL width, fully unrolled so the 2W coalesced slot
-// loads are all in flight before the gathers; W = 0: runtime width.
+// W > 0: compile-time ELL width, fully unrolled; the 2W coalesced slot loads
+// are issued before the scalar prologue so its latency hides behind them.
+// W = 0: runtime width.
 template <typename T, int W>
 __global__ void __launch_bounds__(kPB) k_spmv(EllDev A, FieldBufs f, const T* __restrict__ z,
-                                                  const T* __restrict__ pold, T* __restrict__ pnew,
-                                                  T* __restrict__ q, int it) {
+                                               const T* __restrict__ pold, T* __restrict__ pnew,
+                                               T* __restrict__ q, int it) {
   __shared__ double lds[kPB / 64];
   const int G = gridDim.x;
+  const T* __restrict__ v = vals<T>(A);
+  const int32_t* __restrict__ col = A.col;
+  const int64_t n = A.n;
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x;
+  constexpr int WW = W > 0 ? W : 1;
+  int32_t cj[2][WW];
+  T vj[2][WW];
+  if constexpr (W > 0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t i = r0 + h * kPB;
+      const bool ok = i < n;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        cj[h][k] = ok ? col[(int64_t)k * n + i] : 0;
+        vj[h][k] = ok ? v[(int64_t)k * n + i] : T(0);
+      }
+    }
+  }
   const IterScalars s = iter_scalars(f, it, G, lds, true);
   if (s.done) {
     if (blockIdx.x == 0 && threadIdx.x == 0) f.sc->inner_done = 1;
     return;
   }
   const T beta = it > 0 && s.rz_prev != 0.0 ? (T)(s.rz / s.rz_prev) : T(0);
-  const T* __restrict__ v = vals<T>(A);
-  const int32_t* __restrict__ col = A.col;
-  const int64_t n = A.n;
   double pq = 0.0;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
-  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < n; i += kPB) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = r0 + h * kPB;
+    if (i >= n) continue;
     T acc = T(0);
     if constexpr (W > 0) {
-      int32_t cj[W];
-      T vj[W];
 #pragma unroll
-      for (int k = 0; k < W; ++k) {
-        cj[k] = col[(int64_t)k * n + i];
-        vj[k] = v[(int64_t)k * n + i];
-      }
-#pragma unroll
-      for (int k = 0; k < W; ++k) acc = fma_t(vj[k], fma_t(beta, pold[cj[k]], z[cj[k]]), acc);
+      for (int k = 0; k < W; ++k) acc = fma_t(vj[h][k], fma_t(beta, pold[cj[h][k]], z[cj[h][k]]), acc);
     } else {
       for (int k = 0; k < A.w; ++k) {
         const int64_t o = (int64_t)k * n + i;
@@ -245,9 +257,21 @@ __global__ void __launch_bounds__(kPB) k_spmv(EllDev A, FieldBufs f, const T* __
 
 template <typename T>
 __global__ void __launch_bounds__(kPB) k_update(EllDev A, FieldBufs f, InnerBufs<T> ib,
-                                                    const T* __restrict__ pnew, int it) {
+                                                 const T* __restrict__ pnew, int it) {
   __shared__ double lds[kPB / 64];
   const int G = gridDim.x;
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x;
+  T pn[2], dd[2], rv[2], qv[2], di[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = r0 + h * kPB;
+    const bool ok = i < A.n;
+    pn[h] = ok ? pnew[i] : T(0);
+    dd[h] = ok ? ib.d[i] : T(0);
+    rv[h] = ok ? ib.r[i] : T(0);
+    qv[h] = ok ? ib.q[i] : T(0);
+    di[h] = ok ? dinv<T>(A)[i] : T(0);
+  }
   // update(it) overwrites the other rz/rr parity, so it must not read it
   const IterScalars s = iter_scalars(f, it, G, lds, false);
   if (s.done) return;
@@ -258,11 +282,13 @@ __global__ void __launch_bounds__(kPB) k_update(EllDev A, FieldBufs f, InnerBufs
   }
   const T alpha = (T)(s.rz / pq);
   double rz = 0.0, rr = 0.0;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
-  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
-    const T d = fma_t(alpha, pnew[i], ib.d[i]);
-    const T r = ib.r[i] - alpha * ib.q[i];
-    const T z = dinv<T>(A)[i] * r;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = r0 + h * kPB;
+    if (i >= A.n) continue;
+    const T d = fma_t(alpha, pn[h], dd[h]);
+    const T r = rv[h] - alpha * qv[h];
+    const T z = di[h] * r;
     ib.d[i] = d;
     ib.r[i] = r;
     ib.z[i] = z;
