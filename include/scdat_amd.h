@@ -39,6 +39,9 @@ extern "C" {
 #define SCDAT_BC_REFLECT 2    /* ReflectiveBoundaryCondition (specular)              */
 #define SCDAT_BC_CONDUCTOR 3  /* absorbed onto the floating conductor: its charge
                                  feeds the floating-potential solve                  */
+#define SCDAT_BC_ESCAPE 4     /* no condition set (the default): the particle leaves,
+                                 marked ESCAPED, no ledger entry or surface charge
+                                 (PICCycle.cpp:1254-1257,1355-1377)                 */
 
 /* node kinds */
 #define SCDAT_NODE_FREE 0
@@ -127,6 +130,8 @@ int scdat_set_mesh(scdat_ctx* ctx, int64_t n_nodes, const double* xyz, int64_t n
  * Must be called before scdat_initialize.                                   */
 int scdat_set_boundary(scdat_ctx* ctx, int32_t group, int32_t type, int32_t partner_group,
                        const double* shift);
+/* Particles that left through ESCAPE groups, per species, since the start. */
+int scdat_get_escaped(scdat_ctx* ctx, int32_t species, int64_t* n);
 
 /* Species (ParticleManager::createElectron/createIon semantics,
  * Tools/Particle/include/ParticleManager.h:821-926): charge q (C), mass m (kg),

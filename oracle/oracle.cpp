@@ -30,7 +30,7 @@ namespace {
 constexpr double kE = 1.602176634e-19;        // Tools/Basic/include/Constants.h:46
 constexpr double kEps0 = 8.8541878128e-12;    // Tools/Basic/include/Constants.h:53
 
-enum BcType { BC_ABSORB = 0, BC_PERIODIC = 1, BC_REFLECT = 2, BC_CONDUCTOR = 3 };
+enum BcType { BC_ABSORB = 0, BC_PERIODIC = 1, BC_REFLECT = 2, BC_CONDUCTOR = 3, BC_ESCAPE = 4 };
 constexpr int kMaxGroups = 16;
 
 struct Ledger {
@@ -199,7 +199,9 @@ struct Sim {
   double floor_ratio = 1e-3;
   int max_sub = 100000;
   int max_walk = 256;
-  int deposit_mode = 0;  // 0 = at end position (NearestPoint), 1 = LinearSegmentCloud (compat)
+  // 0 = once at the end position (physics default), 1 = LinearSegmentCloud per
+  // sub-step (reference default), 2 = NearestPoint per sub-step (reference)
+  int deposit_mode = 0;
   int nseg = 10;
   std::array<Ledger, kMaxGroups> ledger{};
   double surface_charge = 0.0;
@@ -541,7 +543,7 @@ void walk(const Sim& s, double* x0, double* x1, double* v, int cell, WalkOut& o,
     const int g = s.fgroup[4 * (size_t)cur + kb];
     double hit[3];
     for (int r = 0; r < 3; ++r) hit[r] = x0[r] + tb * (x1[r] - x0[r]);
-    const int ty = (g >= 0 && g < kMaxGroups) ? s.gtype[g] : BC_ABSORB;
+    const int ty = (g >= 0 && g < kMaxGroups) ? s.gtype[g] : BC_ESCAPE;
     if (ty == BC_PERIODIC) {
       const int pt = s.partner[4 * (size_t)cur + kb];
       if (pt >= 0) {
@@ -639,7 +641,7 @@ void push_species(Sim& s, Species& p, std::vector<double>& rho_acc) {
           } else {
             const int g = w.group;
             const double ke = 0.5 * p.m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            if (g >= 0 && g < kMaxGroups) {
+            if (g >= 0 && g < kMaxGroups && s.gtype[g] != BC_ESCAPE) {
               // Ledger, PICCycle::recordSurfaceBoundaryInteraction :1295-1329.
               Ledger& L = led[g];
               L.energy += ke * std::max(1.0, p.w);
@@ -652,6 +654,17 @@ void push_species(Sim& s, Species& p, std::vector<double>& rho_acc) {
           break;
         }
         cell = w.cell;
+        if (s.deposit_mode == 2) {
+          // NearestPoint: full charge at the sub-step's end position in the
+          // final element, every sub-step (PICParticleAlgorithms.cpp:388-392,
+          // called from the sub-step loop at :686-692).
+          double l[4];
+          bary(s, cell, xn, l);
+          bool in = true;
+          for (int k = 0; k < 4; ++k) in = in && l[k] >= -1e-8 && l[k] <= 1 + 1e-8;
+          if (in)
+            for (int k = 0; k < 4; ++k) rl[s.tet[4 * (size_t)cell + k]] += qw * l[k];
+        }
         if (s.deposit_mode == 1) {
           // LinearSegmentCloud: charge/nseg at each segment midpoint, into the
           // final element only, samples outside dropped
@@ -727,7 +740,8 @@ void* orc_create(int nn, const double* xyz, int nt, const int32_t* tet, const in
   s->E.assign(3 * (size_t)nn, 0.0);
   s->Bn.assign(3 * (size_t)nn, 0.0);
   for (int i = 0; i < nn; ++i) s->phi[i] = s->kind[i] == 1 ? s->dval[i] : 0.0;
-  for (int g = 0; g < kMaxGroups; ++g) s->gtype[g] = BC_ABSORB;
+  // no condition set: escape (PICCycle.cpp:1254-1257)
+  for (int g = 0; g < kMaxGroups; ++g) s->gtype[g] = BC_ESCAPE;
   return s;
 }
 

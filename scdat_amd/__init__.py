@@ -28,7 +28,7 @@ VACUUM_PERMITTIVITY = 8.8541878128e-12
 BOLTZMANN = 1.380649e-23
 
 OK, ERR_ARG, ERR_STATE, ERR_DEVICE, ERR_NOMEM, ERR_SOLVER = 0, -1, -2, -3, -4, -5
-BC_ABSORB, BC_PERIODIC, BC_REFLECT, BC_CONDUCTOR = 0, 1, 2, 3
+BC_ABSORB, BC_PERIODIC, BC_REFLECT, BC_CONDUCTOR, BC_ESCAPE = 0, 1, 2, 3, 4
 NODE_FREE, NODE_DIRICHLET, NODE_CONDUCTOR = 0, 1, 2
 MAX_GROUPS = 16
 
@@ -99,6 +99,7 @@ SIGNATURES = {
     "scdat_device_info": (C.c_int, [C.c_char_p, C.c_int32, _I32]),
     "scdat_set_mesh": (C.c_int, [_P, C.c_int64, _D, C.c_int64, _I32, _I32, _D, _I32]),
     "scdat_set_boundary": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _D]),
+    "scdat_get_escaped": (C.c_int, [_P, C.c_int32, _I64]),
     "scdat_add_species": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_int64, _I32]),
     "scdat_set_particles": (C.c_int, [_P, C.c_int32, C.c_int64, _D, _D, _I32]),
     "scdat_get_particle_count": (C.c_int, [_P, C.c_int32, _I64, _I64]),
@@ -259,6 +260,11 @@ class Context:
     def set_boundary(self, group, bc_type, partner=-1, shift=None):
         sh = None if shift is None else np.ascontiguousarray(shift, np.float64)
         self._chk(lib().scdat_set_boundary(self.h, group, bc_type, partner, _ptr(sh, C.c_double)))
+
+    def escaped(self, species):
+        n = C.c_int64()
+        self._chk(lib().scdat_get_escaped(self.h, species, C.byref(n)))
+        return n.value
 
     def add_species(self, q, m, w, capacity):
         sid = C.c_int32()

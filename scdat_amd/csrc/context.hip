@@ -1043,7 +1043,7 @@ int scdat_create(const scdat_config* cfg, scdat_ctx** out) {
     return SCDAT_ERR_DEVICE;
   }
   for (int g = 0; g < kMaxGroups; ++g) {
-    c->gtype[g] = SCDAT_BC_ABSORB;
+    c->gtype[g] = SCDAT_BC_ESCAPE;
     c->gpartner[g] = -1;
   }
   *out = c;
@@ -1096,13 +1096,27 @@ int scdat_set_mesh(scdat_ctx* c, int64_t n_nodes, const double* xyz, int64_t n_t
 int scdat_set_boundary(scdat_ctx* c, int32_t group, int32_t type, int32_t partner_group,
                        const double* shift) {
   if (!c) return SCDAT_ERR_ARG;
-  if (group < 0 || group >= kMaxGroups || type < 0 || type > 3)
+  if (group < 0 || group >= kMaxGroups || type < 0 || type > 4)
     return fail(c, SCDAT_ERR_ARG, "set_boundary: bad group or type");
   c->gtype[group] = type;
   c->gpartner[group] = partner_group;
   for (int r = 0; r < 3; ++r) c->gshift[group][r] = shift ? shift[r] : 0.0;
   c->initialized = false;
   return SCDAT_OK;
+}
+
+int scdat_get_escaped(scdat_ctx* c, int32_t species, int64_t* n) {
+  if (!c || !n) return SCDAT_ERR_ARG;
+  if (species < 0 || species >= (int)c->sp.size()) return fail(c, SCDAT_ERR_ARG, "bad species");
+  return guarded(c, [&] {
+    std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int64_t e = 0;
+    for (int g = 0; g < kMaxGroups; ++g)
+      if (c->gtype[g] == SCDAT_BC_ESCAPE) e += (int64_t)cnt[g * kMaxSpecies + species];
+    *n = e;
+  });
 }
 
 int scdat_add_species(scdat_ctx* c, double q, double m, double w, int64_t capacity, int32_t* id) {
@@ -1450,6 +1464,7 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
       double surf = 0, qc = c->q_init;
       for (int g = 0; g < kMaxGroups; ++g)
         for (size_t s = 0; s < c->sp.size(); ++s) {
+          if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;
           const double q = (double)cnt[g * kMaxSpecies + s] * c->sp[s].q * c->sp[s].w;
           surf += q;
           if (c->gtype[g] == SCDAT_BC_CONDUCTOR) qc += q;
@@ -1480,6 +1495,7 @@ int scdat_get_ledger(scdat_ctx* c, scdat_ledger_entry* out, int32_t n_groups) {
     for (int g = 0; g < n_groups; ++g) {
       scdat_ledger_entry& L = out[g];
       std::memset(&L, 0, sizeof(L));
+      if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;  // escaped: not recorded
       for (size_t s = 0; s < c->sp.size(); ++s) {
         const unsigned long long k = cnt[g * kMaxSpecies + s];
         const double q = (double)k * c->sp[s].q * c->sp[s].w;

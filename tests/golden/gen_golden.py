@@ -1,0 +1,111 @@
+"""Generate golden vectors from the compiled reference (oracle/_ref).
+
+Runs in the development container, where the reference sources exist; writes
+small .npz fixtures next to this script.  Each fixture holds the inputs and
+the reference's outputs of one routine:
+
+  bary_kuhn4.npz        FieldInterpolator::computeBarycentricCoordinates
+  deposit_kuhn4.npz     ChargeDepositor::depositCharge
+  cross_nearest.npz     CrossTetrahedron::crossTetrahedron, nodal E, NearestPoint
+  cross_segcloud.npz    same, LinearSegmentCloud (10 segments)
+  cross_single_substep.npz  one sub-step per particle, NearestPoint
+  cross_uniform_eb.npz  CrossTetrahedron::crossTetraUniformEwithB (Boris rotation,
+                        adaptive sub-steps)
+  mb_floating_n{4,6}.npz  MultiBoundaryElectricFieldSolver: P1 FEM, Dirichlet,
+                        FLOATING conductor, direct solve
+  poisson_graph_n6.npz  PoissonSolver: graph Laplacian, direct solve, LSQ E
+
+Usage: python tests/golden/gen_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle.ref as R  # noqa: E402
+import scdat_amd as S  # noqa: E402
+
+COND = ((0.4, 0.4, 0.4), (0.6, 0.6, 0.6))
+QE, ME, MP = S.ELEMENTARY_CHARGE, S.ELECTRON_MASS, S.PROTON_MASS
+
+
+def save(name, **kw):
+    np.savez_compressed(os.path.join(HERE, name), **kw)
+    print("wrote", name)
+
+
+def smooth_E(xyz, amp):
+    x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+    return amp * np.stack([np.sin(3 * y + 1) + 0.3, np.cos(2 * z) * x, np.sin(2 * x + z)], 1)
+
+
+def main():
+    m = S.box_mesh(4, conductor=COND)
+    spec = S.box_spec(4, conductor=COND)
+    x, v, c = S.maxwellian_box(spec, 300, 7, 0, 3e5)
+    lam, ins = R.barycentric(m["xyz"], m["tets"], c, x)
+    save("bary_kuhn4.npz", xyz=m["xyz"], tets=m["tets"], cells=c, pts=x, lam=lam, inside=ins)
+
+    rho = R.deposit(m["xyz"], m["tets"], x, c, -QE, 1234.5)
+    save("deposit_kuhn4.npz", xyz=m["xyz"], tets=m["tets"], cells=c, pts=x, charge=-QE, weight=1234.5, rho=rho)
+
+    # one step of independent electrons through a nodal field; dt chosen so
+    # that a fraction of them cross one or more tet faces or leave the mesh
+    E = smooth_E(m["xyz"], 40.0)
+    xs, vs, cs = S.maxwellian_box(spec, 400, 11, 0, 4e5)
+    state = np.concatenate([xs, vs], 1)
+    dt = 2e-7
+    for name, kernel in (("cross_nearest.npz", 0), ("cross_segcloud.npz", 1)):
+        out, el, lost, rr = R.cross(m["xyz"], m["tets"], state, ME, -QE, 100.0, dt, nodal_E=E,
+                                    kernel=kernel, segments=10, hint=cs)
+        save(name, xyz=m["xyz"], tets=m["tets"], kind=m["kind"], group=m["group"], E=E, state=state,
+             cells=cs, mass=ME, charge=-QE, weight=100.0, dt=dt, kernel=kernel, out=out, elem=el,
+             lost=lost, rho=rr)
+
+    # small dt: one sub-step per particle, so deposit-once and per-sub-step agree
+    out, el, lost, rr = R.cross(m["xyz"], m["tets"], state, ME, -QE, 100.0, 2e-9, nodal_E=E * 0.1,
+                                kernel=0, hint=cs)
+    save("cross_single_substep.npz", xyz=m["xyz"], tets=m["tets"], kind=m["kind"], group=m["group"],
+         E=E * 0.1, state=state, cells=cs, mass=ME, charge=-QE, weight=100.0, dt=2e-9, kernel=0, out=out,
+         elem=el, lost=lost, rho=rr)
+
+    # uniform E and B: Boris rotation (|t|^2 < 100) and cyclotron-limited sub-steps
+    uE = np.array([30.0, -10.0, 5.0])
+    uB = np.array([0.0, 2e-4, 1e-4])
+    xs, vs, cs = S.maxwellian_box(spec, 200, 13, 0, 2e5)
+    state = np.concatenate([xs, vs], 1)
+    out, el, lost, rr = R.cross(m["xyz"], m["tets"], state, ME, -QE, 1.0, 1e-7, uniform_E=uE,
+                                uniform_B=uB, kernel=0)
+    save("cross_uniform_eb.npz", xyz=m["xyz"], tets=m["tets"], kind=m["kind"], group=m["group"],
+         uE=uE, uB=uB, state=state, cells=cs, mass=ME, charge=-QE, weight=1.0, dt=1e-7, out=out,
+         elem=el, lost=lost)
+
+    for n in (4, 6):
+        mm = S.box_mesh(n, conductor=COND)
+        nn = len(mm["kind"])
+        rng = np.random.default_rng(100 + n)
+        dens = rng.normal(size=nn) * 1e-9
+        dval = np.where(mm["kind"] == 1, rng.normal(size=nn), 0.0)
+        phi, rc = R.multiboundary(mm["xyz"], mm["tets"], mm["kind"], dval, dens, 2e-12, 1.0, 3, 1e-30)
+        assert rc == 0
+        save(f"mb_floating_n{n}.npz", xyz=mm["xyz"], tets=mm["tets"], kind=mm["kind"], group=mm["group"],
+             dval=dval, density=dens, q_float=2e-12, permittivity=8.8541878128e-12, phi=phi)
+
+    mm = S.box_mesh(6, conductor=COND)
+    nn = len(mm["kind"])
+    rng = np.random.default_rng(7)
+    dens = rng.normal(size=nn) * 1e-9
+    kind = np.where(mm["kind"] == 2, 0, mm["kind"])  # PoissonSolver has no floating conductor
+    dval = np.where(kind == 1, rng.normal(size=nn), 0.0)
+    phi, Eg, rc = R.poisson(mm["xyz"], mm["tets"], kind, dval, dens)
+    assert rc == 0
+    save("poisson_graph_n6.npz", xyz=mm["xyz"], tets=mm["tets"], kind=kind, group=mm["group"], dval=dval,
+         density=dens, phi=phi, E=Eg)
+
+
+if __name__ == "__main__":
+    main()
