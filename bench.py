@@ -3,17 +3,19 @@
 
 Workload (the reference's scaled-up charging case): 1 m cube domain, 0.2 m
 floating conducting cube at its centre, 64^3 hexes split into 6 tets each
-(1,566,336 tets after the conductor cut-out), phi = 0 on the outer walls,
-2^24 macro-particles (2^23 electrons + 2^23 H+), Maxwellian Te = Ti = 1 eV,
-n = 1e10 m^-3, dt = 0.2 / omega_pe, collisionless.  FP32 push / gather /
-deposition, FP32 PCG with FP64 residual refinement to 1e-8.  Inputs are
-resident in HBM before the timed region.
+(1,559,682 tets after the conductor cut-out, 272,897 nodes), phi = 0 on the
+outer walls, 2^24 macro-particles (2^23 electrons + 2^23 H+), Maxwellian
+Te = Ti = 1 eV, n = 1e10 m^-3, dt = 0.2 / omega_pe, collisionless.  FP32 push
+/ gather / deposition, FP32 PCG with FP64 residual refinement to 1e-8.
+Inputs are resident in HBM before the timed region.
 
 One timed step = push + gather + walk + deposit of every particle, boundary
 events, floating-conductor Poisson solve and nodal E (PICCycle order).
 value = particles pushed / wall time of the timed steps.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
+--copy-calibration MB runs only a plain device copy of MB megabytes (the
+reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
 """
 import argparse
 import json
@@ -27,6 +29,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "pmc_traffic_64cube.json")
 
 
 def log(*a):
@@ -56,6 +59,14 @@ def build_case(n_hex, n_part, seed):
                 qm=[(-QE, ME), (QE, MP)])
 
 
+def structural_nnz(tets, n_nodes):
+    """Entries of the node-adjacency pattern (self + neighbours sharing a tet)."""
+    e = np.concatenate([tets[:, [a, b]] for a in range(4) for b in range(a + 1, 4)])
+    e.sort(axis=1)
+    key = e[:, 0].astype(np.int64) * n_nodes + e[:, 1]
+    return int(2 * np.unique(key).size + n_nodes)
+
+
 def run_gpu(case, steps, warmup, sort_interval):
     import scdat_amd as S
     m = case["mesh"]
@@ -73,18 +84,17 @@ def run_gpu(case, steps, warmup, sort_interval):
         ctx.step(1)
     ctx.sync()
     st0 = ctx.stats()
+    ctx.set_kernel_timing(True)
     t1 = time.perf_counter()
     ctx.step(steps)
     ctx.sync()
     wall = time.perf_counter() - t1
     st1 = ctx.stats()
     stages = ctx.stage_times()
-    push_ms, push_launches = ctx.push_kernel_time()
+    kt = {k: ctx.kernel_time(i) for i, k in enumerate(("push", "spmv", "update"))}
     pushes = st1["particle_pushes"] - st0["particle_pushes"]
-    log(f"gpu: {steps} steps in {wall:.3f}s, pushes {pushes}, stages(ms) {stages}, "
-        f"push kernel avg {push_ms:.4f} ms over {push_launches} launches, stats {st1}")
-    return ctx, dict(wall=wall, pushes=pushes, stages=stages, push_ms=push_ms,
-                     push_launches=push_launches, stats=st1, stats0=st0)
+    log(f"gpu: {steps} steps in {wall:.3f}s, pushes {pushes}, stages(ms) {stages}, kernels {kt}, stats {st1}")
+    return ctx, dict(wall=wall, pushes=pushes, stages=stages, kt=kt, stats=st1, stats0=st0)
 
 
 def run_cpu_baseline(case, steps, threads):
@@ -104,22 +114,27 @@ def run_cpu_baseline(case, steps, threads):
     o.efield()
     o.step(1)  # warm start for the timed steps, as on the GPU
     setup = time.time() - t0
-    alive0 = sum(int((o.get_particles(s)[2] >= 0).sum()) for s in range(2))
-    t1 = time.perf_counter()
-    pushes = 0
-    alive = alive0
+    alive = [sum(int((o.get_particles(s)[2] >= 0).sum()) for s in range(2))]
+    wall = 0.0
     for _ in range(steps):
-        pushes += alive
+        t1 = time.perf_counter()
         o.step(1)
-        alive = sum(int((o.get_particles(s)[2] >= 0).sum()) for s in range(2))
-    # the alive counting above is outside the oracle; subtract its cost
-    wall = time.perf_counter() - t1
-    tc = time.perf_counter()
-    for _ in range(steps):
-        sum(int((o.get_particles(s)[2] >= 0).sum()) for s in range(2))
-    wall -= time.perf_counter() - tc
+        wall += time.perf_counter() - t1
+        alive.append(sum(int((o.get_particles(s)[2] >= 0).sum()) for s in range(2)))
+    pushes = sum(alive[:-1])
     log(f"cpu baseline: setup {setup:.1f}s, {steps} steps {wall:.2f}s, threads {oracle.lib().orc_num_threads()}")
     return dict(wall=wall, pushes=pushes, threads=oracle.lib().orc_num_threads())
+
+
+def copy_calibration(mb):
+    import torch
+    n = int(mb * 1e6) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
 
 
 def main():
@@ -134,10 +149,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, default=0.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     import scdat_amd as S
     name, ncu = S.device_info()
@@ -146,14 +165,38 @@ def main():
 
     m = case["mesh"]
     n_nodes, n_tets = len(m["kind"]), len(m["tets"])
-    # Algorithmic bytes of one push launch (one species): particle SoA read +
-    # write 2*28 B, tet record 64 B per tet, nodal E 16 B + charge 8 B per node.
-    pushes_per_launch = g["pushes"] / max(1, g["push_launches"])
-    push_bytes = 56.0 * pushes_per_launch + 64.0 * n_tets + 24.0 * n_nodes
-    push_gbs = push_bytes / (g["push_ms"] * 1e-3) / 1e9 if g["push_ms"] > 0 else 0.0
-    # PCG: FP32 ELL iteration = SpMV (8 B per stored slot + 16 B/row vectors)
-    # + update (32 B/row); reported against the time of the Poisson stage.
+    nnz = structural_nnz(m["tets"], n_nodes)
+    kt = g["kt"]
+    # Algorithmic bytes per launch.
+    #  push (one species): particle SoA read + write 2 x 28 B; tet record 64 B
+    #    per tet; nodal E 16 B + integer charge 8 B per node
+    #  SpMV: column + value 8 B per structural entry; z and p_old read, p_new
+    #    and q written, 4 B each per row
+    #  update: p_new, d, r, q, D^-1 read, d, r, z written, 4 B each per row
+    push_launch = g["pushes"] / max(1, kt["push"][1]) if kt["push"][1] else 0
+    bytes_k = {
+        "push": 56.0 * push_launch + 64.0 * n_tets + 24.0 * n_nodes,
+        "spmv": 8.0 * nnz + 16.0 * n_nodes,
+        "update": 32.0 * n_nodes,
+    }
+    traffic = {}
+    if os.path.exists(TRAFFIC_FILE):
+        traffic = json.load(open(TRAFFIC_FILE)).get("traffic_bytes_per_launch", {})
+    kernels = {}
+    for k, (ms, n) in kt.items():
+        gbs = bytes_k[k] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        kernels[k] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": gbs / HBM_PEAK_GBS, "traffic": traffic.get(k), "avg_launch_ms": ms,
+                      "launches": n, "algorithmic_bytes_per_launch": bytes_k[k],
+                      "total_ms": ms * n}
+    dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    names = {"push": "push_kernel<float>", "spmv": "k_spmv<float,W>", "update": "k_update<float>"}
+    roof = dict(kernels[dominant])
+    roof["kernel"] = names[dominant]
     st = g["stats"]
+    iters = kt["spmv"][1]
+    pcg_ms = g["stages"]["pcg"]
+    pcg_bytes = iters * (bytes_k["spmv"] + bytes_k["update"])
     value = g["pushes"] / g["wall"]
     res = {
         "metric": "particle-steps/s incl. Poisson at 64^3/16M ptcl; CG HBM GB/s vs peak",
@@ -172,20 +215,22 @@ def main():
                                "%d nodes, %d particles (e-/H+), Te=Ti=1eV, n=1e10, dt=0.2/wpe, "
                                "PCG 1e-8" % (args.hex, n_tets, n_nodes, args.particles)},
         "device": name,
+        "roofline": roof,
+        "kernels": kernels,
+        "cg": {"iterations_per_step": iters / args.steps, "gbs_effective": pcg_bytes / (pcg_ms * 1e-3) / 1e9
+               if pcg_ms > 0 else 0.0, "frac_of_hbm_peak": pcg_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+               if pcg_ms > 0 else 0.0, "note": "algorithmic bytes of SpMV + update per inner iteration over the "
+               "whole Poisson stage time (FP64 residuals, launches and host polls included)"},
         "stage_ms_total": g["stages"],
-        "pcg_iterations_last_step": st["pcg_iterations"],
         "floating_potential_v": st["floating_potential"],
-        "roofline": {"bound": "hbm", "kernel": "push_kernel<float>", "achieved": push_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": push_gbs / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": g["push_ms"],
-                     "algorithmic_bytes_per_launch": push_bytes},
+        "pcg_initial_rel_residual": st["pcg_initial_rel_residual"],
     }
     if not args.no_cpu_baseline:
         c = run_cpu_baseline(case, args.cpu_steps, args.cpu_threads)
         res["cpu_baseline"] = {"value": c["pushes"] / c["wall"], "unit": "particle-steps/s",
                                "cores": c["threads"], "kind": "port",
-                               "sample": "%d full steps of the same workload (same inputs), FP64 "
-                                         "oracle with OpenMP, Jacobi-PCG to 1e-8" % args.cpu_steps}
+                               "sample": "%d full steps of the same workload (same inputs, after one warm "
+                                         "step), FP64 oracle with OpenMP, Jacobi-PCG to 1e-8" % args.cpu_steps}
         res["gpu_over_cpu"] = value / res["cpu_baseline"]["value"]
     print(json.dumps(res), flush=True)
 

@@ -123,6 +123,8 @@ SIGNATURES = {
     "scdat_get_ledger": (C.c_int, [_P, C.POINTER(LedgerEntry), C.c_int32]),
     "scdat_get_stage_times": (C.c_int, [_P, _D]),
     "scdat_get_push_kernel_time": (C.c_int, [_P, _D, _I64]),
+    "scdat_set_kernel_timing": (C.c_int, [_P, C.c_int32]),
+    "scdat_get_kernel_time": (C.c_int, [_P, C.c_int32, _D, _I64]),
     "scdat_sync": (C.c_int, [_P]),
 }
 
@@ -385,6 +387,15 @@ class Context:
     def push_kernel_time(self):
         t, n = C.c_double(), C.c_int64()
         self._chk(lib().scdat_get_push_kernel_time(self.h, C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def set_kernel_timing(self, on=True):
+        self._chk(lib().scdat_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self, which):
+        """(average ms, launches) for 0 push, 1 PCG SpMV, 2 PCG update."""
+        t, n = C.c_double(), C.c_int64()
+        self._chk(lib().scdat_get_kernel_time(self.h, which, C.byref(t), C.byref(n)))
         return t.value, n.value
 
     def sync(self):

@@ -394,6 +394,12 @@ struct scdat_ctx {
   double push_ms_sum = 0.0;
   int64_t push_launches = 0;
   std::vector<hipEvent_t> ev_pool;
+  // per-kernel timing (HIP events on the context stream), enabled on request:
+  // [0] push, [1] PCG SpMV, [2] PCG update; sums in ms over executed launches
+  bool kernel_timing = false;
+  std::vector<hipEvent_t> kt_pool;
+  double kt_sum[3] = {0, 0, 0};
+  int64_t kt_n[3] = {0, 0, 0};
   bool is_fp64() const { return cfg.precision == 1; }
 };
 
@@ -545,6 +551,15 @@ FieldBufs fbufs(scdat_ctx* c) {
   return f;
 }
 
+hipEvent_t take_kt_event(scdat_ctx* c, size_t idx) {
+  while (c->kt_pool.size() <= idx) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->kt_pool.push_back(e);
+  }
+  return c->kt_pool[idx];
+}
+
 template <typename T>
 InnerBufs<T> ibufs(scdat_ctx* c) {
   InnerBufs<T> b;
@@ -592,8 +607,31 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
     launch_inner_init<T>(A, f, ib, s);
     int it = 0, batch = 8;
     for (;;) {
-      for (int k = 0; k < batch; ++k, ++it) launch_inner_iter<T>(A, f, ib, it, s);
+      const int it0 = it;
+      for (int k = 0; k < batch; ++k, ++it) {
+        if (c->kernel_timing) {
+          hipEvent_t ev[4] = {take_kt_event(c, 4 * k), take_kt_event(c, 4 * k + 1),
+                              take_kt_event(c, 4 * k + 2), take_kt_event(c, 4 * k + 3)};
+          launch_inner_iter<T>(A, f, ib, it, s, ev);
+        } else {
+          launch_inner_iter<T>(A, f, ib, it, s, nullptr);
+        }
+      }
       read_scalars(c);
+      if (c->kernel_timing) {
+        // iterations it0 .. inner_it-1 ran; later launches of the batch exited
+        // at their first instruction and are not counted
+        const int ran = std::min(batch, std::max(0, c->h_sc->inner_it - it0));
+        for (int k = 0; k < ran; ++k) {
+          float a = 0, b = 0;
+          HIPCHK(hipEventElapsedTime(&a, c->kt_pool[4 * k], c->kt_pool[4 * k + 1]));
+          HIPCHK(hipEventElapsedTime(&b, c->kt_pool[4 * k + 2], c->kt_pool[4 * k + 3]));
+          c->kt_sum[1] += a;
+          c->kt_sum[2] += b;
+          c->kt_n[1] += 1;
+          c->kt_n[2] += 1;
+        }
+      }
       if (c->h_sc->inner_done) break;
       batch = std::min(batch * 2, 64);
     }
@@ -745,9 +783,7 @@ void do_push(scdat_ctx* c, std::vector<std::pair<size_t, size_t>>* ev, size_t* e
     c->sp[si].stat_blocks = push_blocks(a.n);
     if (ev) {
       const size_t e0 = (*evn)++, e1 = (*evn)++;
-      HIPCHK(hipEventRecord(take_event(c, e0), c->stream));
-      launch_push<T>(a, c->stream);
-      HIPCHK(hipEventRecord(take_event(c, e1), c->stream));
+      launch_push<T>(a, c->stream, take_event(c, e0), take_event(c, e1));
       ev->push_back({e0, e1});
     } else {
       launch_push<T>(a, c->stream);
@@ -885,6 +921,10 @@ void do_step(scdat_ctx* c, int n) {
     HIPCHK(hipEventElapsedTime(&t, c->ev_pool[pe.first], c->ev_pool[pe.second]));
     c->push_ms_sum += t;
     c->push_launches++;
+    if (c->kernel_timing) {
+      c->kt_sum[0] += t;
+      c->kt_n[0] += 1;
+    }
   }
 }
 
@@ -1054,6 +1094,7 @@ void scdat_destroy(scdat_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  for (auto e : c->kt_pool) (void)hipEventDestroy(e);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
   hipStream_t s = c->stream;
   delete c;  // DevBuf destructors free device memory
@@ -1524,6 +1565,23 @@ int scdat_get_push_kernel_time(scdat_ctx* c, double* avg_ms, int64_t* launches) 
   if (!c) return SCDAT_ERR_ARG;
   if (avg_ms) *avg_ms = c->push_launches ? c->push_ms_sum / (double)c->push_launches : 0.0;
   if (launches) *launches = c->push_launches;
+  return SCDAT_OK;
+}
+
+int scdat_set_kernel_timing(scdat_ctx* c, int32_t on) {
+  if (!c) return SCDAT_ERR_ARG;
+  c->kernel_timing = on != 0;
+  for (int k = 0; k < 3; ++k) {
+    c->kt_sum[k] = 0.0;
+    c->kt_n[k] = 0;
+  }
+  return SCDAT_OK;
+}
+
+int scdat_get_kernel_time(scdat_ctx* c, int32_t which, double* avg_ms, int64_t* launches) {
+  if (!c || which < 0 || which > 2) return SCDAT_ERR_ARG;
+  if (avg_ms) *avg_ms = c->kt_n[which] ? c->kt_sum[which] / (double)c->kt_n[which] : 0.0;
+  if (launches) *launches = c->kt_n[which];
   return SCDAT_OK;
 }
 

@@ -1,6 +1,7 @@
 // Device-side data layout and per-particle math of the PIC-MCC cycle (gfx950).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 

@@ -415,23 +415,25 @@ void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& 
 }
 template <typename T>
 void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
-                       hipStream_t s) {
+                       hipStream_t s, hipEvent_t* ev) {
   const T* pold = (it & 1) ? ib.p1 : ib.p0;
   T* pnew = (it & 1) ? ib.p0 : ib.p1;
   const int G = grid_rows(A.n);
   const dim3 g(G), b(kPB);
+  hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
   switch (A.w) {
 #define SCDAT_W(w) \
   case w:          \
-    hipLaunchKernelGGL((k_spmv<T, w>), g, b, 0, s, A, f, ib.z, pold, pnew, ib.q, it); \
+    hipExtLaunchKernelGGL((k_spmv<T, w>), g, b, 0, s, e0, e1, 0, A, f, ib.z, pold, pnew, ib.q, it); \
     break;
     SCDAT_W(7) SCDAT_W(8) SCDAT_W(9) SCDAT_W(10) SCDAT_W(11) SCDAT_W(12) SCDAT_W(13) SCDAT_W(14)
     SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18) SCDAT_W(19) SCDAT_W(20)
 #undef SCDAT_W
     default:
-      hipLaunchKernelGGL((k_spmv<T, 0>), g, b, 0, s, A, f, ib.z, pold, pnew, ib.q, it);
+      hipExtLaunchKernelGGL((k_spmv<T, 0>), g, b, 0, s, e0, e1, 0, A, f, ib.z, pold, pnew, ib.q, it);
   }
-  hipLaunchKernelGGL(k_update<T>, dim3(G), dim3(kPB), 0, s, A, f, ib, (const T*)pnew, it);
+  hipExtLaunchKernelGGL(k_update<T>, dim3(G), dim3(kPB), 0, s, ev ? ev[2] : nullptr,
+                        ev ? ev[3] : nullptr, 0, A, f, ib, (const T*)pnew, it);
 }
 template <typename T>
 void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s) {
@@ -459,7 +461,7 @@ void launch_efield_only(const EllDev& A, const T* phiT, Vec4<T>* E, hipStream_t 
   template void launch_inner_init<T>(const EllDev&, const FieldBufs&, const InnerBufs<T>&,     \
                                      hipStream_t);                                             \
   template void launch_inner_iter<T>(const EllDev&, const FieldBufs&, const InnerBufs<T>&, int, \
-                                     hipStream_t);                                             \
+                                     hipStream_t, hipEvent_t*);                                \
   template void launch_apply<T>(const EllDev&, const FieldBufs&, const InnerBufs<T>&,          \
                                 hipStream_t);                                                  \
   template void launch_phi_efield<T>(const EllDev&, const FieldBufs&, T*, double*, Vec4<T>*,   \

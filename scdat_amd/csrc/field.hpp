@@ -70,10 +70,12 @@ void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* r
 void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s);
 template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s);
-// iteration `it` of the current inner solve (0, 1, ...)
+// iteration `it` of the current inner solve (0, 1, ...); with ev non-null the
+// dispatch timestamps of the SpMV land in ev[0], ev[1] and of the update in
+// ev[2], ev[3] (kernel execution only, no launch gaps)
 template <typename T>
 void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
-                       hipStream_t s);
+                       hipStream_t s, hipEvent_t* ev);
 // doubles needed by the partial-sum buffer for n rows
 size_t partial_doubles(int64_t n);
 template <typename T>

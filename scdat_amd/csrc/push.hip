@@ -342,17 +342,18 @@ __global__ void __launch_bounds__(kBlock) deposit_kernel(PushArgs<T> a) {
 int push_blocks(int64_t n) { return (int)((n + (int64_t)kBlock * kChunks - 1) / ((int64_t)kBlock * kChunks)); }
 
 template <typename T>
-void launch_push(const PushArgs<T>& a, hipStream_t s) {
+void launch_push(const PushArgs<T>& a, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   if (a.n <= 0) return;
-  hipLaunchKernelGGL(push_kernel<T>, dim3(push_blocks(a.n)), dim3(kBlock), 0, s, a);
+  hipExtLaunchKernelGGL(push_kernel<T>, dim3(push_blocks(a.n)), dim3(kBlock), 0, s, start, stop, 0,
+                        a);
 }
 template <typename T>
 void launch_deposit(const PushArgs<T>& a, hipStream_t s) {
   if (a.n <= 0) return;
   hipLaunchKernelGGL(deposit_kernel<T>, dim3(push_blocks(a.n)), dim3(kBlock), 0, s, a);
 }
-template void launch_push<float>(const PushArgs<float>&, hipStream_t);
-template void launch_push<double>(const PushArgs<double>&, hipStream_t);
+template void launch_push<float>(const PushArgs<float>&, hipStream_t, hipEvent_t, hipEvent_t);
+template void launch_push<double>(const PushArgs<double>&, hipStream_t, hipEvent_t, hipEvent_t);
 template void launch_deposit<float>(const PushArgs<float>&, hipStream_t);
 template void launch_deposit<double>(const PushArgs<double>&, hipStream_t);
 

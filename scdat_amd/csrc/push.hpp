@@ -50,7 +50,9 @@ struct PushArgs {
 // blocks of one push/deposit launch over n particle slots (one partial each)
 int push_blocks(int64_t n);
 template <typename T>
-void launch_push(const PushArgs<T>& a, hipStream_t s);
+// with start/stop non-null they receive the kernel's dispatch timestamps
+void launch_push(const PushArgs<T>& a, hipStream_t s, hipEvent_t start = nullptr,
+                 hipEvent_t stop = nullptr);
 template <typename T>
 void launch_deposit(const PushArgs<T>& a, hipStream_t s);
 

@@ -280,3 +280,28 @@ def test_device_trajectory_matches_reference_golden():
     rho = c.fields()[0]
     # fixed-point deposition: 2^-24 weight resolution per particle
     assert np.max(np.abs(rho - g["rho"])) < 1e-6 * np.max(np.abs(g["rho"]))
+
+
+def test_statistics_match_particle_state():
+    """getStatistics analogue (PICCycle.cpp:1031-1072): kinetic energy, alive
+    counts and conductor charge equal their closed forms over the particle
+    state read back from the device."""
+    st = plasma_setup(n=8, npart=6000, seed=29)
+    c = make_ctx(st, precision=0, sort_interval=2)
+    c.initialize()
+    c.step(4)
+    s = c.stats()
+    ke, alive = 0.0, 0
+    masses = (S.ELECTRON_MASS, S.PROTON_MASS)
+    for sp in range(2):
+        x, v, cell, _ = c.get_particles(sp)
+        a = cell >= 0
+        alive += int(a.sum())
+        ke += 0.5 * masses[sp] * st["w"] * float((v[a].astype(np.float64) ** 2).sum())
+    print("ke dev %.9g host %.9g alive %d/%d" % (s["total_energy"], ke, s["active_particles"], alive))
+    assert s["active_particles"] == alive
+    assert abs(s["total_energy"] - ke) <= 1e-5 * ke
+    led = c.ledger(12)
+    q = sum(-QE * st["w"] * e["absorbed_electron_particles"] + QE * st["w"] * e["absorbed_ion_particles"]
+            for e in led[6:12])
+    assert abs(s["conductor_charge"] - q) <= 1e-9 * max(abs(q), QE * st["w"])

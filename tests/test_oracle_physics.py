@@ -244,3 +244,85 @@ def _periodic_partners(m):
                     partner[ka] = kb[0]
                     partner[kb] = ka[0]
     return partner
+
+
+def _boundary_oracle(n, cond=None):
+    m = S.box_mesh(n, conductor=cond)
+    o = O.Oracle(m["xyz"], m["tets"], m["kind"], group=m["group"])
+    o.set_poisson(0, EPS0, 1, 1e-10, 100)
+    o.set_push(1e-6, 0, 1e-3, 1, 0)
+    o.set_efield(np.zeros((len(m["xyz"]), 3)))
+    return m, o, S.box_spec(n, conductor=cond)
+
+
+def test_specular_reflection_closed_form():
+    # Mirror image about the wall plane, normal velocity flipped; a corner hit
+    # reflects about both planes (PICCycle.cpp:1221-1293 reflect branch).
+    m, o, spec = _boundary_oracle((2, 2, 6))
+    for g in range(6):
+        o.set_group(g, S.BC_REFLECT)
+    s = o.add_species(-QE, ME, 1.0)
+    x = np.array([[0.5, 0.5, 0.95], [0.95, 0.5, 0.97], [0.3, 0.02, 0.5]])
+    v = np.array([[1e5, 0, 1e5], [1e5, 2e4, 1e5], [0, -7e4, 3e4]])
+    o.set_particles(s, x, v, S.loading.box_locate(spec, x))
+    o.push()
+    xn, vn, cn = o.get_particles(s)
+    want_x = np.array([[0.6, 0.5, 0.95], [0.95, 0.52, 0.93], [0.3, 0.05, 0.53]])
+    want_v = np.array([[1e5, 0, -1e5], [-1e5, 2e4, -1e5], [0, 7e4, 3e4]])
+    assert (cn >= 0).all()
+    np.testing.assert_allclose(xn, want_x, atol=1e-12)
+    np.testing.assert_allclose(vn, want_v, rtol=1e-14)
+    assert np.abs(o.ledger(6)).sum() == 0
+
+
+def test_conductor_absorption_charges_conductor_and_ledger():
+    # An electron and a proton reaching the conductor's x-min face (group 6) are
+    # absorbed there; each adds q*w to the conductor charge and one count to
+    # the face's ledger row (PICCycle.cpp:1295-1329).
+    m, o, spec = _boundary_oracle(10, COND)
+    for g in range(6, 12):
+        o.set_group(g, S.BC_CONDUCTOR)
+    w = 3.0
+    se = o.add_species(-QE, ME, w)
+    si = o.add_species(QE, MP, w)
+    x = np.array([[0.35, 0.5, 0.5]])
+    o.set_particles(se, x, np.array([[1e5, 0, 0]]), S.loading.box_locate(spec, x))
+    xi = np.array([[0.65, 0.45, 0.55]])
+    o.set_particles(si, xi, np.array([[-1e5, 0, 0]]), S.loading.box_locate(spec, xi))
+    o.set_conductor_charge(1e-15)
+    o.push()
+    assert o.get_particles(se)[2][0] == -1 and o.get_particles(si)[2][0] == -1
+    led = o.ledger(12)
+    assert led[6, 4] == 1 and led[6, 5] == 0 and np.isclose(led[6, 0], -QE * w)
+    assert led[7, 5] == 1 and led[7, 4] == 0 and np.isclose(led[7, 1], QE * w)
+    ke_e, ke_i = 0.5 * ME * 1e10 * w, 0.5 * MP * 1e10 * w
+    np.testing.assert_allclose([led[6, 3], led[7, 3]], [ke_e, ke_i], rtol=1e-12)
+    assert np.abs(np.delete(led, [6, 7], axis=0)).sum() == 0
+    np.testing.assert_allclose(o.conductor_charge(), 1e-15, rtol=1e-12)  # +e w - e w
+
+
+def test_escape_default_removes_without_ledger():
+    # A face with no boundary condition: the particle leaves the domain with no
+    # ledger entry and no conductor charge (PICCycle.cpp:1106-1133 default).
+    m, o, spec = _boundary_oracle(10, COND)
+    s = o.add_species(-QE, ME, 1.0)
+    x = np.array([[0.5, 0.5, 0.97], [0.35, 0.5, 0.5]])
+    o.set_particles(s, x, np.array([[0, 0, 1e5], [1e5, 0, 0]]), S.loading.box_locate(spec, x))
+    o.push()
+    _, _, cn = o.get_particles(s)
+    assert (cn == -1).all()
+    assert np.abs(o.ledger(12)).sum() == 0
+    assert o.conductor_charge() == 0.0 and o.lost() == 0
+
+
+def test_kinetic_energy_statistic_closed_form():
+    m, o, spec = _boundary_oracle(4)
+    rng = np.random.default_rng(5)
+    se = o.add_species(-QE, ME, 2.5)
+    si = o.add_species(QE, MP, 0.5)
+    x = rng.uniform(0.1, 0.9, (7, 3))
+    ve, vi = rng.normal(0, 1e5, (7, 3)), rng.normal(0, 1e3, (7, 3))
+    o.set_particles(se, x, ve, S.loading.box_locate(spec, x))
+    o.set_particles(si, x, vi, S.loading.box_locate(spec, x))
+    want = 0.5 * ME * 2.5 * (ve ** 2).sum() + 0.5 * MP * 0.5 * (vi ** 2).sum()
+    np.testing.assert_allclose(o.kinetic_energy(), want, rtol=1e-13)
