@@ -91,7 +91,7 @@ def run_gpu(case, steps, warmup, sort_interval):
     wall = time.perf_counter() - t1
     st1 = ctx.stats()
     stages = ctx.stage_times()
-    kt = {k: ctx.kernel_time(i) for i, k in enumerate(("push", "spmv", "update"))}
+    kt = {"push": ctx.kernel_time(S.KERNEL_PUSH), "pcg_iter": ctx.kernel_time(S.KERNEL_PCG)}
     pushes = st1["particle_pushes"] - st0["particle_pushes"]
     log(f"gpu: {steps} steps in {wall:.3f}s, pushes {pushes}, stages(ms) {stages}, kernels {kt}, stats {st1}")
     return ctx, dict(wall=wall, pushes=pushes, stages=stages, kt=kt, stats=st1, stats0=st0)
@@ -170,14 +170,13 @@ def main():
     # Algorithmic bytes per launch.
     #  push (one species): particle SoA read + write 2 x 28 B; tet record 64 B
     #    per tet; nodal E 16 B + integer charge 8 B per node
-    #  SpMV: column + value 8 B per structural entry; z and p_old read, p_new
-    #    and q written, 4 B each per row
-    #  update: p_new, d, r, q, D^-1 read, d, r, z written, 4 B each per row
+    #  PCG iteration (one fused kernel): column + value 8 B per structural
+    #    entry; r, w, s_old, p, x, D^-1 read and r, w, s, p, x written, 4 B
+    #    each per row
     push_launch = g["pushes"] / max(1, kt["push"][1]) if kt["push"][1] else 0
     bytes_k = {
         "push": 56.0 * push_launch + 64.0 * n_tets + 24.0 * n_nodes,
-        "spmv": 8.0 * nnz + 16.0 * n_nodes,
-        "update": 32.0 * n_nodes,
+        "pcg_iter": 8.0 * nnz + 44.0 * n_nodes,
     }
     traffic = {}
     if os.path.exists(TRAFFIC_FILE):
@@ -190,13 +189,13 @@ def main():
                       "launches": n, "algorithmic_bytes_per_launch": bytes_k[k],
                       "total_ms": ms * n}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    names = {"push": "push_kernel<float>", "spmv": "k_spmv<float,W>", "update": "k_update<float>"}
+    names = {"push": "push_kernel<float>", "pcg_iter": "k_cg_iter<float,W>"}
     roof = dict(kernels[dominant])
     roof["kernel"] = names[dominant]
     st = g["stats"]
-    iters = kt["spmv"][1]
+    iters = kt["pcg_iter"][1]
     pcg_ms = g["stages"]["pcg"]
-    pcg_bytes = iters * (bytes_k["spmv"] + bytes_k["update"])
+    pcg_bytes = iters * bytes_k["pcg_iter"]
     value = g["pushes"] / g["wall"]
     res = {
         "metric": "particle-steps/s incl. Poisson at 64^3/16M ptcl; CG HBM GB/s vs peak",
@@ -219,7 +218,7 @@ def main():
         "kernels": kernels,
         "cg": {"iterations_per_step": iters / args.steps, "gbs_effective": pcg_bytes / (pcg_ms * 1e-3) / 1e9
                if pcg_ms > 0 else 0.0, "frac_of_hbm_peak": pcg_bytes / (pcg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-               if pcg_ms > 0 else 0.0, "note": "algorithmic bytes of SpMV + update per inner iteration over the "
+               if pcg_ms > 0 else 0.0, "note": "algorithmic bytes of the PCG iteration kernel per inner iteration over the "
                "whole Poisson stage time (FP64 residuals, launches and host polls included)"},
         "stage_ms_total": g["stages"],
         "floating_potential_v": st["floating_potential"],

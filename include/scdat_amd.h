@@ -239,12 +239,12 @@ int scdat_get_stage_times(scdat_ctx* ctx, double* ms5);
  * number of launches it averages.                                           */
 int scdat_get_push_kernel_time(scdat_ctx* ctx, double* avg_ms, int64_t* launches);
 
-/* Per-kernel timing with HIP events on the context's stream (off by
- * default; enabling resets the sums).  which: 0 push, 1 PCG SpMV, 2 PCG
- * update; averages over launches that did work since timing was enabled.   */
+/* Per-kernel timing from the kernels' own dispatch timestamps (HIP events
+ * attached to the launches on the context's stream; off by default, enabling
+ * resets the sums).  which: 0 push, 1 PCG iteration (fused SpMV + vector
+ * update); averages over launches that did work since timing was enabled.  */
 #define SCDAT_KERNEL_PUSH 0
-#define SCDAT_KERNEL_SPMV 1
-#define SCDAT_KERNEL_UPDATE 2
+#define SCDAT_KERNEL_PCG 1
 int scdat_set_kernel_timing(scdat_ctx* ctx, int32_t on);
 int scdat_get_kernel_time(scdat_ctx* ctx, int32_t which, double* avg_ms, int64_t* launches);
 

@@ -29,6 +29,7 @@ BOLTZMANN = 1.380649e-23
 
 OK, ERR_ARG, ERR_STATE, ERR_DEVICE, ERR_NOMEM, ERR_SOLVER = 0, -1, -2, -3, -4, -5
 BC_ABSORB, BC_PERIODIC, BC_REFLECT, BC_CONDUCTOR, BC_ESCAPE = 0, 1, 2, 3, 4
+KERNEL_PUSH, KERNEL_PCG = 0, 1
 NODE_FREE, NODE_DIRICHLET, NODE_CONDUCTOR = 0, 1, 2
 MAX_GROUPS = 16
 
@@ -393,7 +394,7 @@ class Context:
         self._chk(lib().scdat_set_kernel_timing(self.h, 1 if on else 0))
 
     def kernel_time(self, which):
-        """(average ms, launches) for 0 push, 1 PCG SpMV, 2 PCG update."""
+        """(average ms, launches) for KERNEL_PUSH or KERNEL_PCG (one PCG iteration)."""
         t, n = C.c_double(), C.c_int64()
         self._chk(lib().scdat_get_kernel_time(self.h, which, C.byref(t), C.byref(n)))
         return t.value, n.value

@@ -372,9 +372,9 @@ struct scdat_ctx {
   // device mesh
   DevBuf d_tets, d_nb, d_groups, d_partner, d_gshift;
   // ELL operator
-  DevBuf d_col, d_v32, d_v64, d_dinv32, d_dinv64, d_g[3];
+  DevBuf d_col, d_v32, d_v64, d_vh32, d_vh64, d_dinv32, d_dinv64, d_g[3];
   DevBuf d_kind, d_lift, d_dval, d_dscale, d_phi1, d_kc_idx, d_kc_val, d_b64, d_x64, d_r64, d_partial, d_sc;
-  DevBuf d_inner[6];
+  DevBuf d_inner[8];
   // fields
   DevBuf d_E, d_phiT, d_phi64, d_rho[2];
   int rho_target = 0, rho_last = 0;
@@ -389,17 +389,18 @@ struct scdat_ctx {
   double sim_time = 0.0;
   PcgScalars* h_sc = nullptr;  // pinned mirror
   int last_iters = 0, last_outer = 0, last_conv = 0;
+  int inner_hist[16] = {};  // inner iterations per outer pass of the last solve
   double last_rel = 0.0, last_init_rel = 0.0;
   double stage_ms[5] = {0, 0, 0, 0, 0};
   double push_ms_sum = 0.0;
   int64_t push_launches = 0;
   std::vector<hipEvent_t> ev_pool;
   // per-kernel timing (HIP events on the context stream), enabled on request:
-  // [0] push, [1] PCG SpMV, [2] PCG update; sums in ms over executed launches
+  // [0] push, [1] PCG iteration; sums in ms over executed launches
   bool kernel_timing = false;
   std::vector<hipEvent_t> kt_pool;
-  double kt_sum[3] = {0, 0, 0};
-  int64_t kt_n[3] = {0, 0, 0};
+  double kt_sum[2] = {0, 0};
+  int64_t kt_n[2] = {0, 0};
   bool is_fp64() const { return cfg.precision == 1; }
 };
 
@@ -451,6 +452,15 @@ void upload_mesh_device(scdat_ctx* c) {
   for (size_t i = 0; i < f32.size(); ++i) f32[i] = (float)h.aval[i];
   c->d_v32.upload(f32, s);
   c->d_v64.upload(h.aval, s);
+  // column-scaled operator A D^-1 of the inner solve (rounded once from FP64)
+  std::vector<double> vh(h.aval.size());
+  for (size_t o = 0; o < vh.size(); ++o) vh[o] = h.aval[o] * h.dinv[h.col[o]];
+  if (sizeof(T) == 4) {
+    for (size_t o = 0; o < vh.size(); ++o) f32[o] = (float)vh[o];
+    c->d_vh32.upload(f32, s);
+  } else {
+    c->d_vh64.upload(vh, s);
+  }
   std::vector<float> di32(h.dinv.size());
   for (size_t i = 0; i < di32.size(); ++i) di32[i] = (float)h.dinv[i];
   c->d_dinv32.upload(di32, s);
@@ -519,6 +529,8 @@ EllDev ell(scdat_ctx* c) {
   A.col = c->d_col.as<int32_t>();
   A.v32 = c->d_v32.as<float>();
   A.v64 = c->d_v64.as<double>();
+  A.vh32 = c->d_vh32.as<float>();
+  A.vh64 = c->d_vh64.as<double>();
   A.dinv32 = c->d_dinv32.as<float>();
   A.dinv64 = c->d_dinv64.as<double>();
   if (c->is_fp64()) {
@@ -563,12 +575,13 @@ hipEvent_t take_kt_event(scdat_ctx* c, size_t idx) {
 template <typename T>
 InnerBufs<T> ibufs(scdat_ctx* c) {
   InnerBufs<T> b;
-  b.r = c->d_inner[0].as<T>();
-  b.z = c->d_inner[1].as<T>();
-  b.d = c->d_inner[2].as<T>();
-  b.p0 = c->d_inner[3].as<T>();
-  b.p1 = c->d_inner[4].as<T>();
-  b.q = c->d_inner[5].as<T>();
+  for (int k = 0; k < 2; ++k) {
+    b.r[k] = c->d_inner[k].as<T>();
+    b.w[k] = c->d_inner[2 + k].as<T>();
+    b.s[k] = c->d_inner[4 + k].as<T>();
+  }
+  b.p = c->d_inner[6].as<T>();
+  b.x = c->d_inner[7].as<T>();
   return b;
 }
 
@@ -581,7 +594,11 @@ void read_scalars(scdat_ctx* c) {
 // Iterations are launched in growing batches; kernels after convergence
 // return immediately, the host polls the done flags between batches.
 template <typename T>
-void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, int max_outer) {
+// `hist` (optional, >= max_outer + 1 entries) holds the inner iteration count
+// of each outer pass of the previous solve; the first batch of a pass is sized
+// from it so that a typical pass needs one or two host polls.
+void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, int max_outer,
+               int* hist) {
   const EllDev A = ell(c);
   const InnerBufs<T> ib = ibufs<T>(c);
   hipStream_t s = c->stream;
@@ -606,12 +623,12 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
     if (c->h_sc->outer_done) break;
     launch_inner_init<T>(A, f, ib, s);
     int it = 0, batch = 8;
+    if (hist && hist[outer] > 16) batch = std::min(1024, hist[outer] * 7 / 8);
     for (;;) {
       const int it0 = it;
       for (int k = 0; k < batch; ++k, ++it) {
         if (c->kernel_timing) {
-          hipEvent_t ev[4] = {take_kt_event(c, 4 * k), take_kt_event(c, 4 * k + 1),
-                              take_kt_event(c, 4 * k + 2), take_kt_event(c, 4 * k + 3)};
+          hipEvent_t ev[2] = {take_kt_event(c, 2 * k), take_kt_event(c, 2 * k + 1)};
           launch_inner_iter<T>(A, f, ib, it, s, ev);
         } else {
           launch_inner_iter<T>(A, f, ib, it, s, nullptr);
@@ -623,18 +640,16 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
         // at their first instruction and are not counted
         const int ran = std::min(batch, std::max(0, c->h_sc->inner_it - it0));
         for (int k = 0; k < ran; ++k) {
-          float a = 0, b = 0;
-          HIPCHK(hipEventElapsedTime(&a, c->kt_pool[4 * k], c->kt_pool[4 * k + 1]));
-          HIPCHK(hipEventElapsedTime(&b, c->kt_pool[4 * k + 2], c->kt_pool[4 * k + 3]));
+          float a = 0;
+          HIPCHK(hipEventElapsedTime(&a, c->kt_pool[2 * k], c->kt_pool[2 * k + 1]));
           c->kt_sum[1] += a;
-          c->kt_sum[2] += b;
           c->kt_n[1] += 1;
-          c->kt_n[2] += 1;
         }
       }
       if (c->h_sc->inner_done) break;
-      batch = std::min(batch * 2, 64);
+      batch = it - it0 > 64 ? 8 : std::min(batch * 2, 64);  // after a predicted batch: small steps
     }
+    if (hist) hist[outer] = c->h_sc->inner_it;
     launch_apply<T>(A, f, ib, s);
   }
   read_scalars(c);
@@ -662,7 +677,7 @@ void solve_phi1(scdat_ctx* c) {
   for (double v : c->h.b1) bb += v * v;
   init.bnorm2 = bb;
   HIPCHK(hipMemcpyAsync(f.sc, &init, sizeof(init), hipMemcpyHostToDevice, c->stream));
-  pcg_drive<T>(c, f, 1e-13, c->is_fp64() ? 0.0 : 1e-5, 12);
+  pcg_drive<T>(c, f, 1e-13, c->is_fp64() ? 0.0 : 1e-5, 12, nullptr);
   std::vector<double> phi1((size_t)c->h.nn);
   HIPCHK(hipMemcpyAsync(phi1.data(), c->d_phi1.p, 8 * phi1.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -692,7 +707,7 @@ void do_solve(scdat_ctx* c, int rho_idx) {
   FieldBufs f = fbufs(c);
   launch_rhs(A, f, c->d_rho[rho_idx].as<unsigned long long>(), c->unit, c->cfg.load_sign, c->stream);
   const bool fp64 = c->is_fp64();
-  pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12);
+  pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, c->inner_hist);
   if (c->h.has_cond) {
     HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, c->stream));
     launch_float(f, float_args(c), c->stream);
@@ -1571,7 +1586,7 @@ int scdat_get_push_kernel_time(scdat_ctx* c, double* avg_ms, int64_t* launches) 
 int scdat_set_kernel_timing(scdat_ctx* c, int32_t on) {
   if (!c) return SCDAT_ERR_ARG;
   c->kernel_timing = on != 0;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 2; ++k) {
     c->kt_sum[k] = 0.0;
     c->kt_n[k] = 0;
   }
@@ -1579,7 +1594,7 @@ int scdat_set_kernel_timing(scdat_ctx* c, int32_t on) {
 }
 
 int scdat_get_kernel_time(scdat_ctx* c, int32_t which, double* avg_ms, int64_t* launches) {
-  if (!c || which < 0 || which > 2) return SCDAT_ERR_ARG;
+  if (!c || which < 0 || which > 1) return SCDAT_ERR_ARG;
   if (avg_ms) *avg_ms = c->kt_n[which] ? c->kt_sum[which] / (double)c->kt_n[which] : 0.0;
   if (launches) *launches = c->kt_n[which];
   return SCDAT_OK;

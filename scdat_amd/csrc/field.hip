@@ -15,9 +15,9 @@
 // Mixed mode runs the inner PCG in FP32 on a unit-norm residual and refines
 // in FP64 (r = b - A x with FP64 values, x += |r| d) until
 // ||r|| <= tol ||b||.  All dot products accumulate in FP64, reduced in a fixed
-// order (wave shuffles, block partials summed by the last block), so a solve
-// is bitwise reproducible.  Per iteration: one fused SpMV kernel (which also
-// forms p = z + beta p on the fly) and one fused update kernel.
+// order (wave shuffles, block partials summed by the next kernel), so a solve
+// is bitwise reproducible.  Per iteration: one kernel (Chronopoulos-Gear CG,
+// below).
 #include "device_common.hpp"
 #include "field.hpp"
 
@@ -26,11 +26,11 @@ namespace scdat {
 namespace {
 
 template <typename T>
-__device__ inline const T* vals(const EllDev& A);
+__device__ inline const T* vhat(const EllDev& A);
 template <>
-__device__ inline const float* vals<float>(const EllDev& A) { return A.v32; }
+__device__ inline const float* vhat<float>(const EllDev& A) { return A.vh32; }
 template <>
-__device__ inline const double* vals<double>(const EllDev& A) { return A.v64; }
+__device__ inline const double* vhat<double>(const EllDev& A) { return A.vh64; }
 template <typename T>
 __device__ inline const T* dinv(const EllDev& A);
 template <>
@@ -48,7 +48,7 @@ inline uint32_t grid_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBloc
 // scalars, so a solve is bitwise reproducible.
 constexpr int kPB = kBlock;           // PCG block: 4 waves, two rows per thread
 constexpr int kRowsPerBlock = 2 * kPB;
-enum Slot { S_BB = 0, S_CL, S_RR, S_RZ0, S_RZ1, S_RR0, S_RR1, S_PQ, S_COUNT };
+enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_COUNT };
 
 __device__ inline double* slot(const FieldBufs& f, int s, int G) { return f.partial + (size_t)s * G; }
 
@@ -151,62 +151,103 @@ __global__ void __launch_bounds__(kPB) k_fin_resid(FieldBufs f, int G) {
   }
 }
 
+// Inner solve: Jacobi-preconditioned CG in the Chronopoulos-Gear form
+// (A. T. Chronopoulos, C. W. Gear, J. Comput. Appl. Math. 25 (1989) 153-168),
+// which needs one global reduction per iteration instead of two:
+//   p_i = u_i + b_i p_{i-1},  s_i = w_i + b_i s_{i-1}  (s = A p),
+//   x += a_i p_i,  r_{i+1} = r_i - a_i s_i,  u = D^-1 r,  w = A u,
+//   g = (r,u), d = (w,u),  b_{i+1} = g_{i+1}/g_i,
+//   a_{i+1} = g_{i+1} / (d_{i+1} - b_{i+1} g_{i+1} / a_i).
+// It is the same Krylov iteration as the PCG of
+// MultiBoundaryElectricFieldSolver.cpp:1432-1513 in exact arithmetic.  One
+// kernel runs a whole iteration: each row forms r_{i+1} of its ELL neighbours
+// from (r_i, w_i, s_{i-1}) with the same fused operations the owner uses, so
+// w_{i+1} = (A D^-1) r_{i+1} needs no second launch.  r, w and s are
+// double-buffered by iteration parity; p and x are owner-only, in place.
 template <typename T>
 __global__ void __launch_bounds__(kPB) k_inner_init(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   if (f.sc->outer_done) return;
-  __shared__ double lds[kPB / 64];
-  const int G = gridDim.x;
   const double inv = f.sc->inv_rnorm;
-  double rz = 0.0, rr = 0.0;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
   for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
-    const T r = (T)(f.r64[i] * inv);
-    const T z = dinv<T>(A)[i] * r;
-    ib.r[i] = r;
-    ib.z[i] = z;
-    ib.d[i] = T(0);
-    ib.p0[i] = T(0);
-    ib.p1[i] = T(0);
-    rz += (double)r * (double)z;
-    rr += (double)r * (double)r;
+    ib.r[0][i] = (T)(f.r64[i] * inv);
+    ib.s[1][i] = T(0);
+    ib.p[i] = T(0);
+    ib.x[i] = T(0);
   }
-  publish(rz, slot(f, S_RZ0, G), lds);
-  publish(rr, slot(f, S_RR0, G), lds);
 }
 
-// Scalars of inner iteration `it`, identical in every block: rz_it, rr_it
-// (from the previous update or the init), rz_{it-1} for beta, and the stop
-// decision.
-struct IterScalars {
-  double rz, rz_prev, rr;
-  bool done;
-};
-
-__device__ inline IterScalars iter_scalars(const FieldBufs& f, int it, int G, double* lds,
-                                           bool need_prev) {
-  IterScalars s;
-  const int cur = it & 1;
-  s.rz = sum_partials(slot(f, cur ? S_RZ1 : S_RZ0, G), G, lds);
-  s.rr = sum_partials(slot(f, cur ? S_RR1 : S_RR0, G), G, lds);
-  s.rz_prev = (need_prev && it > 0) ? sum_partials(slot(f, cur ? S_RZ0 : S_RZ1, G), G, lds) : 0.0;
-  const PcgScalars* sc = f.sc;
-  const double tol = sc->inner_tol_eff;
-  s.done = sc->outer_done || (it > 0 && s.rr <= tol * tol) || (sc->total_it + it >= sc->max_it) ||
-           !(s.rz == s.rz);
-  return s;
+__device__ inline void publish3(double a, double b, double c, double* pa, double* pb, double* pc,
+                                double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+    c += __shfl_xor(c, off, 64);
+  }
+  constexpr int NW = kPB / 64;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) {
+    lds[w] = a;
+    lds[NW + w] = b;
+    lds[2 * NW + w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      x += lds[k];
+      y += lds[NW + k];
+      z += lds[2 * NW + k];
+    }
+    pa[blockIdx.x] = x;
+    pb[blockIdx.x] = y;
+    pc[blockIdx.x] = z;
+  }
 }
 
-// q = A p_new with p_new = z + beta p_old formed on the fly; pq published.
-// W > 0: compile-time ELL width, fully unrolled; the 2W coalesced slot loads
-// are issued before the scalar prologue so its latency hides behind them.
-// W = 0: runtime width.
-template <typename T, int W>
-__global__ void __launch_bounds__(kPB) k_spmv(EllDev A, FieldBufs f, const T* __restrict__ z,
-                                               const T* __restrict__ pold, T* __restrict__ pnew,
-                                               T* __restrict__ q, int it) {
-  __shared__ double lds[kPB / 64];
+template <typename T>
+__device__ inline T r_next(T r, T w, T s, T a, T b) {
+  return fma_t(-a, fma_t(b, s, w), r);
+}
+
+// w_0 = (A D^-1) r_0 and the first dot products.
+template <typename T>
+__global__ void __launch_bounds__(kPB) k_cg_start(EllDev A, FieldBufs f, InnerBufs<T> ib) {
+  if (f.sc->outer_done) return;
+  __shared__ double lds[3 * (kPB / 64)];
   const int G = gridDim.x;
-  const T* __restrict__ v = vals<T>(A);
+  const T* __restrict__ v = vhat<T>(A);
+  const T* __restrict__ r = ib.r[0];
+  double g = 0.0, d = 0.0, rr = 0.0;
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
+    T acc = T(0);
+    for (int k = 0; k < A.w; ++k) {
+      const int64_t o = (int64_t)k * A.n + i;
+      acc = fma_t(v[o], r[A.col[o]], acc);
+    }
+    const T ri = r[i];
+    const T u = dinv<T>(A)[i] * ri;
+    ib.w[0][i] = acc;
+    g += (double)ri * (double)u;
+    d += (double)acc * (double)u;
+    rr += (double)ri * (double)ri;
+  }
+  publish3(g, d, rr, slot(f, S_G0, G), slot(f, S_D0, G), slot(f, S_RR0, G), lds);
+}
+
+// One iteration.  W > 0: compile-time ELL width, fully unrolled, with the 2W
+// coalesced slot loads issued before the scalar prologue; W = 0: runtime width.
+template <typename T, int W>
+__global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBufs<T> ib, int it) {
+  __shared__ double lds[3 * (kPB / 64)];
+  __shared__ double tot[3];
+  const int G = gridDim.x;
+  const int cur = it & 1, nxt = cur ^ 1;
+  const T* __restrict__ v = vhat<T>(A);
   const int32_t* __restrict__ col = A.col;
   const int64_t n = A.n;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x;
@@ -225,13 +266,68 @@ __global__ void __launch_bounds__(kPB) k_spmv(EllDev A, FieldBufs f, const T* __
       }
     }
   }
-  const IterScalars s = iter_scalars(f, it, G, lds, true);
-  if (s.done) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) f.sc->inner_done = 1;
+  // g_i, d_i, rr_i from the previous kernel's block partials
+  {
+    const double* pg = slot(f, cur ? S_G1 : S_G0, G);
+    const double* pd = slot(f, cur ? S_D1 : S_D0, G);
+    const double* pr = slot(f, cur ? S_RR1 : S_RR0, G);
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int j = threadIdx.x; j < G; j += kPB) {
+      a += pg[j];
+      b += pd[j];
+      c += pr[j];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    constexpr int NW = kPB / 64;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+      lds[wv] = a;
+      lds[NW + wv] = b;
+      lds[2 * NW + wv] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        x += lds[k];
+        y += lds[NW + k];
+        z += lds[2 * NW + k];
+      }
+      tot[0] = x;
+      tot[1] = y;
+      tot[2] = z;
+    }
+    __syncthreads();
+  }
+  const double g = tot[0], d = tot[1], rr = tot[2];
+  PcgScalars* sc = f.sc;
+  const double tol = sc->inner_tol_eff;
+  double beta = 0.0, alpha;
+  if (it > 0) {
+    beta = g / sc->hist[1 - cur];
+    alpha = g / (d - beta * g / sc->hist[2 + (1 - cur)]);
+  } else {
+    alpha = g / d;
+  }
+  // inner_done: an earlier launch of this batch already stopped the solve;
+  // the partials of this parity are then stale and must not be used
+  const bool done = sc->inner_done || sc->outer_done || (it > 0 && rr <= tol * tol) ||
+                    (sc->total_it + it >= sc->max_it) || !(alpha > 0.0) || !(alpha < 1e300);
+  if (done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->inner_done = 1;
     return;
   }
-  const T beta = it > 0 && s.rz_prev != 0.0 ? (T)(s.rz / s.rz_prev) : T(0);
-  double pq = 0.0;
+  const T a = (T)alpha, b = (T)beta;
+  const T* __restrict__ rc = ib.r[cur];
+  const T* __restrict__ wc = ib.w[cur];
+  const T* __restrict__ sp = ib.s[nxt];
+  double g1 = 0.0, d1 = 0.0, rr1 = 0.0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int64_t i = r0 + h * kPB;
@@ -239,69 +335,38 @@ __global__ void __launch_bounds__(kPB) k_spmv(EllDev A, FieldBufs f, const T* __
     T acc = T(0);
     if constexpr (W > 0) {
 #pragma unroll
-      for (int k = 0; k < W; ++k) acc = fma_t(vj[h][k], fma_t(beta, pold[cj[h][k]], z[cj[h][k]]), acc);
+      for (int k = 0; k < W; ++k) {
+        const int j = cj[h][k];
+        acc = fma_t(vj[h][k], r_next(rc[j], wc[j], sp[j], a, b), acc);
+      }
     } else {
       for (int k = 0; k < A.w; ++k) {
         const int64_t o = (int64_t)k * n + i;
         const int j = col[o];
-        acc = fma_t(v[o], fma_t(beta, pold[j], z[j]), acc);
+        acc = fma_t(v[o], r_next(rc[j], wc[j], sp[j], a, b), acc);
       }
     }
-    const T pi = fma_t(beta, pold[i], z[i]);
-    pnew[i] = pi;
-    q[i] = acc;
-    pq += (double)pi * (double)acc;
+    const T ri = rc[i], wi = wc[i], so = sp[i];
+    const T si = fma_t(b, so, wi);
+    const T rn = fma_t(-a, si, ri);
+    const T di = dinv<T>(A)[i];
+    const T pi = fma_t(b, ib.p[i], di * ri);
+    ib.x[i] = fma_t(a, pi, ib.x[i]);
+    ib.p[i] = pi;
+    ib.s[cur][i] = si;
+    ib.r[nxt][i] = rn;
+    ib.w[nxt][i] = acc;
+    const T un = di * rn;
+    g1 += (double)rn * (double)un;
+    d1 += (double)acc * (double)un;
+    rr1 += (double)rn * (double)rn;
   }
-  publish(pq, slot(f, S_PQ, G), lds);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kPB) k_update(EllDev A, FieldBufs f, InnerBufs<T> ib,
-                                                 const T* __restrict__ pnew, int it) {
-  __shared__ double lds[kPB / 64];
-  const int G = gridDim.x;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x;
-  T pn[2], dd[2], rv[2], qv[2], di[2];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int64_t i = r0 + h * kPB;
-    const bool ok = i < A.n;
-    pn[h] = ok ? pnew[i] : T(0);
-    dd[h] = ok ? ib.d[i] : T(0);
-    rv[h] = ok ? ib.r[i] : T(0);
-    qv[h] = ok ? ib.q[i] : T(0);
-    di[h] = ok ? dinv<T>(A)[i] : T(0);
-  }
-  // update(it) overwrites the other rz/rr parity, so it must not read it
-  const IterScalars s = iter_scalars(f, it, G, lds, false);
-  if (s.done) return;
-  const double pq = sum_partials(slot(f, S_PQ, G), G, lds);
-  if (!(fabs(pq) > 1e-300)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) f.sc->inner_done = 1;
-    return;
-  }
-  const T alpha = (T)(s.rz / pq);
-  double rz = 0.0, rr = 0.0;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int64_t i = r0 + h * kPB;
-    if (i >= A.n) continue;
-    const T d = fma_t(alpha, pn[h], dd[h]);
-    const T r = rv[h] - alpha * qv[h];
-    const T z = di[h] * r;
-    ib.d[i] = d;
-    ib.r[i] = r;
-    ib.z[i] = z;
-    rz += (double)r * (double)z;
-    rr += (double)r * (double)r;
-  }
-  const int nxt = (it + 1) & 1;
-  publish(rz, slot(f, nxt ? S_RZ1 : S_RZ0, G), lds);
-  publish(rr, slot(f, nxt ? S_RR1 : S_RR0, G), lds);
+  publish3(g1, d1, rr1, slot(f, nxt ? S_G1 : S_G0, G), slot(f, nxt ? S_D1 : S_D0, G),
+           slot(f, nxt ? S_RR1 : S_RR0, G), lds);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    f.sc->inner_it = it + 1;
-    f.sc->rz = s.rz;
-    f.sc->alpha = s.rz / pq;
+    sc->hist[cur] = g;
+    sc->hist[2 + cur] = alpha;
+    sc->inner_it = it + 1;
   }
 }
 
@@ -313,7 +378,7 @@ __global__ void __launch_bounds__(kPB) k_apply(EllDev A, FieldBufs f, InnerBufs<
   const double rn = f.sc->rnorm;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
   for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB)
-    f.x64[i] = fma(rn, (double)ib.d[i], f.x64[i]);
+    f.x64[i] = fma(rn, (double)ib.x[i], f.x64[i]);
 }
 
 __global__ void k_count_outer(FieldBufs f) {
@@ -412,28 +477,24 @@ template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_inner_init<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
+  hipLaunchKernelGGL(k_cg_start<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
 }
 template <typename T>
 void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
                        hipStream_t s, hipEvent_t* ev) {
-  const T* pold = (it & 1) ? ib.p1 : ib.p0;
-  T* pnew = (it & 1) ? ib.p0 : ib.p1;
-  const int G = grid_rows(A.n);
-  const dim3 g(G), b(kPB);
+  const dim3 g(grid_rows(A.n)), b(kPB);
   hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
   switch (A.w) {
 #define SCDAT_W(w) \
   case w:          \
-    hipExtLaunchKernelGGL((k_spmv<T, w>), g, b, 0, s, e0, e1, 0, A, f, ib.z, pold, pnew, ib.q, it); \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
     break;
     SCDAT_W(7) SCDAT_W(8) SCDAT_W(9) SCDAT_W(10) SCDAT_W(11) SCDAT_W(12) SCDAT_W(13) SCDAT_W(14)
     SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18) SCDAT_W(19) SCDAT_W(20)
 #undef SCDAT_W
     default:
-      hipExtLaunchKernelGGL((k_spmv<T, 0>), g, b, 0, s, e0, e1, 0, A, f, ib.z, pold, pnew, ib.q, it);
+      hipExtLaunchKernelGGL((k_cg_iter<T, 0>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
   }
-  hipExtLaunchKernelGGL(k_update<T>, dim3(G), dim3(kPB), 0, s, ev ? ev[2] : nullptr,
-                        ev ? ev[3] : nullptr, 0, A, f, ib, (const T*)pnew, it);
 }
 template <typename T>
 void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s) {

@@ -15,6 +15,7 @@ struct PcgScalars {
   double rz, pq, alpha, beta, rr;
   double vfloat, qcond, ccap;
   double tol, inner_tol, inner_tol_eff;
+  double hist[4];  // (g_i, alpha_i) of the last inner iteration, by parity
   int32_t outer_done, inner_done, inner_it, outer_it, total_it, max_it, max_outer, failed;
   unsigned int ticket[4];
 };
@@ -28,6 +29,8 @@ struct EllDev {
   const int32_t* col;
   const float* v32;
   const double* v64;
+  const float* vh32;   // A D^-1 (column-scaled by the Jacobi inverse), inner solve
+  const double* vh64;
   const float* dinv32;
   const double* dinv64;
   // nodal-gradient weights (LSQ), same slots, 3 components
@@ -55,14 +58,15 @@ struct FieldBufs {
   PcgScalars* sc;
 };
 
+// FP32 (mixed) or FP64 inner-solve vectors: r, w = A D^-1 r and s = A p by
+// iteration parity, the search direction p and the correction x.
 template <typename T>
 struct InnerBufs {
-  T* r;
-  T* z;
-  T* d;
-  T* p0;
-  T* p1;
-  T* q;
+  T* r[2];
+  T* w[2];
+  T* s[2];
+  T* p;
+  T* x;
 };
 
 void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* rho_fx,
@@ -70,9 +74,9 @@ void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* r
 void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s);
 template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s);
-// iteration `it` of the current inner solve (0, 1, ...); with ev non-null the
-// dispatch timestamps of the SpMV land in ev[0], ev[1] and of the update in
-// ev[2], ev[3] (kernel execution only, no launch gaps)
+// iteration `it` of the current inner solve (0, 1, ...), one kernel; with ev
+// non-null its dispatch timestamps land in ev[0], ev[1] (kernel execution
+// only, no launch gaps)
 template <typename T>
 void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
                        hipStream_t s, hipEvent_t* ev);
