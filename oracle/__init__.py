@@ -56,7 +56,72 @@ _SIG = {
     "orc_adaptive_dt": (C.c_double, [_D, _D, _D, C.c_double, C.c_double, C.c_double]),
     "orc_barycentric": (None, [_P, C.c_int, _D, _D]),
     "orc_deposit_fixed_f32": (None, [C.c_int, _F, _I32, C.c_int64, _F, _I32, C.c_int64, _I64]),
+    "orc_set_seed": (None, [_P, C.c_uint32, C.c_uint32]),
+    "orc_set_step": (None, [_P, C.c_int64]),
+    "orc_mcc_configure": (None, [_P, C.c_int, _D]),
+    "orc_set_emission": (None, [_P, C.c_int, _D]),
+    "orc_post_push": (None, [_P]),
+    "orc_mcc_counts": (None, [_P, _I64]),
+    "orc_get_ids": (None, [_P, C.c_int, _I32]),
+    "orc_sigma": (None, [_D, C.c_int64, _D, _D]),
+    "orc_coulomb_log": (C.c_double, [C.c_double, C.c_double]),
+    "orc_see_yield": (None, [C.c_int, C.c_double, C.c_double, C.c_int64, _D, _D, _D]),
+    "orc_photo_current": (None, [C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D]),
+    "orc_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
 }
+
+# Collision process types (CollisionProcessFactory, CollisionAlgorithm.cpp:738-812)
+MCC_ELASTIC, MCC_INELASTIC, MCC_IONIZATION, MCC_EXCITATION = 0, 1, 2, 3
+MCC_CHARGE_EXCHANGE, MCC_RECOMBINATION, MCC_COULOMB = 4, 5, 6
+_PROC_KEYS = ("type", "species", "e_species", "i_species", "density", "threshold", "sigma_max",
+              "energy_loss", "reference_energy", "power_index", "alpha", "beta", "coulomb_log")
+_PROC_DEFAULT = dict(type=0, species=0, e_species=-1, i_species=-1, density=0.0, threshold=0.0,
+                     sigma_max=0.0, energy_loss=0.0, reference_energy=1.0, power_index=0.5,
+                     alpha=1.2, beta=0.35, coulomb_log=10.0)
+
+
+def pack_process(p):
+    d = dict(_PROC_DEFAULT)
+    d.update(p)
+    a = np.zeros(16)
+    for k, key in enumerate(_PROC_KEYS):
+        a[k] = d[key]
+    return a
+
+
+def sigma(proc, e_ev):
+    e = np.ascontiguousarray(e_ev, np.float64)
+    out = np.zeros_like(e)
+    prm = pack_process(proc)
+    lib().orc_sigma(_p(prm, C.c_double), len(e), _p(e, C.c_double), _p(out, C.c_double))
+    return out
+
+
+def coulomb_log(temperature_k, density_m3):
+    return lib().orc_coulomb_log(temperature_k, density_m3)
+
+
+def see_yield(model, dmax, emax, e_ev, cos_inc):
+    e = np.ascontiguousarray(e_ev, np.float64)
+    c = np.ascontiguousarray(np.broadcast_to(cos_inc, e.shape), np.float64)
+    out = np.zeros_like(e)
+    lib().orc_see_yield(model, dmax, emax, len(e), _p(e, C.c_double), _p(c, C.c_double), _p(out, C.c_double))
+    return out
+
+
+def photo_current(j0y, t_ph, v_ref, V):
+    v = np.ascontiguousarray(V, np.float64)
+    out = np.zeros_like(v)
+    lib().orc_photo_current(j0y, t_ph, v_ref, len(v), _p(v, C.c_double), _p(out, C.c_double))
+    return out
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox(c, k, o)
+    return list(o)
 
 _lib = None
 
@@ -245,6 +310,37 @@ class Oracle:
 
     def kinetic_energy(self):
         return lib().orc_kinetic_energy(self.h)
+
+    def set_seed(self, seed):
+        lib().orc_set_seed(self.h, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
+
+    def set_step(self, step):
+        lib().orc_set_step(self.h, step)
+
+    def mcc_configure(self, procs):
+        a = np.concatenate([pack_process(p) for p in procs]) if procs else np.zeros(16)
+        lib().orc_mcc_configure(self.h, len(procs), _p(a, C.c_double))
+
+    def set_emission(self, group, see_model=0, delta_max=0.0, energy_max=400.0, see_temperature=2.0,
+                     photo_j=0.0, photo_temperature=2.0, photo_reference_v=0.0, electron_species=0):
+        a = np.array([see_model, delta_max, energy_max, see_temperature, photo_j, photo_temperature,
+                      photo_reference_v, electron_species], np.float64)
+        lib().orc_set_emission(self.h, group, _p(a, C.c_double))
+
+    def post_push(self):
+        lib().orc_post_push(self.h)
+
+    def mcc_counts(self):
+        out = np.zeros(20, np.int64)
+        lib().orc_mcc_counts(self.h, _p(out, C.c_int64))
+        return dict(collisions=out[:16].copy(), created=int(out[16]), removed=int(out[17]),
+                    see_emitted=int(out[18]), photo_emitted=int(out[19]))
+
+    def get_ids(self, s):
+        n = lib().orc_get_particles(self.h, s, None, None, None)
+        ids = np.zeros(n, np.int32)
+        lib().orc_get_ids(self.h, s, _p(ids, C.c_int32))
+        return ids
 
     def barycentric(self, t, p):
         p = np.ascontiguousarray(p, np.float64)

@@ -42,7 +42,26 @@ struct Species {
   double q = 0, m = 1, w = 1;
   std::vector<double> x, v;  // 3 per particle
   std::vector<int32_t> cell; // -1 = removed
+  std::vector<int32_t> id;   // upload index, or a generated id (bit 31 set)
 };
+
+// Monte Carlo collision process of one incident species with a uniform
+// background.  Types follow CollisionProcessFactory
+// (Tools/Interactions/Collisions/src/CollisionAlgorithm.cpp:738-812).
+enum MccType { MCC_ELASTIC = 0, MCC_INELASTIC, MCC_IONIZATION, MCC_EXCITATION, MCC_CHARGE_EXCHANGE,
+               MCC_RECOMBINATION, MCC_COULOMB };
+struct MccProc {
+  int type = 0, species = 0, e_species = -1, i_species = -1;
+  double n = 0, thr = 0, smax = 0, loss = 0, eref = 1, pidx = 0.5, alpha = 1.2, beta = 0.35, clog = 10;
+};
+// Surface emission of one boundary group.
+struct SurfEmit {
+  int see_model = 0;  // 0 off, 1 BasicSurfaceMaterialModel secondaryYield, 2 Sternglass
+  double dmax = 0, emax = 400, t_see = 2;
+  double photo_j = 0, t_ph = 2, v_ref = 0;  // photo_j = J0 * Y (A/m^2)
+  int e_species = 0;
+};
+struct SeeEvent { int species, id, cell, group, face; double x[3], v[3]; };
 
 struct Csr {
   int n = 0;
@@ -207,6 +226,13 @@ struct Sim {
   double surface_charge = 0.0;
   int64_t lost = 0;
   int64_t steps = 0;
+  // ---- collisions and surface emission ------------------------------------
+  std::vector<MccProc> mcc;
+  uint32_t key[2] = {0, 0};
+  int64_t mcc_count[16] = {};
+  int64_t mcc_created = 0, mcc_removed = 0, see_emitted = 0, photo_emitted = 0;
+  std::array<SurfEmit, kMaxGroups> emit{};
+  std::vector<SeeEvent> events;
 };
 
 // Barycentric coordinates through the affine inverse-Jacobian map, the fast
@@ -497,7 +523,7 @@ PcgResult solve_poisson(Sim& s) {
   return r;
 }
 
-struct WalkOut { int cell; int dead; int group; double hit[3]; };
+struct WalkOut { int cell; int dead; int group; int face; double hit[3]; };
 
 // Straight-segment tet walk from x0 (inside `cell`) to x1.  Replaces
 // CrossTetrahedron::dichotomyBoundaryIntersection + locateParticle
@@ -572,6 +598,7 @@ void walk(const Sim& s, double* x0, double* x1, double* v, int cell, WalkOut& o,
     }
     o.dead = 1;
     o.group = g;
+    o.face = kb;
     o.cell = cur;
     for (int r = 0; r < 3; ++r) o.hit[r] = hit[r];
     return;
@@ -595,6 +622,8 @@ void push_species(Sim& s, Species& p, std::vector<double>& rho_acc) {
   std::vector<std::array<Ledger, kMaxGroups>> led_t(nthr);
   std::vector<double> qc_t(nthr, 0.0);
   std::vector<int64_t> lost_t(nthr, 0);
+  std::vector<std::vector<SeeEvent>> ev_t(nthr);
+  const int si = (int)(&p - s.sp.data());
   // Thread-local charge buffers reduced per node afterwards, as the OpenMP
   // branch of PICCycle::pushParticles, Tools/PICcore/src/PICCycle.cpp:483-563.
 #pragma omp parallel
@@ -648,6 +677,14 @@ void push_species(Sim& s, Species& p, std::vector<double>& rho_acc) {
               if (p.q < 0) { L.abs_e_charge += qw; L.n_abs_e++; }
               else { L.abs_i_charge += qw; L.n_abs_i++; }
               if (s.gtype[g] == BC_CONDUCTOR) qc_t[tid] += qw;
+              // secondary emission is evaluated after the push from the
+              // recorded impact (electrons on absorbing or conductor faces)
+              if (p.q < 0 && s.emit[g].see_model != 0 &&
+                  (s.gtype[g] == BC_CONDUCTOR || s.gtype[g] == BC_ABSORB)) {
+                SeeEvent e{si, p.id.empty() ? (int)i : p.id[i], w.cell, g, w.face, {}, {}};
+                for (int r = 0; r < 3; ++r) { e.x[r] = w.hit[r]; e.v[r] = v[r]; }
+                ev_t[tid].push_back(e);
+              }
             }
             for (int r = 0; r < 3; ++r) x[r] = w.hit[r];
           }
@@ -710,7 +747,379 @@ void push_species(Sim& s, Species& p, std::vector<double>& rho_acc) {
     }
     s.Qcond += qc_t[t];
     s.lost += lost_t[t];
+    s.events.insert(s.events.end(), ev_t[t].begin(), ev_t[t].end());
   }
+}
+
+
+// =========================================================================
+// Monte Carlo collisions and surface emission
+// =========================================================================
+constexpr double kPi = 3.14159265358979323846;
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): ten rounds of the two 32x32->64 multiply-xor
+// bijection with the Weyl key schedule.
+void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+inline double u01(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
+inline void draw(const Sim& s, uint32_t a, uint32_t b, uint32_t c, uint32_t tag, uint32_t out[4]) {
+  const uint32_t ctr[4] = {a, b, c, tag};
+  philox(ctr, s.key, out);
+}
+// stream tags (fourth counter word)
+constexpr uint32_t kTagMccA = 0x4D43, kTagMccB = 0x4D44, kTagMccId = 0x4D45, kTagSee = 0x5345,
+                   kTagSeeK = 0x5346, kTagSeeId = 0x5347, kTagPhoto = 0x5048, kTagPhotoK = 0x5049,
+                   kTagPhotoK2 = 0x504A;
+constexpr int kMaxEmit = 8;
+constexpr double kMe = 9.1093837015e-31;  // Tools/Basic/include/Constants.h:47
+
+// Cross sections in m^2 of energy in eV,
+// Tools/Interactions/Collisions/src/CollisonTypes.cpp:56-176.
+double sigma_of(const MccProc& p, double E) {
+  switch (p.type) {
+    case MCC_ELASTIC:
+    case MCC_INELASTIC:
+    case MCC_CHARGE_EXCHANGE: {  // ElasticCollisionCrossSection :56-77
+      if (E < p.thr) return 0.0;
+      const double sg = p.smax * std::pow(std::max(1e-6, p.eref) / E, p.pidx);
+      return std::max(0.0, std::min(sg, p.smax));
+    }
+    case MCC_IONIZATION: {  // :79-93
+      if (E < p.thr || E <= p.thr) return 0.0;
+      const double I = std::max(1e-6, p.thr);
+      const double sg = p.smax * std::log(E / I) * (1.0 - I / E) / E;
+      return std::max(0.0, std::min(sg, p.smax));
+    }
+    case MCC_EXCITATION: {  // :95-115
+      if (E < p.thr || E <= p.thr) return 0.0;
+      const double x = E / std::max(1e-6, p.thr);
+      const double sg = p.smax * std::pow(std::max(0.0, 1.0 - 1.0 / x), p.alpha) * std::pow(1.0 / x, p.beta);
+      return std::max(0.0, std::min(sg, p.smax));
+    }
+    case MCC_RECOMBINATION: {  // :117-132
+      const double er = std::max(1e-6, p.eref);
+      const double sg = p.smax * std::pow(er / std::max(er, E), p.pidx);
+      return std::max(0.0, std::min(sg, p.smax));
+    }
+    default: {  // MCC_COULOMB :160-176
+      if (E <= 0.0) return 0.0;
+      const double Ej = E * kE;
+      const double k = 1.0 / (4.0 * kPi * kEps0);
+      const double ke2 = k * kE * kE;
+      const double sg = kPi * p.clog * ke2 * ke2 / std::max(1e-30, 4.0 * Ej * Ej);
+      return std::max(0.0, std::min(sg, p.smax));
+    }
+  }
+}
+
+inline void iso_dir(double u2, double u3, double* d) {
+  const double ct = 2.0 * u2 - 1.0;
+  const double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+  const double ph = 2.0 * kPi * u3;
+  d[0] = st * std::cos(ph);
+  d[1] = st * std::sin(ph);
+  d[2] = ct;
+}
+
+void append(Species& t, const double* x, const double* v, int cell, int32_t id) {
+  for (int r = 0; r < 3; ++r) { t.x.push_back(x[r]); t.v.push_back(v[r]); }
+  t.cell.push_back(cell);
+  t.id.push_back(id);
+}
+
+void deposit_point(Sim& s, int cell, const double* x, double qw) {
+  double lam[4];
+  bary(s, cell, x, lam);
+  for (int k = 0; k < 4; ++k) s.rho[s.tet[4 * (size_t)cell + k]] += qw * lam[k];
+}
+
+// One background-collision step over the particles present before it
+// (replaces MonteCarloCollisionHandler::processCollisions,
+// CollisionAlgorithm.cpp:500-538, inside executeBackgroundMccStep,
+// CollisionPicAdapter.cpp:338-434).  Per particle: nu_p = n_p sigma_p(E) |v|,
+// one collision with probability 1 - exp(-sum nu dt), process chosen with
+// probability nu_p / sum nu (null-collision thinning with the particle's own
+// total frequency as the majorant).  The reference's second draw inside
+// executeCollision (effective probability p^2) is not reproduced.  Process
+// outcomes follow CollisionAlgorithm.cpp:189-460: isotropic scatter
+// (elastic, Coulomb), energy loss (inelastic, excitation above threshold),
+// ionisation with a uniform split of E - I and an ion at rest, removal
+// (charge exchange turns the ion neutral, recombination the electron).
+void mcc_step(Sim& s) {
+  if (s.mcc.empty()) return;
+  const uint32_t step = (uint32_t)s.steps;
+  std::vector<size_t> n0(s.sp.size());
+  for (size_t k = 0; k < s.sp.size(); ++k) n0[k] = s.sp[k].cell.size();
+  for (size_t si = 0; si < s.sp.size(); ++si) {
+    std::vector<int> pr;
+    for (size_t k = 0; k < s.mcc.size(); ++k)
+      if (s.mcc[k].species == (int)si) pr.push_back((int)k);
+    if (pr.empty()) continue;
+    const double m = s.sp[si].m;
+    for (size_t i = 0; i < n0[si]; ++i) {
+      Species& P = s.sp[si];
+      if (P.cell[i] < 0) continue;
+      double v[3] = {P.v[3 * i], P.v[3 * i + 1], P.v[3 * i + 2]};
+      const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+      const double speed = std::sqrt(v2);
+      const double E = 0.5 * m * v2 / kE;
+      double nu[16], tot = 0.0;
+      for (size_t j = 0; j < pr.size(); ++j) {
+        nu[j] = s.mcc[pr[j]].n * sigma_of(s.mcc[pr[j]], E) * speed;
+        tot += nu[j];
+      }
+      if (!(tot > 0.0)) continue;
+      const uint32_t id = (uint32_t)P.id[i];
+      uint32_t a[4];
+      draw(s, id, step, (uint32_t)si, kTagMccA, a);
+      if (!(u01(a[0]) < -std::expm1(-tot * s.dt))) continue;
+      const double R = u01(a[1]) * tot;
+      int j = -1;
+      double cum = 0.0;
+      for (size_t q = 0; q < pr.size(); ++q) {
+        cum += nu[q];
+        if (nu[q] > 0.0) j = (int)q;
+        if (R < cum) break;
+      }
+      const MccProc& q = s.mcc[pr[j]];
+      s.mcc_count[pr[j]]++;
+      double d[3];
+      switch (q.type) {
+        case MCC_ELASTIC:
+        case MCC_COULOMB:
+          iso_dir(u01(a[2]), u01(a[3]), d);
+          for (int r = 0; r < 3; ++r) P.v[3 * i + r] = speed * d[r];
+          break;
+        case MCC_INELASTIC:
+        case MCC_EXCITATION: {
+          if (q.type == MCC_EXCITATION && E <= q.thr) break;
+          const double loss = q.type == MCC_INELASTIC ? std::max(0.0, q.loss) : (q.loss > 0.0 ? q.loss : q.thr);
+          const double sp = std::sqrt(2.0 * std::max(1e-6, E - loss) * kE / m);
+          iso_dir(u01(a[2]), u01(a[3]), d);
+          for (int r = 0; r < 3; ++r) P.v[3 * i + r] = sp * d[r];
+          break;
+        }
+        case MCC_IONIZATION: {
+          if (E <= q.thr) break;
+          uint32_t b[4], c[4];
+          draw(s, id, step, (uint32_t)si, kTagMccB, b);
+          draw(s, id, step, (uint32_t)si, kTagMccId, c);
+          const double avail = E - q.thr;
+          const double sec = u01(b[0]) * avail;
+          const double prim = avail - sec;
+          const double sp = std::sqrt(2.0 * prim * kE / m);
+          iso_dir(u01(a[2]), u01(a[3]), d);
+          for (int r = 0; r < 3; ++r) P.v[3 * i + r] = sp * d[r];
+          const double xp[3] = {P.x[3 * i], P.x[3 * i + 1], P.x[3 * i + 2]};
+          const int cell = P.cell[i];
+          Species& Se = s.sp[q.e_species];
+          const double se = std::sqrt(2.0 * sec * kE / Se.m);
+          iso_dir(u01(b[1]), u01(b[2]), d);
+          const double ve[3] = {se * d[0], se * d[1], se * d[2]};
+          append(Se, xp, ve, cell, (int32_t)(0x80000000u | (c[0] & 0x7FFFFFFFu)));
+          deposit_point(s, cell, xp, Se.q * Se.w);
+          Species& Si = s.sp[q.i_species];
+          const double vi[3] = {0.0, 0.0, 0.0};
+          append(Si, xp, vi, cell, (int32_t)(0x80000000u | (c[1] & 0x7FFFFFFFu)));
+          deposit_point(s, cell, xp, Si.q * Si.w);
+          s.mcc_created += 2;
+          break;
+        }
+        default:  // charge exchange, recombination
+          P.cell[i] = -1;
+          s.mcc_removed++;
+          break;
+      }
+    }
+  }
+}
+
+// Secondary-electron yield.  Model 1: secondaryYield of
+// Tools/Material/src/SurfaceMaterialModel.cpp:163-170
+// (delta_max x e^{1-x} / sqrt(max(0.1, cos theta)), x = E / max(1, E_max)).
+// Model 2: Sternglass, delta_max (E/E_max) exp(2 - 2 sqrt(E/E_max)).
+double see_yield(int model, double dmax, double emax, double E, double cos_inc) {
+  if (model == 1) {
+    const double x = std::max(1e-6, E / std::max(1.0, emax));
+    return std::max(0.0, dmax * x * std::exp(1.0 - x) / std::sqrt(std::max(0.1, cos_inc)));
+  }
+  if (model == 2) {
+    if (!(E > 0.0)) return 0.0;
+    const double r = E / emax;
+    return dmax * r * std::exp(2.0 - 2.0 * std::sqrt(r));
+  }
+  return 0.0;
+}
+
+// Photoelectron current density, SurfaceMaterialModel.cpp:199-208.
+double photo_current(double j0y, double t_ph, double v_ref, double V) {
+  return std::max(0.0, j0y) * std::exp(-std::max(0.0, V - v_ref) / std::max(1e-3, t_ph));
+}
+
+// Inward unit normal of face k of tet t (gradient of lambda_k).
+void inward_normal(const Sim& s, int t, int k, double* n) {
+  const double* a = &s.A[9 * (size_t)t];
+  if (k == 0) for (int r = 0; r < 3; ++r) n[r] = -(a[r] + a[3 + r] + a[6 + r]);
+  else for (int r = 0; r < 3; ++r) n[r] = a[3 * (k - 1) + r];
+  const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int r = 0; r < 3; ++r) n[r] /= l;
+}
+
+// Emission point: the surface point moved 1e-6 of the way to the tet centroid.
+void nudge(const Sim& s, int t, const double* p, double* x) {
+  for (int r = 0; r < 3; ++r) {
+    double c = 0.0;
+    for (int k = 0; k < 4; ++k) c += s.xyz[3 * (size_t)s.tet[4 * (size_t)t + k] + r];
+    x[r] = p[r] + 1e-6 * (0.25 * c - p[r]);
+  }
+}
+
+// Emitted velocity: flux-weighted Maxwellian of temperature T (eV), energy
+// -T ln(u_a u_b) (Gamma(2, T)), cos(theta) = sqrt(u_c) about the inward
+// normal, azimuth 2 pi u_d in the tangent frame (t1 = n x a / |n x a|,
+// a = x-axis unless |n_x| >= 0.9, then y-axis; t2 = n x t1).
+void emit_velocity(const double* n, double T, double m, double ua, double ub, double uc, double ud,
+                   double* v) {
+  const double E = -T * (std::log(ua) + std::log(ub));
+  const double sp = std::sqrt(2.0 * E * kE / m);
+  const double ax[3] = {std::abs(n[0]) < 0.9 ? 1.0 : 0.0, std::abs(n[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+  double t1[3] = {n[1] * ax[2] - n[2] * ax[1], n[2] * ax[0] - n[0] * ax[2], n[0] * ax[1] - n[1] * ax[0]};
+  const double l = std::sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+  for (int r = 0; r < 3; ++r) t1[r] /= l;
+  const double t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+  const double ct = std::sqrt(uc), st = std::sqrt(1.0 - uc), ph = 2.0 * kPi * ud;
+  const double cp = std::cos(ph), sn = std::sin(ph);
+  for (int r = 0; r < 3; ++r) v[r] = sp * (st * cp * t1[r] + st * sn * t2[r] + ct * n[r]);
+}
+
+void record_emission(Sim& s, int g, int count, const Species& T) {
+  Ledger& L = s.ledger[g];
+  L.n_emit_e += count;
+  L.emit_e_charge += count * T.q * T.w;
+  if (s.gtype[g] == BC_CONDUCTOR) s.Qcond -= count * T.q * T.w;
+}
+
+// Secondary emission from the impacts recorded by the last push.
+void see_emit(Sim& s) {
+  const uint32_t step = (uint32_t)s.steps;
+  for (const SeeEvent& e : s.events) {
+    const SurfEmit& cf = s.emit[e.group];
+    const Species& P = s.sp[e.species];
+    double n[3];
+    inward_normal(s, e.cell, e.face, n);
+    const double v2 = e.v[0] * e.v[0] + e.v[1] * e.v[1] + e.v[2] * e.v[2];
+    const double sp = std::sqrt(v2);
+    const double E = 0.5 * P.m * v2 / kE;
+    const double cosi = sp > 0.0 ? std::abs(e.v[0] * n[0] + e.v[1] * n[1] + e.v[2] * n[2]) / sp : 1.0;
+    const double dl = see_yield(cf.see_model, cf.dmax, cf.emax, E, cosi);
+    uint32_t a[4];
+    draw(s, (uint32_t)e.id, step, (uint32_t)e.species, kTagSee, a);
+    const double fl = std::floor(dl);
+    int nem = (int)std::min(fl, (double)kMaxEmit) + (u01(a[0]) < dl - fl ? 1 : 0);
+    nem = std::min(nem, kMaxEmit);
+    if (nem <= 0) continue;
+    double xe[3];
+    nudge(s, e.cell, e.x, xe);
+    Species& T = s.sp[cf.e_species];
+    for (int k = 0; k < nem; ++k) {
+      const uint32_t sk = ((uint32_t)e.species << 8) | (uint32_t)k;
+      uint32_t b[4], c[4];
+      draw(s, (uint32_t)e.id, step, sk, kTagSeeK, b);
+      draw(s, (uint32_t)e.id, step, sk, kTagSeeId, c);
+      double v[3];
+      emit_velocity(n, cf.t_see, T.m, u01(b[0]), u01(b[1]), u01(b[2]), u01(b[3]), v);
+      append(T, xe, v, e.cell, (int32_t)(0x80000000u | (c[0] & 0x7FFFFFFFu)));
+      deposit_point(s, e.cell, xe, T.q * T.w);
+    }
+    record_emission(s, e.group, nem, T);
+    s.see_emitted += nem;
+  }
+  s.events.clear();
+}
+
+// Boundary faces of group g in (tet, face) order with cumulative areas.
+struct FaceList { std::vector<int32_t> tet, face; std::vector<double> cum; };
+FaceList group_faces(const Sim& s, int g) {
+  FaceList f;
+  double acc = 0.0;
+  for (int t = 0; t < s.nt; ++t)
+    for (int k = 0; k < 4; ++k) {
+      if (s.fgroup[4 * (size_t)t + k] != g || s.nb[4 * (size_t)t + k] >= 0) continue;
+      const double* p[3];
+      int m = 0;
+      for (int q = 0; q < 4; ++q)
+        if (q != k) p[m++] = &s.xyz[3 * (size_t)s.tet[4 * (size_t)t + q]];
+      const double u[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+      const double w[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+      const double c[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+      acc += 0.5 * std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      f.tet.push_back(t);
+      f.face.push_back(k);
+      f.cum.push_back(acc);
+    }
+  return f;
+}
+
+// Photoemission: per group and step, J A dt / (|q| w) macro-electrons
+// (integer part plus one with the fractional probability), uniform over the
+// group's faces, J from photo_current at the group's potential (the floating
+// potential of the last solve on conductor faces, 0 V elsewhere).
+void photo_emit(Sim& s) {
+  const uint32_t step = (uint32_t)s.steps;
+  for (int g = 0; g < kMaxGroups; ++g) {
+    const SurfEmit& cf = s.emit[g];
+    if (!(cf.photo_j > 0.0)) continue;
+    const FaceList f = group_faces(s, g);
+    if (f.cum.empty()) continue;
+    const double area = f.cum.back();
+    Species& T = s.sp[cf.e_species];
+    const double V = s.gtype[g] == BC_CONDUCTOR ? s.Vfloat : 0.0;
+    const double expect = photo_current(cf.photo_j, cf.t_ph, cf.v_ref, V) * area * s.dt /
+                          (std::abs(T.q) * T.w);
+    uint32_t a[4];
+    draw(s, (uint32_t)g, step, 0u, kTagPhoto, a);
+    const double fl = std::floor(expect);
+    const int64_t nem = (int64_t)fl + (u01(a[0]) < expect - fl ? 1 : 0);
+    for (int64_t k = 0; k < nem; ++k) {
+      uint32_t b[4], c[4];
+      draw(s, (uint32_t)k, step, (uint32_t)g, kTagPhotoK, b);
+      draw(s, (uint32_t)k, step, (uint32_t)g, kTagPhotoK2, c);
+      const double r = u01(b[0]) * area;
+      size_t fi = (size_t)(std::upper_bound(f.cum.begin(), f.cum.end(), r) - f.cum.begin());
+      if (fi >= f.cum.size()) fi = f.cum.size() - 1;
+      const int t = f.tet[fi], kf = f.face[fi];
+      const double* p[3];
+      int m = 0;
+      for (int q = 0; q < 4; ++q)
+        if (q != kf) p[m++] = &s.xyz[3 * (size_t)s.tet[4 * (size_t)t + q]];
+      const double sq = std::sqrt(u01(b[1])), u2 = u01(b[2]);
+      double pt[3], xe[3], n[3], v[3];
+      for (int q = 0; q < 3; ++q) pt[q] = (1.0 - sq) * p[0][q] + sq * (1.0 - u2) * p[1][q] + sq * u2 * p[2][q];
+      nudge(s, t, pt, xe);
+      inward_normal(s, t, kf, n);
+      emit_velocity(n, cf.t_ph, T.m, u01(b[3]), u01(c[0]), u01(c[1]), u01(c[2]), v);
+      append(T, xe, v, t, (int32_t)(0x80000000u | (c[3] & 0x7FFFFFFFu)));
+      deposit_point(s, t, xe, T.q * T.w);
+    }
+    record_emission(s, g, (int)nem, T);
+    s.photo_emitted += nem;
+  }
+}
+
+void post_push(Sim& s) {
+  mcc_step(s);
+  see_emit(s);
+  photo_emit(s);
 }
 
 }  // namespace
@@ -834,6 +1243,8 @@ void orc_set_particles(void* h, int si, int64_t n, const double* x, const double
   p.x.assign(x, x + 3 * n);
   p.v.assign(v, v + 3 * n);
   p.cell.assign(cell, cell + n);
+  p.id.resize(n);
+  for (int64_t i = 0; i < n; ++i) p.id[i] = (int32_t)i;
 }
 
 int64_t orc_get_particles(void* h, int si, double* x, double* v, int32_t* cell) {
@@ -887,6 +1298,7 @@ void orc_efield(void* h) { compute_efield(*static_cast<Sim*>(h)); }
 // Push all species one step (deposit into rho, which is cleared first).
 void orc_push(void* h) {
   auto* s = static_cast<Sim*>(h);
+  s->events.clear();
   std::fill(s->rho.begin(), s->rho.end(), 0.0);
   for (auto& p : s->sp) push_species(*s, p, s->rho);
 }
@@ -909,12 +1321,15 @@ void orc_deposit(void* h) {
 
 // One full cycle step in the order of PICCycle::executeTimeStep
 // (Tools/PICcore/src/PICCycle.cpp:311-416): clear rho, push + deposit with the
-// previous E, Poisson, nodal E.  Returns PCG iterations.
+// previous E, then (when configured) background collisions, secondary and
+// photo-emission with their new particles deposited, Poisson, nodal E.
+// Returns PCG iterations.
 int orc_step(void* h, int nsteps) {
   auto* s = static_cast<Sim*>(h);
   int it = 0;
   for (int k = 0; k < nsteps; ++k) {
     orc_push(h);
+    post_push(*s);
     it = solve_poisson(*s).iters;
     compute_efield(*s);
     s->steps++;
@@ -989,5 +1404,73 @@ void orc_deposit_fixed_f32(int nt, const float* geo, const int32_t* tet, int64_t
     for (int k = 0; k < 4; ++k) rho_fx[tet[4 * (size_t)c + k]] += cs * (int64_t)f[k];
   }
 }
+
+
+// ---- collisions and surface emission -------------------------------------
+void orc_set_seed(void* h, uint32_t lo, uint32_t hi) {
+  auto* s = static_cast<Sim*>(h);
+  s->key[0] = lo;
+  s->key[1] = hi;
+}
+void orc_set_step(void* h, int64_t step) { static_cast<Sim*>(h)->steps = step; }
+
+static MccProc unpack_proc(const double* q) {
+  MccProc p;
+  p.type = (int)q[0]; p.species = (int)q[1]; p.e_species = (int)q[2]; p.i_species = (int)q[3];
+  p.n = q[4]; p.thr = q[5]; p.smax = q[6]; p.loss = q[7]; p.eref = q[8]; p.pidx = q[9];
+  p.alpha = q[10]; p.beta = q[11]; p.clog = q[12];
+  return p;
+}
+// 16 doubles per process: type, species, e_species, i_species, density,
+// threshold, sigma_max, energy_loss, reference_energy, power_index, alpha,
+// beta, coulomb_log, (3 unused).
+void orc_mcc_configure(void* h, int n, const double* prm) {
+  auto* s = static_cast<Sim*>(h);
+  s->mcc.clear();
+  for (int k = 0; k < n && k < 16; ++k) s->mcc.push_back(unpack_proc(prm + 16 * k));
+}
+// 8 doubles: see_model, delta_max, energy_max, see_temperature, photo_j,
+// photo_temperature, photo_reference_v, electron_species.
+void orc_set_emission(void* h, int g, const double* q) {
+  auto* s = static_cast<Sim*>(h);
+  SurfEmit& e = s->emit[g];
+  e.see_model = (int)q[0]; e.dmax = q[1]; e.emax = q[2]; e.t_see = q[3];
+  e.photo_j = q[4]; e.t_ph = q[5]; e.v_ref = q[6]; e.e_species = (int)q[7];
+}
+// Collisions, secondary and photo-emission after a push (deposits into rho).
+void orc_post_push(void* h) { post_push(*static_cast<Sim*>(h)); }
+// out: 16 per-process collision counts, created, removed, SEE emitted, photo emitted
+void orc_mcc_counts(void* h, int64_t* out) {
+  auto* s = static_cast<Sim*>(h);
+  for (int k = 0; k < 16; ++k) out[k] = s->mcc_count[k];
+  out[16] = s->mcc_created; out[17] = s->mcc_removed; out[18] = s->see_emitted; out[19] = s->photo_emitted;
+}
+void orc_get_ids(void* h, int si, int32_t* ids) {
+  auto* s = static_cast<Sim*>(h);
+  const Species& p = s->sp[si];
+  std::memcpy(ids, p.id.data(), sizeof(int32_t) * p.id.size());
+}
+void orc_sigma(const double* prm, int64_t n, const double* e_ev, double* out) {
+  const MccProc p = unpack_proc(prm);
+  for (int64_t i = 0; i < n; ++i) out[i] = sigma_of(p, e_ev[i]);
+}
+// Coulomb logarithm of CoulombCollisionCrossSection (CollisonTypes.cpp:178-198):
+// ln(lambda_D / max(1e-12, 1e-4 lambda_D)) clamped to [1, 30],
+// lambda_D = sqrt(eps0 k T / (n e^2)) (1e-3 m for T or n <= 0).
+double orc_coulomb_log(double temperature_k, double density_m3) {
+  const double kB = 1.380649e-23;  // Tools/Basic/include/Constants.h:51
+  const double ld = (temperature_k <= 0.0 || density_m3 <= 0.0)
+                        ? 1e-3 : std::sqrt(kEps0 * kB * temperature_k / (density_m3 * kE * kE));
+  const double bmin = std::max(1e-12, ld * 1e-4);
+  return std::min(30.0, std::max(1.0, std::log(std::max(1.000001, ld / bmin))));
+}
+void orc_see_yield(int model, double dmax, double emax, int64_t n, const double* e_ev,
+                   const double* cos_inc, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = see_yield(model, dmax, emax, e_ev[i], cos_inc[i]);
+}
+void orc_photo_current(double j0y, double t_ph, double v_ref, int64_t n, const double* V, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = photo_current(j0y, t_ph, v_ref, V[i]);
+}
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
 
 }  // extern "C"

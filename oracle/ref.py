@@ -110,3 +110,50 @@ def poisson(xyz, tets, kind, dval, density):
     rc = lib().ref_poisson(len(xyz), _p(xyz, C.c_double), len(tets), _p(tets, C.c_int32), _p(kind, C.c_int32),
                            _p(dv, C.c_double), _p(de, C.c_double), _p(phi, C.c_double), _p(E, C.c_double))
     return phi, E, rc
+
+
+SURF_LIB = os.path.join(_HERE, "_ref", "libscdat_ref_surface.so")
+_slib = None
+
+
+def surface_lib():
+    """Cross sections and surface yields of the reference (a second library,
+    linked with --gc-sections; see oracle/ref/Makefile)."""
+    global _slib
+    if _slib is None:
+        if not os.path.exists(SURF_LIB):
+            subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "ref"), "-j8"], check=True)
+        L = C.CDLL(SURF_LIB)
+        L.ref_sigma.argtypes = [C.c_int, _D, C.c_int64, _D, _D]
+        L.ref_coulomb_log.argtypes = [C.c_double, C.c_double]
+        L.ref_coulomb_log.restype = C.c_double
+        L.ref_surface.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D, _D,
+                                  C.c_double, C.c_double, _D, _D]
+        _slib = L
+    return _slib
+
+
+def sigma(kind, prm, e_ev):
+    """kind: 0 elastic, 1 ionisation, 2 excitation, 3 recombination, 4 Coulomb;
+    prm: threshold, sigma_max, reference_energy, power_index, alpha, beta,
+    electron_density, electron_temperature_K."""
+    e = np.ascontiguousarray(e_ev, np.float64)
+    p = np.ascontiguousarray(prm, np.float64)
+    out = np.zeros_like(e)
+    surface_lib().ref_sigma(kind, _p(p, C.c_double), len(e), _p(e, C.c_double), _p(out, C.c_double))
+    return out
+
+
+def coulomb_log(temperature_k, density_m3):
+    return surface_lib().ref_coulomb_log(temperature_k, density_m3)
+
+
+def surface(see_max, see_peak_ev, photo_yield, photo_temp_ev, e_ev, angle_rad, surf_v, ref_v, source_j):
+    e = np.ascontiguousarray(e_ev, np.float64)
+    a = np.ascontiguousarray(angle_rad, np.float64)
+    v = np.ascontiguousarray(surf_v, np.float64)
+    y, j = np.zeros_like(e), np.zeros_like(e)
+    surface_lib().ref_surface(see_max, see_peak_ev, photo_yield, photo_temp_ev, len(e), _p(e, C.c_double),
+                              _p(a, C.c_double), _p(v, C.c_double), ref_v, source_j, _p(y, C.c_double),
+                              _p(j, C.c_double))
+    return y, j

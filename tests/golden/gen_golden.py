@@ -14,8 +14,12 @@ the reference's outputs of one routine:
   mb_floating_n{4,6}.npz  MultiBoundaryElectricFieldSolver: P1 FEM, Dirichlet,
                         FLOATING conductor, direct solve
   poisson_graph_n6.npz  PoissonSolver: graph Laplacian, direct solve, LSQ E
+  mcc_sigma.npz         collision cross sections (CollisonTypes.cpp) and the
+                        Coulomb logarithm
+  surface_yield.npz     BasicSurfaceMaterialModel secondary yield and
+                        photoelectron current density
 
-Usage: python tests/golden/gen_golden.py
+Usage: python tests/golden/gen_golden.py [field|surface]   (default: both)
 """
 import os
 import sys
@@ -107,5 +111,57 @@ def main():
          density=dens, phi=phi, E=Eg)
 
 
+# Parameter sets: the reference defaults of
+# MonteCarloCollisionHandler::initializeDefaultCollisionParameters
+# (CollisionAlgorithm.cpp:590-646) and one variant each.
+# Columns: threshold, sigma_max, reference_energy, power_index, alpha, beta,
+# electron_density, electron_temperature_K.
+SIGMA_SETS = [
+    (0, [0.0, 2.0e-19, 10.0, 0.3, 1.2, 0.35, 1e16, 1000.0]),     # elastic
+    (0, [1.0, 8.0e-20, 1.0, 0.5, 1.2, 0.35, 1e16, 1000.0]),      # inelastic (elastic form)
+    (0, [0.0, 1.2e-19, 1.0, 0.5, 1.2, 0.35, 1e16, 1000.0]),      # charge exchange (elastic form)
+    (0, [2.0, 5.0e-19, 0.5, 1.1, 1.2, 0.35, 1e16, 1000.0]),      # variant
+    (1, [15.76, 3.0e-20, 1.0, 0.5, 1.2, 0.35, 1e16, 1000.0]),    # ionisation
+    (1, [13.6, 1.0e-17, 1.0, 0.5, 1.2, 0.35, 1e16, 1000.0]),     # variant (oxygen-like)
+    (2, [11.5, 2.5e-20, 1.0, 0.5, 1.2, 0.35, 1e16, 1000.0]),     # excitation
+    (2, [10.0, 3.2e-20, 1.0, 0.5, 0.8, 0.6, 1e16, 1000.0]),      # variant
+    (3, [0.0, 5.0e-19, 1.0, 0.7, 1.2, 0.35, 1e16, 1000.0]),      # recombination
+    (3, [0.0, 5.0e-19, 3.0, 0.2, 1.2, 0.35, 1e16, 1000.0]),      # variant
+    (4, [0.0, 1.0e-18, 1.0, 0.5, 1.2, 0.35, 1e16, 1000.0]),      # Coulomb
+    (4, [0.0, 1.0e-15, 1.0, 0.5, 1.2, 0.35, 1e10, 1.16e4]),      # variant
+]
+
+
+def gen_surface():
+    e = np.concatenate([np.logspace(-3, 6, 181), [1.0, 2.0, 10.0, 11.5, 13.6, 15.76, 27.0, 100.0, 400.0]])
+    kinds = np.array([k for k, _ in SIGMA_SETS], np.int32)
+    prm = np.array([p for _, p in SIGMA_SETS])
+    sig = np.stack([R.sigma(k, p, e) for k, p in SIGMA_SETS])
+    tn = np.array([[1000.0, 1e16], [1.16e4, 1e10], [1.16e8, 1e6], [0.0, 1e10], [300.0, 1e24]])
+    clog = np.array([R.coulomb_log(t, n) for t, n in tn])
+    save("mcc_sigma.npz", energy_ev=e, kind=kinds, prm=prm, sigma=sig, tn=tn, coulomb_log=clog)
+
+    rng = np.random.default_rng(11)
+    n = 400
+    ee = np.concatenate([np.logspace(-1, 5, n // 2), rng.uniform(1, 5000, n // 2)])
+    ang = rng.uniform(-0.2, 1.8, n)  # includes angles clamped to [0, pi/2]
+    sv = rng.uniform(-10, 30, n)
+    cases = []
+    for see_max, peak, pyield, ptemp, vref, src in ((2.0, 400.0, 0.5, 2.0, 0.0, 2e-5),
+                                                    (1.3, 250.0, 1.0, 1.5, 5.0, 4e-5),
+                                                    (3.0, 0.5, 0.2, 3.0, -2.0, 1e-5)):
+        y, j = R.surface(see_max, peak, pyield, ptemp, ee, ang, sv, vref, src)
+        cases.append(dict(see_max=see_max, peak=peak, photo_yield=pyield, photo_temp=ptemp, v_ref=vref,
+                          source_j=src, yield_=y, photo_j=j))
+    save("surface_yield.npz", energy_ev=ee, angle=ang, surface_v=sv,
+         params=np.array([[c["see_max"], c["peak"], c["photo_yield"], c["photo_temp"], c["v_ref"],
+                           c["source_j"]] for c in cases]),
+         see_yield=np.stack([c["yield_"] for c in cases]), photo_j=np.stack([c["photo_j"] for c in cases]))
+
+
 if __name__ == "__main__":
-    main()
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("all", "field"):
+        main()
+    if what in ("all", "surface"):
+        gen_surface()
