@@ -230,6 +230,96 @@ typedef struct {
  * per boundary group (n_groups <= SCDAT_MAX_GROUPS).                        */
 int scdat_get_ledger(scdat_ctx* ctx, scdat_ledger_entry* out, int32_t n_groups);
 
+/* ---- Background Monte Carlo collisions -----------------------------------
+ * Replace MonteCarloCollisionHandler(seed) + addCollisionProcess +
+ * processCollisions (Tools/Interactions/Collisions/include/CollisionAlgorithm.h:
+ * 176-226) as driven by executeBackgroundMccStep (CollisionPicAdapter.h:16-75,
+ * CollisionPicAdapter.cpp:338-434).  Run inside scdat_step after the push and
+ * before the Poisson solve over the particles present at the push; products
+ * are deposited into the same step's charge.  Per particle one collision with
+ * probability 1 - exp(-sum_p nu_p dt), nu_p = n_p sigma_p(E)|v|, process p
+ * with probability nu_p / sum nu (the reference's per-process p^2 draw is not
+ * reproduced).  Random numbers: Philox4x32-10 keyed by the seed, counter
+ * (particle id, step, species, stream).                                     */
+#define SCDAT_MCC_ELASTIC 0         /* sigma_max (E_ref/E)^p, isotropic scatter */
+#define SCDAT_MCC_INELASTIC 1       /* elastic form, fixed energy loss          */
+#define SCDAT_MCC_IONIZATION 2      /* sigma_max ln(E/I)(1-I/E)/E, e- + ion     */
+#define SCDAT_MCC_EXCITATION 3      /* sigma_max (1-1/x)^a x^-b, energy loss    */
+#define SCDAT_MCC_CHARGE_EXCHANGE 4 /* elastic form, ion removed (neutralised)  */
+#define SCDAT_MCC_RECOMBINATION 5   /* sigma_max (E_ref/max(E_ref,E))^p, removed */
+#define SCDAT_MCC_COULOMB 6         /* pi lnL (e^2/4pi eps0)^2/(2E)^2, scatter  */
+#define SCDAT_MAX_MCC_PROCESSES 16
+
+typedef struct {
+  int32_t type;
+  int32_t species;                  /* incident species                      */
+  int32_t product_electron_species; /* ionisation products                   */
+  int32_t product_ion_species;
+  double target_density;            /* background density, m^-3              */
+  double threshold_ev;
+  double sigma_max_m2;
+  double energy_loss_ev;
+  double reference_energy_ev;       /* elastic form and recombination        */
+  double power_index;
+  double alpha, beta;               /* excitation                            */
+  double coulomb_log;               /* Coulomb                               */
+} scdat_mcc_process;
+
+/* The reference's default parameters of one process type
+ * (MonteCarloCollisionHandler::initializeDefaultCollisionParameters,
+ * CollisionAlgorithm.cpp:590-646); species fields are left for the caller.  */
+int scdat_mcc_process_default(int32_t type, scdat_mcc_process* p);
+/* Replaces the process set (n = 0 turns collisions off); processes are
+ * counted in this order by scdat_get_event_counts.                         */
+int scdat_mcc_configure(scdat_ctx* ctx, const scdat_mcc_process* procs, int32_t n);
+/* Background density update (reinitialize_configuration with new effective
+ * densities, CollisionPicAdapter.cpp:357-363).                              */
+int scdat_mcc_set_density(scdat_ctx* ctx, int32_t process, double density);
+/* Seed of every random stream (collisions, emission). */
+int scdat_set_seed(scdat_ctx* ctx, uint64_t seed);
+
+/* ---- Surface emission ------------------------------------------------------
+ * Secondary electrons from electrons absorbed on a CONDUCTOR or ABSORB face
+ * and photoelectrons from the face, per boundary group.  Yields: the
+ * reference's secondaryYield (Tools/Material/src/SurfaceMaterialModel.cpp:
+ * 163-170, delta_max x e^(1-x)/sqrt(max(0.1, cos theta)), x = E/max(1, E_max))
+ * or Sternglass delta_max (E/E_max) e^(2 - 2 sqrt(E/E_max)); photo current
+ * J0 Y exp(-max(0, V - V_ref)/T_ph) (:199-208), V = floating potential of the
+ * last solve on conductor faces, 0 elsewhere.  floor(delta) secondaries plus
+ * one with probability frac(delta) (at most 8), J A dt/(|q| w) photo-
+ * electrons per step; emitted with a flux Maxwellian of the given
+ * temperature (cosine law about the inward normal) just inside the face.
+ * Emission from conductor faces adds -q w per electron to the conductor.   */
+#define SCDAT_SEE_OFF 0
+#define SCDAT_SEE_REFERENCE 1
+#define SCDAT_SEE_STERNGLASS 2
+typedef struct {
+  int32_t see_model;
+  int32_t electron_species;     /* species the emitted electrons join       */
+  double see_delta_max;
+  double see_energy_max_ev;
+  double see_temperature_ev;
+  double photo_current_density; /* J0 * Y, A/m^2 (0: no photoemission)    */
+  double photo_temperature_ev;
+  double photo_reference_v;
+} scdat_surface_emission;
+int scdat_set_surface_emission(scdat_ctx* ctx, int32_t group, const scdat_surface_emission* e);
+
+/* Cumulative event counts (replaces MonteCarloCollisionHandler::getStatistics,
+ * CollisionAlgorithm.h:176-226, and the CollisionStepResult counters).
+ * overflow: products dropped at a full species; impacts_dropped: impacts
+ * beyond the per-step impact buffer (no emission for them).                */
+typedef struct {
+  int64_t collisions[SCDAT_MAX_MCC_PROCESSES];
+  int64_t created;
+  int64_t removed;
+  int64_t see_emitted;
+  int64_t photo_emitted;
+  int64_t overflow;
+  int64_t impacts_dropped;
+} scdat_event_counts;
+int scdat_get_event_counts(scdat_ctx* ctx, scdat_event_counts* out);
+
 /* Per-stage device time of the last scdat_step call, milliseconds:
  * [push, rhs, pcg, field, sort].                                            */
 int scdat_get_stage_times(scdat_ctx* ctx, double* ms5);

@@ -78,6 +78,41 @@ class LedgerEntry(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+MCC_ELASTIC, MCC_INELASTIC, MCC_IONIZATION, MCC_EXCITATION = 0, 1, 2, 3
+MCC_CHARGE_EXCHANGE, MCC_RECOMBINATION, MCC_COULOMB = 4, 5, 6
+MAX_MCC_PROCESSES = 16
+SEE_OFF, SEE_REFERENCE, SEE_STERNGLASS = 0, 1, 2
+
+
+class MccProcess(C.Structure):
+    _fields_ = [("type", C.c_int32), ("species", C.c_int32), ("product_electron_species", C.c_int32),
+                ("product_ion_species", C.c_int32), ("target_density", C.c_double),
+                ("threshold_ev", C.c_double), ("sigma_max_m2", C.c_double), ("energy_loss_ev", C.c_double),
+                ("reference_energy_ev", C.c_double), ("power_index", C.c_double), ("alpha", C.c_double),
+                ("beta", C.c_double), ("coulomb_log", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SurfaceEmission(C.Structure):
+    _fields_ = [("see_model", C.c_int32), ("electron_species", C.c_int32), ("see_delta_max", C.c_double),
+                ("see_energy_max_ev", C.c_double), ("see_temperature_ev", C.c_double),
+                ("photo_current_density", C.c_double), ("photo_temperature_ev", C.c_double),
+                ("photo_reference_v", C.c_double)]
+
+
+class EventCounts(C.Structure):
+    _fields_ = [("collisions", C.c_int64 * MAX_MCC_PROCESSES), ("created", C.c_int64), ("removed", C.c_int64),
+                ("see_emitted", C.c_int64), ("photo_emitted", C.c_int64), ("overflow", C.c_int64),
+                ("impacts_dropped", C.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "collisions"}
+        d["collisions"] = list(self.collisions)
+        return d
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _F = C.POINTER(C.c_float)
@@ -127,7 +162,24 @@ SIGNATURES = {
     "scdat_set_kernel_timing": (C.c_int, [_P, C.c_int32]),
     "scdat_get_kernel_time": (C.c_int, [_P, C.c_int32, _D, _I64]),
     "scdat_sync": (C.c_int, [_P]),
+    "scdat_mcc_process_default": (C.c_int, [C.c_int32, C.POINTER(MccProcess)]),
+    "scdat_mcc_configure": (C.c_int, [_P, C.POINTER(MccProcess), C.c_int32]),
+    "scdat_mcc_set_density": (C.c_int, [_P, C.c_int32, C.c_double]),
+    "scdat_set_seed": (C.c_int, [_P, C.c_uint64]),
+    "scdat_set_surface_emission": (C.c_int, [_P, C.c_int32, C.POINTER(SurfaceEmission)]),
+    "scdat_get_event_counts": (C.c_int, [_P, C.POINTER(EventCounts)]),
 }
+
+
+def mcc_process(type, species, **kw):
+    """A collision process with the reference defaults of its type, overridden by kw."""
+    p = MccProcess()
+    if lib().scdat_mcc_process_default(type, C.byref(p)) != OK:
+        raise ValueError(f"unknown collision type {type}")
+    p.species = species
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 
 _lib = None
 
@@ -369,6 +421,28 @@ class Context:
         v = C.c_int64()
         self._chk(lib().scdat_get_species_charge_fixed(self.h, s, C.byref(v)))
         return v.value
+
+    def set_seed(self, seed):
+        self._chk(lib().scdat_set_seed(self.h, seed))
+
+    def mcc_configure(self, procs):
+        arr = (MccProcess * max(1, len(procs)))(*procs)
+        self._chk(lib().scdat_mcc_configure(self.h, arr, len(procs)))
+
+    def mcc_set_density(self, process, density):
+        self._chk(lib().scdat_mcc_set_density(self.h, process, density))
+
+    def set_surface_emission(self, group, see_model=SEE_OFF, electron_species=0, delta_max=0.0,
+                             energy_max=400.0, see_temperature=2.0, photo_j=0.0, photo_temperature=2.0,
+                             photo_reference_v=0.0):
+        e = SurfaceEmission(see_model, electron_species, delta_max, energy_max, see_temperature, photo_j,
+                            photo_temperature, photo_reference_v)
+        self._chk(lib().scdat_set_surface_emission(self.h, group, C.byref(e)))
+
+    def event_counts(self):
+        e = EventCounts()
+        self._chk(lib().scdat_get_event_counts(self.h, C.byref(e)))
+        return e.as_dict()
 
     def stats(self):
         st = Stats()

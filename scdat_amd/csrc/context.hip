@@ -25,6 +25,7 @@
 #include "field.hpp"
 #include "mesh_host.hpp"
 #include "push.hpp"
+#include "emit.hpp"
 #include "sort.hpp"
 
 namespace scdat {
@@ -388,6 +389,18 @@ struct scdat_ctx {
   int64_t steps = 0;
   double sim_time = 0.0;
   PcgScalars* h_sc = nullptr;  // pinned mirror
+  // background collisions and surface emission
+  std::vector<scdat_mcc_process> mcc;
+  uint32_t key[2] = {0x5CDA7u, 0x2026u};
+  std::array<scdat_surface_emission, kMaxGroups> emit{};
+  DevBuf d_counts, d_append, d_emit_count, d_events, d_n_events, d_xyz64, d_tet4;
+  DevBuf d_face_tet, d_face_k, d_face_cum;
+  std::array<int64_t, kMaxGroups + 1> face_off{};
+  std::array<double, kMaxGroups> face_area{};
+  int64_t events_dropped = 0;
+  int64_t event_cap = 0;
+  unsigned long long* h_append = nullptr;  // pinned, kMaxSpecies + 1 (last: impacts)
+  bool append_pending = false;
   int last_iters = 0, last_outer = 0, last_conv = 0;
   int inner_hist[16] = {};  // inner iterations per outer pass of the last solve
   double last_rel = 0.0, last_init_rel = 0.0;
@@ -444,6 +457,44 @@ void upload_mesh_device(scdat_ctx* c) {
     for (int k = 0; k < 4; ++k) m.node[k] = h.tets[4 * t + k];
   }
   c->d_tets.upload(tm, s);
+  c->d_xyz64.upload(h.xyz, s);
+  c->d_tet4.upload(h.tets, s);
+  {
+    // boundary faces per group in (tet, face) order with cumulative areas,
+    // the photoemission source table
+    std::vector<int32_t> ft, fk;
+    std::vector<double> fc;
+    for (int g = 0; g < kMaxGroups; ++g) {
+      c->face_off[g] = (int64_t)ft.size();
+      double acc = 0.0;
+      c->face_area[g] = 0.0;
+      for (int64_t t = 0; t < h.nt; ++t)
+        for (int k = 0; k < 4; ++k) {
+          if (h.group[4 * t + k] != g || h.nb[4 * t + k] >= 0) continue;
+          const double* q[3];
+          int m = 0;
+          for (int r = 0; r < 4; ++r)
+            if (r != k) q[m++] = &h.xyz[3 * (size_t)h.tets[4 * t + r]];
+          const double u[3] = {q[1][0] - q[0][0], q[1][1] - q[0][1], q[1][2] - q[0][2]};
+          const double w[3] = {q[2][0] - q[0][0], q[2][1] - q[0][1], q[2][2] - q[0][2]};
+          const double cr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+          acc += 0.5 * std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+          ft.push_back((int32_t)t);
+          fk.push_back(k);
+          fc.push_back(acc);
+          c->face_area[g] = acc;
+        }
+    }
+    c->face_off[kMaxGroups] = (int64_t)ft.size();
+    if (ft.empty()) {
+      ft.push_back(0);
+      fk.push_back(0);
+      fc.push_back(0.0);
+    }
+    c->d_face_tet.upload(ft, s);
+    c->d_face_k.upload(fk, s);
+    c->d_face_cum.upload(fc, s);
+  }
   c->d_nb.upload(h.nb, s);
   c->d_groups.upload(h.group, s);
   c->d_partner.upload(h.partner, s);
@@ -508,6 +559,14 @@ void upload_mesh_device(scdat_ctx* c) {
   }
   c->d_ledger_count.alloc(8 * kMaxGroups * kMaxSpecies);
   c->d_ledger_energy.alloc(8 * kMaxGroups * kMaxSpecies);
+  c->d_counts.alloc(8 * C_COUNT);
+  c->d_append.alloc(8 * (kMaxSpecies + 1));
+  c->d_emit_count.alloc(8 * kMaxGroups * kMaxSpecies);
+  c->d_n_events.alloc(8);
+  HIPCHK(hipMemsetAsync(c->d_counts.p, 0, 8 * C_COUNT, s));
+  HIPCHK(hipMemsetAsync(c->d_append.p, 0, 8 * (kMaxSpecies + 1), s));
+  HIPCHK(hipMemsetAsync(c->d_emit_count.p, 0, 8 * kMaxGroups * kMaxSpecies, s));
+  HIPCHK(hipMemsetAsync(c->d_n_events.p, 0, 8, s));
   c->d_lost.alloc(8);
   c->d_pushes.alloc(8);
   HIPCHK(hipMemsetAsync(c->d_pushes.p, 0, 8, s));
@@ -693,7 +752,7 @@ FloatArgs float_args(scdat_ctx* c) {
   FloatArgs fa{};
   fa.q_init = c->q_init;
   fa.ledger_count = c->d_ledger_count.as<unsigned long long>();
-  fa.emit_count = nullptr;
+  fa.emit_count = c->d_emit_count.as<unsigned long long>();
   for (size_t s = 0; s < c->sp.size() && s < (size_t)kMaxSpecies; ++s) fa.qw[s] = c->sp[s].q * c->sp[s].w;
   fa.cond_mask = 0;
   for (int g = 0; g < kMaxGroups; ++g)
@@ -768,6 +827,16 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   a.pushes = c->d_pushes.as<unsigned long long>();
   a.ke_partial = S.ke_partial.as<double>();
   a.alive_partial = S.alive_partial.as<int32_t>();
+  a.id = S.buf[S.cur][7].as<int32_t>();
+  a.see_mask = 0;
+  if (S.q < 0 && c->event_cap > 0)
+    for (int g = 0; g < kMaxGroups; ++g)
+      if (c->emit[g].see_model != SCDAT_SEE_OFF &&
+          (c->gtype[g] == SCDAT_BC_CONDUCTOR || c->gtype[g] == SCDAT_BC_ABSORB))
+        a.see_mask |= 1u << g;
+  a.events = c->d_events.as<SeeEvent>();
+  a.n_events = c->d_n_events.as<unsigned long long>();
+  a.event_cap = c->event_cap;
   return a;
 }
 
@@ -883,6 +952,161 @@ void do_initialize(scdat_ctx* c) {
   c->initialized = true;
 }
 
+bool has_post_push(const scdat_ctx* c) {
+  if (!c->mcc.empty() || c->event_cap > 0) return true;
+  for (const auto& e : c->emit)
+    if (e.photo_current_density > 0.0) return true;
+  return false;
+}
+
+template <typename T>
+SpeciesOut<T> species_out(scdat_ctx* c, int si) {
+  SpeciesState& S = c->sp[si];
+  SpeciesOut<T> o{};
+  o.x = S.buf[S.cur][0].as<T>();
+  o.y = S.buf[S.cur][1].as<T>();
+  o.z = S.buf[S.cur][2].as<T>();
+  o.vx = S.buf[S.cur][3].as<T>();
+  o.vy = S.buf[S.cur][4].as<T>();
+  o.vz = S.buf[S.cur][5].as<T>();
+  o.cell = S.buf[S.cur][6].as<int32_t>();
+  o.id = S.buf[S.cur][7].as<int32_t>();
+  o.base = S.n;
+  o.cap = S.cap;
+  o.append = c->d_append.as<unsigned long long>() + si;
+  o.m = S.m;
+  o.cs = S.cs;
+  return o;
+}
+
+// Collisions, secondary emission from the push's impacts and photoemission,
+// after the push and before the Poisson solve; new particles are deposited
+// into the charge of this step.  Species slot counts are updated once the
+// stream has synchronised (apply_appends).
+template <typename T>
+void post_push(scdat_ctx* c) {
+  hipStream_t s = c->stream;
+  unsigned long long* counts = c->d_counts.as<unsigned long long>();
+  launch_fill_u64(c->d_append.as<unsigned long long>(), kMaxSpecies, s);
+  const uint32_t step = (uint32_t)c->steps;
+  unsigned long long* rho = c->d_rho[c->rho_last].as<unsigned long long>();
+  const int ns = (int)c->sp.size();
+  for (int si = 0; si < ns; ++si) {
+    MccArgs<T> a{};
+    int np = 0;
+    for (size_t k = 0; k < c->mcc.size(); ++k) {
+      const scdat_mcc_process& q = c->mcc[k];
+      if (q.species != si) continue;
+      MccProcDev& d = a.proc[np++];
+      d.type = q.type;
+      d.e_species = q.product_electron_species;
+      d.i_species = q.product_ion_species;
+      d.slot = (int)k;
+      d.n = q.target_density;
+      d.thr = q.threshold_ev;
+      d.smax = q.sigma_max_m2;
+      d.loss = q.energy_loss_ev;
+      d.eref = q.reference_energy_ev;
+      d.pidx = q.power_index;
+      d.alpha = q.alpha;
+      d.beta = q.beta;
+      d.clog = q.coulomb_log;
+    }
+    SpeciesState& S = c->sp[si];
+    if (np == 0 || S.n == 0) continue;
+    a.x = S.buf[S.cur][0].as<T>();
+    a.y = S.buf[S.cur][1].as<T>();
+    a.z = S.buf[S.cur][2].as<T>();
+    a.vx = S.buf[S.cur][3].as<T>();
+    a.vy = S.buf[S.cur][4].as<T>();
+    a.vz = S.buf[S.cur][5].as<T>();
+    a.cell = S.buf[S.cur][6].as<int32_t>();
+    a.id = S.buf[S.cur][7].as<int32_t>();
+    a.n = S.n;
+    a.species = si;
+    a.np = np;
+    a.mass = S.m;
+    a.dt = c->cfg.dt;
+    a.step = step;
+    a.key0 = c->key[0];
+    a.key1 = c->key[1];
+    for (int o = 0; o < ns; ++o) a.out[o] = species_out<T>(c, o);
+    a.tets = c->d_tets.as<TetMap<T>>();
+    a.rho = rho;
+    a.counts = counts;
+    launch_mcc<T>(a, s);
+    HIPCHK(hipGetLastError());
+  }
+  EmitArgs<T> e{};
+  for (int g = 0; g < kMaxGroups; ++g) {
+    const scdat_surface_emission& q = c->emit[g];
+    e.g[g].see_model = q.see_model;
+    e.g[g].e_species = q.electron_species;
+    e.g[g].dmax = q.see_delta_max;
+    e.g[g].emax = q.see_energy_max_ev;
+    e.g[g].t_see = q.see_temperature_ev;
+    e.g[g].photo_j = q.photo_current_density;
+    e.g[g].t_ph = q.photo_temperature_ev;
+    e.g[g].v_ref = q.photo_reference_v;
+  }
+  for (int o = 0; o < ns; ++o) {
+    e.out[o] = species_out<T>(c, o);
+    e.q_abs_w[o] = std::abs(c->sp[o].q) * c->sp[o].w;
+    e.mass[o] = c->sp[o].m;
+  }
+  e.tets = c->d_tets.as<TetMap<T>>();
+  e.xyz = c->d_xyz64.as<double>();
+  e.tet4 = c->d_tet4.as<int32_t>();
+  e.rho = rho;
+  e.counts = counts;
+  e.emit_count = c->d_emit_count.as<unsigned long long>();
+  e.step = step;
+  e.key0 = c->key[0];
+  e.key1 = c->key[1];
+  e.dt = c->cfg.dt;
+  e.events = c->d_events.as<SeeEvent>();
+  e.n_events = c->d_n_events.as<unsigned long long>();
+  e.event_cap = c->event_cap;
+  e.face_tet = c->d_face_tet.as<int32_t>();
+  e.face_k = c->d_face_k.as<int32_t>();
+  e.face_cum = c->d_face_cum.as<double>();
+  e.vfloat = &c->d_sc.as<PcgScalars>()->vfloat;
+  if (c->event_cap > 0) launch_see<T>(e, c->event_cap, s);
+  for (int g = 0; g < kMaxGroups; ++g) {
+    const scdat_surface_emission& q = c->emit[g];
+    const int64_t f0 = c->face_off[g], f1 = c->face_off[g + 1];
+    if (!(q.photo_current_density > 0.0) || f1 <= f0) continue;
+    const double area = c->face_area[g];
+    const SpeciesState& S = c->sp[q.electron_species];
+    const double bound = q.photo_current_density * area * c->cfg.dt / (std::abs(S.q) * S.w);
+    if (bound > 4.0e9) throw Error(SCDAT_ERR_ARG, "photoemission: too many macro-electrons per step");
+    EmitArgs<T> eg = e;
+    eg.f0 = f0;
+    eg.f1 = f1;
+    eg.group = g;
+    eg.is_conductor = c->gtype[g] == SCDAT_BC_CONDUCTOR && c->h.has_cond ? 1 : 0;
+    launch_photo<T>(eg, (int64_t)std::floor(bound) + 1, s);
+  }
+  HIPCHK(hipGetLastError());
+  // appended counts and impacts, read once the stream has synchronised
+  HIPCHK(hipMemcpyAsync(c->h_append, c->d_append.p, 8 * kMaxSpecies, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(c->h_append + kMaxSpecies, c->d_n_events.p, 8, hipMemcpyDeviceToHost, s));
+  c->append_pending = true;
+}
+
+// After a stream synchronisation: new slot counts (capped at the capacity,
+// overflow is counted on the device) and dropped impacts.
+void apply_appends(scdat_ctx* c) {
+  if (!c->append_pending) return;
+  for (size_t si = 0; si < c->sp.size(); ++si) {
+    SpeciesState& S = c->sp[si];
+    S.n = std::min<int64_t>(S.cap, S.n + (int64_t)c->h_append[si]);
+  }
+  const int64_t ne = (int64_t)c->h_append[kMaxSpecies];
+  if (ne > c->event_cap) c->events_dropped += ne - c->event_cap;
+  c->append_pending = false;
+}
+
 template <typename T>
 void do_step(scdat_ctx* c, int n) {
   std::vector<std::pair<size_t, size_t>> pev;
@@ -895,10 +1119,14 @@ void do_step(scdat_ctx* c, int n) {
     StageEv se;
     se.a = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.a), c->stream));
+    const bool post = has_post_push(c);
+    if (c->event_cap > 0) launch_fill_u64(c->d_n_events.as<unsigned long long>(), 1, c->stream);
     do_push<T>(c, &pev, &evn);
+    if (post) post_push<T>(c);
     se.b = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.b), c->stream));
     do_solve<T>(c, c->rho_last);
+    apply_appends(c);
     se.cc = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.cc), c->stream));
     // E for the next push; clears the buffer the next push deposits into.
@@ -1092,7 +1320,9 @@ int scdat_create(const scdat_config* cfg, scdat_ctx** out) {
     delete c;
     return SCDAT_ERR_DEVICE;
   }
-  if (hipHostMalloc((void**)&c->h_sc, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&c->h_sc, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_append, 8 * (kMaxSpecies + 1), hipHostMallocDefault) != hipSuccess) {
+    if (c->h_sc) (void)hipHostFree(c->h_sc);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return SCDAT_ERR_DEVICE;
@@ -1111,6 +1341,7 @@ void scdat_destroy(scdat_ctx* c) {
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   for (auto e : c->kt_pool) (void)hipEventDestroy(e);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
+  if (c->h_append) (void)hipHostFree(c->h_append);
   hipStream_t s = c->stream;
   delete c;  // DevBuf destructors free device memory
   if (s) (void)hipStreamDestroy(s);
@@ -1509,6 +1740,8 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
       HIPCHK(hipMemcpyAsync(fx.data(), c->d_rho[c->rho_last].p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
       std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
       HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
+      std::vector<unsigned long long> emi(kMaxGroups * kMaxSpecies);
+      HIPCHK(hipMemcpyAsync(emi.data(), c->d_emit_count.p, 8 * emi.size(), hipMemcpyDeviceToHost, c->stream));
       unsigned long long lost = 0, pushes = 0;
       HIPCHK(hipMemcpyAsync(&lost, c->d_lost.p, 8, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipMemcpyAsync(&pushes, c->d_pushes.p, 8, hipMemcpyDeviceToHost, c->stream));
@@ -1523,7 +1756,8 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
           if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;
           const double q = (double)cnt[g * kMaxSpecies + s] * c->sp[s].q * c->sp[s].w;
           surf += q;
-          if (c->gtype[g] == SCDAT_BC_CONDUCTOR) qc += q;
+          if (c->gtype[g] == SCDAT_BC_CONDUCTOR)
+            qc += q - (double)emi[g * kMaxSpecies + s] * c->sp[s].q * c->sp[s].w;
         }
       st->surface_deposited_charge = surf;
       st->conductor_charge = qc;
@@ -1545,12 +1779,19 @@ int scdat_get_ledger(scdat_ctx* c, scdat_ledger_entry* out, int32_t n_groups) {
   return guarded(c, [&] {
     std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
     std::vector<double> en(kMaxGroups * kMaxSpecies);
+    std::vector<unsigned long long> emi(kMaxGroups * kMaxSpecies);
     HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(en.data(), c->d_ledger_energy.p, 8 * en.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(emi.data(), c->d_emit_count.p, 8 * emi.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int g = 0; g < n_groups; ++g) {
       scdat_ledger_entry& L = out[g];
       std::memset(&L, 0, sizeof(L));
+      for (size_t s = 0; s < c->sp.size(); ++s) {
+        const unsigned long long k = emi[g * kMaxSpecies + s];
+        L.emitted_electron_particles += (int64_t)k;
+        L.emitted_electron_charge_c += (double)k * c->sp[s].q * c->sp[s].w;
+      }
       if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;  // escaped: not recorded
       for (size_t s = 0; s < c->sp.size(); ++s) {
         const unsigned long long k = cnt[g * kMaxSpecies + s];
@@ -1567,6 +1808,132 @@ int scdat_get_ledger(scdat_ctx* c, scdat_ledger_entry* out, int32_t n_groups) {
         }
       }
     }
+  });
+}
+
+// ---- background collisions and surface emission -----------------------------
+
+int scdat_set_seed(scdat_ctx* c, uint64_t seed) {
+  if (!c) return SCDAT_ERR_ARG;
+  c->key[0] = (uint32_t)seed;
+  c->key[1] = (uint32_t)(seed >> 32);
+  return SCDAT_OK;
+}
+
+int scdat_mcc_process_default(int32_t type, scdat_mcc_process* p) {
+  if (!p || type < SCDAT_MCC_ELASTIC || type > SCDAT_MCC_COULOMB) return SCDAT_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->type = type;
+  p->product_electron_species = -1;
+  p->product_ion_species = -1;
+  p->reference_energy_ev = 1.0;
+  p->power_index = 0.5;
+  p->alpha = 1.2;
+  p->beta = 0.35;
+  p->coulomb_log = std::log(1.0e4);  // lambda_D / b_min with b_min = 1e-4 lambda_D
+  // MonteCarloCollisionHandler::initializeDefaultCollisionParameters,
+  // CollisionAlgorithm.cpp:590-646
+  switch (type) {
+    case SCDAT_MCC_ELASTIC:
+      p->sigma_max_m2 = 2.0e-19;
+      p->reference_energy_ev = 10.0;
+      p->power_index = 0.3;
+      break;
+    case SCDAT_MCC_INELASTIC:
+      p->threshold_ev = 1.0;
+      p->sigma_max_m2 = 8.0e-20;
+      p->energy_loss_ev = 1.0;
+      break;
+    case SCDAT_MCC_IONIZATION:
+      p->threshold_ev = 15.76;
+      p->sigma_max_m2 = 3.0e-20;
+      p->energy_loss_ev = 15.76;
+      break;
+    case SCDAT_MCC_EXCITATION:
+      p->threshold_ev = 11.5;
+      p->sigma_max_m2 = 2.5e-20;
+      p->energy_loss_ev = 11.5;
+      break;
+    case SCDAT_MCC_CHARGE_EXCHANGE:
+      p->sigma_max_m2 = 1.2e-19;
+      p->energy_loss_ev = 0.2;
+      break;
+    case SCDAT_MCC_RECOMBINATION:
+      p->sigma_max_m2 = 5.0e-19;
+      p->power_index = 0.7;
+      break;
+    default:
+      p->sigma_max_m2 = 1.0e-18;
+      break;
+  }
+  return SCDAT_OK;
+}
+
+int scdat_mcc_configure(scdat_ctx* c, const scdat_mcc_process* procs, int32_t n) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (n < 0 || n > SCDAT_MAX_MCC_PROCESSES || (n > 0 && !procs))
+    return fail(c, SCDAT_ERR_ARG, "mcc_configure: bad process count");
+  const int ns = (int)c->sp.size();
+  for (int k = 0; k < n; ++k) {
+    const scdat_mcc_process& q = procs[k];
+    if (q.type < SCDAT_MCC_ELASTIC || q.type > SCDAT_MCC_COULOMB || q.species < 0 || q.species >= ns ||
+        !(q.target_density >= 0.0) || !(q.sigma_max_m2 >= 0.0))
+      return fail(c, SCDAT_ERR_ARG, "mcc_configure: bad process " + std::to_string(k));
+    if (q.type == SCDAT_MCC_IONIZATION &&
+        (q.product_electron_species < 0 || q.product_electron_species >= ns || q.product_ion_species < 0 ||
+         q.product_ion_species >= ns))
+      return fail(c, SCDAT_ERR_ARG, "mcc_configure: ionisation needs product species");
+  }
+  c->mcc.assign(procs, procs + n);
+  return SCDAT_OK;
+}
+
+int scdat_mcc_set_density(scdat_ctx* c, int32_t process, double density) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (process < 0 || process >= (int)c->mcc.size() || !(density >= 0.0))
+    return fail(c, SCDAT_ERR_ARG, "mcc_set_density: bad process or density");
+  c->mcc[process].target_density = density;
+  return SCDAT_OK;
+}
+
+int scdat_set_surface_emission(scdat_ctx* c, int32_t group, const scdat_surface_emission* e) {
+  if (!c || !e) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_surface_emission: set the mesh first");
+  if (group < 0 || group >= kMaxGroups || e->see_model < SCDAT_SEE_OFF || e->see_model > SCDAT_SEE_STERNGLASS ||
+      e->electron_species < 0 || e->electron_species >= (int)c->sp.size() || c->sp[e->electron_species].q >= 0)
+    return fail(c, SCDAT_ERR_ARG, "set_surface_emission: bad group, model or electron species");
+  if (e->see_model == SCDAT_SEE_STERNGLASS && !(e->see_energy_max_ev > 0.0))
+    return fail(c, SCDAT_ERR_ARG, "set_surface_emission: Sternglass needs energy_max > 0");
+  return guarded(c, [&] {
+    c->emit[group] = *e;
+    bool see = false;
+    for (const auto& q : c->emit) see = see || q.see_model != SCDAT_SEE_OFF;
+    if (see && c->event_cap == 0) {
+      // impacts per step: a 1/64 of the electron capacity, at least 65536
+      int64_t cap = 0;
+      for (const auto& S : c->sp)
+        if (S.q < 0) cap += S.cap;
+      c->event_cap = std::max<int64_t>(65536, cap / 64);
+      c->d_events.alloc(sizeof(SeeEvent) * (size_t)c->event_cap);
+    }
+  });
+}
+
+int scdat_get_event_counts(scdat_ctx* c, scdat_event_counts* out) {
+  if (!c || !out) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "get_event_counts: no mesh");
+  return guarded(c, [&] {
+    unsigned long long v[C_COUNT];
+    HIPCHK(hipMemcpyAsync(v, c->d_counts.p, 8 * C_COUNT, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::memset(out, 0, sizeof(*out));
+    for (int k = 0; k < SCDAT_MAX_MCC_PROCESSES; ++k) out->collisions[k] = (int64_t)v[k];
+    out->created = (int64_t)v[C_CREATED];
+    out->removed = (int64_t)v[C_REMOVED];
+    out->see_emitted = (int64_t)v[C_SEE];
+    out->photo_emitted = (int64_t)v[C_PHOTO];
+    out->overflow = (int64_t)v[C_OVERFLOW];
+    out->impacts_dropped = c->events_dropped;
   });
 }
 

@@ -1,0 +1,378 @@
+// Background Monte Carlo collisions, secondary-electron emission and
+// photoemission on gfx950.
+//
+// Collisions replace MonteCarloCollisionHandler::processCollisions
+// (Tools/Interactions/Collisions/src/CollisionAlgorithm.cpp:500-538) inside
+// executeBackgroundMccStep (CollisionPicAdapter.cpp:338-434).  Per particle:
+// nu_p = n_p sigma_p(E) |v| with the cross sections of CollisonTypes.cpp:56-176,
+// one collision with probability 1 - exp(-sum nu dt) and the process chosen
+// with probability nu_p / sum nu (null-collision thinning with the particle's
+// own total frequency as majorant).  Outcomes follow CollisionAlgorithm.cpp:
+// 189-460: isotropic scatter (elastic, Coulomb), energy loss (inelastic,
+// excitation above threshold), ionisation (uniform split of E - I, secondary
+// electron and ion at rest at the parent's position), removal (charge
+// exchange, recombination).  The reference's second draw inside
+// executeCollision (effective probability p^2) is not reproduced.
+//
+// All arithmetic of these rare events is FP64 whatever the particle storage
+// precision; random numbers come from Philox keyed by (seed, particle id,
+// step, species), so results do not depend on thread order.  New particles
+// are appended with one atomic per particle and deposited into the step's
+// charge at once (fixed point, so the order of deposits does not matter).
+#include "emit.hpp"
+#include "rng.hpp"
+
+namespace scdat {
+
+namespace {
+
+constexpr double kE = 1.602176634e-19;
+constexpr double kEps0 = 8.8541878128e-12;
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ double sigma_of(const MccProcDev& p, double E) {
+  switch (p.type) {
+    case MCC_ELASTIC:
+    case MCC_INELASTIC:
+    case MCC_CHARGE_EXCHANGE: {
+      if (E < p.thr) return 0.0;
+      const double sg = p.smax * pow(fmax(1e-6, p.eref) / E, p.pidx);
+      return fmax(0.0, fmin(sg, p.smax));
+    }
+    case MCC_IONIZATION: {
+      if (E <= p.thr) return 0.0;
+      const double I = fmax(1e-6, p.thr);
+      const double sg = p.smax * log(E / I) * (1.0 - I / E) / E;
+      return fmax(0.0, fmin(sg, p.smax));
+    }
+    case MCC_EXCITATION: {
+      if (E <= p.thr) return 0.0;
+      const double x = E / fmax(1e-6, p.thr);
+      const double sg = p.smax * pow(fmax(0.0, 1.0 - 1.0 / x), p.alpha) * pow(1.0 / x, p.beta);
+      return fmax(0.0, fmin(sg, p.smax));
+    }
+    case MCC_RECOMBINATION: {
+      const double er = fmax(1e-6, p.eref);
+      const double sg = p.smax * pow(er / fmax(er, E), p.pidx);
+      return fmax(0.0, fmin(sg, p.smax));
+    }
+    default: {
+      if (E <= 0.0) return 0.0;
+      const double Ej = E * kE;
+      const double ke2 = (1.0 / (4.0 * kPi * kEps0)) * kE * kE;
+      const double sg = kPi * p.clog * ke2 * ke2 / fmax(1e-30, 4.0 * Ej * Ej);
+      return fmax(0.0, fmin(sg, p.smax));
+    }
+  }
+}
+
+__device__ __forceinline__ void iso_dir(double u2, double u3, double d[3]) {
+  const double ct = 2.0 * u2 - 1.0;
+  const double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+  const double ph = 2.0 * kPi * u3;
+  d[0] = st * cos(ph);
+  d[1] = st * sin(ph);
+  d[2] = ct;
+}
+
+// Fixed-point deposit of one particle (the push kernel's weights).
+template <typename T>
+__device__ void deposit_one(const TetMap<T>* tets, unsigned long long* rho, int cell, const T p[3],
+                            int64_t cs) {
+  const TetMap<T> m = tets[cell];
+  T lam[4];
+  bary(m, p, lam);
+  int32_t f[4];
+  lambda_fixed(lam, f);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    atomicAdd(&rho[m.node[k]], (unsigned long long)(cs * (int64_t)f[k]));
+}
+
+// Appends one particle; returns false (and counts an overflow) when full.
+template <typename T>
+__device__ bool append(const SpeciesOut<T>& o, const T x[3], const double v[3], int cell, int32_t id,
+                       const TetMap<T>* tets, unsigned long long* rho, unsigned long long* counts) {
+  const int64_t slot = o.base + (int64_t)atomicAdd(o.append, 1ull);
+  if (slot >= o.cap) {
+    atomicAdd(&counts[C_OVERFLOW], 1ull);
+    return false;
+  }
+  o.x[slot] = x[0];
+  o.y[slot] = x[1];
+  o.z[slot] = x[2];
+  o.vx[slot] = (T)v[0];
+  o.vy[slot] = (T)v[1];
+  o.vz[slot] = (T)v[2];
+  o.cell[slot] = cell;
+  o.id[slot] = id;
+  deposit_one(tets, rho, cell, x, o.cs);
+  return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mcc(MccArgs<T> a) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int cell = a.cell[i];
+  if (cell < 0) return;
+  double v[3] = {(double)a.vx[i], (double)a.vy[i], (double)a.vz[i]};
+  const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const double speed = sqrt(v2);
+  const double E = 0.5 * a.mass * v2 / kE;
+  double nu[kMaxProc];
+  double tot = 0.0;
+#pragma unroll
+  for (int j = 0; j < kMaxProc; ++j) {
+    nu[j] = j < a.np ? a.proc[j].n * sigma_of(a.proc[j], E) * speed : 0.0;
+    tot += nu[j];
+  }
+  if (!(tot > 0.0)) return;
+  const uint32_t id = (uint32_t)a.id[i];
+  const U4 r = philox4x32(id, a.step, (uint32_t)a.species, kTagMccA, a.key0, a.key1);
+  if (!(u01(r.x) < -expm1(-tot * a.dt))) return;
+  // the first process whose cumulative frequency exceeds R (zero frequencies
+  // are never chosen), the checker's rule
+  const double R = u01(r.y) * tot;
+  int j = -1;
+  double cum = 0.0;
+  bool stop = false;
+#pragma unroll
+  for (int q = 0; q < kMaxProc; ++q) {
+    if (!stop && q < a.np) {
+      cum += nu[q];
+      if (nu[q] > 0.0) j = q;
+      if (R < cum) stop = true;
+    }
+  }
+  const MccProcDev& p = a.proc[j];
+  atomicAdd(&a.counts[p.slot], 1ull);
+  double d[3];
+  switch (p.type) {
+    case MCC_ELASTIC:
+    case MCC_COULOMB:
+      iso_dir(u01(r.z), u01(r.w), d);
+      a.vx[i] = (T)(speed * d[0]);
+      a.vy[i] = (T)(speed * d[1]);
+      a.vz[i] = (T)(speed * d[2]);
+      break;
+    case MCC_INELASTIC:
+    case MCC_EXCITATION: {
+      if (p.type == MCC_EXCITATION && E <= p.thr) break;
+      const double loss = p.type == MCC_INELASTIC ? fmax(0.0, p.loss) : (p.loss > 0.0 ? p.loss : p.thr);
+      const double sp = sqrt(2.0 * fmax(1e-6, E - loss) * kE / a.mass);
+      iso_dir(u01(r.z), u01(r.w), d);
+      a.vx[i] = (T)(sp * d[0]);
+      a.vy[i] = (T)(sp * d[1]);
+      a.vz[i] = (T)(sp * d[2]);
+      break;
+    }
+    case MCC_IONIZATION: {
+      if (E <= p.thr) break;
+      const U4 b = philox4x32(id, a.step, (uint32_t)a.species, kTagMccB, a.key0, a.key1);
+      const U4 c = philox4x32(id, a.step, (uint32_t)a.species, kTagMccId, a.key0, a.key1);
+      const double avail = E - p.thr;
+      const double sec = u01(b.x) * avail;
+      const double prim = avail - sec;
+      const double sp = sqrt(2.0 * prim * kE / a.mass);
+      iso_dir(u01(r.z), u01(r.w), d);
+      a.vx[i] = (T)(sp * d[0]);
+      a.vy[i] = (T)(sp * d[1]);
+      a.vz[i] = (T)(sp * d[2]);
+      const T xp[3] = {a.x[i], a.y[i], a.z[i]};
+      const SpeciesOut<T>& oe = a.out[p.e_species];
+      const double se = sqrt(2.0 * sec * kE / oe.m);
+      iso_dir(u01(b.y), u01(b.z), d);
+      const double ve[3] = {se * d[0], se * d[1], se * d[2]};
+      int made = append(oe, xp, ve, cell, gen_id(c.x), a.tets, a.rho, a.counts) ? 1 : 0;
+      const double vi[3] = {0.0, 0.0, 0.0};
+      made += append(a.out[p.i_species], xp, vi, cell, gen_id(c.y), a.tets, a.rho, a.counts) ? 1 : 0;
+      atomicAdd(&a.counts[C_CREATED], (unsigned long long)made);
+      break;
+    }
+    default:  // charge exchange, recombination
+      a.cell[i] = -1;
+      atomicAdd(&a.counts[C_REMOVED], 1ull);
+      break;
+  }
+}
+
+// ---- surface emission ------------------------------------------------------
+__device__ double see_yield(int model, double dmax, double emax, double E, double cos_inc) {
+  if (model == 1) {  // SurfaceMaterialModel.cpp:163-170
+    const double x = fmax(1e-6, E / fmax(1.0, emax));
+    return fmax(0.0, dmax * x * exp(1.0 - x) / sqrt(fmax(0.1, cos_inc)));
+  }
+  if (model == 2) {  // Sternglass
+    if (!(E > 0.0)) return 0.0;
+    const double r = E / emax;
+    return dmax * r * exp(2.0 - 2.0 * sqrt(r));
+  }
+  return 0.0;
+}
+
+template <typename T>
+__device__ void inward_normal(const TetMap<T>& m, int k, double n[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double g1 = m.a[r], g2 = m.a[3 + r], g3 = m.a[6 + r];
+    n[r] = k == 0 ? -(g1 + g2 + g3) : (k == 1 ? g1 : (k == 2 ? g2 : g3));
+  }
+  const double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) n[r] /= l;
+}
+
+// Surface point moved 1e-6 of the way to the tet centroid (inside the tet).
+__device__ void nudge(const double* xyz, const int32_t* tet4, int t, const double p[3], double x[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double c = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c += xyz[3 * (int64_t)tet4[4 * (int64_t)t + k] + r];
+    x[r] = p[r] + 1e-6 * (0.25 * c - p[r]);
+  }
+}
+
+// Flux-weighted Maxwellian of temperature T (eV) about the inward normal:
+// energy -T ln(ua ub), cos(theta) = sqrt(uc), azimuth 2 pi ud.
+__device__ void emit_velocity(const double n[3], double T, double m, double ua, double ub, double uc,
+                              double ud, double v[3]) {
+  const double E = -T * (log(ua) + log(ub));
+  const double sp = sqrt(2.0 * E * kE / m);
+  const bool xa = fabs(n[0]) < 0.9;
+  const double ax[3] = {xa ? 1.0 : 0.0, xa ? 0.0 : 1.0, 0.0};
+  double t1[3] = {n[1] * ax[2] - n[2] * ax[1], n[2] * ax[0] - n[0] * ax[2], n[0] * ax[1] - n[1] * ax[0]};
+  const double l = sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) t1[r] /= l;
+  const double t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+  const double ct = sqrt(uc), st = sqrt(1.0 - uc), ph = 2.0 * kPi * ud;
+  const double cp = cos(ph), sn = sin(ph);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) v[r] = sp * (st * cp * t1[r] + st * sn * t2[r] + ct * n[r]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const unsigned long long ne = *a.n_events;
+  const int64_t nev = (int64_t)ne < a.event_cap ? (int64_t)ne : a.event_cap;
+  if (e >= nev) return;
+  const SeeEvent ev = a.events[e];
+  const int g = ev.group_face & 0xFF, face = ev.group_face >> 8;
+  const SurfEmitDev& cf = a.g[g];
+  const TetMap<T> m = a.tets[ev.cell];
+  double n[3];
+  inward_normal(m, face, n);
+  const double v2 = ev.v[0] * ev.v[0] + ev.v[1] * ev.v[1] + ev.v[2] * ev.v[2];
+  const double sp = sqrt(v2);
+  const double E = 0.5 * a.mass[ev.species] * v2 / kE;
+  const double cosi = sp > 0.0 ? fabs(ev.v[0] * n[0] + ev.v[1] * n[1] + ev.v[2] * n[2]) / sp : 1.0;
+  const double dl = see_yield(cf.see_model, cf.dmax, cf.emax, E, cosi);
+  const U4 r = philox4x32((uint32_t)ev.id, a.step, (uint32_t)ev.species, kTagSee, a.key0, a.key1);
+  const double fl = floor(dl);
+  int nem = (int)fmin(fl, (double)kMaxEmit) + (u01(r.x) < dl - fl ? 1 : 0);
+  nem = nem < kMaxEmit ? nem : kMaxEmit;
+  if (nem <= 0) return;
+  double xe[3];
+  nudge(a.xyz, a.tet4, ev.cell, ev.x, xe);
+  const T xt[3] = {(T)xe[0], (T)xe[1], (T)xe[2]};
+  const SpeciesOut<T>& o = a.out[cf.e_species];
+  int made = 0;
+  for (int k = 0; k < nem; ++k) {
+    const uint32_t sk = ((uint32_t)ev.species << 8) | (uint32_t)k;
+    const U4 b = philox4x32((uint32_t)ev.id, a.step, sk, kTagSeeK, a.key0, a.key1);
+    const U4 c = philox4x32((uint32_t)ev.id, a.step, sk, kTagSeeId, a.key0, a.key1);
+    double v[3];
+    emit_velocity(n, cf.t_see, o.m, u01(b.x), u01(b.y), u01(b.z), u01(b.w), v);
+    made += append(o, xt, v, ev.cell, gen_id(c.x), a.tets, a.rho, a.counts) ? 1 : 0;
+  }
+  atomicAdd(&a.emit_count[g * kMaxSpecies + cf.e_species], (unsigned long long)made);
+  atomicAdd(&a.counts[C_SEE], (unsigned long long)made);
+}
+
+// Photoemission of one group, SurfaceMaterialModel.cpp:199-208 current
+// density at the group's potential; J A dt / (|q| w) macro-electrons per step.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_photo(EmitArgs<T> a) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const SurfEmitDev& cf = a.g[a.group];
+  const double area = a.face_cum[a.f1 - 1];
+  const double V = a.is_conductor ? *a.vfloat : 0.0;
+  const double j = fmax(0.0, cf.photo_j) * exp(-fmax(0.0, V - cf.v_ref) / fmax(1e-3, cf.t_ph));
+  const double expect = j * area * a.dt / a.q_abs_w[cf.e_species];
+  const U4 r0 = philox4x32((uint32_t)a.group, a.step, 0u, kTagPhoto, a.key0, a.key1);
+  const double fl = floor(expect);
+  const int64_t nem = (int64_t)fl + (u01(r0.x) < expect - fl ? 1 : 0);
+  if (k >= nem) return;
+  const U4 b = philox4x32((uint32_t)k, a.step, (uint32_t)a.group, kTagPhotoK, a.key0, a.key1);
+  const U4 c = philox4x32((uint32_t)k, a.step, (uint32_t)a.group, kTagPhotoK2, a.key0, a.key1);
+  // face: first cumulative area above u * area
+  const double rr = u01(b.x) * area;
+  int64_t lo = a.f0, hi = a.f1;  // search [lo, hi)
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (a.face_cum[mid] > rr) hi = mid;
+    else lo = mid + 1;
+  }
+  const int64_t fi = lo < a.f1 ? lo : a.f1 - 1;
+  const int t = a.face_tet[fi], kf = a.face_k[fi];
+  const double* p[3];
+  int mm = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (q != kf) p[mm++] = &a.xyz[3 * (int64_t)a.tet4[4 * (int64_t)t + q]];
+  const double sq = sqrt(u01(b.y)), u2 = u01(b.z);
+  double pt[3], xe[3], n[3], v[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) pt[q] = (1.0 - sq) * p[0][q] + sq * (1.0 - u2) * p[1][q] + sq * u2 * p[2][q];
+  nudge(a.xyz, a.tet4, t, pt, xe);
+  inward_normal(a.tets[t], kf, n);
+  const SpeciesOut<T>& o = a.out[cf.e_species];
+  emit_velocity(n, cf.t_ph, o.m, u01(b.w), u01(c.x), u01(c.y), u01(c.z), v);
+  const T xt[3] = {(T)xe[0], (T)xe[1], (T)xe[2]};
+  if (append(o, xt, v, t, gen_id(c.w), a.tets, a.rho, a.counts)) {
+    atomicAdd(&a.emit_count[a.group * kMaxSpecies + cf.e_species], 1ull);
+    atomicAdd(&a.counts[C_PHOTO], 1ull);
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+inline uint32_t blocks_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+template <typename T>
+void launch_mcc(const MccArgs<T>& a, hipStream_t s) {
+  if (a.n <= 0 || a.np <= 0) return;
+  hipLaunchKernelGGL(k_mcc<T>, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+}
+template <typename T>
+void launch_see(const EmitArgs<T>& a, int64_t max_events, hipStream_t s) {
+  if (max_events <= 0) return;
+  hipLaunchKernelGGL(k_see<T>, dim3(blocks_for(max_events)), dim3(kBlock), 0, s, a);
+}
+template <typename T>
+void launch_photo(const EmitArgs<T>& a, int64_t max_count, hipStream_t s) {
+  if (max_count <= 0 || a.f1 <= a.f0) return;
+  hipLaunchKernelGGL(k_photo<T>, dim3(blocks_for(max_count)), dim3(kBlock), 0, s, a);
+}
+void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, n);
+}
+
+#define SCDAT_EMIT_INST(T)                                                        \
+  template void launch_mcc<T>(const MccArgs<T>&, hipStream_t);                    \
+  template void launch_see<T>(const EmitArgs<T>&, int64_t, hipStream_t);          \
+  template void launch_photo<T>(const EmitArgs<T>&, int64_t, hipStream_t);
+SCDAT_EMIT_INST(float)
+SCDAT_EMIT_INST(double)
+#undef SCDAT_EMIT_INST
+
+}  // namespace scdat

@@ -1,0 +1,100 @@
+// Launch interface of the background-collision and surface-emission kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+
+namespace scdat {
+
+constexpr int kMaxProc = 16;
+constexpr int kMaxEmit = 8;  // secondaries per impact
+// counter slots: [0, kMaxProc) collisions per process, then:
+enum CountSlot { C_CREATED = kMaxProc, C_REMOVED, C_SEE, C_PHOTO, C_OVERFLOW, C_EVENT_DROP, C_COUNT };
+
+enum MccType { MCC_ELASTIC = 0, MCC_INELASTIC, MCC_IONIZATION, MCC_EXCITATION, MCC_CHARGE_EXCHANGE,
+               MCC_RECOMBINATION, MCC_COULOMB };
+
+struct MccProcDev {
+  int type, e_species, i_species, slot;
+  double n, thr, smax, loss, eref, pidx, alpha, beta, clog;
+};
+
+// Append target: one species' SoA.  New slots are base + atomicAdd(append).
+template <typename T>
+struct SpeciesOut {
+  T *x, *y, *z, *vx, *vy, *vz;
+  int32_t* cell;
+  int32_t* id;
+  int64_t base, cap;
+  unsigned long long* append;
+  double m;
+  int64_t cs;  // integer charge per unit weight (deposit)
+};
+
+// Impact of an electron on a face with secondary emission, recorded by the push.
+struct SeeEvent {
+  double x[3], v[3];
+  int32_t id, cell, group_face, species;  // group_face = group | face << 8
+};
+
+template <typename T>
+struct MccArgs {
+  T *x, *y, *z, *vx, *vy, *vz;
+  int32_t* cell;
+  const int32_t* id;
+  int64_t n;
+  int species, np;
+  double mass, dt;
+  uint32_t step, key0, key1;
+  MccProcDev proc[kMaxProc];
+  SpeciesOut<T> out[kMaxSpecies];
+  const TetMap<T>* tets;
+  unsigned long long* rho;
+  unsigned long long* counts;
+};
+
+struct SurfEmitDev {
+  int see_model, e_species;
+  double dmax, emax, t_see, photo_j, t_ph, v_ref;
+};
+
+template <typename T>
+struct EmitArgs {
+  SurfEmitDev g[kMaxGroups];
+  SpeciesOut<T> out[kMaxSpecies];
+  double q_abs_w[kMaxSpecies];  // |q| w of each species
+  double mass[kMaxSpecies];
+  const TetMap<T>* tets;
+  const double* xyz;  // node coordinates (FP64)
+  const int32_t* tet4;
+  unsigned long long* rho;
+  unsigned long long* counts;
+  unsigned long long* emit_count;  // [kMaxGroups][kMaxSpecies]
+  uint32_t step, key0, key1;
+  double dt;
+  // impacts
+  const SeeEvent* events;
+  const unsigned long long* n_events;
+  int64_t event_cap;
+  // photoemission of one group: faces [f0, f1) of the face table
+  const int32_t* face_tet;
+  const int32_t* face_k;
+  const double* face_cum;  // cumulative area within the group
+  int64_t f0, f1;
+  int group;
+  const double* vfloat;  // floating potential of the last solve
+  int is_conductor;
+};
+
+template <typename T>
+void launch_mcc(const MccArgs<T>& a, hipStream_t s);
+template <typename T>
+void launch_see(const EmitArgs<T>& a, int64_t max_events, hipStream_t s);
+// max_count: upper bound of the group's emissions this step (grid size)
+template <typename T>
+void launch_photo(const EmitArgs<T>& a, int64_t max_count, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s);
+
+}  // namespace scdat

@@ -31,11 +31,11 @@ __device__ __forceinline__ void gather_E(const PushArgs<T>& a, const TetMap<T>& 
 }
 
 // Straight-segment walk from x0 (inside cell `cur`) to x1.  Returns the final
-// cell, -1 when the particle left through an absorbing/conductor face (group
-// and hit point set), -2 when the walk limit was exceeded.
+// cell, -1 when the particle left through an absorbing/conductor face (group,
+// hit point, exit face and its tet set), -2 when the walk limit was exceeded.
 template <typename T>
 __device__ __forceinline__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1[3], T v[3], int& group,
-                    T hit[3], TetMap<T>& m) {
+                    T hit[3], TetMap<T>& m, int& face, int& last) {
   const T tol = a.tol;
   for (int it = 0; it < a.max_walk; ++it) {
     T l1[4];
@@ -114,6 +114,8 @@ __device__ __forceinline__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1
       continue;
     }
     group = g;
+    face = kb;
+    last = cur;
     return -1;
   }
   return -2;
@@ -231,7 +233,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
       T remaining = a.dt;
       int sub = 0;
       int dead = 0;
-      int group = -1;
+      int group = -1, face = 0, last = c;
       T hit[3] = {x[0], x[1], x[2]};
       // Sub-step loop, PICParticleAlgorithms.cpp:593-701.
       while (remaining > T(1e-20) && sub < a.max_sub) {
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
         boris(v, E, a.B, a.has_b != 0, a.qm, h);
         T xs[3] = {x[0], x[1], x[2]};
         T xn[3] = {fma_t(v[0], h, x[0]), fma_t(v[1], h, x[1]), fma_t(v[2], h, x[2])};
-        const int r = walk(a, c, xs, xn, v, group, hit, m);
+        const int r = walk(a, c, xs, xn, v, group, hit, m, face, last);
         if (r < 0) {
           dead = r;
           break;
@@ -267,6 +269,24 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
                            ((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]);
           atomicAdd(&led_n[g], 1u);
           atomicAdd(&led_e[g], k * a.energy_weight);
+          // impact recorded for secondary emission after the push
+          if ((a.see_mask >> g) & 1u) {
+            const unsigned long long e = atomicAdd(a.n_events, 1ull);
+            if ((int64_t)e < a.event_cap) {
+              SeeEvent ev;
+              ev.x[0] = hit[0];
+              ev.x[1] = hit[1];
+              ev.x[2] = hit[2];
+              ev.v[0] = v[0];
+              ev.v[1] = v[1];
+              ev.v[2] = v[2];
+              ev.id = a.id[i];
+              ev.cell = last;
+              ev.group_face = g | (face << 8);
+              ev.species = a.species;
+              a.events[e] = ev;
+            }
+          }
         }
         a.cell[i] = -1;
         a.x[i] = hit[0];

@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "device_common.hpp"
+#include "emit.hpp"
 
 namespace scdat {
 
@@ -45,6 +46,12 @@ struct PushArgs {
   unsigned long long* pushes;        // cumulative particles pushed
   double* ke_partial;                // per block
   int32_t* alive_partial;            // per block
+  // secondary emission: impacts on groups in see_mask are recorded
+  const int32_t* id;
+  uint32_t see_mask;
+  SeeEvent* events;
+  unsigned long long* n_events;
+  int64_t event_cap;
 };
 
 // blocks of one push/deposit launch over n particle slots (one partial each)

@@ -120,3 +120,20 @@ def test_maxwellian_loader_moments_and_determinism():
     assert np.array_equal(x2, x[:1000]) and np.array_equal(v2, v[:1000])
     x3, _, _ = S.maxwellian_box(spec, 1000, 42, 1, 2.0)
     assert not np.array_equal(x3, x[:1000])
+
+
+def test_collision_process_defaults_are_the_reference_defaults():
+    # MonteCarloCollisionHandler::initializeDefaultCollisionParameters
+    # (CollisionAlgorithm.cpp:590-646); the same sets drive mcc_sigma.npz
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mcc_sigma.npz"))
+    prm = g["prm"]
+    cases = {S.MCC_ELASTIC: prm[0], S.MCC_INELASTIC: prm[1], S.MCC_CHARGE_EXCHANGE: prm[2],
+             S.MCC_IONIZATION: prm[4], S.MCC_EXCITATION: prm[6], S.MCC_RECOMBINATION: prm[8]}
+    for t, p in cases.items():
+        d = S.mcc_process(t, 0)
+        assert (d.threshold_ev, d.sigma_max_m2) == (p[0], p[1]), t
+        if t in (S.MCC_ELASTIC, S.MCC_INELASTIC, S.MCC_CHARGE_EXCHANGE, S.MCC_RECOMBINATION):
+            assert (d.reference_energy_ev, d.power_index) == (p[2], p[3]), t
+    assert S.mcc_process(S.MCC_EXCITATION, 0).alpha == 1.2 and S.mcc_process(S.MCC_EXCITATION, 0).beta == 0.35
+    assert S.mcc_process(S.MCC_IONIZATION, 0).energy_loss_ev == 15.76
+    assert abs(S.mcc_process(S.MCC_COULOMB, 0).coulomb_log - g["coulomb_log"][0]) < 1e-12
