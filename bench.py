@@ -146,7 +146,7 @@ def main():
     ap.add_argument("--particles", type=int, default=1 << 24)
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--sort-interval", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=float, default=0.0)

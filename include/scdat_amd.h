@@ -50,6 +50,12 @@ extern "C" {
 
 typedef struct scdat_ctx scdat_ctx;
 
+/* Physical constants of Tools/Basic/include/Constants.h:46-53 (SI). */
+#define SCDAT_ELEMENTARY_CHARGE 1.602176634e-19
+#define SCDAT_ELECTRON_MASS 9.1093837015e-31
+#define SCDAT_PROTON_MASS 1.67262192369e-27
+#define SCDAT_VACUUM_PERMITTIVITY 8.8541878128e-12
+
 /* ---- structured mesh generation (host only, no device needed) ----------
  * Replaces GeometryMeshGenerator::generateBoxMesh
  * (Tools/Mesh/src/MeshParsing.cpp:1208-1261): same lexicographic node order,
@@ -178,6 +184,9 @@ int scdat_sort(scdat_ctx* ctx);
 
 /* Inject a nodal field (V/m, 3 per node) / potential (V per node); the
  * latter mirrors PICCycle::setPotential (PICCycle.h:90-105) and recomputes E. */
+/* Replaces PICCycle::setTimeStep (PICCycle.h:86-92): ignored unless dt > 0. */
+int scdat_set_time_step(scdat_ctx* ctx, double dt);
+
 int scdat_set_efield(scdat_ctx* ctx, const double* E);
 int scdat_set_potential(scdat_ctx* ctx, const double* phi);
 int scdat_set_rho(scdat_ctx* ctx, const double* rho);

@@ -148,6 +148,7 @@ SIGNATURES = {
     "scdat_solve": (C.c_int, [_P]),
     "scdat_compute_efield": (C.c_int, [_P]),
     "scdat_sort": (C.c_int, [_P]),
+    "scdat_set_time_step": (C.c_int, [_P, C.c_double]),
     "scdat_set_efield": (C.c_int, [_P, _D]),
     "scdat_set_potential": (C.c_int, [_P, _D]),
     "scdat_set_rho": (C.c_int, [_P, _D]),
@@ -443,6 +444,9 @@ class Context:
         e = EventCounts()
         self._chk(lib().scdat_get_event_counts(self.h, C.byref(e)))
         return e.as_dict()
+
+    def set_time_step(self, dt):
+        self._chk(lib().scdat_set_time_step(self.h, dt))
 
     def stats(self):
         st = Stats()

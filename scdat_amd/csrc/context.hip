@@ -1484,6 +1484,12 @@ int scdat_get_particles(scdat_ctx* c, int32_t species, double* x, double* v, int
   });
 }
 
+int scdat_set_time_step(scdat_ctx* c, double dt) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (dt > 0.0 && std::isfinite(dt)) c->cfg.dt = dt;
+  return SCDAT_OK;
+}
+
 int scdat_set_conductor_charge(scdat_ctx* c, double q) {
   if (!c) return SCDAT_ERR_ARG;
   c->q_init = q;

@@ -129,7 +129,9 @@ class PICCycle:
         self._ctx.set_potential(phi)
 
     def setTimeStep(self, dt):
-        raise NotImplementedError("the device context fixes dt at construction; build a new PICCycle")
+        if dt > 0.0:
+            self.params.time_step = dt
+            self._ctx.set_time_step(dt)
 
     def getTimeStep(self):
         return self.params.time_step
