@@ -329,6 +329,44 @@ typedef struct {
 } scdat_event_counts;
 int scdat_get_event_counts(scdat_ctx* ctx, scdat_event_counts* out);
 
+/* ---- Surface circuit (host, FP64) -----------------------------------------
+ * Replaces SurfaceCircuitCoupling (Tools/Coupling/include/SurfaceCircuitCoupling.h:
+ * 13-45,324-361): nodes with capacitance, shunt conductance or a fixed
+ * potential, branches with conductance and bias, advanced by backward Euler
+ * with a linearised plasma current per node (advanceImplicit,
+ * SurfaceCircuitCoupling.cpp:897-1127).  Independent of any device context. */
+typedef struct scdat_circuit scdat_circuit;
+typedef struct {
+  int32_t row, column;   /* node indices of an extra matrix entry (A/V)     */
+  double coefficient;
+} scdat_circuit_entry;
+int scdat_circuit_create(scdat_circuit** out);
+void scdat_circuit_destroy(scdat_circuit* c);
+const char* scdat_circuit_last_error(const scdat_circuit* c);
+/* addNode (SurfaceCircuitCoupling.h:329) */
+int scdat_circuit_add_node(scdat_circuit* c, double potential_v, double capacitance_f,
+                           double shunt_conductance_s, int32_t fixed, double fixed_value_v, int32_t* index);
+/* addBranch (:330) */
+int scdat_circuit_add_branch(scdat_circuit* c, int32_t from, int32_t to, double conductance_s, double bias_v,
+                             int32_t* index);
+/* setNodePotential / setNodeCapacitance / setNodeShuntConductance /
+ * setNodeFixedPotential (:332-335) in one call */
+int scdat_circuit_set_node(scdat_circuit* c, int32_t node, double potential_v, double capacitance_f,
+                           double shunt_conductance_s, int32_t fixed, double fixed_value_v);
+/* setBranchConductance (:336) */
+int scdat_circuit_set_branch_conductance(scdat_circuit* c, int32_t branch, double conductance_s);
+int scdat_circuit_get_potentials(const scdat_circuit* c, double* v);
+/* advanceImplicit(dt, linearization, max_delta) (:348-353).  Per-node arrays
+ * of length n_nodes, any may be NULL (zeros); max_delta_v = INFINITY for no
+ * step limit.  Outputs (any NULL): new potentials, linearised plasma
+ * currents I + dI/dV (V_new - V_old), branch currents G (V_from - V_to - bias),
+ * largest |dV|.                                                              */
+int scdat_circuit_advance(scdat_circuit* c, double dt, const double* plasma_current, const double* plasma_didv,
+                          const double* additional_rhs, const double* additional_diag,
+                          const scdat_circuit_entry* entries, int32_t n_entries, double max_delta_v,
+                          double* node_potentials, double* plasma_currents, double* branch_currents,
+                          double* max_delta_out);
+
 /* Per-stage device time of the last scdat_step call, milliseconds:
  * [push, rhs, pcg, field, sort].                                            */
 int scdat_get_stage_times(scdat_ctx* ctx, double* ms5);

@@ -129,6 +129,8 @@ def surface_lib():
         L.ref_coulomb_log.restype = C.c_double
         L.ref_surface.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D, _D,
                                   C.c_double, C.c_double, _D, _D]
+        L.ref_circuit_advance.argtypes = [C.c_int, _D, C.c_int, _D, C.c_double, _D, _D, _D, _D, C.c_int, _D,
+                                          C.c_double, _D, _D, _D]
         _slib = L
     return _slib
 
@@ -157,3 +159,23 @@ def surface(see_max, see_peak_ev, photo_yield, photo_temp_ev, e_ev, angle_rad, s
                               _p(a, C.c_double), _p(v, C.c_double), ref_v, source_j, _p(y, C.c_double),
                               _p(j, C.c_double))
     return y, j
+
+
+def circuit_advance(nodes, branches, dt, current, didv, rhs=None, diag=None, entries=(), max_delta=np.inf):
+    """One SurfaceCircuitCoupling::advanceImplicit.  nodes: (v, C, G, fixed, fixed_v)
+    rows; branches: (from, to, G, bias) rows; entries: (row, col, coef) rows."""
+    nd = np.ascontiguousarray(nodes, np.float64).reshape(-1, 5)
+    br = np.ascontiguousarray(branches, np.float64).reshape(-1, 4)
+    en = np.ascontiguousarray(entries, np.float64).reshape(-1, 3)
+    n = len(nd)
+    z = np.zeros(n)
+    cur = np.ascontiguousarray(current, np.float64)
+    dv = np.ascontiguousarray(didv, np.float64)
+    rh = np.ascontiguousarray(z if rhs is None else rhs, np.float64)
+    dg = np.ascontiguousarray(z if diag is None else diag, np.float64)
+    v, i, b = np.zeros(n), np.zeros(n), np.zeros(max(1, len(br)))
+    rc = surface_lib().ref_circuit_advance(n, _p(nd, C.c_double), len(br), _p(br, C.c_double), dt,
+                                           _p(cur, C.c_double), _p(dv, C.c_double), _p(rh, C.c_double),
+                                           _p(dg, C.c_double), len(en), _p(en, C.c_double), max_delta,
+                                           _p(v, C.c_double), _p(i, C.c_double), _p(b, C.c_double))
+    return rc, v, i, b[:len(br)]

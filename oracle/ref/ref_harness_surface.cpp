@@ -13,9 +13,11 @@
 //   Tools/Material/src/SurfaceMaterialModel.cpp:163-208,369-402
 //       secondaryYield and photoCurrentDensity through
 //       BasicSurfaceMaterialModel::evaluateCurrents
+//   Tools/Coupling/src/SurfaceCircuitCoupling.cpp:801-1127  advanceImplicit
 #include <cstdint>
 
 #include "Tools/Interactions/Collisions/include/CollisonTypes.h"
+#include "Tools/Coupling/include/SurfaceCircuitCoupling.h"
 #include "Tools/Material/include/SurfaceMaterialModel.h"
 
 namespace C = SCDAT::Collision;
@@ -79,6 +81,54 @@ void ref_surface(double see_max, double see_peak_ev, double photo_yield, double 
                    : 0.0;
     photo_j[i] = c.photoelectron_emission_a_per_m2;
   }
+}
+
+// One advanceImplicit call.  node: 5 per node (potential, capacitance,
+// shunt conductance, fixed flag, fixed value); branch: 4 per branch (from,
+// to, conductance, bias); entry: 3 per extra matrix entry (row, column,
+// coefficient).  Returns 0, or 1 when the reference throws.
+int ref_circuit_advance(int nn, const double* node, int nb, const double* branch, double dt,
+                        const double* current, const double* didv, const double* rhs, const double* diag,
+                        int ne, const double* entry, double max_delta, double* v_out, double* i_out,
+                        double* b_out) {
+  namespace Cp = SCDAT::Coupling;
+  Cp::SurfaceCircuitCoupling circ;
+  for (int i = 0; i < nn; ++i) {
+    Cp::SurfaceCircuitNode n;
+    n.potential_v = node[5 * i];
+    n.capacitance_f = node[5 * i + 1];
+    n.shunt_conductance_s = node[5 * i + 2];
+    n.fixed_potential = node[5 * i + 3] != 0.0;
+    n.fixed_value_v = node[5 * i + 4];
+    circ.addNode(n);
+  }
+  for (int k = 0; k < nb; ++k) {
+    Cp::SurfaceCircuitBranch b;
+    b.from_node = (std::size_t)branch[4 * k];
+    b.to_node = (std::size_t)branch[4 * k + 1];
+    b.conductance_s = branch[4 * k + 2];
+    b.bias_v = branch[4 * k + 3];
+    circ.addBranch(b);
+  }
+  Cp::SurfaceCircuitLinearization lin;
+  lin.plasma_current_a.assign(current, current + nn);
+  lin.plasma_didv_a_per_v.assign(didv, didv + nn);
+  lin.additional_rhs_a.assign(rhs, rhs + nn);
+  lin.additional_diagonal_a_per_v.assign(diag, diag + nn);
+  for (int e = 0; e < ne; ++e)
+    lin.additional_off_diagonal_entries.push_back(
+        {(std::size_t)entry[3 * e], (std::size_t)entry[3 * e + 1], entry[3 * e + 2]});
+  try {
+    const Cp::SurfaceCircuitAdvanceResult r = circ.advanceImplicit(dt, lin, max_delta);
+    for (int i = 0; i < nn; ++i) {
+      v_out[i] = r.node_potentials_v[i];
+      i_out[i] = r.plasma_currents_a[i];
+    }
+    for (int k = 0; k < nb; ++k) b_out[k] = r.branch_currents_a[k];
+  } catch (...) {
+    return 1;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -169,6 +169,16 @@ SIGNATURES = {
     "scdat_set_seed": (C.c_int, [_P, C.c_uint64]),
     "scdat_set_surface_emission": (C.c_int, [_P, C.c_int32, C.POINTER(SurfaceEmission)]),
     "scdat_get_event_counts": (C.c_int, [_P, C.POINTER(EventCounts)]),
+    "scdat_circuit_create": (C.c_int, [C.POINTER(_P)]),
+    "scdat_circuit_destroy": (None, [_P]),
+    "scdat_circuit_last_error": (C.c_char_p, [_P]),
+    "scdat_circuit_add_node": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double, _I32]),
+    "scdat_circuit_add_branch": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, C.c_double, _I32]),
+    "scdat_circuit_set_node": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double]),
+    "scdat_circuit_set_branch_conductance": (C.c_int, [_P, C.c_int32, C.c_double]),
+    "scdat_circuit_get_potentials": (C.c_int, [_P, _D]),
+    "scdat_circuit_advance": (C.c_int, [_P, C.c_double, _D, _D, _D, _D, C.c_void_p, C.c_int32, C.c_double,
+                                        _D, _D, _D, _D]),
 }
 
 
@@ -483,3 +493,78 @@ class Context:
 
 from .loading import maxwellian_box, philox4x32  # noqa: E402,F401
 from .piccycle import PICCycle  # noqa: E402,F401
+
+
+class CircuitEntry(C.Structure):
+    _fields_ = [("row", C.c_int32), ("column", C.c_int32), ("coefficient", C.c_double)]
+
+
+class Circuit:
+    """SurfaceCircuitCoupling (Tools/Coupling/include/SurfaceCircuitCoupling.h:324-361)
+    over the C ABI: host FP64 backward-Euler circuit of surface patches."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        if lib().scdat_circuit_create(C.byref(h)) != OK:
+            raise ScdatError(ERR_ARG, "circuit_create failed")
+        self.h = h
+        self.n_nodes = 0
+        self.n_branches = 0
+
+    def close(self):
+        if self.h:
+            lib().scdat_circuit_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise ScdatError(rc, lib().scdat_circuit_last_error(self.h).decode())
+
+    def addNode(self, potential_v=0.0, capacitance_f=0.0, shunt_conductance_s=0.0, fixed=False,
+                fixed_value_v=0.0):
+        i = C.c_int32()
+        self._chk(lib().scdat_circuit_add_node(self.h, potential_v, capacitance_f, shunt_conductance_s,
+                                               int(bool(fixed)), fixed_value_v, C.byref(i)))
+        self.n_nodes += 1
+        return i.value
+
+    def addBranch(self, from_node, to_node, conductance_s, bias_v=0.0):
+        i = C.c_int32()
+        self._chk(lib().scdat_circuit_add_branch(self.h, from_node, to_node, conductance_s, bias_v, C.byref(i)))
+        self.n_branches += 1
+        return i.value
+
+    def setNode(self, node, potential_v=0.0, capacitance_f=0.0, shunt_conductance_s=0.0, fixed=False,
+                fixed_value_v=0.0):
+        self._chk(lib().scdat_circuit_set_node(self.h, node, potential_v, capacitance_f, shunt_conductance_s,
+                                               int(bool(fixed)), fixed_value_v))
+
+    def setBranchConductance(self, branch, conductance_s):
+        self._chk(lib().scdat_circuit_set_branch_conductance(self.h, branch, conductance_s))
+
+    def potentials(self):
+        v = np.zeros(self.n_nodes)
+        self._chk(lib().scdat_circuit_get_potentials(self.h, _ptr(v, C.c_double)))
+        return v
+
+    def advanceImplicit(self, dt, plasma_current, plasma_didv, additional_rhs=None, additional_diagonal=None,
+                        entries=(), max_delta_potential_v=float("inf")):
+        """Returns (node potentials, plasma currents, branch currents, max |dV|)."""
+        n = self.n_nodes
+
+        def arr(a):
+            return None if a is None else np.ascontiguousarray(a, np.float64)
+        cur, dv, rh, dg = arr(plasma_current), arr(plasma_didv), arr(additional_rhs), arr(additional_diagonal)
+        ent = (CircuitEntry * max(1, len(entries)))(*[CircuitEntry(int(r), int(c), float(x)) for r, c, x in entries])
+        v, i, b = np.zeros(n), np.zeros(n), np.zeros(max(1, self.n_branches))
+        md = C.c_double()
+        p = lambda a: None if a is None else _ptr(a, C.c_double)  # noqa: E731
+        self._chk(lib().scdat_circuit_advance(self.h, dt, p(cur), p(dv), p(rh), p(dg), C.cast(ent, C.c_void_p),
+                                              len(entries), max_delta_potential_v, p(v), p(i), p(b), C.byref(md)))
+        return v, i, b[:self.n_branches], md.value

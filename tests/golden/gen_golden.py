@@ -18,8 +18,11 @@ the reference's outputs of one routine:
                         Coulomb logarithm
   surface_yield.npz     BasicSurfaceMaterialModel secondary yield and
                         photoelectron current density
+  circuit_cases.npz     SurfaceCircuitCoupling::advanceImplicit on random
+                        circuits (fixed nodes, biased branches, extra entries,
+                        step limits)
 
-Usage: python tests/golden/gen_golden.py [field|surface]   (default: both)
+Usage: python tests/golden/gen_golden.py [field|surface|circuit]   (default: all)
 """
 import os
 import sys
@@ -159,9 +162,44 @@ def gen_surface():
          see_yield=np.stack([c["yield_"] for c in cases]), photo_j=np.stack([c["photo_j"] for c in cases]))
 
 
+def gen_circuit():
+    rng = np.random.default_rng(2026)
+    out = {}
+    ncase = 12
+    for k in range(ncase):
+        n = int(rng.integers(2, 8))
+        nodes = np.zeros((n, 5))
+        nodes[:, 0] = rng.normal(0, 5, n)
+        nodes[:, 1] = 10.0 ** rng.uniform(-12, -8, n)
+        nodes[:, 2] = np.where(rng.random(n) < 0.5, 10.0 ** rng.uniform(-12, -9, n), 0.0)
+        nodes[0, 3] = 1.0  # node 0 fixed (body / ground)
+        nodes[0, 4] = rng.normal(0, 2)
+        nb = int(rng.integers(1, 2 * n))
+        br = np.zeros((nb, 4))
+        for b in range(nb):
+            f, t = rng.choice(n, 2, replace=False)
+            br[b] = (f, t, 10.0 ** rng.uniform(-11, -8) if rng.random() < 0.9 else 0.0, rng.normal(0, 1))
+        ne = int(rng.integers(0, 4))
+        en = np.array([(int(rng.integers(0, n)), int(rng.integers(0, n)), rng.normal(0, 1e-9)) for _ in range(ne)])
+        cur = rng.normal(0, 1e-9, n)
+        didv = -np.abs(rng.normal(0, 1e-11, n))
+        rhs = rng.normal(0, 1e-10, n) * (rng.random() < 0.5)
+        diag = np.abs(rng.normal(0, 1e-10, n)) * (rng.random() < 0.5)
+        dt = 10.0 ** rng.uniform(-3, 0)
+        md = float("inf") if k % 3 else float(rng.uniform(0.1, 3.0))
+        rc, v, i, bc = R.circuit_advance(nodes, br, dt, cur, didv, rhs, diag, en.reshape(-1, 3), md)
+        assert rc == 0
+        for key, val in dict(nodes=nodes, branches=br, entries=en.reshape(-1, 3), current=cur, didv=didv,
+                             rhs=rhs, diag=diag, dt=dt, max_delta=md, v=v, i=i, b=bc).items():
+            out[f"c{k}_{key}"] = val
+    save("circuit_cases.npz", n_cases=ncase, **out)
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     if what in ("all", "field"):
         main()
     if what in ("all", "surface"):
         gen_surface()
+    if what in ("all", "circuit"):
+        gen_circuit()

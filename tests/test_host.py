@@ -137,3 +137,15 @@ def test_collision_process_defaults_are_the_reference_defaults():
     assert S.mcc_process(S.MCC_EXCITATION, 0).alpha == 1.2 and S.mcc_process(S.MCC_EXCITATION, 0).beta == 0.35
     assert S.mcc_process(S.MCC_IONIZATION, 0).energy_loss_ev == 15.76
     assert abs(S.mcc_process(S.MCC_COULOMB, 0).coulomb_log - g["coulomb_log"][0]) < 1e-12
+
+
+def test_cpp_facade_builds_and_fails_loudly_without_a_device():
+    # the facade links against the library; with no GPU the context cannot be
+    # created and the program stops with the library's error, no CPU fallback
+    here = os.path.dirname(os.path.abspath(__file__))
+    subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp")], check=True)
+    if os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU may be visible here")
+    r = subprocess.run([os.path.join(here, "cpp", "picycle_facade_test")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0 and "scdat_create failed" in r.stderr
