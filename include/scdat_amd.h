@@ -329,6 +329,56 @@ typedef struct {
 } scdat_event_counts;
 int scdat_get_event_counts(scdat_ctx* ctx, scdat_event_counts* out);
 
+/* ---- Live-PIC surface current sampler -------------------------------------
+ * Replaces PicMccSurfaceCurrentSampler::sample / sampleWithDerivative
+ * ("Toolkit/Surface Charging/include/PicMccSurfaceCurrentSampler.h:20-100",
+ * PicMccSurfaceCurrentSampler.cpp:489-670): a PIC-MCC window in a plate gap
+ * (surface at z = 0 held at surface_potential_v, z = gap at the reference,
+ * periodic sides, absorbing z faces); current densities from the charge
+ * collected at z = 0.  probe_delta_v != 0 also runs V +/- max(0.1, |d|)
+ * windows concurrently (three contexts) and returns dJ/dV.               */
+typedef struct {
+  double surface_area_m2;
+  double gap_distance_m;
+  double surface_aspect_ratio;
+  double surface_potential_v;
+  double plasma_reference_potential_v;
+  double electron_density_m3, electron_temperature_ev;
+  double ion_density_m3, ion_temperature_ev, ion_mass_amu;
+  double neutral_density_m3;  /* collisions (SurfacePicV1 set) when > 1e6   */
+  int32_t enable_mcc;
+  int32_t nx, ny, z_layers;    /* plate mesh nx x ny x max(4, z_layers) hexes */
+  int32_t particles_per_element;
+  int32_t window_steps;
+  int32_t precision;           /* 0 mixed, 1 FP64                          */
+  uint64_t seed;
+  double time_step;            /* 0: gap / (40 max thermal speed), [5e-12, 5e-9] s */
+} scdat_sampler_config;
+
+typedef struct {
+  int32_t valid;
+  double sampled_surface_potential_v;
+  double electron_collection_current_density_a_per_m2;
+  double ion_collection_current_density_a_per_m2;
+  double emitted_electron_current_density_a_per_m2;
+  double net_collection_current_density_a_per_m2;
+  double current_derivative_a_per_m2_per_v;
+  int64_t total_collisions;
+  double time_step;
+  int64_t macro_electrons, macro_ions;
+} scdat_current_sample;
+
+/* PicMccSurfaceSamplerConfig defaults (PicMccSurfaceCurrentSampler.h:20-49). */
+void scdat_sampler_config_default(scdat_sampler_config* c);
+int scdat_sample_surface_current(const scdat_sampler_config* c, double probe_delta_v, scdat_current_sample* out);
+/* The window's inputs, for checkers: plate box spec, macro-particle counts
+ * and weight (electrons from the per-element count, ions at the same
+ * weight), and the velocity direction step (v_z -> -|v_z|, toward the
+ * surface) applied after scdat_load_maxwellian(spec, n, seed, 0 | 1, ...). */
+void scdat_sampler_plate(const scdat_sampler_config* c, scdat_box_spec* spec);
+void scdat_sampler_counts(const scdat_sampler_config* c, int64_t* n_electrons, int64_t* n_ions, double* weight);
+void scdat_sampler_direct(int64_t n, double* v);
+
 /* ---- Surface circuit (host, FP64) -----------------------------------------
  * Replaces SurfaceCircuitCoupling (Tools/Coupling/include/SurfaceCircuitCoupling.h:
  * 13-45,324-361): nodes with capacitance, shunt conductance or a fixed

@@ -169,6 +169,11 @@ SIGNATURES = {
     "scdat_set_seed": (C.c_int, [_P, C.c_uint64]),
     "scdat_set_surface_emission": (C.c_int, [_P, C.c_int32, C.POINTER(SurfaceEmission)]),
     "scdat_get_event_counts": (C.c_int, [_P, C.POINTER(EventCounts)]),
+    "scdat_sampler_config_default": (None, [C.c_void_p]),
+    "scdat_sample_surface_current": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
+    "scdat_sampler_plate": (None, [C.c_void_p, C.POINTER(BoxSpec)]),
+    "scdat_sampler_counts": (None, [C.c_void_p, _I64, _I64, _D]),
+    "scdat_sampler_direct": (None, [C.c_int64, _D]),
     "scdat_circuit_create": (C.c_int, [C.POINTER(_P)]),
     "scdat_circuit_destroy": (None, [_P]),
     "scdat_circuit_last_error": (C.c_char_p, [_P]),
@@ -568,3 +573,63 @@ class Circuit:
         self._chk(lib().scdat_circuit_advance(self.h, dt, p(cur), p(dv), p(rh), p(dg), C.cast(ent, C.c_void_p),
                                               len(entries), max_delta_potential_v, p(v), p(i), p(b), C.byref(md)))
         return v, i, b[:self.n_branches], md.value
+
+
+class SamplerConfig(C.Structure):
+    _fields_ = [("surface_area_m2", C.c_double), ("gap_distance_m", C.c_double),
+                ("surface_aspect_ratio", C.c_double), ("surface_potential_v", C.c_double),
+                ("plasma_reference_potential_v", C.c_double), ("electron_density_m3", C.c_double),
+                ("electron_temperature_ev", C.c_double), ("ion_density_m3", C.c_double),
+                ("ion_temperature_ev", C.c_double), ("ion_mass_amu", C.c_double),
+                ("neutral_density_m3", C.c_double), ("enable_mcc", C.c_int32), ("nx", C.c_int32),
+                ("ny", C.c_int32), ("z_layers", C.c_int32), ("particles_per_element", C.c_int32),
+                ("window_steps", C.c_int32), ("precision", C.c_int32), ("seed", C.c_uint64),
+                ("time_step", C.c_double)]
+
+
+class CurrentSample(C.Structure):
+    _fields_ = [("valid", C.c_int32), ("sampled_surface_potential_v", C.c_double),
+                ("electron_collection_current_density_a_per_m2", C.c_double),
+                ("ion_collection_current_density_a_per_m2", C.c_double),
+                ("emitted_electron_current_density_a_per_m2", C.c_double),
+                ("net_collection_current_density_a_per_m2", C.c_double),
+                ("current_derivative_a_per_m2_per_v", C.c_double), ("total_collisions", C.c_int64),
+                ("time_step", C.c_double), ("macro_electrons", C.c_int64), ("macro_ions", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def sampler_config(**kw):
+    """PicMccSurfaceSamplerConfig with the reference defaults, overridden by kw."""
+    c = SamplerConfig()
+    lib().scdat_sampler_config_default(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def sample_surface_current(cfg, probe_delta_v=0.0):
+    """PicMccSurfaceCurrentSampler::sample (probe_delta_v = 0) or
+    sampleWithDerivative (three concurrent windows)."""
+    out = CurrentSample()
+    rc = lib().scdat_sample_surface_current(C.byref(cfg), probe_delta_v, C.byref(out))
+    if rc != OK:
+        raise ScdatError(rc, "sample_surface_current failed")
+    return out.as_dict()
+
+
+def sampler_inputs(cfg):
+    """The window's mesh spec, macro-particle counts/weight and initial
+    particles (for checkers)."""
+    spec = BoxSpec()
+    lib().scdat_sampler_plate(C.byref(cfg), C.byref(spec))
+    ne, ni, w = C.c_int64(), C.c_int64(), C.c_double()
+    lib().scdat_sampler_counts(C.byref(cfg), C.byref(ne), C.byref(ni), C.byref(w))
+    return spec, ne.value, ni.value, w.value
+
+
+def sampler_direct(v):
+    v = np.ascontiguousarray(v, np.float64)
+    lib().scdat_sampler_direct(len(v), _ptr(v, C.c_double))
+    return v

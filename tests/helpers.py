@@ -63,3 +63,23 @@ def smooth_field(xyz, amp=50.0):
 
 def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def periodic_partners(m, pairs=((0, 1, 0), (2, 3, 1))):
+    """Partner tet across periodic wall pairs (group_a, group_b, axis), by
+    matching face centroids in the other two coordinates."""
+    xyz, tets, grp = m["xyz"], m["tets"], m["group"]
+    partner = -np.ones_like(tets)
+    faces = {}
+    for t, k in zip(*np.nonzero(grp >= 0)):
+        faces.setdefault(int(grp[t, k]), []).append((t, xyz[np.delete(tets[t], k)].mean(0)))
+    for ga, gb, ax in pairs:
+        keep = [r for r in range(3) if r != ax]
+        B = {tuple(np.round(c[keep], 9)): t for t, c in faces.get(gb, [])}
+        for t, c in faces.get(ga, []):
+            tb = B[tuple(np.round(c[keep], 9))]
+            ka = [k for k in range(4) if grp[t, k] == ga][0]
+            kb = [k for k in range(4) if grp[tb, k] == gb][0]
+            partner[t, ka] = tb
+            partner[tb, kb] = t
+    return partner
