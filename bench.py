@@ -67,7 +67,7 @@ def structural_nnz(tets, n_nodes):
     return int(2 * np.unique(key).size + n_nodes)
 
 
-def run_gpu(case, steps, warmup, sort_interval):
+def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True):
     import scdat_amd as S
     m = case["mesh"]
     t0 = time.time()
@@ -84,7 +84,9 @@ def run_gpu(case, steps, warmup, sort_interval):
         ctx.step(1)
     ctx.sync()
     st0 = ctx.stats()
-    ctx.set_kernel_timing(True)
+    # every push launch and every 16th PCG iteration carry dispatch-timestamp
+    # events (timing every launch would slow the host's launch loop)
+    ctx.set_kernel_timing(kernel_timing, period=16)
     t1 = time.perf_counter()
     ctx.step(steps)
     ctx.sync()
@@ -149,6 +151,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="no per-launch events in the timed region (no roofline figures)")
     ap.add_argument("--copy-calibration", type=float, default=0.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -161,7 +165,7 @@ def main():
     import scdat_amd as S
     name, ncu = S.device_info()
     case = build_case(args.hex, args.particles, args.seed)
-    ctx, g = run_gpu(case, args.steps, args.warmup, args.sort_interval)
+    ctx, g = run_gpu(case, args.steps, args.warmup, args.sort_interval, not args.no_kernel_timing)
 
     m = case["mesh"]
     n_nodes, n_tets = len(m["kind"]), len(m["tets"])
@@ -181,19 +185,22 @@ def main():
     traffic = {}
     if os.path.exists(TRAFFIC_FILE):
         traffic = json.load(open(TRAFFIC_FILE)).get("traffic_bytes_per_launch", {})
+    # launches that did work in the timed region (PCG: one per inner iteration;
+    # every 16th carries timing events)
+    iters = g["stats"]["pcg_iterations_total"] - g["stats0"]["pcg_iterations_total"]
+    launches = {"push": kt["push"][1], "pcg_iter": iters}
     kernels = {}
     for k, (ms, n) in kt.items():
         gbs = bytes_k[k] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         kernels[k] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": gbs / HBM_PEAK_GBS, "traffic": traffic.get(k), "avg_launch_ms": ms,
-                      "launches": n, "algorithmic_bytes_per_launch": bytes_k[k],
-                      "total_ms": ms * n}
+                      "launches": launches[k], "launches_timed": n,
+                      "algorithmic_bytes_per_launch": bytes_k[k], "total_ms": ms * launches[k]}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
     names = {"push": "push_kernel<float>", "pcg_iter": "k_cg_iter<float,W>"}
     roof = dict(kernels[dominant])
     roof["kernel"] = names[dominant]
     st = g["stats"]
-    iters = kt["pcg_iter"][1]
     pcg_ms = g["stages"]["pcg"]
     pcg_bytes = iters * bytes_k["pcg_iter"]
     value = g["pushes"] / g["wall"]

@@ -220,6 +220,7 @@ typedef struct {
   int64_t particle_pushes;     /* cumulative particles pushed by scdat_step/
                                   scdat_push (alive at the start of a push)  */
   double pcg_initial_rel_residual; /* last solve, warm-start ||r0|| / ||b||  */
+  int64_t pcg_iterations_total;    /* inner PCG iterations of all solves     */
 } scdat_stats;
 
 /* Replaces PICCycle::getStatistics (PICCycle.h:135-149). */
@@ -428,11 +429,14 @@ int scdat_get_push_kernel_time(scdat_ctx* ctx, double* avg_ms, int64_t* launches
 
 /* Per-kernel timing from the kernels' own dispatch timestamps (HIP events
  * attached to the launches on the context's stream; off by default, enabling
- * resets the sums).  which: 0 push, 1 PCG iteration (fused SpMV + vector
- * update); averages over launches that did work since timing was enabled.  */
+ * resets the sums).  period 0: off; n > 0: every push launch and every n-th
+ * PCG iteration launch is timed (attached events cost host time, so a
+ * period of 8-16 keeps the measured loop close to the untimed one).
+ * which: 0 push, 1 PCG iteration (fused SpMV + vector update); averages over
+ * timed launches that did work since timing was enabled.                   */
 #define SCDAT_KERNEL_PUSH 0
 #define SCDAT_KERNEL_PCG 1
-int scdat_set_kernel_timing(scdat_ctx* ctx, int32_t on);
+int scdat_set_kernel_timing(scdat_ctx* ctx, int32_t period);
 int scdat_get_kernel_time(scdat_ctx* ctx, int32_t which, double* avg_ms, int64_t* launches);
 
 /* Synchronise the context's stream. */

@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("pcg_iterations", C.c_int32), ("pcg_outer", C.c_int32),
                 ("pcg_rel_residual", C.c_double), ("pcg_converged", C.c_int32),
                 ("lost_particles", C.c_int64), ("particle_pushes", C.c_int64),
-                ("pcg_initial_rel_residual", C.c_double)]
+                ("pcg_initial_rel_residual", C.c_double), ("pcg_iterations_total", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -483,8 +483,9 @@ class Context:
         self._chk(lib().scdat_get_push_kernel_time(self.h, C.byref(t), C.byref(n)))
         return t.value, n.value
 
-    def set_kernel_timing(self, on=True):
-        self._chk(lib().scdat_set_kernel_timing(self.h, 1 if on else 0))
+    def set_kernel_timing(self, on=True, period=1):
+        """Time every push launch and every `period`-th PCG iteration launch."""
+        self._chk(lib().scdat_set_kernel_timing(self.h, int(period) if on else 0))
 
     def kernel_time(self, which):
         """(average ms, launches) for KERNEL_PUSH or KERNEL_PCG (one PCG iteration)."""

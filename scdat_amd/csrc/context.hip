@@ -403,6 +403,7 @@ struct scdat_ctx {
   bool append_pending = false;
   int last_iters = 0, last_outer = 0, last_conv = 0;
   int inner_hist[16] = {};  // inner iterations per outer pass of the last solve
+  int64_t iters_total = 0;  // inner iterations of all solves
   double last_rel = 0.0, last_init_rel = 0.0;
   double stage_ms[5] = {0, 0, 0, 0, 0};
   double push_ms_sum = 0.0;
@@ -410,7 +411,7 @@ struct scdat_ctx {
   std::vector<hipEvent_t> ev_pool;
   // per-kernel timing (HIP events on the context stream), enabled on request:
   // [0] push, [1] PCG iteration; sums in ms over executed launches
-  bool kernel_timing = false;
+  int kernel_timing = 0;  // 0 off, else time every kernel_timing-th PCG iteration
   std::vector<hipEvent_t> kt_pool;
   double kt_sum[2] = {0, 0};
   int64_t kt_n[2] = {0, 0};
@@ -684,35 +685,37 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
     int it = 0, batch = 8;
     if (hist && hist[outer] > 16) batch = std::min(1024, hist[outer] * 7 / 8);
     for (;;) {
-      const int it0 = it;
+      // timed launches of this batch (every kernel_timing-th iteration)
+      std::vector<int> timed;
       for (int k = 0; k < batch; ++k, ++it) {
-        if (c->kernel_timing) {
-          hipEvent_t ev[2] = {take_kt_event(c, 2 * k), take_kt_event(c, 2 * k + 1)};
+        if (c->kernel_timing > 0 && it % c->kernel_timing == 0) {
+          const size_t j = timed.size();
+          hipEvent_t ev[2] = {take_kt_event(c, 2 * j), take_kt_event(c, 2 * j + 1)};
           launch_inner_iter<T>(A, f, ib, it, s, ev);
+          timed.push_back(it);
         } else {
           launch_inner_iter<T>(A, f, ib, it, s, nullptr);
         }
       }
       read_scalars(c);
-      if (c->kernel_timing) {
-        // iterations it0 .. inner_it-1 ran; later launches of the batch exited
-        // at their first instruction and are not counted
-        const int ran = std::min(batch, std::max(0, c->h_sc->inner_it - it0));
-        for (int k = 0; k < ran; ++k) {
-          float a = 0;
-          HIPCHK(hipEventElapsedTime(&a, c->kt_pool[2 * k], c->kt_pool[2 * k + 1]));
-          c->kt_sum[1] += a;
-          c->kt_n[1] += 1;
-        }
+      // iterations below inner_it ran; later launches of the batch exited
+      // early and are not counted
+      for (size_t j = 0; j < timed.size(); ++j) {
+        if (timed[j] >= c->h_sc->inner_it) break;
+        float a = 0;
+        HIPCHK(hipEventElapsedTime(&a, c->kt_pool[2 * j], c->kt_pool[2 * j + 1]));
+        c->kt_sum[1] += a;
+        c->kt_n[1] += 1;
       }
       if (c->h_sc->inner_done) break;
-      batch = it - it0 > 64 ? 8 : std::min(batch * 2, 64);  // after a predicted batch: small steps
+      batch = batch > 64 ? 8 : std::min(batch * 2, 64);  // after a predicted batch: small steps
     }
     if (hist) hist[outer] = c->h_sc->inner_it;
     launch_apply<T>(A, f, ib, s);
   }
   read_scalars(c);
   c->last_iters = c->h_sc->total_it;
+  c->iters_total += c->h_sc->total_it;
   c->last_outer = c->h_sc->outer_it;
   const double bn = std::sqrt(c->h_sc->bnorm2);
   c->last_rel = bn > 0 ? c->h_sc->rnorm / bn : c->h_sc->rnorm;
@@ -1776,6 +1779,7 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
     st->pcg_rel_residual = c->last_rel;
     st->pcg_converged = c->last_conv;
     st->pcg_initial_rel_residual = c->last_init_rel;
+    st->pcg_iterations_total = c->iters_total;
   });
 }
 
@@ -1958,7 +1962,7 @@ int scdat_get_push_kernel_time(scdat_ctx* c, double* avg_ms, int64_t* launches) 
 
 int scdat_set_kernel_timing(scdat_ctx* c, int32_t on) {
   if (!c) return SCDAT_ERR_ARG;
-  c->kernel_timing = on != 0;
+  c->kernel_timing = on > 0 ? on : 0;
   for (int k = 0; k < 2; ++k) {
     c->kt_sum[k] = 0.0;
     c->kt_n[k] = 0;
