@@ -407,6 +407,30 @@ int scdat_circuit_set_node(scdat_circuit* c, int32_t node, double potential_v, d
 /* setBranchConductance (:336) */
 int scdat_circuit_set_branch_conductance(scdat_circuit* c, int32_t branch, double conductance_s);
 int scdat_circuit_get_potentials(const scdat_circuit* c, double* v);
+/* getNodes().size() / getBranches().size() (:338-346) */
+int scdat_circuit_size(const scdat_circuit* c, int32_t* n_nodes, int32_t* n_branches);
+
+/* ---- Conductor patches on a surface circuit ------------------------------
+ * The conductor's nodes split into patches (node_patch[n], -1 off the
+ * conductor; NULL: the six faces, boundary groups 6..11 -> patches 0..5,
+ * and the body, patch 6 = nodes on more than one face, n_patches = 7).
+ * Each patch is held at the potential of circuit node patch_circuit_node[k];
+ * after every push the charge collected by its groups (group_patch[g], NULL:
+ * group 6 + k -> patch k) over dt is that node's plasma current and the
+ * circuit takes one backward-Euler step (SurfaceCircuitCoupling::
+ * advanceImplicit, SurfaceCircuitCoupling.cpp:897-1127) before the field:
+ * phi = phi_0 + sum_k V_k phi_k, phi_k the unit solution of patch k (solved
+ * at scdat_initialize).  This replaces the Gauss-law floating conductor; the
+ * circuit is caller-owned and must outlive the context's use.  n_patches = 0
+ * returns to the floating conductor.                                        */
+#define SCDAT_MAX_PATCHES 16
+int scdat_set_conductor_circuit(scdat_ctx* ctx, int32_t n_patches, const int32_t* node_patch, scdat_circuit* circuit,
+                                const int32_t* patch_circuit_node, const int32_t* group_patch);
+/* dI/dV per patch (A/V) for the circuit's linearised plasma current. */
+int scdat_set_patch_didv(scdat_ctx* ctx, const double* didv);
+int scdat_get_patch_potentials(scdat_ctx* ctx, double* v);
+/* Unit-potential solution of one patch (FP64, n_nodes), after initialize. */
+int scdat_get_patch_basis(scdat_ctx* ctx, int32_t patch, double* phi);
 /* advanceImplicit(dt, linearization, max_delta) (:348-353).  Per-node arrays
  * of length n_nodes, any may be NULL (zeros); max_delta_v = INFINITY for no
  * step limit.  Outputs (any NULL): new potentials, linearised plasma

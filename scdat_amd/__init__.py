@@ -174,6 +174,11 @@ SIGNATURES = {
     "scdat_sampler_plate": (None, [C.c_void_p, C.POINTER(BoxSpec)]),
     "scdat_sampler_counts": (None, [C.c_void_p, _I64, _I64, _D]),
     "scdat_sampler_direct": (None, [C.c_int64, _D]),
+    "scdat_circuit_size": (C.c_int, [_P, _I32, _I32]),
+    "scdat_set_conductor_circuit": (C.c_int, [_P, C.c_int32, _I32, _P, _I32, _I32]),
+    "scdat_set_patch_didv": (C.c_int, [_P, _D]),
+    "scdat_get_patch_potentials": (C.c_int, [_P, _D]),
+    "scdat_get_patch_basis": (C.c_int, [_P, C.c_int32, _D]),
     "scdat_circuit_create": (C.c_int, [C.POINTER(_P)]),
     "scdat_circuit_destroy": (None, [_P]),
     "scdat_circuit_last_error": (C.c_char_p, [_P]),
@@ -460,6 +465,30 @@ class Context:
         self._chk(lib().scdat_get_event_counts(self.h, C.byref(e)))
         return e.as_dict()
 
+    def set_conductor_circuit(self, circuit, patch_circuit_node, n_patches=7, node_patch=None, group_patch=None):
+        """Hold conductor patches at circuit potentials (default: six faces + body)."""
+        pc = np.ascontiguousarray(patch_circuit_node, np.int32)
+        npa = None if node_patch is None else np.ascontiguousarray(node_patch, np.int32)
+        gp = None if group_patch is None else np.ascontiguousarray(group_patch, np.int32)
+        self._circuit = circuit  # keep alive
+        self._n_patch = n_patches
+        self._chk(lib().scdat_set_conductor_circuit(self.h, n_patches, _ptr(npa, C.c_int32), circuit.h,
+                                                     _ptr(pc, C.c_int32), _ptr(gp, C.c_int32)))
+
+    def set_patch_didv(self, didv):
+        d = np.ascontiguousarray(didv, np.float64)
+        self._chk(lib().scdat_set_patch_didv(self.h, _ptr(d, C.c_double)))
+
+    def patch_potentials(self):
+        v = np.zeros(max(1, getattr(self, "_n_patch", 0)))
+        self._chk(lib().scdat_get_patch_potentials(self.h, _ptr(v, C.c_double)))
+        return v
+
+    def patch_basis(self, k):
+        phi = np.zeros(self.n_nodes)
+        self._chk(lib().scdat_get_patch_basis(self.h, k, _ptr(phi, C.c_double)))
+        return phi
+
     def set_time_step(self, dt):
         self._chk(lib().scdat_set_time_step(self.h, dt))
 
@@ -553,6 +582,11 @@ class Circuit:
 
     def setBranchConductance(self, branch, conductance_s):
         self._chk(lib().scdat_circuit_set_branch_conductance(self.h, branch, conductance_s))
+
+    def size(self):
+        n, b = C.c_int32(), C.c_int32()
+        self._chk(lib().scdat_circuit_size(self.h, C.byref(n), C.byref(b)))
+        return n.value, b.value
 
     def potentials(self):
         v = np.zeros(self.n_nodes)

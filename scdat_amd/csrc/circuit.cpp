@@ -125,6 +125,13 @@ int scdat_circuit_set_branch_conductance(scdat_circuit* c, int32_t branch, doubl
   return SCDAT_OK;
 }
 
+int scdat_circuit_size(const scdat_circuit* c, int32_t* n_nodes, int32_t* n_branches) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (n_nodes) *n_nodes = (int32_t)c->nodes.size();
+  if (n_branches) *n_branches = (int32_t)c->branches.size();
+  return SCDAT_OK;
+}
+
 int scdat_circuit_get_potentials(const scdat_circuit* c, double* v) {
   if (!c || !v) return SCDAT_ERR_ARG;
   for (size_t i = 0; i < c->nodes.size(); ++i) v[i] = c->nodes[i].v;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -342,6 +343,8 @@ static void host_locate(const HostMesh& h, int64_t n, const double* x, std::vect
   }
 }
 
+constexpr int kMaxPatches = SCDAT_MAX_PATCHES;
+
 struct SpeciesState {
   double q = 0, m = 0, w = 0;
   int64_t cap = 0, n = 0;  // n = slots in use
@@ -401,6 +404,18 @@ struct scdat_ctx {
   int64_t event_cap = 0;
   unsigned long long* h_append = nullptr;  // pinned, kMaxSpecies + 1 (last: impacts)
   bool append_pending = false;
+  // conductor patches held at the potentials of a surface circuit
+  int n_patch = 0;
+  std::vector<int32_t> node_patch;
+  scdat_circuit* circuit = nullptr;  // caller-owned
+  std::array<int32_t, kMaxPatches> patch_cnode{};
+  std::array<int32_t, kMaxGroups> group_patch{};
+  std::array<double, kMaxPatches> patch_q{};  // collected charge so far
+  std::array<double, kMaxPatches> patch_didv{};
+  DevBuf d_phik, d_patch_v;
+  double* h_patch_v = nullptr;               // pinned, kMaxPatches
+  unsigned long long* h_ledger = nullptr;    // pinned, 2 kMaxGroups kMaxSpecies
+  bool ledger_pending = false;
   int last_iters = 0, last_outer = 0, last_conv = 0;
   int inner_hist[16] = {};  // inner iterations per outer pass of the last solve
   int64_t iters_total = 0;  // inner iterations of all solves
@@ -620,6 +635,9 @@ FieldBufs fbufs(scdat_ctx* c) {
   f.r64 = c->d_r64.as<double>();
   f.partial = c->d_partial.as<double>();
   f.sc = c->d_sc.as<PcgScalars>();
+  f.n_basis = c->n_patch;
+  f.basis = c->d_phik.as<double>();
+  f.coef = c->d_patch_v.as<double>();
   return f;
 }
 
@@ -727,33 +745,69 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
   c->last_conv = c->h_sc->failed ? 0 : 1;
 }
 
+// Unit-potential solution: the nodes with in_set[i] at 1 V, every other
+// constrained node at 0 V, no charge.  Written to out (FP64, n).
 template <typename T>
-void solve_phi1(scdat_ctx* c) {
-  if (!c->h.has_cond) return;
+void solve_unit(scdat_ctx* c, const std::vector<char>& in_set, double* out) {
+  const HostMesh& h = c->h;
+  const int64_t nn = h.nn;
   FieldBufs f = fbufs(c);
-  f.x64 = c->d_phi1.as<double>();
-  HIPCHK(hipMemcpyAsync(c->d_b64.p, c->h.b1.data(), 8 * (size_t)c->h.nn, hipMemcpyHostToDevice, c->stream));
-  // start from the constrained values (1 on the conductor, 0 on the walls) so
-  // constrained rows have zero residual and only free rows iterate
-  std::vector<double> x0((size_t)c->h.nn, 0.0);
-  for (size_t i = 0; i < x0.size(); ++i)
-    if (c->h.kind[i] == SCDAT_NODE_CONDUCTOR) x0[i] = 1.0;
-  HIPCHK(hipMemcpyAsync(c->d_phi1.p, x0.data(), 8 * x0.size(), hipMemcpyHostToDevice, c->stream));
+  f.x64 = out;
+  // right-hand side: K_ii on the set's rows, minus the set's columns of K on
+  // free rows (the lift of a unit Dirichlet value)
+  std::vector<double> b((size_t)nn, 0.0), x0((size_t)nn, 0.0);
+  for (int64_t i = 0; i < nn; ++i) {
+    if (h.kind[i] != SCDAT_NODE_FREE) {
+      if (in_set[i]) {
+        b[i] = h.dscale[i];
+        x0[i] = 1.0;
+      }
+      continue;
+    }
+    for (int sl = 1; sl < h.W; ++sl) {
+      const size_t o = (size_t)sl * nn + i;
+      const int32_t j = h.col[o];
+      if (j != i && in_set[j] && h.kind[j] != SCDAT_NODE_FREE) b[i] -= h.kfull[o];
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->d_b64.p, b.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(out, x0.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
   PcgScalars init{};
   double bb = 0;
-  for (double v : c->h.b1) bb += v * v;
+  for (double v : b) bb += v * v;
   init.bnorm2 = bb;
   HIPCHK(hipMemcpyAsync(f.sc, &init, sizeof(init), hipMemcpyHostToDevice, c->stream));
   pcg_drive<T>(c, f, 1e-13, c->is_fp64() ? 0.0 : 1e-5, 12, nullptr);
-  std::vector<double> phi1((size_t)c->h.nn);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->last_conv) throw Error(SCDAT_ERR_SOLVER, "unit-potential solve did not converge");
+}
+
+template <typename T>
+void solve_phi1(scdat_ctx* c) {
+  if (!c->h.has_cond) return;
+  const int64_t nn = c->h.nn;
+  std::vector<char> cond((size_t)nn, 0);
+  for (int64_t i = 0; i < nn; ++i) cond[i] = c->h.kind[i] == SCDAT_NODE_CONDUCTOR;
+  solve_unit<T>(c, cond, c->d_phi1.as<double>());
+  std::vector<double> phi1((size_t)nn);
   HIPCHK(hipMemcpyAsync(phi1.data(), c->d_phi1.p, 8 * phi1.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   double cap = 0;
   for (size_t k = 0; k < c->h.kc_idx.size(); ++k) cap += c->h.kc_val[k] * phi1[c->h.kc_idx[k]];
   c->ccap = cap;
   if (!(cap > 0)) throw Error(SCDAT_ERR_SOLVER, "conductor capacitance not positive");
+  // patch basis for a circuit-driven conductor
+  if (c->n_patch > 0) {
+    c->d_phik.alloc(8 * (size_t)nn * c->n_patch);
+    for (int k = 0; k < c->n_patch; ++k) {
+      std::vector<char> in((size_t)nn, 0);
+      for (int64_t i = 0; i < nn; ++i) in[i] = c->node_patch[i] == k;
+      solve_unit<T>(c, in, c->d_phik.as<double>() + (size_t)k * nn);
+    }
+  }
   PcgScalars z{};
-  HIPCHK(hipMemcpyAsync(f.sc, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_sc.p, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 FloatArgs float_args(scdat_ctx* c) {
@@ -775,10 +829,62 @@ void do_solve(scdat_ctx* c, int rho_idx) {
   launch_rhs(A, f, c->d_rho[rho_idx].as<unsigned long long>(), c->unit, c->cfg.load_sign, c->stream);
   const bool fp64 = c->is_fp64();
   pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, c->inner_hist);
-  if (c->h.has_cond) {
+  if (c->h.has_cond && c->n_patch == 0) {
     HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, c->stream));
     launch_float(f, float_args(c), c->stream);
   }
+}
+
+// Snapshot of the cumulative ledger and emission counts (read after the next
+// stream synchronisation) for the patch currents.
+void snapshot_ledger(scdat_ctx* c) {
+  if (c->n_patch == 0) return;
+  const size_t m = (size_t)kMaxGroups * kMaxSpecies;
+  HIPCHK(hipMemcpyAsync(c->h_ledger, c->d_ledger_count.p, 8 * m, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_ledger + m, c->d_emit_count.p, 8 * m, hipMemcpyDeviceToHost, c->stream));
+  c->ledger_pending = true;
+}
+
+// Collected charge per patch since the last step -> plasma current -> one
+// backward-Euler circuit step over dt -> patch potentials for the field.
+void advance_patches(scdat_ctx* c, bool advance) {
+  if (c->n_patch == 0) return;
+  if (c->ledger_pending) {
+    const size_t m = (size_t)kMaxGroups * kMaxSpecies;
+    std::array<double, kMaxPatches> q{};
+    for (int g = 0; g < kMaxGroups; ++g) {
+      const int k = c->group_patch[g];
+      if (k < 0 || k >= c->n_patch) continue;
+      for (size_t sp = 0; sp < c->sp.size(); ++sp) {
+        const double qw = c->sp[sp].q * c->sp[sp].w;
+        q[k] += ((double)c->h_ledger[g * kMaxSpecies + sp] - (double)c->h_ledger[m + g * kMaxSpecies + sp]) * qw;
+      }
+    }
+    if (advance) {
+      int32_t nc = 0;
+      std::vector<double> v;
+      // circuit size: the largest node index used
+      for (int k = 0; k < c->n_patch; ++k) nc = std::max(nc, c->patch_cnode[k] + 1);
+      std::vector<double> cur(nc, 0.0), didv(nc, 0.0), pot(nc, 0.0);
+      for (int k = 0; k < c->n_patch; ++k) {
+        cur[c->patch_cnode[k]] += (q[k] - c->patch_q[k]) / c->cfg.dt;
+        didv[c->patch_cnode[k]] += c->patch_didv[k];
+      }
+      const int rc = scdat_circuit_advance(c->circuit, c->cfg.dt, cur.data(), didv.data(), nullptr, nullptr, nullptr,
+                                           0, std::numeric_limits<double>::infinity(), nullptr, nullptr, nullptr,
+                                           nullptr);
+      if (rc != SCDAT_OK) throw Error(rc, std::string("surface circuit: ") + scdat_circuit_last_error(c->circuit));
+    }
+    c->patch_q = q;
+    c->ledger_pending = false;
+  }
+  int32_t ncn = 0, nbr = 0;
+  scdat_circuit_size(c->circuit, &ncn, &nbr);
+  std::vector<double> all((size_t)std::max(1, ncn), 0.0);
+  if (scdat_circuit_get_potentials(c->circuit, all.data()) != SCDAT_OK)
+    throw Error(SCDAT_ERR_STATE, "surface circuit: cannot read potentials");
+  for (int k = 0; k < c->n_patch; ++k) c->h_patch_v[k] = all[c->patch_cnode[k]];
+  HIPCHK(hipMemcpyAsync(c->d_patch_v.p, c->h_patch_v, 8 * (size_t)c->n_patch, hipMemcpyHostToDevice, c->stream));
 }
 
 template <typename T>
@@ -943,6 +1049,11 @@ void do_initialize(scdat_ctx* c) {
     c->d_gshift.upload(sh, c->stream);
   }
   solve_phi1<T>(c);
+  if (c->n_patch > 0) {
+    c->patch_q.fill(0.0);
+    c->ledger_pending = false;
+    advance_patches(c, false);
+  }
   // Initial deposit -> Poisson -> E (PICCycle::initialize + first field).
   zero_rho<T>(c, 0);
   c->rho_target = 0;
@@ -1131,10 +1242,12 @@ void do_step(scdat_ctx* c, int n) {
     if (c->event_cap > 0) launch_fill_u64(c->d_n_events.as<unsigned long long>(), 1, c->stream);
     do_push<T>(c, &pev, &evn);
     if (post) post_push<T>(c);
+    snapshot_ledger(c);
     se.b = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.b), c->stream));
     do_solve<T>(c, c->rho_last);
     apply_appends(c);
+    advance_patches(c, true);
     se.cc = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.cc), c->stream));
     // E for the next push; clears the buffer the next push deposits into.
@@ -1329,7 +1442,11 @@ int scdat_create(const scdat_config* cfg, scdat_ctx** out) {
     return SCDAT_ERR_DEVICE;
   }
   if (hipHostMalloc((void**)&c->h_sc, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_append, 8 * (kMaxSpecies + 1), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_append, 8 * (kMaxSpecies + 1), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_patch_v, 8 * kMaxPatches, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_ledger, 16 * kMaxGroups * kMaxSpecies, hipHostMallocDefault) != hipSuccess) {
+    if (c->h_append) (void)hipHostFree(c->h_append);
+    if (c->h_patch_v) (void)hipHostFree(c->h_patch_v);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1350,6 +1467,8 @@ void scdat_destroy(scdat_ctx* c) {
   for (auto e : c->kt_pool) (void)hipEventDestroy(e);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
   if (c->h_append) (void)hipHostFree(c->h_append);
+  if (c->h_patch_v) (void)hipHostFree(c->h_patch_v);
+  if (c->h_ledger) (void)hipHostFree(c->h_ledger);
   hipStream_t s = c->stream;
   delete c;  // DevBuf destructors free device memory
   if (s) (void)hipStreamDestroy(s);
@@ -1949,6 +2068,88 @@ int scdat_get_event_counts(scdat_ctx* c, scdat_event_counts* out) {
     out->photo_emitted = (int64_t)v[C_PHOTO];
     out->overflow = (int64_t)v[C_OVERFLOW];
     out->impacts_dropped = c->events_dropped;
+  });
+}
+
+// ---- conductor patches driven by a surface circuit ----------------------------
+
+int scdat_set_conductor_circuit(scdat_ctx* c, int32_t n_patches, const int32_t* node_patch, scdat_circuit* circuit,
+                                const int32_t* patch_circuit_node, const int32_t* group_patch) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_conductor_circuit: set the mesh first");
+  if (n_patches == 0) {
+    c->n_patch = 0;
+    c->circuit = nullptr;
+    c->initialized = false;
+    return SCDAT_OK;
+  }
+  if (n_patches < 0 || n_patches > kMaxPatches || !circuit || !patch_circuit_node)
+    return fail(c, SCDAT_ERR_ARG, "set_conductor_circuit: bad patch count, circuit or node map");
+  if (!c->h.has_cond) return fail(c, SCDAT_ERR_STATE, "set_conductor_circuit: the mesh has no conductor nodes");
+  int32_t ncn = 0, nbr = 0;
+  scdat_circuit_size(circuit, &ncn, &nbr);
+  for (int k = 0; k < n_patches; ++k)
+    if (patch_circuit_node[k] < 0 || patch_circuit_node[k] >= ncn)
+      return fail(c, SCDAT_ERR_ARG, "set_conductor_circuit: patch mapped to a missing circuit node");
+  return guarded(c, [&] {
+    const HostMesh& h = c->h;
+    std::vector<int32_t> np((size_t)h.nn, -1);
+    if (node_patch) {
+      np.assign(node_patch, node_patch + h.nn);
+    } else {
+      // six conductor faces (groups 6..11 -> patches 0..5) and the body
+      // (patch 6: nodes on more than one face)
+      if (n_patches != 7) throw Error(SCDAT_ERR_ARG, "set_conductor_circuit: default patches need n_patches = 7");
+      std::vector<uint32_t> bits((size_t)h.nn, 0u);
+      for (int64_t t = 0; t < h.nt; ++t)
+        for (int k = 0; k < 4; ++k) {
+          const int g = h.group[4 * t + k];
+          if (g < 6 || g > 11) continue;
+          for (int q = 0; q < 4; ++q)
+            if (q != k) bits[h.tets[4 * t + q]] |= 1u << (g - 6);
+        }
+      for (int64_t i = 0; i < h.nn; ++i) {
+        if (h.kind[i] != SCDAT_NODE_CONDUCTOR || bits[i] == 0u) continue;
+        np[i] = __builtin_popcount(bits[i]) == 1 ? __builtin_ctz(bits[i]) : 6;
+      }
+    }
+    for (int64_t i = 0; i < h.nn; ++i) {
+      if (h.kind[i] == SCDAT_NODE_CONDUCTOR && (np[i] < 0 || np[i] >= n_patches))
+        throw Error(SCDAT_ERR_ARG, "set_conductor_circuit: conductor node " + std::to_string(i) + " has no patch");
+      if (h.kind[i] != SCDAT_NODE_CONDUCTOR) np[i] = -1;
+    }
+    c->node_patch = np;
+    c->n_patch = n_patches;
+    c->circuit = circuit;
+    for (int k = 0; k < n_patches; ++k) c->patch_cnode[k] = patch_circuit_node[k];
+    for (int g = 0; g < kMaxGroups; ++g)
+      c->group_patch[g] = group_patch ? group_patch[g] : (g >= 6 && g < 12 && g - 6 < n_patches ? g - 6 : -1);
+    c->patch_didv.fill(0.0);
+    c->d_patch_v.alloc(8 * kMaxPatches);
+    c->initialized = false;
+  });
+}
+
+int scdat_set_patch_didv(scdat_ctx* c, const double* didv) {
+  if (!c || !didv) return SCDAT_ERR_ARG;
+  for (int k = 0; k < c->n_patch; ++k) c->patch_didv[k] = didv[k];
+  return SCDAT_OK;
+}
+
+int scdat_get_patch_potentials(scdat_ctx* c, double* v) {
+  if (!c || !v) return SCDAT_ERR_ARG;
+  for (int k = 0; k < c->n_patch; ++k) v[k] = c->h_patch_v[k];
+  return SCDAT_OK;
+}
+
+int scdat_get_patch_basis(scdat_ctx* c, int32_t patch, double* phi) {
+  if (!c || !phi) return SCDAT_ERR_ARG;
+  if (patch < 0 || patch >= c->n_patch || !c->initialized)
+    return fail(c, SCDAT_ERR_STATE, "get_patch_basis: no such patch or not initialised");
+  return guarded(c, [&] {
+    HIPCHK(hipMemcpyAsync(phi, c->d_phik.as<double>() + (size_t)patch * c->h.nn, 8 * (size_t)c->h.nn,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
 

@@ -414,7 +414,12 @@ __global__ void __launch_bounds__(kBlock) k_phi(int64_t n, FieldBufs f, T* phiT,
   const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
   if (i >= n) return;
-  const double p = f.phi1 ? fma(f.sc->vfloat, f.phi1[i], f.x64[i]) : f.x64[i];
+  double p = f.x64[i];
+  if (f.n_basis > 0) {
+    for (int k = 0; k < f.n_basis; ++k) p = fma(f.coef[k], f.basis[(int64_t)k * n + i], p);
+  } else if (f.phi1) {
+    p = fma(f.sc->vfloat, f.phi1[i], p);
+  }
   phi64[i] = p;
   phiT[i] = (T)p;
 }

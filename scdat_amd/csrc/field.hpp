@@ -56,6 +56,11 @@ struct FieldBufs {
   double* r64;
   double* partial;          // >= 2 * grid doubles
   PcgScalars* sc;
+  // conductor patches held at circuit potentials: phi = x + sum_k coef[k] basis_k
+  // (n_basis > 0 replaces the single floating conductor phi1 / vfloat)
+  int n_basis;
+  const double* basis;      // n_basis x n, patch k at 1 V, everything else 0
+  const double* coef;       // n_basis potentials (device)
 };
 
 // FP32 (mixed) or FP64 inner-solve vectors: r, w = A D^-1 r and s = A p by
