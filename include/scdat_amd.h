@@ -410,6 +410,41 @@ int scdat_circuit_get_potentials(const scdat_circuit* c, double* v);
 /* getNodes().size() / getBranches().size() (:338-346) */
 int scdat_circuit_size(const scdat_circuit* c, int32_t* n_nodes, int32_t* n_branches);
 
+/* ---- Batched scenarios ----------------------------------------------------
+ * One context holds n independent charging scenarios whose meshes are
+ * concatenated into the context's mesh: scenario s owns the nodes
+ * [node_start[s], node_start[s+1]) (each start a multiple of
+ * SCDAT_SEGMENT_ALIGN; nodes that no tetrahedron uses are padding, held at
+ * 0 V) and the tetrahedra on them.  Every step advances all scenarios with one
+ * launch per kernel: one push over the particles of all scenarios, one PCG
+ * whose scalars, convergence test and floating conductor are per scenario.
+ * Scenario s has its own time step, weight multiplier (every species' weight
+ * times weight_scale[s]), random key (seed[s]) and surface emission; its
+ * results equal those of the same scenario run alone in its own context.
+ * Any array may be NULL (cfg.dt, 1, the context seed).  Call after
+ * scdat_set_mesh and before scdat_set_particles; particle ids then count
+ * within each scenario.  Background collisions, photoemission and conductor
+ * patches need a single scenario.  This batched form has no counterpart in
+ * the reference, which runs one PICCycle per case
+ * (PicMccSurfaceCurrentSampler.cpp:519-586 builds one per window).        */
+#define SCDAT_SEGMENT_ALIGN 256
+int scdat_set_scenarios(scdat_ctx* ctx, int32_t n, const int64_t* node_start, const double* weight_scale,
+                        const double* time_step, const uint64_t* seed);
+/* Surface emission of one boundary group of one scenario
+ * (scdat_set_surface_emission sets the group in every scenario). */
+int scdat_set_scenario_emission(scdat_ctx* ctx, int32_t scenario, int32_t group, const scdat_surface_emission* e);
+typedef struct {
+  int64_t completed_steps;
+  double simulation_time, time_step, weight_scale;
+  double floating_potential, conductor_charge;
+  int32_t pcg_iterations, pcg_converged;   /* of the last solve */
+  double pcg_rel_residual;
+  int64_t first_node, n_nodes;
+} scdat_scenario_stats;
+int scdat_get_scenario_stats(scdat_ctx* ctx, int32_t scenario, scdat_scenario_stats* out);
+/* getSurfaceCurrentLedger of one scenario (scdat_get_ledger sums them). */
+int scdat_get_scenario_ledger(scdat_ctx* ctx, int32_t scenario, scdat_ledger_entry* out, int32_t n_groups);
+
 /* ---- Conductor patches on a surface circuit ------------------------------
  * The conductor's nodes split into patches (node_patch[n], -1 off the
  * conductor; NULL: the six faces, boundary groups 6..11 -> patches 0..5,

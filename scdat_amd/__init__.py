@@ -102,6 +102,19 @@ class SurfaceEmission(C.Structure):
                 ("photo_reference_v", C.c_double)]
 
 
+SEGMENT_ALIGN = 256
+
+
+class ScenarioStats(C.Structure):
+    _fields_ = [("completed_steps", C.c_int64), ("simulation_time", C.c_double), ("time_step", C.c_double),
+                ("weight_scale", C.c_double), ("floating_potential", C.c_double),
+                ("conductor_charge", C.c_double), ("pcg_iterations", C.c_int32), ("pcg_converged", C.c_int32),
+                ("pcg_rel_residual", C.c_double), ("first_node", C.c_int64), ("n_nodes", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class EventCounts(C.Structure):
     _fields_ = [("collisions", C.c_int64 * MAX_MCC_PROCESSES), ("created", C.c_int64), ("removed", C.c_int64),
                 ("see_emitted", C.c_int64), ("photo_emitted", C.c_int64), ("overflow", C.c_int64),
@@ -176,6 +189,10 @@ SIGNATURES = {
     "scdat_sampler_direct": (None, [C.c_int64, _D]),
     "scdat_circuit_size": (C.c_int, [_P, _I32, _I32]),
     "scdat_set_conductor_circuit": (C.c_int, [_P, C.c_int32, _I32, _P, _I32, _I32]),
+    "scdat_set_scenarios": (C.c_int, [_P, C.c_int32, _I64, _D, _D, C.POINTER(C.c_uint64)]),
+    "scdat_set_scenario_emission": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(SurfaceEmission)]),
+    "scdat_get_scenario_stats": (C.c_int, [_P, C.c_int32, C.POINTER(ScenarioStats)]),
+    "scdat_get_scenario_ledger": (C.c_int, [_P, C.c_int32, C.POINTER(LedgerEntry), C.c_int32]),
     "scdat_set_patch_didv": (C.c_int, [_P, _D]),
     "scdat_get_patch_potentials": (C.c_int, [_P, _D]),
     "scdat_get_patch_basis": (C.c_int, [_P, C.c_int32, _D]),
@@ -459,6 +476,31 @@ class Context:
         e = SurfaceEmission(see_model, electron_species, delta_max, energy_max, see_temperature, photo_j,
                             photo_temperature, photo_reference_v)
         self._chk(lib().scdat_set_surface_emission(self.h, group, C.byref(e)))
+
+    def set_scenarios(self, node_start, weight_scale=None, time_step=None, seed=None):
+        """Split the mesh into independent scenarios (batched charging cases)."""
+        ns = np.ascontiguousarray(node_start, np.int64)
+        ws = None if weight_scale is None else np.ascontiguousarray(weight_scale, np.float64)
+        dt = None if time_step is None else np.ascontiguousarray(time_step, np.float64)
+        sd = None if seed is None else np.ascontiguousarray(seed, np.uint64)
+        self._chk(lib().scdat_set_scenarios(self.h, len(ns) - 1, _ptr(ns, C.c_int64), _ptr(ws, C.c_double),
+                                            _ptr(dt, C.c_double), _ptr(sd, C.c_uint64)))
+        self.n_scenarios = len(ns) - 1
+
+    def set_scenario_emission(self, scenario, group, see_model=SEE_OFF, electron_species=0, delta_max=0.0,
+                              energy_max=400.0, see_temperature=2.0):
+        e = SurfaceEmission(see_model, electron_species, delta_max, energy_max, see_temperature, 0.0, 2.0, 0.0)
+        self._chk(lib().scdat_set_scenario_emission(self.h, scenario, group, C.byref(e)))
+
+    def scenario_stats(self, scenario):
+        st = ScenarioStats()
+        self._chk(lib().scdat_get_scenario_stats(self.h, scenario, C.byref(st)))
+        return st.as_dict()
+
+    def scenario_ledger(self, scenario, n_groups=12):
+        arr = (LedgerEntry * n_groups)()
+        self._chk(lib().scdat_get_scenario_ledger(self.h, scenario, arr, n_groups))
+        return [e.as_dict() for e in arr]
 
     def event_counts(self):
         e = EventCounts()

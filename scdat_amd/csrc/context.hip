@@ -204,6 +204,14 @@ static void prepare_mesh(HostMesh& h, int op, double eps) {
         }
     }
   }
+  // Nodes that no tetrahedron uses (padding between batched scenarios) get a
+  // unit row and are held at 0 V.
+  for (int64_t i = 0; i < nn; ++i)
+    if (rp[i + 1] - rp[i] == 1) {
+      h.kfull[i] = 1.0;
+      h.kind[i] = SCDAT_NODE_DIRICHLET;
+      h.dval[i] = 0.0;
+    }
   // Symmetric elimination of Dirichlet and conductor nodes.
   h.aval = h.kfull;
   h.lift.assign(nn, 0.0);
@@ -416,6 +424,18 @@ struct scdat_ctx {
   double* h_patch_v = nullptr;               // pinned, kMaxPatches
   unsigned long long* h_ledger = nullptr;    // pinned, 2 kMaxGroups kMaxSpecies
   bool ledger_pending = false;
+  // batched scenarios (segments); empty seg_node0: one scenario
+  std::vector<int64_t> seg_node0;  // n_seg + 1 node offsets
+  std::vector<int32_t> tet_seg_h, blk_seg_h, seg_b0_h;
+  std::vector<double> seg_wscale, seg_dt, seg_ccap;
+  std::vector<uint32_t> seg_key;
+  std::vector<int64_t> kc_off_h;
+  std::vector<std::array<scdat_surface_emission, kMaxGroups>> seg_emit;
+  DevBuf d_tet_seg, d_blk_seg, d_seg_b0, d_seg_unit, d_seg_dt, d_seg_wscale, d_seg_qinit, d_seg_ccap, d_kc_off,
+      d_seg_emit, d_seg_key;
+  bool seg_emit_dirty = false;
+  int n_seg() const { return seg_node0.empty() ? 1 : (int)seg_node0.size() - 1; }
+  bool segmented() const { return !seg_node0.empty(); }
   int last_iters = 0, last_outer = 0, last_conv = 0;
   int inner_hist[16] = {};  // inner iterations per outer pass of the last solve
   int64_t iters_total = 0;  // inner iterations of all solves
@@ -638,8 +658,19 @@ FieldBufs fbufs(scdat_ctx* c) {
   f.n_basis = c->n_patch;
   f.basis = c->d_phik.as<double>();
   f.coef = c->d_patch_v.as<double>();
+  f.n_seg = c->n_seg();
+  if (c->segmented()) {
+    f.blk_seg = c->d_blk_seg.as<int32_t>();
+    f.seg_b0 = c->d_seg_b0.as<int32_t>();
+    f.seg_unit = c->d_seg_unit.as<double>();
+    f.kc_off = c->d_kc_off.as<int64_t>();
+  }
   return f;
 }
+
+// Scalars slots on the device: one per segment, plus the summary slot when
+// there are segments.
+size_t n_scalars(const scdat_ctx* c) { return c->segmented() ? (size_t)c->n_seg() + 1 : 1; }
 
 // Timing-only events: no system-scope fence (cache writeback and invalidate)
 // when they complete, so the timed kernel runs as its untimed neighbours do.
@@ -666,8 +697,14 @@ InnerBufs<T> ibufs(scdat_ctx* c) {
   return b;
 }
 
+// The host reads one PcgScalars: the solve's own, or the segments' summary.
 void read_scalars(scdat_ctx* c) {
-  HIPCHK(hipMemcpyAsync(c->h_sc, c->d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
+  const PcgScalars* src = c->d_sc.as<PcgScalars>();
+  if (c->segmented()) {
+    launch_seg_summary(fbufs(c), c->stream);
+    src += c->n_seg();
+  }
+  HIPCHK(hipMemcpyAsync(c->h_sc, src, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
@@ -683,16 +720,18 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
   const EllDev A = ell(c);
   const InnerBufs<T> ib = ibufs<T>(c);
   hipStream_t s = c->stream;
-  // scalar controls
+  // scalar controls (every segment)
   {
-    PcgScalars ctl;
-    HIPCHK(hipMemcpyAsync(&ctl, f.sc, sizeof(ctl), hipMemcpyDeviceToHost, s));
+    std::vector<PcgScalars> ctl(n_scalars(c));
+    HIPCHK(hipMemcpyAsync(ctl.data(), f.sc, sizeof(PcgScalars) * ctl.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    ctl.tol = tol;
-    ctl.inner_tol = inner_tol;
-    ctl.max_it = c->cfg.pcg_max_iter;
-    ctl.max_outer = max_outer;
-    HIPCHK(hipMemcpyAsync(f.sc, &ctl, sizeof(ctl), hipMemcpyHostToDevice, s));
+    for (auto& q : ctl) {
+      q.tol = tol;
+      q.inner_tol = inner_tol;
+      q.max_it = c->cfg.pcg_max_iter;
+      q.max_outer = max_outer;
+    }
+    HIPCHK(hipMemcpyAsync(f.sc, ctl.data(), sizeof(PcgScalars) * ctl.size(), hipMemcpyHostToDevice, s));
   }
   for (int outer = 0; outer <= max_outer; ++outer) {
     launch_resid64(A, f, s);
@@ -772,11 +811,15 @@ void solve_unit(scdat_ctx* c, const std::vector<char>& in_set, double* out) {
   }
   HIPCHK(hipMemcpyAsync(c->d_b64.p, b.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(out, x0.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
-  PcgScalars init{};
-  double bb = 0;
-  for (double v : b) bb += v * v;
-  init.bnorm2 = bb;
-  HIPCHK(hipMemcpyAsync(f.sc, &init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  // |b|^2 per segment
+  std::vector<PcgScalars> init(n_scalars(c));
+  for (int sg = 0; sg < c->n_seg(); ++sg) {
+    const int64_t i0 = c->segmented() ? c->seg_node0[sg] : 0, i1 = c->segmented() ? c->seg_node0[sg + 1] : nn;
+    double bb = 0;
+    for (int64_t i = i0; i < i1; ++i) bb += b[i] * b[i];
+    init[sg].bnorm2 = bb;
+  }
+  HIPCHK(hipMemcpyAsync(f.sc, init.data(), sizeof(PcgScalars) * init.size(), hipMemcpyHostToDevice, c->stream));
   pcg_drive<T>(c, f, 1e-13, c->is_fp64() ? 0.0 : 1e-5, 12, nullptr);
   HIPCHK(hipStreamSynchronize(c->stream));
   if (!c->last_conv) throw Error(SCDAT_ERR_SOLVER, "unit-potential solve did not converge");
@@ -792,10 +835,24 @@ void solve_phi1(scdat_ctx* c) {
   std::vector<double> phi1((size_t)nn);
   HIPCHK(hipMemcpyAsync(phi1.data(), c->d_phi1.p, 8 * phi1.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  double cap = 0;
-  for (size_t k = 0; k < c->h.kc_idx.size(); ++k) cap += c->h.kc_val[k] * phi1[c->h.kc_idx[k]];
-  c->ccap = cap;
-  if (!(cap > 0)) throw Error(SCDAT_ERR_SOLVER, "conductor capacitance not positive");
+  if (c->segmented()) {
+    // capacitance per scenario over its own conductor rows
+    c->seg_ccap.assign((size_t)c->n_seg(), 0.0);
+    for (int sg = 0; sg < c->n_seg(); ++sg) {
+      double cap = 0;
+      for (int64_t k = c->kc_off_h[sg]; k < c->kc_off_h[sg + 1]; ++k) cap += c->h.kc_val[k] * phi1[c->h.kc_idx[k]];
+      if (c->kc_off_h[sg + 1] > c->kc_off_h[sg] && !(cap > 0))
+        throw Error(SCDAT_ERR_SOLVER, "conductor capacitance not positive in scenario " + std::to_string(sg));
+      c->seg_ccap[sg] = cap;
+    }
+    c->d_seg_ccap.upload(c->seg_ccap, c->stream);
+    c->ccap = c->seg_ccap[0];
+  } else {
+    double cap = 0;
+    for (size_t k = 0; k < c->h.kc_idx.size(); ++k) cap += c->h.kc_val[k] * phi1[c->h.kc_idx[k]];
+    c->ccap = cap;
+    if (!(cap > 0)) throw Error(SCDAT_ERR_SOLVER, "conductor capacitance not positive");
+  }
   // patch basis for a circuit-driven conductor
   if (c->n_patch > 0) {
     c->d_phik.alloc(8 * (size_t)nn * c->n_patch);
@@ -805,8 +862,8 @@ void solve_phi1(scdat_ctx* c) {
       solve_unit<T>(c, in, c->d_phik.as<double>() + (size_t)k * nn);
     }
   }
-  PcgScalars z{};
-  HIPCHK(hipMemcpyAsync(c->d_sc.p, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  std::vector<PcgScalars> z(n_scalars(c));
+  HIPCHK(hipMemcpyAsync(c->d_sc.p, z.data(), sizeof(PcgScalars) * z.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
@@ -819,6 +876,11 @@ FloatArgs float_args(scdat_ctx* c) {
   fa.cond_mask = 0;
   for (int g = 0; g < kMaxGroups; ++g)
     if (c->gtype[g] == SCDAT_BC_CONDUCTOR) fa.cond_mask |= 1u << g;
+  if (c->segmented()) {
+    fa.seg_qinit = c->d_seg_qinit.as<double>();
+    fa.seg_wscale = c->d_seg_wscale.as<double>();
+    fa.seg_ccap = c->d_seg_ccap.as<double>();
+  }
   return fa;
 }
 
@@ -830,7 +892,8 @@ void do_solve(scdat_ctx* c, int rho_idx) {
   const bool fp64 = c->is_fp64();
   pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, c->inner_hist);
   if (c->h.has_cond && c->n_patch == 0) {
-    HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (!c->segmented())
+      HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, c->stream));
     launch_float(f, float_args(c), c->stream);
   }
 }
@@ -944,13 +1007,19 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   a.id = S.buf[S.cur][7].as<int32_t>();
   a.see_mask = 0;
   if (S.q < 0 && c->event_cap > 0)
-    for (int g = 0; g < kMaxGroups; ++g)
-      if (c->emit[g].see_model != SCDAT_SEE_OFF &&
-          (c->gtype[g] == SCDAT_BC_CONDUCTOR || c->gtype[g] == SCDAT_BC_ABSORB))
-        a.see_mask |= 1u << g;
+    for (int g = 0; g < kMaxGroups; ++g) {
+      bool on = c->emit[g].see_model != SCDAT_SEE_OFF;
+      for (const auto& se : c->seg_emit) on = on || se[g].see_model != SCDAT_SEE_OFF;
+      if (on && (c->gtype[g] == SCDAT_BC_CONDUCTOR || c->gtype[g] == SCDAT_BC_ABSORB)) a.see_mask |= 1u << g;
+    }
   a.events = c->d_events.as<SeeEvent>();
   a.n_events = c->d_n_events.as<unsigned long long>();
   a.event_cap = c->event_cap;
+  if (c->segmented()) {
+    a.tet_seg = c->d_tet_seg.as<int32_t>();
+    a.seg_dt = c->d_seg_dt.as<T>();
+    a.seg_wscale = c->d_seg_wscale.as<double>();
+  }
   return a;
 }
 
@@ -1026,6 +1095,68 @@ void do_sort(scdat_ctx* c) {
   for (size_t si = 0; si < c->sp.size(); ++si) do_sort_species<T>(c, (int)si);
 }
 
+SurfEmitDev emit_dev(const scdat_surface_emission& q) {
+  SurfEmitDev d{};
+  d.see_model = q.see_model;
+  d.e_species = q.electron_species;
+  d.dmax = q.see_delta_max;
+  d.emax = q.see_energy_max_ev;
+  d.t_see = q.see_temperature_ev;
+  d.photo_j = q.photo_current_density;
+  d.t_ph = q.photo_temperature_ev;
+  d.v_ref = q.photo_reference_v;
+  return d;
+}
+
+void upload_seg_emit(scdat_ctx* c) {
+  std::vector<SurfEmitDev> v((size_t)c->n_seg() * kMaxGroups);
+  for (int sg = 0; sg < c->n_seg(); ++sg)
+    for (int g = 0; g < kMaxGroups; ++g) v[(size_t)sg * kMaxGroups + g] = emit_dev(c->seg_emit[sg][g]);
+  c->d_seg_emit.upload(v, c->stream);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->seg_emit_dirty = false;
+}
+
+// Per-scenario device tables (after the species are known: the charge unit
+// of scenario s is that of the reference weight times its multiplier).
+template <typename T>
+void upload_segments(scdat_ctx* c) {
+  if (!c->segmented()) return;
+  for (const auto& q : c->mcc)
+    if (q.target_density > 0.0) throw Error(SCDAT_ERR_STATE, "background collisions need a single scenario");
+  if (c->n_patch > 0) throw Error(SCDAT_ERR_STATE, "conductor patches need a single scenario");
+  for (const auto& q : c->emit)
+    if (q.photo_current_density > 0.0) throw Error(SCDAT_ERR_STATE, "photoemission needs a single scenario");
+  for (const auto& se : c->seg_emit)
+    for (const auto& q : se)
+      if (q.photo_current_density > 0.0) throw Error(SCDAT_ERR_STATE, "photoemission needs a single scenario");
+  const int ns = c->n_seg();
+  hipStream_t s = c->stream;
+  c->d_tet_seg.upload(c->tet_seg_h, s);
+  c->d_blk_seg.upload(c->blk_seg_h, s);
+  c->d_seg_b0.upload(c->seg_b0_h, s);
+  std::vector<double> unit(ns), qinit(ns, c->q_init);
+  std::vector<T> dt(ns);
+  for (int sg = 0; sg < ns; ++sg) {
+    unit[sg] = kElem * (c->w_ref * c->seg_wscale[sg]) / 4294967296.0;
+    dt[sg] = (T)c->seg_dt[sg];
+  }
+  c->d_seg_unit.upload(unit, s);
+  c->d_seg_dt.upload(dt, s);
+  c->d_seg_wscale.upload(c->seg_wscale, s);
+  c->d_seg_qinit.upload(qinit, s);
+  c->d_seg_key.upload(c->seg_key, s);
+  // conductor-row sums kc are sorted by node: split them by scenario
+  c->kc_off_h.assign((size_t)ns + 1, 0);
+  for (int sg = 0; sg <= ns; ++sg)
+    c->kc_off_h[sg] = std::lower_bound(c->h.kc_idx.begin(), c->h.kc_idx.end(), (int32_t)c->seg_node0[sg]) -
+                      c->h.kc_idx.begin();
+  c->d_kc_off.upload(c->kc_off_h, s);
+  c->seg_ccap.assign((size_t)ns, 0.0);
+  c->d_seg_ccap.upload(c->seg_ccap, s);
+  upload_seg_emit(c);
+}
+
 template <typename T>
 void do_initialize(scdat_ctx* c) {
   // Periodic partners.
@@ -1048,6 +1179,7 @@ void do_initialize(scdat_ctx* c) {
       for (int r = 0; r < 3; ++r) sh[3 * g + r] = (T)c->gshift[g][r];
     c->d_gshift.upload(sh, c->stream);
   }
+  upload_segments<T>(c);
   solve_phi1<T>(c);
   if (c->n_patch > 0) {
     c->patch_q.fill(0.0);
@@ -1190,6 +1322,12 @@ void post_push(scdat_ctx* c) {
   e.face_k = c->d_face_k.as<int32_t>();
   e.face_cum = c->d_face_cum.as<double>();
   e.vfloat = &c->d_sc.as<PcgScalars>()->vfloat;
+  if (c->segmented()) {
+    if (c->seg_emit_dirty) upload_seg_emit(c);
+    e.tet_seg = c->d_tet_seg.as<int32_t>();
+    e.gseg = c->d_seg_emit.as<SurfEmitDev>();
+    e.seg_key = c->d_seg_key.as<uint32_t>();
+  }
   if (c->event_cap > 0) launch_see<T>(e, c->event_cap, s);
   for (int g = 0; g < kMaxGroups; ++g) {
     const scdat_surface_emission& q = c->emit[g];
@@ -1256,7 +1394,7 @@ void do_step(scdat_ctx* c, int n) {
     se.d = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.d), c->stream));
     c->steps++;
-    c->sim_time += c->cfg.dt;
+    c->sim_time += c->segmented() ? c->seg_dt[0] : c->cfg.dt;
     if (c->cfg.sort_interval > 0 && c->steps % c->cfg.sort_interval == 0) do_sort<T>(c);
     se.e = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.e), c->stream));
@@ -1310,7 +1448,13 @@ void set_particles_t(scdat_ctx* c, int si, int64_t n, const double* x, const dou
       f[3 + r][i] = (T)v[3 * i + r];
     }
   std::vector<int32_t> id(n);
-  for (int64_t i = 0; i < n; ++i) id[i] = (int32_t)i;
+  if (c->segmented()) {
+    // ids count within each scenario, as in a context of that scenario alone
+    std::vector<int32_t> next((size_t)c->n_seg(), 0);
+    for (int64_t i = 0; i < n; ++i) id[i] = cl[i] >= 0 ? next[c->tet_seg_h[cl[i]]]++ : (int32_t)i;
+  } else {
+    for (int64_t i = 0; i < n; ++i) id[i] = (int32_t)i;
+  }
   S.cur = 0;
   for (int r = 0; r < 6; ++r)
     if (n) HIPCHK(hipMemcpyAsync(S.buf[0][r].p, f[r].data(), sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
@@ -1342,6 +1486,63 @@ void get_particles_t(scdat_ctx* c, int si, double* x, double* v, int32_t* cell, 
     }
   if (cell) std::memcpy(cell, cl.data(), 4 * n);
   if (id) std::memcpy(id, ids.data(), 4 * n);
+}
+
+// Device ledger, emission counts and energies of every scenario.
+struct LedgerHost {
+  std::vector<unsigned long long> cnt, emi;
+  std::vector<double> en;
+};
+LedgerHost read_ledger(scdat_ctx* c) {
+  const size_t tot = (size_t)kMaxGroups * kMaxSpecies * c->n_seg();
+  LedgerHost L;
+  L.cnt.resize(tot);
+  L.emi.resize(tot);
+  L.en.resize(tot);
+  HIPCHK(hipMemcpyAsync(L.cnt.data(), c->d_ledger_count.p, 8 * tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(L.emi.data(), c->d_emit_count.p, 8 * tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(L.en.data(), c->d_ledger_energy.p, 8 * tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return L;
+}
+double seg_scale(const scdat_ctx* c, int sg) { return c->segmented() ? c->seg_wscale[sg] : 1.0; }
+// Charge of one fixed-point unit at node i (its scenario's weight).
+double node_unit(const scdat_ctx* c, int64_t i) {
+  if (!c->segmented()) return c->unit;
+  const int sg = (int)(std::upper_bound(c->seg_node0.begin(), c->seg_node0.end(), i) - c->seg_node0.begin()) - 1;
+  return kElem * (c->w_ref * c->seg_wscale[sg]) / 4294967296.0;
+}
+
+// Ledger entries of scenarios [s0, s1) summed, PICCycle::getSurfaceCurrentLedger
+// (PICCycle.cpp:1295-1329) fields.
+void fill_ledger(scdat_ctx* c, const LedgerHost& L, int s0, int s1, scdat_ledger_entry* out, int n_groups) {
+  for (int g = 0; g < n_groups; ++g) std::memset(&out[g], 0, sizeof(out[g]));
+  for (int sg = s0; sg < s1; ++sg) {
+    const size_t lo = (size_t)sg * kMaxGroups * kMaxSpecies;
+    const double ws = seg_scale(c, sg);
+    for (int g = 0; g < n_groups; ++g) {
+      scdat_ledger_entry& E = out[g];
+      for (size_t sp = 0; sp < c->sp.size(); ++sp) {
+        const unsigned long long k = L.emi[lo + g * kMaxSpecies + sp];
+        E.emitted_electron_particles += (int64_t)k;
+        E.emitted_electron_charge_c += (double)k * c->sp[sp].q * (c->sp[sp].w * ws);
+      }
+      if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;  // escaped: not recorded
+      for (size_t sp = 0; sp < c->sp.size(); ++sp) {
+        const unsigned long long k = L.cnt[lo + g * kMaxSpecies + sp];
+        const double q = (double)k * c->sp[sp].q * (c->sp[sp].w * ws);
+        E.incident_energy_j += L.en[lo + g * kMaxSpecies + sp];
+        // electron-like = negative charge
+        if (c->sp[sp].q < 0) {
+          E.absorbed_electron_charge_c += q;
+          E.absorbed_electron_particles += (int64_t)k;
+        } else {
+          E.absorbed_ion_charge_c += q;
+          E.absorbed_ion_particles += (int64_t)k;
+        }
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -1486,6 +1687,8 @@ int scdat_set_mesh(scdat_ctx* c, int64_t n_nodes, const double* xyz, int64_t n_t
   return guarded(c, [&] {
     HostMesh& h = c->h;
     h = HostMesh();
+    c->seg_node0.clear();
+    c->seg_emit.clear();
     h.nn = n_nodes;
     h.nt = n_tets;
     h.xyz.assign(xyz, xyz + 3 * n_nodes);
@@ -1523,12 +1726,11 @@ int scdat_get_escaped(scdat_ctx* c, int32_t species, int64_t* n) {
   if (!c || !n) return SCDAT_ERR_ARG;
   if (species < 0 || species >= (int)c->sp.size()) return fail(c, SCDAT_ERR_ARG, "bad species");
   return guarded(c, [&] {
-    std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
-    HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    const LedgerHost L = read_ledger(c);
     int64_t e = 0;
-    for (int g = 0; g < kMaxGroups; ++g)
-      if (c->gtype[g] == SCDAT_BC_ESCAPE) e += (int64_t)cnt[g * kMaxSpecies + species];
+    for (int sg = 0; sg < c->n_seg(); ++sg)
+      for (int g = 0; g < kMaxGroups; ++g)
+        if (c->gtype[g] == SCDAT_BC_ESCAPE) e += (int64_t)L.cnt[((size_t)sg * kMaxGroups + g) * kMaxSpecies + species];
     *n = e;
   });
 }
@@ -1765,7 +1967,7 @@ int scdat_set_rho(scdat_ctx* c, const double* rho) {
   return guarded(c, [&] {
     const size_t nn = (size_t)c->h.nn;
     std::vector<long long> fx(nn);
-    for (size_t i = 0; i < nn; ++i) fx[i] = std::llround(rho[i] / c->unit);
+    for (size_t i = 0; i < nn; ++i) fx[i] = std::llround(rho[i] / node_unit(c, (int64_t)i));
     const int k = c->rho_target;
     HIPCHK(hipMemcpyAsync(c->d_rho[k].p, fx.data(), 8 * nn, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1784,7 +1986,7 @@ int scdat_get_fields(scdat_ctx* c, double* rho, double* phi, double* E) {
       std::vector<long long> fx(nn);
       HIPCHK(hipMemcpyAsync(fx.data(), c->d_rho[c->rho_last].p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
-      for (size_t i = 0; i < nn; ++i) rho[i] = (double)fx[i] * c->unit;
+      for (size_t i = 0; i < nn; ++i) rho[i] = (double)fx[i] * node_unit(c, (int64_t)i);
     }
     if (phi) {
       HIPCHK(hipMemcpyAsync(phi, c->d_phi64.p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
@@ -1871,27 +2073,33 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
       const size_t nn = (size_t)c->h.nn;
       std::vector<long long> fx(nn);
       HIPCHK(hipMemcpyAsync(fx.data(), c->d_rho[c->rho_last].p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
-      std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
-      HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
-      std::vector<unsigned long long> emi(kMaxGroups * kMaxSpecies);
-      HIPCHK(hipMemcpyAsync(emi.data(), c->d_emit_count.p, 8 * emi.size(), hipMemcpyDeviceToHost, c->stream));
+      const LedgerHost L = read_ledger(c);
       unsigned long long lost = 0, pushes = 0;
       HIPCHK(hipMemcpyAsync(&lost, c->d_lost.p, 8, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipMemcpyAsync(&pushes, c->d_pushes.p, 8, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipMemcpyAsync(c->h_sc, c->d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       long double tq = 0;
-      for (long long v : fx) tq += (long double)v;
-      st->total_charge = (double)(tq * (long double)c->unit);
+      for (int sg = 0; sg < c->n_seg(); ++sg) {
+        const int64_t i0 = c->segmented() ? c->seg_node0[sg] : 0;
+        const int64_t i1 = c->segmented() ? c->seg_node0[sg + 1] : (int64_t)nn;
+        long double t = 0;
+        for (int64_t i = i0; i < i1; ++i) t += (long double)fx[i];
+        tq += t * (long double)(c->unit * seg_scale(c, sg));
+      }
+      st->total_charge = (double)tq;
+      // surface charge of every scenario; conductor charge of the first
       double surf = 0, qc = c->q_init;
-      for (int g = 0; g < kMaxGroups; ++g)
-        for (size_t s = 0; s < c->sp.size(); ++s) {
-          if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;
-          const double q = (double)cnt[g * kMaxSpecies + s] * c->sp[s].q * c->sp[s].w;
-          surf += q;
-          if (c->gtype[g] == SCDAT_BC_CONDUCTOR)
-            qc += q - (double)emi[g * kMaxSpecies + s] * c->sp[s].q * c->sp[s].w;
-        }
+      for (int sg = 0; sg < c->n_seg(); ++sg)
+        for (int g = 0; g < kMaxGroups; ++g)
+          for (size_t s = 0; s < c->sp.size(); ++s) {
+            if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;
+            const size_t o = ((size_t)sg * kMaxGroups + g) * kMaxSpecies + s;
+            const double qw = c->sp[s].q * (c->sp[s].w * seg_scale(c, sg));
+            const double q = (double)L.cnt[o] * qw;
+            surf += q;
+            if (sg == 0 && c->gtype[g] == SCDAT_BC_CONDUCTOR) qc += q - (double)L.emi[o] * qw;
+          }
       st->surface_deposited_charge = surf;
       st->conductor_charge = qc;
       st->floating_potential = c->h.has_cond ? c->h_sc->vfloat : 0.0;
@@ -1910,39 +2118,7 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
 int scdat_get_ledger(scdat_ctx* c, scdat_ledger_entry* out, int32_t n_groups) {
   if (!c || !out || n_groups < 0 || n_groups > kMaxGroups) return SCDAT_ERR_ARG;
   if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "get_ledger: no mesh");
-  return guarded(c, [&] {
-    std::vector<unsigned long long> cnt(kMaxGroups * kMaxSpecies);
-    std::vector<double> en(kMaxGroups * kMaxSpecies);
-    std::vector<unsigned long long> emi(kMaxGroups * kMaxSpecies);
-    HIPCHK(hipMemcpyAsync(cnt.data(), c->d_ledger_count.p, 8 * cnt.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(en.data(), c->d_ledger_energy.p, 8 * en.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(emi.data(), c->d_emit_count.p, 8 * emi.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (int g = 0; g < n_groups; ++g) {
-      scdat_ledger_entry& L = out[g];
-      std::memset(&L, 0, sizeof(L));
-      for (size_t s = 0; s < c->sp.size(); ++s) {
-        const unsigned long long k = emi[g * kMaxSpecies + s];
-        L.emitted_electron_particles += (int64_t)k;
-        L.emitted_electron_charge_c += (double)k * c->sp[s].q * c->sp[s].w;
-      }
-      if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;  // escaped: not recorded
-      for (size_t s = 0; s < c->sp.size(); ++s) {
-        const unsigned long long k = cnt[g * kMaxSpecies + s];
-        const double q = (double)k * c->sp[s].q * c->sp[s].w;
-        L.incident_energy_j += en[g * kMaxSpecies + s];
-        // PICCycle::recordSurfaceBoundaryInteraction (PICCycle.cpp:1295-1329):
-        // electron-like = negative charge.
-        if (c->sp[s].q < 0) {
-          L.absorbed_electron_charge_c += q;
-          L.absorbed_electron_particles += (int64_t)k;
-        } else {
-          L.absorbed_ion_charge_c += q;
-          L.absorbed_ion_particles += (int64_t)k;
-        }
-      }
-    }
-  });
+  return guarded(c, [&] { fill_ledger(c, read_ledger(c), 0, c->n_seg(), out, n_groups); });
 }
 
 // ---- background collisions and surface emission -----------------------------
@@ -2040,6 +2216,8 @@ int scdat_set_surface_emission(scdat_ctx* c, int32_t group, const scdat_surface_
     return fail(c, SCDAT_ERR_ARG, "set_surface_emission: Sternglass needs energy_max > 0");
   return guarded(c, [&] {
     c->emit[group] = *e;
+    for (auto& se : c->seg_emit) se[group] = *e;
+    c->seg_emit_dirty = c->segmented();
     bool see = false;
     for (const auto& q : c->emit) see = see || q.see_model != SCDAT_SEE_OFF;
     if (see && c->event_cap == 0) {
@@ -2071,12 +2249,143 @@ int scdat_get_event_counts(scdat_ctx* c, scdat_event_counts* out) {
   });
 }
 
+// ---- batched scenarios ----------------------------------------------------------
+
+int scdat_set_scenarios(scdat_ctx* c, int32_t n, const int64_t* node_start, const double* weight_scale,
+                        const double* time_step, const uint64_t* seed) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_scenarios: set the mesh first");
+  if (n < 1 || !node_start) return fail(c, SCDAT_ERR_ARG, "set_scenarios: need n >= 1 and node_start");
+  for (const auto& S : c->sp)
+    if (S.n > 0) return fail(c, SCDAT_ERR_STATE, "set_scenarios: call before set_particles");
+  if (c->n_patch > 0) return fail(c, SCDAT_ERR_STATE, "set_scenarios: conductor patches need a single scenario");
+  return guarded(c, [&] {
+    const HostMesh& h = c->h;
+    const int64_t align = SCDAT_SEGMENT_ALIGN;
+    static_assert(SCDAT_SEGMENT_ALIGN == kBlock, "scenario rows align with the row blocks");
+    if (node_start[0] != 0 || node_start[n] != h.nn)
+      throw Error(SCDAT_ERR_ARG, "set_scenarios: node_start must run from 0 to the node count");
+    for (int sg = 0; sg < n; ++sg) {
+      if (node_start[sg + 1] <= node_start[sg]) throw Error(SCDAT_ERR_ARG, "set_scenarios: empty scenario");
+      if (node_start[sg] % align) throw Error(SCDAT_ERR_ARG, "set_scenarios: scenario starts must be multiples of 256");
+      if (weight_scale && !(weight_scale[sg] > 0.0)) throw Error(SCDAT_ERR_ARG, "set_scenarios: weight scale <= 0");
+      if (time_step && !(time_step[sg] > 0.0 && std::isfinite(time_step[sg])))
+        throw Error(SCDAT_ERR_ARG, "set_scenarios: bad time step");
+    }
+    std::vector<int32_t> tseg((size_t)h.nt);
+    for (int64_t t = 0; t < h.nt; ++t) {
+      const int32_t a = h.tets[4 * t];
+      const int sg = (int)(std::upper_bound(node_start, node_start + n + 1, (int64_t)a) - node_start) - 1;
+      for (int k = 1; k < 4; ++k) {
+        const int64_t b = h.tets[4 * t + k];
+        if (b < node_start[sg] || b >= node_start[sg + 1])
+          throw Error(SCDAT_ERR_ARG, "set_scenarios: tetrahedron " + std::to_string(t) + " spans two scenarios");
+      }
+      tseg[t] = sg;
+    }
+    const int64_t G = (h.nn + align - 1) / align;
+    std::vector<int32_t> bseg((size_t)G), b0((size_t)n + 1);
+    for (int sg = 0; sg <= n; ++sg) b0[sg] = (int32_t)((node_start[sg] + align - 1) / align);
+    for (int sg = 0; sg < n; ++sg)
+      for (int32_t b = b0[sg]; b < b0[sg + 1]; ++b) bseg[b] = sg;
+    c->seg_node0.assign(node_start, node_start + n + 1);
+    c->tet_seg_h = std::move(tseg);
+    c->blk_seg_h = std::move(bseg);
+    c->seg_b0_h = std::move(b0);
+    c->seg_wscale.assign((size_t)n, 1.0);
+    c->seg_dt.assign((size_t)n, c->cfg.dt);
+    c->seg_key.assign(2 * (size_t)n, 0);
+    for (int sg = 0; sg < n; ++sg) {
+      if (weight_scale) c->seg_wscale[sg] = weight_scale[sg];
+      if (time_step) c->seg_dt[sg] = time_step[sg];
+      c->seg_key[2 * sg] = seed ? (uint32_t)seed[sg] : c->key[0];
+      c->seg_key[2 * sg + 1] = seed ? (uint32_t)(seed[sg] >> 32) : c->key[1];
+    }
+    c->seg_emit.assign((size_t)n, c->emit);
+    const size_t m = (size_t)kMaxGroups * kMaxSpecies * n;
+    for (DevBuf* b : {&c->d_ledger_count, &c->d_ledger_energy, &c->d_emit_count}) {
+      b->alloc(8 * m);
+      HIPCHK(hipMemsetAsync(b->p, 0, 8 * m, c->stream));
+    }
+    c->d_sc.alloc(sizeof(PcgScalars) * ((size_t)n + 1));
+    HIPCHK(hipMemsetAsync(c->d_sc.p, 0, sizeof(PcgScalars) * ((size_t)n + 1), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->initialized = false;
+  });
+}
+
+int scdat_set_scenario_emission(scdat_ctx* c, int32_t scenario, int32_t group, const scdat_surface_emission* e) {
+  if (!c || !e) return SCDAT_ERR_ARG;
+  if (!c->segmented() || scenario < 0 || scenario >= c->n_seg())
+    return fail(c, SCDAT_ERR_ARG, "set_scenario_emission: no such scenario");
+  if (group < 0 || group >= kMaxGroups || e->see_model < SCDAT_SEE_OFF || e->see_model > SCDAT_SEE_STERNGLASS ||
+      e->electron_species < 0 || e->electron_species >= (int)c->sp.size() || c->sp[e->electron_species].q >= 0)
+    return fail(c, SCDAT_ERR_ARG, "set_scenario_emission: bad group, model or electron species");
+  if (e->see_model == SCDAT_SEE_STERNGLASS && !(e->see_energy_max_ev > 0.0))
+    return fail(c, SCDAT_ERR_ARG, "set_scenario_emission: Sternglass needs energy_max > 0");
+  if (e->photo_current_density > 0.0)
+    return fail(c, SCDAT_ERR_ARG, "set_scenario_emission: photoemission needs a single scenario");
+  return guarded(c, [&] {
+    c->seg_emit[scenario][group] = *e;
+    c->seg_emit_dirty = true;
+    if (e->see_model != SCDAT_SEE_OFF && c->event_cap == 0) {
+      int64_t cap = 0;
+      for (const auto& S : c->sp)
+        if (S.q < 0) cap += S.cap;
+      c->event_cap = std::max<int64_t>(65536, cap / 64);
+      c->d_events.alloc(sizeof(SeeEvent) * (size_t)c->event_cap);
+    }
+  });
+}
+
+int scdat_get_scenario_stats(scdat_ctx* c, int32_t scenario, scdat_scenario_stats* out) {
+  if (!c || !out) return SCDAT_ERR_ARG;
+  if (scenario < 0 || scenario >= c->n_seg()) return fail(c, SCDAT_ERR_ARG, "get_scenario_stats: no such scenario");
+  if (!c->initialized) return fail(c, SCDAT_ERR_STATE, "get_scenario_stats: not initialized");
+  return guarded(c, [&] {
+    PcgScalars q;
+    HIPCHK(hipMemcpyAsync(&q, c->d_sc.as<PcgScalars>() + scenario, sizeof(q), hipMemcpyDeviceToHost, c->stream));
+    const LedgerHost L = read_ledger(c);
+    std::memset(out, 0, sizeof(*out));
+    const double bn = std::sqrt(q.bnorm2);
+    out->completed_steps = c->steps;
+    out->simulation_time = (double)c->steps * (c->segmented() ? c->seg_dt[scenario] : c->cfg.dt);
+    out->floating_potential = c->h.has_cond ? q.vfloat : 0.0;
+    out->pcg_iterations = q.total_it;
+    out->pcg_converged = q.failed ? 0 : 1;
+    out->pcg_rel_residual = bn > 0 ? q.rnorm / bn : q.rnorm;
+    double qc = c->q_init;
+    const size_t lo = (size_t)scenario * kMaxGroups * kMaxSpecies;
+    for (int g = 0; g < kMaxGroups; ++g) {
+      if (c->gtype[g] != SCDAT_BC_CONDUCTOR) continue;
+      for (size_t sp = 0; sp < c->sp.size(); ++sp) {
+        const double qw = c->sp[sp].q * (c->sp[sp].w * seg_scale(c, scenario));
+        qc += ((double)L.cnt[lo + g * kMaxSpecies + sp] - (double)L.emi[lo + g * kMaxSpecies + sp]) * qw;
+      }
+    }
+    out->conductor_charge = qc;
+    out->time_step = c->segmented() ? c->seg_dt[scenario] : c->cfg.dt;
+    out->weight_scale = seg_scale(c, scenario);
+    out->first_node = c->segmented() ? c->seg_node0[scenario] : 0;
+    out->n_nodes = c->segmented() ? c->seg_node0[scenario + 1] - c->seg_node0[scenario] : c->h.nn;
+  });
+}
+
+int scdat_get_scenario_ledger(scdat_ctx* c, int32_t scenario, scdat_ledger_entry* out, int32_t n_groups) {
+  if (!c || !out || n_groups < 0 || n_groups > kMaxGroups) return SCDAT_ERR_ARG;
+  if (!c->has_mesh || scenario < 0 || scenario >= c->n_seg())
+    return fail(c, SCDAT_ERR_ARG, "get_scenario_ledger: no such scenario");
+  return guarded(c, [&] { fill_ledger(c, read_ledger(c), scenario, scenario + 1, out, n_groups); });
+}
+
 // ---- conductor patches driven by a surface circuit ----------------------------
 
 int scdat_set_conductor_circuit(scdat_ctx* c, int32_t n_patches, const int32_t* node_patch, scdat_circuit* circuit,
                                 const int32_t* patch_circuit_node, const int32_t* group_patch) {
   if (!c) return SCDAT_ERR_ARG;
   if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_conductor_circuit: set the mesh first");
+  if (c->segmented() && n_patches != 0)
+    return fail(c, SCDAT_ERR_STATE, "set_conductor_circuit: conductor patches need a single scenario");
   if (n_patches == 0) {
     c->n_patch = 0;
     c->circuit = nullptr;

@@ -261,7 +261,14 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
   if (e >= nev) return;
   const SeeEvent ev = a.events[e];
   const int g = ev.group_face & 0xFF, face = ev.group_face >> 8;
-  const SurfEmitDev& cf = a.g[g];
+  int sgm = 0;
+  uint32_t k0 = a.key0, k1 = a.key1;
+  if (a.tet_seg) {
+    sgm = a.tet_seg[ev.cell];
+    k0 = a.seg_key[2 * sgm];
+    k1 = a.seg_key[2 * sgm + 1];
+  }
+  const SurfEmitDev& cf = a.tet_seg ? a.gseg[sgm * kMaxGroups + g] : a.g[g];
   const TetMap<T> m = a.tets[ev.cell];
   double n[3];
   inward_normal(m, face, n);
@@ -270,7 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
   const double E = 0.5 * a.mass[ev.species] * v2 / kE;
   const double cosi = sp > 0.0 ? fabs(ev.v[0] * n[0] + ev.v[1] * n[1] + ev.v[2] * n[2]) / sp : 1.0;
   const double dl = see_yield(cf.see_model, cf.dmax, cf.emax, E, cosi);
-  const U4 r = philox4x32((uint32_t)ev.id, a.step, (uint32_t)ev.species, kTagSee, a.key0, a.key1);
+  const U4 r = philox4x32((uint32_t)ev.id, a.step, (uint32_t)ev.species, kTagSee, k0, k1);
   const double fl = floor(dl);
   int nem = (int)fmin(fl, (double)kMaxEmit) + (u01(r.x) < dl - fl ? 1 : 0);
   nem = nem < kMaxEmit ? nem : kMaxEmit;
@@ -282,13 +289,13 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
   int made = 0;
   for (int k = 0; k < nem; ++k) {
     const uint32_t sk = ((uint32_t)ev.species << 8) | (uint32_t)k;
-    const U4 b = philox4x32((uint32_t)ev.id, a.step, sk, kTagSeeK, a.key0, a.key1);
-    const U4 c = philox4x32((uint32_t)ev.id, a.step, sk, kTagSeeId, a.key0, a.key1);
+    const U4 b = philox4x32((uint32_t)ev.id, a.step, sk, kTagSeeK, k0, k1);
+    const U4 c = philox4x32((uint32_t)ev.id, a.step, sk, kTagSeeId, k0, k1);
     double v[3];
     emit_velocity(n, cf.t_see, o.m, u01(b.x), u01(b.y), u01(b.z), u01(b.w), v);
     made += append(o, xt, v, ev.cell, gen_id(c.x), a.tets, a.rho, a.counts) ? 1 : 0;
   }
-  atomicAdd(&a.emit_count[g * kMaxSpecies + cf.e_species], (unsigned long long)made);
+  atomicAdd(&a.emit_count[((size_t)sgm * kMaxGroups + g) * kMaxSpecies + cf.e_species], (unsigned long long)made);
   atomicAdd(&a.counts[C_SEE], (unsigned long long)made);
 }
 

@@ -86,6 +86,12 @@ struct EmitArgs {
   int group;
   const double* vfloat;  // floating potential of the last solve
   int is_conductor;
+  // batched scenarios (null: one scenario): segment of each tet, emission
+  // settings [segment][kMaxGroups], random key (2 words) per segment;
+  // emit_count is then [segment][kMaxGroups][kMaxSpecies]
+  const int32_t* tet_seg;
+  const SurfEmitDev* gseg;
+  const uint32_t* seg_key;
 };
 
 template <typename T>

@@ -53,6 +53,24 @@ enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_COUNT 
 
 __device__ inline double* slot(const FieldBufs& f, int s, int G) { return f.partial + (size_t)s * G; }
 
+// Row blocks [b0, b1) of the segment that block `blk` belongs to, and its
+// scalars.  Sums over a segment run over its own blocks in the same order
+// as a one-segment solve of that segment alone, so a batched scenario
+// reproduces its stand-alone solve bit for bit.
+struct SegRange {
+  int s, b0, b1;
+};
+__device__ inline SegRange seg_of_block(const FieldBufs& f, int blk, int G) {
+  if (!f.blk_seg) return {0, 0, G};
+  const int s = f.blk_seg[blk];
+  return {s, f.seg_b0[s], f.seg_b0[s + 1]};
+}
+// segment handled by block `blk` of a one-block-per-segment kernel
+__device__ inline SegRange seg_by_index(const FieldBufs& f, int s, int G) {
+  if (!f.blk_seg) return {0, 0, G};
+  return {s, f.seg_b0[s], f.seg_b0[s + 1]};
+}
+
 // Sum of G partials, identical in every block; result broadcast to all threads.
 __device__ inline double sum_partials(const double* p, int G, double* lds) {
   double x = 0.0;
@@ -74,6 +92,7 @@ __global__ void __launch_bounds__(kPB) k_rhs(EllDev A, FieldBufs f,
                                                  int load_sign) {
   __shared__ double lds[kPB / 64];
   const int G = gridDim.x;
+  if (f.seg_unit) unit = f.seg_unit[seg_of_block(f, blockIdx.x, G).s];
   double bb = 0.0, cl = 0.0;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
   for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
@@ -93,12 +112,14 @@ __global__ void __launch_bounds__(kPB) k_rhs(EllDev A, FieldBufs f,
   publish(cl, slot(f, S_CL, G), lds);
 }
 
+// one block per segment
 __global__ void __launch_bounds__(kPB) k_fin_rhs(FieldBufs f, int G) {
   __shared__ double lds[kPB / 64];
-  const double bb = sum_partials(slot(f, S_BB, G), G, lds);
-  const double cl = sum_partials(slot(f, S_CL, G), G, lds);
+  const SegRange sg = seg_by_index(f, blockIdx.x, G);
+  const double bb = sum_partials(slot(f, S_BB, G) + sg.b0, sg.b1 - sg.b0, lds);
+  const double cl = sum_partials(slot(f, S_CL, G) + sg.b0, sg.b1 - sg.b0, lds);
   if (threadIdx.x == 0) {
-    PcgScalars* sc = f.sc;
+    PcgScalars* sc = f.sc + sg.s;
     sc->bnorm2 = bb;
     sc->cond_load = cl;
     sc->outer_done = 0;
@@ -110,9 +131,9 @@ __global__ void __launch_bounds__(kPB) k_fin_rhs(FieldBufs f, int G) {
 }
 
 __global__ void __launch_bounds__(kPB) k_resid64(EllDev A, FieldBufs f) {
-  if (f.sc->outer_done) return;
-  __shared__ double lds[kPB / 64];
   const int G = gridDim.x;
+  if (f.sc[seg_of_block(f, blockIdx.x, G).s].outer_done) return;
+  __shared__ double lds[kPB / 64];
   double rr = 0.0;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
   for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
@@ -128,12 +149,14 @@ __global__ void __launch_bounds__(kPB) k_resid64(EllDev A, FieldBufs f) {
   publish(rr, slot(f, S_RR, G), lds);
 }
 
+// one block per segment
 __global__ void __launch_bounds__(kPB) k_fin_resid(FieldBufs f, int G) {
-  if (f.sc->outer_done) return;
+  const SegRange sg = seg_by_index(f, blockIdx.x, G);
+  if (f.sc[sg.s].outer_done) return;
   __shared__ double lds[kPB / 64];
-  const double rr = sum_partials(slot(f, S_RR, G), G, lds);
+  const double rr = sum_partials(slot(f, S_RR, G) + sg.b0, sg.b1 - sg.b0, lds);
   if (threadIdx.x == 0) {
-    PcgScalars* sc = f.sc;
+    PcgScalars* sc = f.sc + sg.s;
     const double rn = sqrt(rr);
     const double bn = sqrt(sc->bnorm2);
     sc->rnorm = rn;
@@ -167,8 +190,9 @@ __global__ void __launch_bounds__(kPB) k_fin_resid(FieldBufs f, int G) {
 // double-buffered by iteration parity; p and x are owner-only, in place.
 template <typename T>
 __global__ void __launch_bounds__(kPB) k_inner_init(EllDev A, FieldBufs f, InnerBufs<T> ib) {
-  if (f.sc->outer_done) return;
-  const double inv = f.sc->inv_rnorm;
+  const PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
+  if (sc->outer_done) return;
+  const double inv = sc->inv_rnorm;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
   for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
     ib.r[0][i] = (T)(f.r64[i] * inv);
@@ -217,9 +241,9 @@ __device__ inline T r_next(T r, T w, T s, T a, T b) {
 // w_0 = (A D^-1) r_0 and the first dot products.
 template <typename T>
 __global__ void __launch_bounds__(kPB) k_cg_start(EllDev A, FieldBufs f, InnerBufs<T> ib) {
-  if (f.sc->outer_done) return;
-  __shared__ double lds[3 * (kPB / 64)];
   const int G = gridDim.x;
+  if (f.sc[seg_of_block(f, blockIdx.x, G).s].outer_done) return;
+  __shared__ double lds[3 * (kPB / 64)];
   const T* __restrict__ v = vhat<T>(A);
   const T* __restrict__ r = ib.r[0];
   double g = 0.0, d = 0.0, rr = 0.0;
@@ -267,13 +291,15 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
       }
     }
   }
-  // g_i, d_i, rr_i from the previous kernel's block partials
+  // g_i, d_i, rr_i from the previous kernel's block partials of this segment
+  const SegRange sg = seg_of_block(f, blockIdx.x, G);
   {
-    const double* pg = slot(f, cur ? S_G1 : S_G0, G);
-    const double* pd = slot(f, cur ? S_D1 : S_D0, G);
-    const double* pr = slot(f, cur ? S_RR1 : S_RR0, G);
+    const double* pg = slot(f, cur ? S_G1 : S_G0, G) + sg.b0;
+    const double* pd = slot(f, cur ? S_D1 : S_D0, G) + sg.b0;
+    const double* pr = slot(f, cur ? S_RR1 : S_RR0, G) + sg.b0;
+    const int nb = sg.b1 - sg.b0;
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int j = threadIdx.x; j < G; j += kPB) {
+    for (int j = threadIdx.x; j < nb; j += kPB) {
       a += pg[j];
       b += pd[j];
       c += pr[j];
@@ -307,7 +333,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
     __syncthreads();
   }
   const double g = tot[0], d = tot[1], rr = tot[2];
-  PcgScalars* sc = f.sc;
+  PcgScalars* sc = f.sc + sg.s;
   const double tol = sc->inner_tol_eff;
   double beta = 0.0, alpha;
   if (it > 0) {
@@ -321,7 +347,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
   const bool done = sc->inner_done || sc->outer_done || (it > 0 && rr <= tol * tol) ||
                     (sc->total_it + it >= sc->max_it) || !(alpha > 0.0) || !(alpha < 1e300);
   if (done) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->inner_done = 1;
+    if ((int)blockIdx.x == sg.b0 && threadIdx.x == 0) sc->inner_done = 1;
     return;
   }
   const T a = (T)alpha, b = (T)beta;
@@ -364,7 +390,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
   }
   publish3(g1, d1, rr1, slot(f, nxt ? S_G1 : S_G0, G), slot(f, nxt ? S_D1 : S_D0, G),
            slot(f, nxt ? S_RR1 : S_RR0, G), lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if ((int)blockIdx.x == sg.b0 && threadIdx.x == 0) {
     sc->hist[cur] = g;
     sc->hist[2 + cur] = alpha;
     sc->inner_it = it + 1;
@@ -375,37 +401,93 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
 // x += |r| d (skipped when the outer loop has already converged).
 template <typename T>
 __global__ void __launch_bounds__(kPB) k_apply(EllDev A, FieldBufs f, InnerBufs<T> ib) {
-  if (f.sc->outer_done) return;
-  const double rn = f.sc->rnorm;
+  const PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
+  if (sc->outer_done) return;
+  const double rn = sc->rnorm;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
   for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB)
     f.x64[i] = fma(rn, (double)ib.x[i], f.x64[i]);
 }
 
-__global__ void k_count_outer(FieldBufs f) {
-  if (f.sc->outer_done) return;
-  f.sc->outer_it += 1;
-  f.sc->total_it += f.sc->inner_it;
+// one thread per segment
+__global__ void k_count_outer(FieldBufs f, int n_seg) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_seg) return;
+  PcgScalars* sc = f.sc + s;
+  if (sc->outer_done) return;
+  sc->outer_it += 1;
+  sc->total_it += sc->inner_it;
 }
 
-// Floating potential from Gauss's law on the conductor (one block).
+// Host-polled summary of the segments: done when every segment is done,
+// largest iteration counts, largest relative residual (as rnorm, bnorm2 = 1).
+__global__ void __launch_bounds__(kPB) k_seg_summary(FieldBufs f) {
+  __shared__ int flags[3];
+  __shared__ int mx[3];
+  __shared__ double lds[kPB / 64];
+  if (threadIdx.x == 0) {
+    flags[0] = 1;
+    flags[1] = 1;
+    flags[2] = 0;
+    mx[0] = mx[1] = mx[2] = 0;
+  }
+  __syncthreads();
+  double rel = 0.0;
+  for (int s = threadIdx.x; s < f.n_seg; s += kPB) {
+    const PcgScalars& q = f.sc[s];
+    if (!q.outer_done) atomicAnd(&flags[0], 0);
+    if (!q.inner_done) atomicAnd(&flags[1], 0);
+    if (q.failed) atomicOr(&flags[2], 1);
+    atomicMax(&mx[0], q.inner_it);
+    atomicMax(&mx[1], q.total_it);
+    atomicMax(&mx[2], q.outer_it);
+    const double bn = sqrt(q.bnorm2);
+    rel = fmax(rel, q.rnorm / (bn > 0.0 ? bn : 1.0));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) rel = fmax(rel, __shfl_xor(rel, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = rel;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kPB / 64; ++w) rel = fmax(rel, lds[w]);
+    rel = fmax(rel, lds[0]);
+    PcgScalars* o = f.sc + f.n_seg;
+    o->outer_done = flags[0];
+    o->inner_done = flags[1];
+    o->failed = flags[2];
+    o->inner_it = mx[0];
+    o->total_it = mx[1];
+    o->outer_it = mx[2];
+    o->rnorm = rel;
+    o->bnorm2 = 1.0;
+  }
+}
+
+// Floating potential from Gauss's law on the conductor (one block per segment).
 __global__ void __launch_bounds__(kBlock) k_float(FieldBufs f, FloatArgs fa) {
   __shared__ double lds[kBlock / 64];
+  const int sg = blockIdx.x;
+  const int64_t j0 = f.kc_off ? f.kc_off[sg] : 0, j1 = f.kc_off ? f.kc_off[sg + 1] : f.n_kc;
   double s = 0.0;
-  for (int64_t j = threadIdx.x; j < f.n_kc; j += blockDim.x) s += f.kc_val[j] * f.x64[f.kc_idx[j]];
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) s += f.kc_val[j] * f.x64[f.kc_idx[j]];
   const double tot = block_sum(s, lds);
   if (threadIdx.x == 0) {
-    PcgScalars* sc = f.sc;
-    double q = fa.q_init;
+    PcgScalars* sc = f.sc + sg;
+    const size_t lo = (size_t)sg * kMaxGroups * kMaxSpecies;
+    const double ws = fa.seg_wscale ? fa.seg_wscale[sg] : 1.0;
+    double q = fa.seg_qinit ? fa.seg_qinit[sg] : fa.q_init;
     for (int g = 0; g < kMaxGroups; ++g) {
       if (!((fa.cond_mask >> g) & 1u)) continue;
       for (int sp = 0; sp < kMaxSpecies; ++sp) {
-        q += (double)fa.ledger_count[g * kMaxSpecies + sp] * fa.qw[sp];
-        if (fa.emit_count) q -= (double)fa.emit_count[g * kMaxSpecies + sp] * fa.qw[sp];
+        const double qw = fa.qw[sp] * ws;
+        q += (double)fa.ledger_count[lo + g * kMaxSpecies + sp] * qw;
+        if (fa.emit_count) q -= (double)fa.emit_count[lo + g * kMaxSpecies + sp] * qw;
       }
     }
+    const double cap = fa.seg_ccap ? fa.seg_ccap[sg] : sc->ccap;
+    sc->ccap = cap;
     sc->qcond = q;
-    sc->vfloat = sc->ccap > 0.0 ? (q + sc->cond_load - tot) / sc->ccap : 0.0;
+    sc->vfloat = cap > 0.0 ? (q + sc->cond_load - tot) / cap : 0.0;
   }
 }
 
@@ -418,7 +500,8 @@ __global__ void __launch_bounds__(kBlock) k_phi(int64_t n, FieldBufs f, T* phiT,
   if (f.n_basis > 0) {
     for (int k = 0; k < f.n_basis; ++k) p = fma(f.coef[k], f.basis[(int64_t)k * n + i], p);
   } else if (f.phi1) {
-    p = fma(f.sc->vfloat, f.phi1[i], p);
+    // kBlock rows per block, the PCG row blocks
+    p = fma(f.sc[f.blk_seg ? f.blk_seg[lb] : 0].vfloat, f.phi1[i], p);
   }
   phi64[i] = p;
   phiT[i] = (T)p;
@@ -472,12 +555,12 @@ void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* r
                 double unit, int load_sign, hipStream_t s) {
   const int G = grid_rows(A.n);
   hipLaunchKernelGGL(k_rhs, dim3(G), dim3(kPB), 0, s, A, f, rho_fx, unit, load_sign);
-  hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(kPB), 0, s, f, G);
+  hipLaunchKernelGGL(k_fin_rhs, dim3(f.blk_seg ? f.n_seg : 1), dim3(kPB), 0, s, f, G);
 }
 void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s) {
   const int G = grid_rows(A.n);
   hipLaunchKernelGGL(k_resid64, dim3(G), dim3(kPB), 0, s, A, f);
-  hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(kPB), 0, s, f, G);
+  hipLaunchKernelGGL(k_fin_resid, dim3(f.blk_seg ? f.n_seg : 1), dim3(kPB), 0, s, f, G);
 }
 template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib,
@@ -505,11 +588,16 @@ void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& 
 template <typename T>
 void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s) {
   hipLaunchKernelGGL(k_apply<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
-  hipLaunchKernelGGL(k_count_outer, dim3(1), dim3(1), 0, s, f);
+  const int ns = f.blk_seg ? f.n_seg : 1;
+  hipLaunchKernelGGL(k_count_outer, dim3((ns + kPB - 1) / kPB), dim3(kPB), 0, s, f, ns);
 }
 size_t partial_doubles(int64_t n) { return (size_t)S_COUNT * (size_t)grid_rows(n) + 16; }
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s) {
-  hipLaunchKernelGGL(k_float, dim3(1), dim3(kBlock), 0, s, f, fa);
+  hipLaunchKernelGGL(k_float, dim3(f.blk_seg ? f.n_seg : 1), dim3(kBlock), 0, s, f, fa);
+}
+void launch_seg_summary(const FieldBufs& f, hipStream_t s) {
+  if (!f.blk_seg) return;
+  hipLaunchKernelGGL(k_seg_summary, dim3(1), dim3(kPB), 0, s, f);
 }
 template <typename T>
 void launch_phi_efield(const EllDev& A, const FieldBufs& f, T* phiT, double* phi64, Vec4<T>* E,

@@ -61,6 +61,16 @@ struct FieldBufs {
   int n_basis;
   const double* basis;      // n_basis x n, patch k at 1 V, everything else 0
   const double* coef;       // n_basis potentials (device)
+  // Independent segments (batched scenarios).  The rows of segment s are the
+  // row blocks [seg_b0[s], seg_b0[s+1]) and blk_seg[b] names the segment of
+  // block b; sc[s], the charge unit and the conductor sums are per segment,
+  // and sc[n_seg] is a summary the host polls (all done, largest counts).
+  // blk_seg null: one segment over all rows, sc[0] only.
+  int n_seg;
+  const int32_t* blk_seg;
+  const int32_t* seg_b0;
+  const double* seg_unit;
+  const int64_t* kc_off;     // n_seg + 1 offsets into kc_idx / kc_val
 };
 
 // FP32 (mixed) or FP64 inner-solve vectors: r, w = A D^-1 r and s = A p by
@@ -93,12 +103,19 @@ void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, h
 // absorbed count * q_s w_s; floating potential from Gauss's law.
 struct FloatArgs {
   double q_init;
-  const unsigned long long* ledger_count;  // [kMaxGroups][kMaxSpecies]
-  const unsigned long long* emit_count;    // [kMaxGroups][kMaxSpecies] or null
+  const unsigned long long* ledger_count;  // [segment][kMaxGroups][kMaxSpecies]
+  const unsigned long long* emit_count;    // [segment][kMaxGroups][kMaxSpecies] or null
   double qw[kMaxSpecies];
   uint32_t cond_mask;
+  // per segment (null without segments): initial charge, weight multiplier
+  // of every species' qw, capacitance sum_C K phi_1
+  const double* seg_qinit;
+  const double* seg_wscale;
+  const double* seg_ccap;
 };
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s);
+// Summary of the per-segment scalars into sc[n_seg] (no-op for one segment).
+void launch_seg_summary(const FieldBufs& f, hipStream_t s);
 template <typename T>
 void launch_phi_efield(const EllDev& A, const FieldBufs& f, T* phiT, double* phi64, Vec4<T>* E,
                        unsigned long long* rho_clear, hipStream_t s);

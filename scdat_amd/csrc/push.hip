@@ -230,7 +230,14 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
       T v[3] = {a.vx[i], a.vy[i], a.vz[i]};
       const T v0[3] = {v[0], v[1], v[2]};
       TetMap<T> m = a.tets[c];
+      int sg = 0;
       T remaining = a.dt;
+      double wt = a.weight;
+      if (a.tet_seg) {
+        sg = a.tet_seg[c];
+        remaining = a.seg_dt[sg];
+        wt = a.weight * a.seg_wscale[sg];
+      }
       int sub = 0;
       int dead = 0;
       int group = -1, face = 0, last = c;
@@ -267,8 +274,15 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
           const int g = (group >= 0 && group < kMaxGroups) ? group : 0;
           const double k = 0.5 * (double)a.mass *
                            ((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]);
-          atomicAdd(&led_n[g], 1u);
-          atomicAdd(&led_e[g], k * a.energy_weight);
+          if (a.tet_seg) {
+            // scenarios may share a block: straight to the scenario's slots
+            const size_t o = ((size_t)sg * kMaxGroups + g) * kMaxSpecies + a.species;
+            atomicAdd(&a.ledger_count[o], 1ull);
+            unsafeAtomicAdd(&a.ledger_energy[o], k * fmax(1.0, wt));
+          } else {
+            atomicAdd(&led_n[g], 1u);
+            atomicAdd(&led_e[g], k * a.energy_weight);
+          }
           // impact recorded for secondary emission after the push
           if ((a.see_mask >> g) & 1u) {
             const unsigned long long e = atomicAdd(a.n_events, 1ull);
@@ -298,7 +312,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
         lambda_fixed(lam, f);
         key = c;
         alive += 1;
-        ke += 0.5 * (double)a.mass * (double)a.weight *
+        ke += 0.5 * (double)a.mass * wt *
               ((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]);
         a.x[i] = x[0];
         a.y[i] = x[1];

@@ -52,6 +52,12 @@ struct PushArgs {
   SeeEvent* events;
   unsigned long long* n_events;
   int64_t event_cap;
+  // batched scenarios: segment of each tet, time step and weight multiplier
+  // per segment; ledger slots [segment][kMaxGroups][kMaxSpecies] (null: one
+  // scenario, a.dt and a.weight)
+  const int32_t* tet_seg;
+  const T* seg_dt;
+  const double* seg_wscale;
 };
 
 // blocks of one push/deposit launch over n particle slots (one partial each)
