@@ -14,6 +14,14 @@ events, floating-conductor Poisson solve and nodal E (PICCycle order).
 value = particles pushed / wall time of the timed steps.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
+
+--workload sweep runs the batched charging sweep instead: --scenarios cases
+(default 256) with mesh edge 8..32 hexes and 2e4..2e6 particles per species
+(log-uniform), Te = Ti log-uniform in 1 eV..20 keV, n log-uniform in
+1e6..1e11 m^-3, Sternglass SEE delta_max uniform in 1..3 on the conductor,
+FP32 with FP64 refinement; all cases advance together in one context, and
+the same cases are then run one after another in their own contexts.
+value = particle pushes of all cases / wall time of the batched steps.
 --copy-calibration MB runs only a plain device copy of MB megabytes (the
 reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
 """
@@ -128,6 +136,76 @@ def run_cpu_baseline(case, steps, threads):
     return dict(wall=wall, pushes=pushes, threads=oracle.lib().orc_num_threads())
 
 
+def run_sweep(args):
+    import scdat_amd as S
+    import scdat_amd.batch as B
+    name, _ = S.device_info()
+    t0 = time.time()
+    sweep = B.sweep(args.scenarios, args.sweep_seed)
+    cases = [B.case_inputs(sc) for sc in sweep]
+    n_part = sum(2 * sc["n_particles"] for sc in sweep)
+    log(f"sweep: {len(sweep)} cases, {n_part} particles, "
+        f"{sum(len(c['mesh']['tets']) for c in cases)} tets, inputs in {time.time() - t0:.1f}s")
+
+    def timed(ctx):
+        for _ in range(args.warmup):
+            ctx.step(1)
+        ctx.sync()
+        p0 = ctx.stats()["particle_pushes"]
+        t1 = time.perf_counter()
+        ctx.step(args.steps)
+        ctx.sync()
+        wall = time.perf_counter() - t1
+        return wall, ctx.stats()["particle_pushes"] - p0
+
+    t0 = time.time()
+    bc, u = B.batch_context(sweep, cases, precision=0, tol=1e-8, sort_interval=args.sort_interval)
+    log(f"batched setup {time.time() - t0:.1f}s")
+    bwall, bpush = timed(bc)
+    stages = bc.stage_times()
+    conv = [bc.scenario_stats(k) for k in range(len(sweep))]
+    del bc
+    log(f"batched: {args.steps} steps {bwall:.3f}s, pushes {bpush}, stages {stages}")
+    swall, spush = 0.0, 0
+    t0 = time.time()
+    for k, (sc, st) in enumerate(zip(sweep, cases)):
+        c = B.standalone_context(sc, st, precision=0, tol=1e-8, sort_interval=args.sort_interval)
+        w, p = timed(c)
+        swall += w
+        spush += p
+        del c
+        if k % 32 == 31:
+            log(f"sequential: {k + 1} cases, {swall:.3f}s timed so far")
+    log(f"sequential: {args.steps} steps per case {swall:.3f}s, pushes {spush}, total {time.time() - t0:.1f}s")
+    value = bpush / bwall
+    res = {
+        "metric": "batched charging sweep: particle-steps/s over all cases; speed-up vs the cases run one by one",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * bwall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (push/SpMV) + f64 (refinement, dots, conductor)",
+        "data": "synthetic Maxwellian plasmas (sweep seed %d)" % args.sweep_seed,
+        "config": {"workload": "%d charging cases, box 1m with floating 0.2m conductor, edge 8..32 hexes, "
+                               "2e4..2e6 particles per species, Te 1eV..20keV, n 1e6..1e11, Sternglass SEE "
+                               "delta_max 1..3, PCG 1e-8; %d nodes, %d particles in all"
+                               % (len(sweep), int(u["node_start"][-1]), n_part)},
+        "device": name,
+        "sequential": {"value": spush / swall, "ms_per_step": 1e3 * swall / args.steps,
+                       "note": "the same cases, each in its own context, one after another (timed steps only)"},
+        "speedup_vs_sequential": value / (spush / swall),
+        "stage_ms_total": stages,
+        "cases_converged": int(sum(1 for q in conv if q["pcg_converged"])),
+        "pcg_iterations_max": int(max(q["pcg_iterations"] for q in conv)),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def copy_calibration(mb):
     import torch
     n = int(mb * 1e6) // 4
@@ -154,12 +232,18 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no per-launch events in the timed region (no roofline figures)")
     ap.add_argument("--copy-calibration", type=float, default=0.0)
+    ap.add_argument("--workload", choices=("box", "sweep"), default="box")
+    ap.add_argument("--scenarios", type=int, default=256)
+    ap.add_argument("--sweep-seed", type=int, default=20261206)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "sweep":
+        run_sweep(args)
         return
 
     import scdat_amd as S
