@@ -445,6 +445,45 @@ int scdat_get_scenario_stats(scdat_ctx* ctx, int32_t scenario, scdat_scenario_st
 /* getSurfaceCurrentLedger of one scenario (scdat_get_ledger sums them). */
 int scdat_get_scenario_ledger(scdat_ctx* ctx, int32_t scenario, scdat_ledger_entry* out, int32_t n_groups);
 
+/* ---- Discrete adjoint of the charging cycle -------------------------------
+ * Objective: the floating conductor's potential after `steps` collisionless
+ * steps from the initialized state (before any scdat_step).  Forward model
+ * per step: B = 0 push (one sub-step), FP64 charge deposition, and the
+ * conductor charge
+ *   Q_{n+1} = Q_n + sum over conductor hits (q w + [electrons] e w delta)
+ *             + dt (J_ph(V_n) A_sunlit - G V_n),
+ * delta = see_delta_max x face_yield[face] x the Sternglass shape
+ * (E/E_max) exp(2 - 2 sqrt(E/E_max)) of the impact energy, J_ph =
+ * J0 exp(-max(0, V - V_ref) / T_ph) (SurfaceMaterialModel.cpp:199-208) on
+ * the sunlit faces, G the conductance from the conductor to the 0 V walls:
+ * secondary and photoelectrons leave as surface current, so the yields
+ * enter continuously.  The adjoint runs the transposed steps backwards
+ * (Poisson backward = one PCG solve with the same SPD operator, push
+ * backward from states recomputed per checkpoint segment) and returns
+ * dV/d(delta_max, J0, G, face_yield[0..5]).  The context's fields and
+ * particles are left as they were.  No counterpart in the reference, which
+ * integrates surface charging without gradients.                         */
+typedef struct {
+  double see_delta_max;          /* Sternglass delta_max                          */
+  double see_energy_max_ev;      /* Sternglass E_max                              */
+  double face_yield[6];          /* delta_max multiplier of conductor face k (group 6 + k) */
+  double photo_current_density;  /* J0 (A/m^2)                                    */
+  double photo_temperature_ev, photo_reference_v;
+  int32_t sunlit_faces;          /* bit k: conductor face k is sunlit             */
+  double surface_conductance_s;  /* conductor -> outer walls                      */
+} scdat_charging_params;
+#define SCDAT_GRAD_DELTA_MAX 0
+#define SCDAT_GRAD_PHOTO_J 1
+#define SCDAT_GRAD_CONDUCTANCE 2
+#define SCDAT_GRAD_FACE_YIELD 3 /* .. 8 */
+#define SCDAT_N_GRAD 9
+/* delta_max 2 at 400 eV, yields 1, 20 uA/m^2 at 2 eV on the XMax face, G 0 */
+void scdat_charging_params_default(scdat_charging_params* p);
+int scdat_charging_forward(scdat_ctx* ctx, const scdat_charging_params* p, int32_t steps, double* v_final);
+/* grad[SCDAT_N_GRAD]; states are kept every checkpoint_interval steps. */
+int scdat_charging_adjoint(scdat_ctx* ctx, const scdat_charging_params* p, int32_t steps, int32_t checkpoint_interval,
+                           double* v_final, double* grad);
+
 /* ---- Conductor patches on a surface circuit ------------------------------
  * The conductor's nodes split into patches (node_patch[n], -1 off the
  * conductor; NULL: the six faces, boundary groups 6..11 -> patches 0..5,

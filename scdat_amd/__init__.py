@@ -115,6 +115,39 @@ class ScenarioStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+GRAD_DELTA_MAX, GRAD_PHOTO_J, GRAD_CONDUCTANCE, GRAD_FACE_YIELD, N_GRAD = 0, 1, 2, 3, 9
+
+
+class ChargingParams(C.Structure):
+    _fields_ = [("see_delta_max", C.c_double), ("see_energy_max_ev", C.c_double), ("face_yield", C.c_double * 6),
+                ("photo_current_density", C.c_double), ("photo_temperature_ev", C.c_double),
+                ("photo_reference_v", C.c_double), ("sunlit_faces", C.c_int32),
+                ("surface_conductance_s", C.c_double)]
+
+    def vector(self):
+        """(delta_max, J0, G, face_yield[6]) in gradient order."""
+        return np.array([self.see_delta_max, self.photo_current_density, self.surface_conductance_s]
+                        + list(self.face_yield))
+
+    def set_vector(self, theta):
+        self.see_delta_max, self.photo_current_density, self.surface_conductance_s = (float(t) for t in theta[:3])
+        for k in range(6):
+            self.face_yield[k] = float(theta[3 + k])
+        return self
+
+
+def charging_params(**kw):
+    p = ChargingParams()
+    lib().scdat_charging_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "face_yield":
+            for i in range(6):
+                p.face_yield[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
 class EventCounts(C.Structure):
     _fields_ = [("collisions", C.c_int64 * MAX_MCC_PROCESSES), ("created", C.c_int64), ("removed", C.c_int64),
                 ("see_emitted", C.c_int64), ("photo_emitted", C.c_int64), ("overflow", C.c_int64),
@@ -190,6 +223,9 @@ SIGNATURES = {
     "scdat_circuit_size": (C.c_int, [_P, _I32, _I32]),
     "scdat_set_conductor_circuit": (C.c_int, [_P, C.c_int32, _I32, _P, _I32, _I32]),
     "scdat_set_scenarios": (C.c_int, [_P, C.c_int32, _I64, _D, _D, C.POINTER(C.c_uint64)]),
+    "scdat_charging_params_default": (None, [C.POINTER(ChargingParams)]),
+    "scdat_charging_forward": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, _D]),
+    "scdat_charging_adjoint": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, C.c_int32, _D, _D]),
     "scdat_set_scenario_emission": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(SurfaceEmission)]),
     "scdat_get_scenario_stats": (C.c_int, [_P, C.c_int32, C.POINTER(ScenarioStats)]),
     "scdat_get_scenario_ledger": (C.c_int, [_P, C.c_int32, C.POINTER(LedgerEntry), C.c_int32]),
@@ -501,6 +537,21 @@ class Context:
         arr = (LedgerEntry * n_groups)()
         self._chk(lib().scdat_get_scenario_ledger(self.h, scenario, arr, n_groups))
         return [e.as_dict() for e in arr]
+
+    def charging_forward(self, params, steps):
+        """Floating potential after `steps` steps of the differentiable charging
+        model, from the initialized state (the context is left unchanged)."""
+        v = C.c_double()
+        self._chk(lib().scdat_charging_forward(self.h, C.byref(params), steps, C.byref(v)))
+        return v.value
+
+    def charging_adjoint(self, params, steps, checkpoint_interval=20):
+        """(V_final, dV/d(delta_max, J0, G, face_yield[0..5])) by the discrete adjoint."""
+        v = C.c_double()
+        g = np.zeros(N_GRAD)
+        self._chk(lib().scdat_charging_adjoint(self.h, C.byref(params), steps, checkpoint_interval, C.byref(v),
+                                               _ptr(g, C.c_double)))
+        return v.value, g
 
     def event_counts(self):
         e = EventCounts()

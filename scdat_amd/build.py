@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libscdat_amd.so")
 
-HIP_SOURCES = ["context.hip", "push.hip", "field.hip", "sort.hip", "emit.hip"]
+HIP_SOURCES = ["context.hip", "push.hip", "field.hip", "sort.hip", "emit.hip", "adjoint.hip"]
 CXX_SOURCES = ["mesh_host.cpp", "host_tools.cpp", "circuit.cpp", "sampler.cpp"]
 
 ARCH = "gfx950"

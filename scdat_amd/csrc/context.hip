@@ -28,6 +28,7 @@
 #include "push.hpp"
 #include "emit.hpp"
 #include "sort.hpp"
+#include "adjoint.hpp"
 
 namespace scdat {
 
@@ -368,6 +369,12 @@ struct Engine {
   virtual ~Engine() = default;
 };
 
+// Particle state of one step of the charging adjoint plus that step's nodal E.
+struct AdjStep {
+  DevBuf v[7];  // x, y, z, vx, vy, vz, cell
+  DevBuf E;
+};
+
 }  // namespace scdat
 
 using namespace scdat;
@@ -434,6 +441,12 @@ struct scdat_ctx {
   DevBuf d_tet_seg, d_blk_seg, d_seg_b0, d_seg_unit, d_seg_dt, d_seg_wscale, d_seg_qinit, d_seg_ccap, d_kc_off,
       d_seg_emit, d_seg_key;
   bool seg_emit_dirty = false;
+  // charging adjoint: working steps, checkpoints, adjoint vectors, snapshot
+  std::deque<AdjStep> adj_st, adj_ck;
+  std::deque<DevBuf> adj_ck_x64;
+  DevBuf d_adj_sp, d_adj_rho, d_adj_part, d_adj_xb, d_adj_vb, d_adj_phibar, d_adj_y, d_adj_rhobar, d_adj_Ebar[2],
+      d_adj_dotp, d_snap_E, d_snap_x64, d_snap_phi64, d_snap_phiT, d_snap_sc;
+  int64_t adj_np = 0;
   int n_seg() const { return seg_node0.empty() ? 1 : (int)seg_node0.size() - 1; }
   bool segmented() const { return !seg_node0.empty(); }
   int last_iters = 0, last_outer = 0, last_conv = 0;
@@ -1513,6 +1526,345 @@ double node_unit(const scdat_ctx* c, int64_t i) {
   return kElem * (c->w_ref * c->seg_wscale[sg]) / 4294967296.0;
 }
 
+// ---- discrete adjoint of the charging cycle ---------------------------------
+
+struct ChargingRun {
+  scdat_charging_params p;
+  double a_lit = 0.0;  // sunlit conductor area
+  uint32_t cond_mask = 0;
+};
+
+double photo_j(const ChargingRun& r, double V, double* dj_dv) {
+  const double t = std::max(1e-3, r.p.photo_temperature_ev);
+  const double over = V - r.p.photo_reference_v;
+  const double e = std::exp(-std::max(0.0, over) / t);
+  if (dj_dv) *dj_dv = over > 0.0 ? -r.p.photo_current_density * e / t : 0.0;
+  return r.p.photo_current_density * e;
+}
+
+template <typename T>
+void adj_alloc_step(scdat_ctx* c, AdjStep& st) {
+  const int64_t np = c->adj_np;
+  for (int r = 0; r < 6; ++r) st.v[r].alloc(sizeof(T) * (size_t)np);
+  st.v[6].alloc(4 * (size_t)np);
+  st.E.alloc(sizeof(Vec4<T>) * (size_t)c->h.nn);
+}
+
+template <typename T>
+AdjState<T> adj_state(AdjStep& st) {
+  return {st.v[0].as<T>(), st.v[1].as<T>(), st.v[2].as<T>(), st.v[3].as<T>(),
+          st.v[4].as<T>(), st.v[5].as<T>(), st.v[6].as<int32_t>()};
+}
+
+template <typename T>
+void adj_copy_step(scdat_ctx* c, AdjStep& dst, const AdjStep& src) {
+  for (int r = 0; r < 7; ++r)
+    HIPCHK(hipMemcpyAsync(dst.v[r].p, src.v[r].p, src.v[r].bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dst.E.p, src.E.p, src.E.bytes, hipMemcpyDeviceToDevice, c->stream));
+}
+
+// Checks, buffers for `steps_kept` working steps, the concatenated initial
+// particles in adj_st[0] and a snapshot of the context's field state.
+template <typename T>
+ChargingRun adj_begin(scdat_ctx* c, const scdat_charging_params* p, int steps_kept) {
+  if (!c->initialized) throw Error(SCDAT_ERR_STATE, "charging adjoint: call scdat_initialize first");
+  if (c->steps != 0) throw Error(SCDAT_ERR_STATE, "charging adjoint: starts from the initialized state (no steps)");
+  if (c->segmented() || c->n_patch > 0 || !c->h.has_cond)
+    throw Error(SCDAT_ERR_STATE, "charging adjoint: needs one scenario with a floating conductor");
+  for (int g = 0; g < kMaxGroups; ++g)
+    if (c->gtype[g] == SCDAT_BC_PERIODIC || c->gtype[g] == SCDAT_BC_REFLECT)
+      throw Error(SCDAT_ERR_STATE, "charging adjoint: periodic and reflecting groups are not differentiated");
+  for (int r = 0; r < 3; ++r)
+    if (c->cfg.bfield[r] != 0.0) throw Error(SCDAT_ERR_STATE, "charging adjoint: needs B = 0");
+  if (!(p->see_energy_max_ev > 0.0) || p->see_delta_max < 0.0 || p->surface_conductance_s < 0.0)
+    throw Error(SCDAT_ERR_ARG, "charging adjoint: bad parameters");
+  ChargingRun run;
+  run.p = *p;
+  for (int k = 0; k < 6; ++k)
+    if ((p->sunlit_faces >> k) & 1) run.a_lit += c->face_area[kAdjFaceGroup0 + k];
+  for (int g = 0; g < kMaxGroups; ++g)
+    if (c->gtype[g] == SCDAT_BC_CONDUCTOR) run.cond_mask |= 1u << g;
+  int64_t np = 0;
+  for (const auto& S : c->sp) np += S.n;
+  const bool realloc = np != c->adj_np || (int)c->adj_st.size() < steps_kept;
+  c->adj_np = np;
+  hipStream_t s = c->stream;
+  if (realloc) {
+    c->adj_st.clear();
+    c->adj_ck.clear();
+    c->adj_ck_x64.clear();
+  }
+  while ((int)c->adj_st.size() < steps_kept) {
+    c->adj_st.emplace_back();
+    adj_alloc_step<T>(c, c->adj_st.back());
+  }
+  // species index and the concatenated initial state
+  std::vector<int32_t> sp((size_t)np);
+  int64_t off = 0;
+  AdjStep& s0 = c->adj_st[0];
+  for (size_t si = 0; si < c->sp.size(); ++si) {
+    SpeciesState& S = c->sp[si];
+    std::fill(sp.begin() + off, sp.begin() + off + S.n, (int32_t)si);
+    for (int r = 0; r < 7; ++r) {
+      const size_t es = r < 6 ? sizeof(T) : 4;
+      if (S.n)
+        HIPCHK(hipMemcpyAsync(static_cast<char*>(s0.v[r].p) + es * off, S.buf[S.cur][r].p, es * S.n,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    off += S.n;
+  }
+  c->d_adj_sp.upload(sp, s);
+  HIPCHK(hipMemcpyAsync(s0.E.p, c->d_E.p, s0.E.bytes, hipMemcpyDeviceToDevice, s));
+  const size_t nn = (size_t)c->h.nn;
+  c->d_adj_rho.alloc(8 * nn);
+  c->d_adj_part.alloc(8 * (size_t)kAdjPartials * (size_t)std::max(1, adj_blocks(std::max(np, c->h.nn))));
+  c->d_adj_dotp.alloc(8 * (size_t)adj_blocks(c->h.nn));
+  // snapshot of the context's field state (restored by adj_end)
+  c->d_snap_E.alloc(c->d_E.bytes);
+  c->d_snap_x64.alloc(c->d_x64.bytes);
+  c->d_snap_phi64.alloc(c->d_phi64.bytes);
+  c->d_snap_phiT.alloc(c->d_phiT.bytes);
+  c->d_snap_sc.alloc(sizeof(PcgScalars));
+  HIPCHK(hipMemcpyAsync(c->d_snap_E.p, c->d_E.p, c->d_E.bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_snap_x64.p, c->d_x64.p, c->d_x64.bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_snap_phi64.p, c->d_phi64.p, c->d_phi64.bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_snap_phiT.p, c->d_phiT.p, c->d_phiT.bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_snap_sc.p, c->d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return run;
+}
+
+void adj_end(scdat_ctx* c) {
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(c->d_E.p, c->d_snap_E.p, c->d_E.bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_x64.p, c->d_snap_x64.p, c->d_x64.bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_phi64.p, c->d_snap_phi64.p, c->d_phi64.bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_phiT.p, c->d_snap_phiT.p, c->d_phiT.bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->d_sc.p, c->d_snap_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+template <typename T>
+AdjArgs<T> adj_args(scdat_ctx* c, const ChargingRun& run) {
+  AdjArgs<T> a{};
+  a.mesh = push_args<T>(c, 0);
+  a.sp = c->d_adj_sp.as<int32_t>();
+  a.n = c->adj_np;
+  for (size_t si = 0; si < c->sp.size(); ++si) {
+    const SpeciesState& S = c->sp[si];
+    a.qm[si] = (T)(S.q / S.m);
+    a.qw[si] = S.q * S.w;
+    a.mass[si] = S.m;
+    a.electron[si] = S.q < 0 ? 1 : 0;
+  }
+  a.dt = (T)c->cfg.dt;
+  a.cond_mask = run.cond_mask;
+  for (int k = 0; k < 6; ++k) {
+    const int g = kAdjFaceGroup0 + k;
+    a.yield[g] = run.p.face_yield[k];
+    a.dmax[g] = run.p.see_delta_max * run.p.face_yield[k];
+  }
+  a.delta_max = run.p.see_delta_max;
+  a.emax = run.p.see_energy_max_ev;
+  a.rho = c->d_adj_rho.as<double>();
+  a.partial = c->d_adj_part.as<double>();
+  return a;
+}
+
+// Block partials of the last adjoint kernel, summed in block order.
+std::array<double, kAdjPartials> adj_partials(scdat_ctx* c, int blocks) {
+  std::vector<double> h((size_t)blocks * kAdjPartials);
+  HIPCHK(hipMemcpyAsync(h.data(), c->d_adj_part.p, 8 * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::array<double, kAdjPartials> t{};
+  for (int b = 0; b < blocks; ++b)
+    for (int k = 0; k < kAdjPartials; ++k) t[k] += h[(size_t)b * kAdjPartials + k];
+  return t;
+}
+
+// One forward step s0 -> s1: push + conductor charge (Q, V of step n in,
+// of step n + 1 out) + Poisson + E of step n + 1 into s1.E.
+template <typename T>
+void adj_forward_step(scdat_ctx* c, const ChargingRun& run, AdjArgs<T> a, AdjStep& s0, AdjStep& s1, double& Q,
+                      double& V) {
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemsetAsync(c->d_adj_rho.p, 0, 8 * (size_t)c->h.nn, s));
+  a.mesh.E = s0.E.as<Vec4<T>>();
+  a.s0 = adj_state<T>(s0);
+  a.s1 = adj_state<T>(s1);
+  launch_adj_forward<T>(a, s);
+  HIPCHK(hipGetLastError());
+  const auto part = adj_partials(c, adj_blocks(a.n));
+  double hit = 0.0;
+  for (int k = 0; k < kAdjPartials; ++k) hit += part[k];
+  const double dt = c->cfg.dt;
+  Q += hit + dt * (photo_j(run, V, nullptr) * run.a_lit - run.p.surface_conductance_s * V);
+  const EllDev A = ell(c);
+  FieldBufs f = fbufs(c);
+  launch_rhs_real(A, f, c->d_adj_rho.as<double>(), 1.0, c->cfg.load_sign, 0, s);
+  const bool fp64 = c->is_fp64();
+  pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, c->inner_hist);
+  if (!c->last_conv) throw Error(SCDAT_ERR_SOLVER, "charging adjoint: forward Poisson solve did not converge");
+  HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, s));
+  FloatArgs fa = float_args(c);
+  fa.q_init = Q;
+  fa.cond_mask = 0;  // the charge is Q itself
+  fa.emit_count = nullptr;
+  launch_float(f, fa, s);
+  launch_phi_efield<T>(A, f, c->d_phiT.as<T>(), c->d_phi64.as<double>(), s1.E.as<Vec4<T>>(), nullptr, s);
+  HIPCHK(hipMemcpyAsync(&V, &f.sc->vfloat, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+template <typename T>
+double charging_forward(scdat_ctx* c, const scdat_charging_params* p, int steps) {
+  const ChargingRun run = adj_begin<T>(c, p, 2);
+  AdjArgs<T> a = adj_args<T>(c, run);
+  double Q = c->q_init, V = c->h_sc->vfloat;
+  {
+    PcgScalars q;
+    HIPCHK(hipMemcpy(&q, c->d_sc.p, sizeof(q), hipMemcpyDeviceToHost));
+    V = q.vfloat;
+  }
+  for (int n = 0; n < steps; ++n) adj_forward_step<T>(c, run, a, c->adj_st[n & 1], c->adj_st[(n + 1) & 1], Q, V);
+  adj_end(c);
+  return V;
+}
+
+// Adjoint of one step n (state s0 -> s1, V_n = Vn): in, Ebar of E_{n+1}
+// (ebar_in), Vbar of V_{n+1}, Qbar of Q_{n+2}; out, Ebar of E_n (ebar_out),
+// Vbar of V_n, Qbar of Q_{n+1}, gradient contributions.
+template <typename T>
+void adj_backward_step(scdat_ctx* c, const ChargingRun& run, AdjArgs<T> a, AdjStep& s0, AdjStep& s1, double Vn,
+                       double* ebar_in, double* ebar_out, double& Vbar, double& Qbar, double* grad) {
+  hipStream_t s = c->stream;
+  const int64_t nn = c->h.nn;
+  const EllDev A = ell(c);
+  double* phibar = c->d_adj_phibar.as<double>();
+  HIPCHK(hipMemsetAsync(phibar, 0, 8 * (size_t)nn, s));
+  launch_adj_field_t<T>(A, ebar_in, phibar, s);
+  launch_adj_dot(nn, c->d_phi1.as<double>(), phibar, c->d_adj_dotp.as<double>(), s);
+  {
+    const int nb = adj_blocks(nn);
+    std::vector<double> h((size_t)nb);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_adj_dotp.p, 8 * h.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (double v : h) Vbar += v;
+  }
+  // V = (Q + sum_C rho - kc.phi_0) / C
+  const double vc = Vbar / c->ccap;
+  Qbar += vc;
+  launch_adj_scatter((int64_t)c->h.kc_idx.size(), c->d_kc_idx.as<int32_t>(), c->d_kc_val.as<double>(), -vc, phibar,
+                     s);
+  // phi_0 = A^-1 b(rho): rhobar = A^-1 phibar_0 on free rows (A symmetric)
+  FieldBufs f = fbufs(c);
+  f.x64 = c->d_adj_y.as<double>();
+  launch_rhs_real(A, f, phibar, 1.0, c->cfg.load_sign, 1, s);
+  const bool fp64 = c->is_fp64();
+  pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, nullptr);
+  if (!c->last_conv) throw Error(SCDAT_ERR_SOLVER, "charging adjoint: adjoint Poisson solve did not converge");
+  launch_adj_rhobar(nn, c->d_kind.as<int32_t>(), c->d_adj_y.as<double>(), vc * (double)c->cfg.load_sign,
+                    c->d_adj_rhobar.as<double>(), s);
+  // particles: deposit, secondary charge, drift and kick of step n
+  HIPCHK(hipMemsetAsync(ebar_out, 0, 24 * (size_t)nn, s));
+  a.mesh.E = s0.E.as<Vec4<T>>();
+  a.s0 = adj_state<T>(s0);
+  a.s1 = adj_state<T>(s1);
+  a.xb = c->d_adj_xb.as<double>();
+  a.vb = c->d_adj_vb.as<double>();
+  a.rhobar = c->d_adj_rhobar.as<double>();
+  a.Ebar = ebar_out;
+  a.qbar = Qbar;
+  launch_adj_backward<T>(a, s);
+  HIPCHK(hipGetLastError());
+  const auto part = adj_partials(c, adj_blocks(a.n));
+  grad[SCDAT_GRAD_DELTA_MAX] += part[0];
+  for (int k = 0; k < 6; ++k) grad[SCDAT_GRAD_FACE_YIELD + k] += part[1 + k];
+  // Q_{n+1} = Q_n + ... + dt (J(V_n) A_lit - G V_n)
+  const double dt = c->cfg.dt;
+  double djdv = 0.0;
+  const double j = photo_j(run, Vn, &djdv);
+  if (run.p.photo_current_density != 0.0) grad[SCDAT_GRAD_PHOTO_J] += Qbar * dt * run.a_lit * j / run.p.photo_current_density;
+  else {
+    const double t = std::max(1e-3, run.p.photo_temperature_ev);
+    grad[SCDAT_GRAD_PHOTO_J] += Qbar * dt * run.a_lit * std::exp(-std::max(0.0, Vn - run.p.photo_reference_v) / t);
+  }
+  grad[SCDAT_GRAD_CONDUCTANCE] -= Qbar * dt * Vn;
+  Vbar = Qbar * dt * (run.a_lit * djdv - run.p.surface_conductance_s);
+}
+
+template <typename T>
+double charging_adjoint(scdat_ctx* c, const scdat_charging_params* p, int steps, int interval, double* grad) {
+  const int K = std::max(1, std::min(interval, steps));
+  const ChargingRun run = adj_begin<T>(c, p, K + 1);
+  AdjArgs<T> a = adj_args<T>(c, run);
+  hipStream_t s = c->stream;
+  const int64_t nn = c->h.nn, np = c->adj_np;
+  const int nseg = (steps + K - 1) / K;
+  double Q = c->q_init, V;
+  {
+    PcgScalars q;
+    HIPCHK(hipMemcpy(&q, c->d_sc.p, sizeof(q), hipMemcpyDeviceToHost));
+    V = q.vfloat;
+  }
+  while ((int)c->adj_ck.size() < nseg) {
+    c->adj_ck.emplace_back();
+    adj_alloc_step<T>(c, c->adj_ck.back());
+    c->adj_ck_x64.emplace_back();
+    c->adj_ck_x64.back().alloc(8 * (size_t)nn);
+  }
+  std::vector<double> ckQ(nseg), ckV(nseg);
+  // forward sweep, checkpoint at the start of every segment
+  for (int k = 0; k < nseg; ++k) {
+    adj_copy_step<T>(c, c->adj_ck[k], c->adj_st[0]);
+    HIPCHK(hipMemcpyAsync(c->adj_ck_x64[k].p, c->d_x64.p, 8 * (size_t)nn, hipMemcpyDeviceToDevice, s));
+    ckQ[k] = Q;
+    ckV[k] = V;
+    const int L = std::min(K, steps - k * K);
+    for (int j = 0; j < L; ++j) {
+      AdjStep& from = c->adj_st[j & 1];
+      AdjStep& to = c->adj_st[(j + 1) & 1];
+      adj_forward_step<T>(c, run, a, from, to, Q, V);
+    }
+    if (L & 1) adj_copy_step<T>(c, c->adj_st[0], c->adj_st[1]);
+  }
+  const double v_final = V;
+  // reverse sweep: recompute each segment from its checkpoint, then go back
+  c->d_adj_xb.alloc(24 * (size_t)np);
+  c->d_adj_vb.alloc(24 * (size_t)np);
+  c->d_adj_phibar.alloc(8 * (size_t)nn);
+  c->d_adj_rhobar.alloc(8 * (size_t)nn);
+  c->d_adj_y.alloc(8 * (size_t)nn);
+  for (int k = 0; k < 2; ++k) c->d_adj_Ebar[k].alloc(24 * (size_t)nn);
+  HIPCHK(hipMemsetAsync(c->d_adj_xb.p, 0, 24 * (size_t)np, s));
+  HIPCHK(hipMemsetAsync(c->d_adj_vb.p, 0, 24 * (size_t)np, s));
+  HIPCHK(hipMemsetAsync(c->d_adj_y.p, 0, 8 * (size_t)nn, s));
+  HIPCHK(hipMemsetAsync(c->d_adj_Ebar[0].p, 0, 24 * (size_t)nn, s));
+  for (int q = 0; q < SCDAT_N_GRAD; ++q) grad[q] = 0.0;
+  double Vbar = 1.0, Qbar = 0.0;  // objective: V at the last step
+  int eb = 0;                      // d_adj_Ebar[eb]: adjoint of E_{n+1}
+  std::vector<double> Vs((size_t)K + 1);
+  for (int k = nseg - 1; k >= 0; --k) {
+    const int L = std::min(K, steps - k * K);
+    adj_copy_step<T>(c, c->adj_st[0], c->adj_ck[k]);
+    HIPCHK(hipMemcpyAsync(c->d_x64.p, c->adj_ck_x64[k].p, 8 * (size_t)nn, hipMemcpyDeviceToDevice, s));
+    Q = ckQ[k];
+    V = ckV[k];
+    Vs[0] = V;
+    for (int j = 0; j < L; ++j) {
+      adj_forward_step<T>(c, run, a, c->adj_st[j], c->adj_st[j + 1], Q, V);
+      Vs[j + 1] = V;
+    }
+    for (int j = L - 1; j >= 0; --j) {
+      adj_backward_step<T>(c, run, a, c->adj_st[j], c->adj_st[j + 1], Vs[j], c->d_adj_Ebar[eb].as<double>(),
+                           c->d_adj_Ebar[1 - eb].as<double>(), Vbar, Qbar, grad);
+      eb = 1 - eb;
+    }
+  }
+  adj_end(c);
+  return v_final;
+}
+
 // Ledger entries of scenarios [s0, s1) summed, PICCycle::getSurfaceCurrentLedger
 // (PICCycle.cpp:1295-1329) fields.
 void fill_ledger(scdat_ctx* c, const LedgerHost& L, int s0, int s1, scdat_ledger_entry* out, int n_groups) {
@@ -2376,6 +2728,37 @@ int scdat_get_scenario_ledger(scdat_ctx* c, int32_t scenario, scdat_ledger_entry
   if (!c->has_mesh || scenario < 0 || scenario >= c->n_seg())
     return fail(c, SCDAT_ERR_ARG, "get_scenario_ledger: no such scenario");
   return guarded(c, [&] { fill_ledger(c, read_ledger(c), scenario, scenario + 1, out, n_groups); });
+}
+
+// ---- discrete adjoint of the charging cycle ------------------------------------
+
+void scdat_charging_params_default(scdat_charging_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->see_delta_max = 2.0;
+  p->see_energy_max_ev = 400.0;
+  for (int k = 0; k < 6; ++k) p->face_yield[k] = 1.0;
+  p->photo_current_density = 2.0e-5;
+  p->photo_temperature_ev = 2.0;
+  p->photo_reference_v = 0.0;
+  p->sunlit_faces = 1 << 1;  // XMax face of the conductor (group 7)
+  p->surface_conductance_s = 0.0;
+}
+
+int scdat_charging_forward(scdat_ctx* c, const scdat_charging_params* p, int32_t steps, double* v_final) {
+  if (!c || !p || !v_final || steps < 0) return SCDAT_ERR_ARG;
+  return guarded(c, [&] {
+    *v_final = c->is_fp64() ? charging_forward<double>(c, p, steps) : charging_forward<float>(c, p, steps);
+  });
+}
+
+int scdat_charging_adjoint(scdat_ctx* c, const scdat_charging_params* p, int32_t steps, int32_t checkpoint_interval,
+                           double* v_final, double* grad) {
+  if (!c || !p || !v_final || !grad || steps < 1 || checkpoint_interval < 1) return SCDAT_ERR_ARG;
+  return guarded(c, [&] {
+    *v_final = c->is_fp64() ? charging_adjoint<double>(c, p, steps, checkpoint_interval, grad)
+                            : charging_adjoint<float>(c, p, steps, checkpoint_interval, grad);
+  });
 }
 
 // ---- conductor patches driven by a surface circuit ----------------------------

@@ -112,6 +112,36 @@ __global__ void __launch_bounds__(kPB) k_rhs(EllDev A, FieldBufs f,
   publish(cl, slot(f, S_CL, G), lds);
 }
 
+// Right-hand side from an FP64 nodal vector: src as deposited charge (scaled
+// like k_rhs), or, homogeneous, the load on free rows and zero on
+// constrained rows (no lift): the adjoint solve A y = P load(src).
+__global__ void __launch_bounds__(kPB) k_rhs_real(EllDev A, FieldBufs f, const double* src, double scale,
+                                                  int load_sign, int homogeneous) {
+  __shared__ double lds[kPB / 64];
+  const int G = gridDim.x;
+  double bb = 0.0, cl = 0.0;
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
+    const int k = f.kind[i];
+    const double load = (double)load_sign * scale * src[i];
+    double b;
+    if (homogeneous) {
+      b = k == 0 ? load : 0.0;
+    } else {
+      if (k == 0) b = load + f.lift[i];
+      else if (k == 1) b = f.dscale[i] * f.dval[i];
+      else {
+        b = 0.0;
+        cl += load;
+      }
+    }
+    f.b64[i] = b;
+    bb += b * b;
+  }
+  publish(bb, slot(f, S_BB, G), lds);
+  publish(cl, slot(f, S_CL, G), lds);
+}
+
 // one block per segment
 __global__ void __launch_bounds__(kPB) k_fin_rhs(FieldBufs f, int G) {
   __shared__ double lds[kPB / 64];
@@ -555,6 +585,12 @@ void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* r
                 double unit, int load_sign, hipStream_t s) {
   const int G = grid_rows(A.n);
   hipLaunchKernelGGL(k_rhs, dim3(G), dim3(kPB), 0, s, A, f, rho_fx, unit, load_sign);
+  hipLaunchKernelGGL(k_fin_rhs, dim3(f.blk_seg ? f.n_seg : 1), dim3(kPB), 0, s, f, G);
+}
+void launch_rhs_real(const EllDev& A, const FieldBufs& f, const double* src, double scale, int load_sign,
+                     int homogeneous, hipStream_t s) {
+  const int G = grid_rows(A.n);
+  hipLaunchKernelGGL(k_rhs_real, dim3(G), dim3(kPB), 0, s, A, f, src, scale, load_sign, homogeneous);
   hipLaunchKernelGGL(k_fin_rhs, dim3(f.blk_seg ? f.n_seg : 1), dim3(kPB), 0, s, f, G);
 }
 void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s) {

@@ -86,6 +86,10 @@ struct InnerBufs {
 
 void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* rho_fx,
                 double unit, int load_sign, hipStream_t s);
+// b from an FP64 nodal vector: deposited charge (x scale, like launch_rhs),
+// or homogeneous: src on free rows, 0 on constrained rows
+void launch_rhs_real(const EllDev& A, const FieldBufs& f, const double* src, double scale, int load_sign,
+                     int homogeneous, hipStream_t s);
 void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s);
 template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s);
