@@ -61,6 +61,7 @@ _SIG = {
     "orc_mcc_configure": (None, [_P, C.c_int, _D]),
     "orc_set_emission": (None, [_P, C.c_int, _D]),
     "orc_post_push": (None, [_P]),
+    "orc_charging_forward": (C.c_double, [_P, _D, C.c_int]),
     "orc_mcc_counts": (None, [_P, _I64]),
     "orc_get_ids": (None, [_P, C.c_int, _I32]),
     "orc_sigma": (None, [_D, C.c_int64, _D, _D]),
@@ -326,6 +327,14 @@ class Oracle:
         a = np.array([see_model, delta_max, energy_max, see_temperature, photo_j, photo_temperature,
                       photo_reference_v, electron_species], np.float64)
         lib().orc_set_emission(self.h, group, _p(a, C.c_double))
+
+    def charging_forward(self, params, steps):
+        """Floating potential after `steps` steps of the differentiable
+        charging model; params has the fields of scdat_charging_params."""
+        q = np.array([params.see_delta_max, params.see_energy_max_ev] + list(params.face_yield)
+                     + [params.photo_current_density, params.photo_temperature_ev, params.photo_reference_v,
+                        float(params.sunlit_faces), params.surface_conductance_s], dtype=np.float64)
+        return lib().orc_charging_forward(self.h, _p(q, C.c_double), int(steps))
 
     def post_push(self):
         lib().orc_post_push(self.h)

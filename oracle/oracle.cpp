@@ -1337,6 +1337,55 @@ int orc_step(void* h, int nsteps) {
   return it;
 }
 
+// Differentiable charging model (the forward that the device's discrete
+// adjoint differentiates), FP64: steps of push + deposit with the previous E
+// (no sub-steps when the push is configured non-adaptive), conductor charge
+//   Q += hits (q w) + sum over electron hits on conductor group 6 + k of
+//        |q| w delta_max yield_k s(E),  s(E) = (E/E_max) exp(2 - 2 sqrt(E/E_max))
+//      + dt (J0 exp(-max(0, V - V_ref) / T_ph) A_sunlit - G V),
+// then the floating-conductor solve (anchor collapse) and nodal E.
+// p = {delta_max, E_max, yield[6], J0, T_ph, V_ref, sunlit bits, G}.
+// Returns the final floating potential; the particles and fields advance.
+double orc_charging_forward(void* h, const double* p, int steps) {
+  auto* s = static_cast<Sim*>(h);
+  const double dmax = p[0], emax = p[1], j0 = p[8], tph = std::max(1e-3, p[9]), vref = p[10], G = p[12];
+  const int sunlit = (int)p[11];
+  double area = 0.0;
+  for (int k = 0; k < 6; ++k)
+    if ((sunlit >> k) & 1) {
+      const FaceList f = group_faces(*s, 6 + k);
+      if (!f.cum.empty()) area += f.cum.back();
+    }
+  // impacts on the conductor faces are recorded as emission events
+  std::array<SurfEmit, kMaxGroups> saved = s->emit;
+  for (int k = 0; k < 6; ++k) {
+    s->emit[6 + k].see_model = 2;
+    s->emit[6 + k].dmax = dmax * p[2 + k];
+    s->emit[6 + k].emax = emax;
+  }
+  for (int n = 0; n < steps; ++n) {
+    const double V = s->Vfloat;
+    orc_push(h);  // Qcond += q w of every conductor hit
+    double see = 0.0;
+    for (const SeeEvent& e : s->events) {
+      if (e.group < 6 || e.group >= 12 || s->gtype[e.group] != BC_CONDUCTOR) continue;
+      const Species& sp = s->sp[e.species];
+      const double E = 0.5 * sp.m * (e.v[0] * e.v[0] + e.v[1] * e.v[1] + e.v[2] * e.v[2]) / kE;
+      if (!(E > 0.0)) continue;
+      const double r = E / emax;
+      see += std::abs(sp.q * sp.w) * dmax * p[2 + e.group - 6] * r * std::exp(2.0 - 2.0 * std::sqrt(r));
+    }
+    s->events.clear();
+    const double jph = j0 * std::exp(-std::max(0.0, V - vref) / tph);
+    s->Qcond += see + s->dt * (jph * area - G * V);
+    solve_poisson(*s);
+    compute_efield(*s);
+    s->steps++;
+  }
+  s->emit = saved;
+  return s->Vfloat;
+}
+
 // Ledger: 7 values per group (abs_e_charge, abs_i_charge, emit_e_charge,
 // energy, n_abs_e, n_abs_i, n_emit_e).
 void orc_get_ledger(void* h, double* out, int ngroups) {

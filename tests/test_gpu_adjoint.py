@@ -126,3 +126,20 @@ def test_mixed_precision_adjoint_error_against_fp64():
     print("V fp64", v64, "mixed", v32, "sensitivity error (max abs / max |theta g|)", err)
     assert abs(v32 - v64) <= 1e-4 * abs(v64)
     assert err < 1e-2
+
+
+def test_forward_matches_oracle_fp64():
+    """The differentiable forward on the device against the oracle's FP64
+    restatement (anchor-collapse floating conductor, CPU walk and deposit)."""
+    from helpers import make_oracle
+    st = _case()
+    p = S.charging_params(**THETA_KW)
+    steps = 30
+    v_dev = _ctx(st, 1).charging_forward(p, steps)
+    o = make_oracle(st, tol=1e-13, adaptive=0, max_sub=1)
+    o.deposit()
+    o.solve()
+    o.efield()
+    v_orc = o.charging_forward(p, steps)
+    print("V device", v_dev, "oracle", v_orc)
+    assert v_dev == pytest.approx(v_orc, rel=1e-9)

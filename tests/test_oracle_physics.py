@@ -326,3 +326,62 @@ def test_kinetic_energy_statistic_closed_form():
     o.set_particles(si, x, vi, S.loading.box_locate(spec, x))
     want = 0.5 * ME * 2.5 * (ve ** 2).sum() + 0.5 * MP * 0.5 * (vi ** 2).sum()
     np.testing.assert_allclose(o.kinetic_energy(), want, rtol=1e-13)
+
+
+def _charging_params(**kw):
+    p = S.charging_params(photo_current_density=0.0, surface_conductance_s=0.0)
+    for k, v in kw.items():
+        if k == "face_yield":
+            for i in range(6):
+                p.face_yield[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def test_charging_forward_conductor_relaxation_closed_form():
+    # No particles: Q_{n+1} = Q_n + dt (J0 A_sunlit - G V_n), V = Q / C.
+    m, o, spec = _boundary_oracle(8, COND)
+    for g in range(6, 12):
+        o.set_group(g, S.BC_CONDUCTOR)
+    o.set_poisson(0, EPS0, 1, 1e-13, 5000)
+    o.add_species(-QE, ME, 1.0)
+    o.set_particles(0, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.int32))
+    q0, dt = -2e-11, 1e-6
+    o.set_push(dt, 0, 1e-3, 1, 0)
+    o.set_conductor_charge(q0)
+    o.solve()
+    cap = q0 / o.floating_potential()
+    assert 1e-11 < cap < 3e-11  # a 0.2 m cube in a 1 m box: ~ 4 pi eps0 0.13 m
+    lit = 0b000100  # face of group 8
+    area = 0.2 * 0.2
+    p = _charging_params(photo_current_density=3e-6, photo_reference_v=10.0, surface_conductance_s=2e-6,
+                         sunlit_faces=lit)
+    V = q0 / cap
+    for _ in range(12):
+        V = V + dt * (3e-6 * area - 2e-6 * V) / cap
+    assert o.charging_forward(p, 12) == pytest.approx(V, rel=1e-9)
+
+
+def test_charging_forward_secondary_charge_of_one_impact():
+    # One electron hits the conductor's x-min face (group 6) at 1e7 m/s: the
+    # conductor gains -e w plus the Sternglass secondary charge e w delta.
+    m, o, spec = _boundary_oracle(10, COND)
+    for g in range(6, 12):
+        o.set_group(g, S.BC_CONDUCTOR)
+    o.set_poisson(0, EPS0, 1, 1e-13, 5000)
+    w = 5.0
+    se = o.add_species(-QE, ME, w)
+    x = np.array([[0.35, 0.5, 0.5]])
+    o.set_particles(se, x, np.array([[1e7, 0, 0]]), S.loading.box_locate(spec, x))
+    o.set_push(1e-8, 0, 1e-3, 1, 0)
+    o.solve()
+    o.set_efield(np.zeros((len(m["xyz"]), 3)))
+    yields = [0.7, 1.0, 1.0, 1.0, 1.0, 1.0]
+    p = _charging_params(see_delta_max=2.5, see_energy_max_ev=400.0, face_yield=yields)
+    o.charging_forward(p, 1)
+    E = 0.5 * ME * 1e14 / QE
+    r = E / 400.0
+    delta = 2.5 * 0.7 * r * np.exp(2 - 2 * np.sqrt(r))
+    assert o.get_particles(se)[2][0] == -1
+    np.testing.assert_allclose(o.conductor_charge(), -QE * w + QE * w * delta, rtol=1e-12)
