@@ -95,7 +95,8 @@ def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True):
         sid = ctx.add_species(q, mass, case["w"], len(x))
         ctx.set_particles(sid, x, v, c)
     ctx.initialize()
-    log(f"gpu setup {time.time() - t0:.1f}s; warmup {warmup} steps")
+    chunks = ctx.autotune()
+    log(f"gpu setup {time.time() - t0:.1f}s (push tile: {chunks} particles per thread); warmup {warmup} steps")
     for _ in range(warmup):
         ctx.step(1)
     ctx.sync()
@@ -112,7 +113,7 @@ def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True):
     kt = {"push": ctx.kernel_time(S.KERNEL_PUSH), "pcg_iter": ctx.kernel_time(S.KERNEL_PCG)}
     pushes = st1["particle_pushes"] - st0["particle_pushes"]
     log(f"gpu: {steps} steps in {wall:.3f}s, pushes {pushes}, stages(ms) {stages}, kernels {kt}, stats {st1}")
-    return ctx, dict(wall=wall, pushes=pushes, stages=stages, kt=kt, stats=st1, stats0=st0)
+    return ctx, dict(wall=wall, pushes=pushes, stages=stages, kt=kt, stats=st1, stats0=st0, chunks=chunks)
 
 
 def run_cpu_baseline(case, steps, threads):
@@ -167,7 +168,7 @@ def run_sweep(args):
         return wall, ctx.stats()["particle_pushes"] - p0
 
     t0 = time.time()
-    bc, u = B.batch_context(sweep, cases, precision=0, tol=1e-8, sort_interval=args.sort_interval)
+    bc, u = B.batch_context(sweep, cases, precision=0, tol=1e-8, sort_interval=args.sort_interval, autotune=True)
     log(f"batched setup {time.time() - t0:.1f}s")
     bwall, bpush = timed(bc)
     stages = bc.stage_times()
@@ -177,7 +178,7 @@ def run_sweep(args):
     swall, spush = 0.0, 0
     t0 = time.time()
     for k, (sc, st) in enumerate(zip(sweep, cases)):
-        c = B.standalone_context(sc, st, precision=0, tol=1e-8, sort_interval=args.sort_interval)
+        c = B.standalone_context(sc, st, precision=0, tol=1e-8, sort_interval=args.sort_interval, autotune=True)
         w, p = timed(c)
         swall += w
         spush += p
@@ -410,6 +411,7 @@ def main():
                if pcg_ms > 0 else 0.0, "note": "algorithmic bytes of the PCG iteration kernel per inner iteration over the "
                "whole Poisson stage time (FP64 residuals, launches and host polls included)"},
         "stage_ms_total": g["stages"],
+        "launch_shape": {"push_particles_per_thread": g["chunks"], "chosen_by": "scdat_autotune"},
         "floating_potential_v": st["floating_potential"],
         "pcg_initial_rel_residual": st["pcg_initial_rel_residual"],
     }

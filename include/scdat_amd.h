@@ -445,6 +445,19 @@ int scdat_get_scenario_stats(scdat_ctx* ctx, int32_t scenario, scdat_scenario_st
 /* getSurfaceCurrentLedger of one scenario (scdat_get_ledger sums them). */
 int scdat_get_scenario_ledger(scdat_ctx* ctx, int32_t scenario, scdat_ledger_entry* out, int32_t n_groups);
 
+/* ---- Launch shapes -----------------------------------------------------------
+ * SCDAT_SHAPE_PUSH_CHUNKS: particles per thread of the push kernel, whose
+ * deposit tile is chunks x 256 consecutive (cell-sorted) particles: 1, 2, 4
+ * (default) or 8.  scdat_autotune (after scdat_initialize) times the push at
+ * every shape on scratch copies of all species, after half a sort interval
+ * of pushes in the current field, and keeps the fastest; the simulation
+ * state is untouched.  Every shape gives the same bits.  No counterpart in
+ * the reference.                                                           */
+#define SCDAT_SHAPE_PUSH_CHUNKS 0
+int scdat_set_launch_shape(scdat_ctx* ctx, int32_t what, int32_t value);
+int scdat_get_launch_shape(scdat_ctx* ctx, int32_t what, int32_t* value);
+int scdat_autotune(scdat_ctx* ctx);
+
 /* ---- Discrete adjoint of the charging cycle -------------------------------
  * Objective: the floating conductor's potential after `steps` collisionless
  * steps from the initialized state (before any scdat_step).  Forward model

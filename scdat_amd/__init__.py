@@ -103,6 +103,7 @@ class SurfaceEmission(C.Structure):
 
 
 SEGMENT_ALIGN = 256
+SHAPE_PUSH_CHUNKS = 0
 
 
 class ScenarioStats(C.Structure):
@@ -224,6 +225,9 @@ SIGNATURES = {
     "scdat_set_conductor_circuit": (C.c_int, [_P, C.c_int32, _I32, _P, _I32, _I32]),
     "scdat_set_scenarios": (C.c_int, [_P, C.c_int32, _I64, _D, _D, C.POINTER(C.c_uint64)]),
     "scdat_charging_params_default": (None, [C.POINTER(ChargingParams)]),
+    "scdat_set_launch_shape": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "scdat_get_launch_shape": (C.c_int, [_P, C.c_int32, _I32]),
+    "scdat_autotune": (C.c_int, [_P]),
     "scdat_charging_forward": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, _D]),
     "scdat_charging_adjoint": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, C.c_int32, _D, _D]),
     "scdat_set_scenario_emission": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(SurfaceEmission)]),
@@ -537,6 +541,22 @@ class Context:
         arr = (LedgerEntry * n_groups)()
         self._chk(lib().scdat_get_scenario_ledger(self.h, scenario, arr, n_groups))
         return [e.as_dict() for e in arr]
+
+    def autotune(self):
+        """Time the push kernel's launch shapes on a scratch copy and keep the
+        fastest; returns the chosen particles per thread."""
+        self._chk(lib().scdat_autotune(self.h))
+        return self.push_chunks
+
+    @property
+    def push_chunks(self):
+        v = C.c_int32()
+        self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_PUSH_CHUNKS, C.byref(v)))
+        return v.value
+
+    @push_chunks.setter
+    def push_chunks(self, n):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_PUSH_CHUNKS, int(n)))
 
     def charging_forward(self, params, steps):
         """Floating potential after `steps` steps of the differentiable charging

@@ -86,7 +86,8 @@ def _emission(ctx, scenario, sc):
             ctx.set_scenario_emission(scenario, g, SEE_STERNGLASS, 0, sc["delta_max"], 400.0, 2.0)
 
 
-def standalone_context(sc, inputs=None, precision=0, tol=1e-8, sort_interval=20, see=True, capacity_factor=2):
+def standalone_context(sc, inputs=None, precision=0, tol=1e-8, sort_interval=20, see=True, capacity_factor=2,
+                       autotune=False):
     """The single-case context that batched case `sc` reproduces."""
     st = inputs or case_inputs(sc)
     m = st["mesh"]
@@ -103,6 +104,8 @@ def standalone_context(sc, inputs=None, precision=0, tol=1e-8, sort_interval=20,
     c.set_particles(0, *st["e"])
     c.set_particles(1, *st["i"])
     c.initialize()
+    if autotune:
+        c.autotune()
     return c
 
 
@@ -137,7 +140,8 @@ def concat(cases):
                 e=parts[0], i=parts[1])
 
 
-def batch_context(scenarios, inputs=None, precision=0, tol=1e-8, sort_interval=20, see=True, capacity_factor=2):
+def batch_context(scenarios, inputs=None, precision=0, tol=1e-8, sort_interval=20, see=True, capacity_factor=2,
+                  autotune=False):
     """One context advancing every case of `scenarios` together; returns
     (context, union) where union holds the node and tet offsets per case."""
     cases = inputs or [case_inputs(sc) for sc in scenarios]
@@ -157,6 +161,8 @@ def batch_context(scenarios, inputs=None, precision=0, tol=1e-8, sort_interval=2
     c.set_particles(0, *u["e"])
     c.set_particles(1, *u["i"])
     c.initialize()
+    if autotune:
+        c.autotune()
     return c, u
 
 

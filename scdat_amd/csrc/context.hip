@@ -441,6 +441,7 @@ struct scdat_ctx {
   DevBuf d_tet_seg, d_blk_seg, d_seg_b0, d_seg_unit, d_seg_dt, d_seg_wscale, d_seg_qinit, d_seg_ccap, d_kc_off,
       d_seg_emit, d_seg_key;
   bool seg_emit_dirty = false;
+  int push_chunks = 4;  // particles per thread of the push kernel (scdat_autotune)
   // charging adjoint: working steps, checkpoints, adjoint vectors, snapshot
   std::deque<AdjStep> adj_st, adj_ck;
   std::deque<DevBuf> adj_ck_x64;
@@ -1033,7 +1034,93 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
     a.seg_dt = c->d_seg_dt.as<T>();
     a.seg_wscale = c->d_seg_wscale.as<double>();
   }
+  a.chunks = c->push_chunks;
   return a;
+}
+
+// Launch-shape tuner: for 1, 2, 4 and 8 particles per thread, pushes scratch
+// copies of every species for half a sort interval (2..10 pushes, so the
+// particle order is as disordered as it is on average between sorts) in the
+// current field and times the last two pushes; keeps the fastest.  The
+// context's particles, charge, ledger and statistics are untouched.
+template <typename T>
+void autotune_push(scdat_ctx* c) {
+  const int ns = (int)c->sp.size();
+  hipStream_t s = c->stream;
+  const size_t m = (size_t)kMaxGroups * kMaxSpecies * c->n_seg();
+  DevBuf rho, led_c, led_e, cnt, ke, alive;
+  size_t kb = 16, ab = 16;
+  for (const auto& S : c->sp) {
+    kb = std::max(kb, S.ke_partial.bytes);
+    ab = std::max(ab, S.alive_partial.bytes);
+  }
+  rho.alloc(8 * (size_t)c->h.nn);
+  led_c.alloc(8 * m);
+  led_e.alloc(8 * m);
+  cnt.alloc(32);
+  ke.alloc(kb);
+  alive.alloc(ab);
+  HIPCHK(hipMemsetAsync(led_c.p, 0, led_c.bytes, s));
+  HIPCHK(hipMemsetAsync(led_e.p, 0, led_e.bytes, s));
+  HIPCHK(hipMemsetAsync(cnt.p, 0, 32, s));
+  const int P = std::max(2, std::min(10, c->cfg.sort_interval / 2));
+  const int cand[4] = {1, 2, 4, 8};
+  double best = 1e300;
+  int pick = c->push_chunks;
+  for (int k = 0; k < 4; ++k) {
+    for (int si = 0; si < ns; ++si) {
+      SpeciesState& S = c->sp[si];
+      for (int r = 0; r < 8 && S.n; ++r) {
+        const size_t es = r < 6 ? sizeof(T) : 4;
+        HIPCHK(hipMemcpyAsync(S.buf[1 - S.cur][r].p, S.buf[S.cur][r].p, es * (size_t)S.n, hipMemcpyDeviceToDevice,
+                              s));
+      }
+    }
+    double tsum = 0.0;
+    for (int p = 0; p < P; ++p) {
+      HIPCHK(hipMemsetAsync(rho.p, 0, rho.bytes, s));
+      for (int si = 0; si < ns; ++si) {
+        SpeciesState& S = c->sp[si];
+        if (!S.n) continue;
+        const int dst = 1 - S.cur;
+        PushArgs<T> a = push_args<T>(c, si);
+        a.x = S.buf[dst][0].as<T>();
+        a.y = S.buf[dst][1].as<T>();
+        a.z = S.buf[dst][2].as<T>();
+        a.vx = S.buf[dst][3].as<T>();
+        a.vy = S.buf[dst][4].as<T>();
+        a.vz = S.buf[dst][5].as<T>();
+        a.cell = S.buf[dst][6].as<int32_t>();
+        a.id = S.buf[dst][7].as<int32_t>();
+        a.rho = rho.as<unsigned long long>();
+        a.ledger_count = led_c.as<unsigned long long>();
+        a.ledger_energy = led_e.as<double>();
+        a.lost = cnt.as<unsigned long long>();
+        a.pushes = cnt.as<unsigned long long>() + 1;
+        a.n_events = cnt.as<unsigned long long>() + 2;
+        a.see_mask = 0;
+        a.ke_partial = ke.as<double>();
+        a.alive_partial = alive.as<int32_t>();
+        a.chunks = cand[k];
+        hipEvent_t e0 = take_kt_event(c, 2 * si), e1 = take_kt_event(c, 2 * si + 1);
+        launch_push<T>(a, s, e0, e1);
+        HIPCHK(hipGetLastError());
+      }
+      HIPCHK(hipStreamSynchronize(s));
+      if (p < P - 2) continue;  // time the last two pushes
+      for (int si = 0; si < ns; ++si) {
+        if (!c->sp[si].n) continue;
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, c->kt_pool[2 * si], c->kt_pool[2 * si + 1]));
+        tsum += t;
+      }
+    }
+    if (tsum < best) {
+      best = tsum;
+      pick = cand[k];
+    }
+  }
+  c->push_chunks = pick;
 }
 
 template <typename T>
@@ -1060,7 +1147,7 @@ void do_push(scdat_ctx* c, std::vector<std::pair<size_t, size_t>>* ev, size_t* e
   for (size_t si = 0; si < c->sp.size(); ++si) {
     if (c->sp[si].n == 0) continue;
     const PushArgs<T> a = push_args<T>(c, (int)si);
-    c->sp[si].stat_blocks = push_blocks(a.n);
+    c->sp[si].stat_blocks = push_blocks(a.n, a.chunks);
     if (ev) {
       const size_t e0 = (*evn)++, e1 = (*evn)++;
       launch_push<T>(a, c->stream, take_event(c, e0), take_event(c, e1));
@@ -2728,6 +2815,31 @@ int scdat_get_scenario_ledger(scdat_ctx* c, int32_t scenario, scdat_ledger_entry
   if (!c->has_mesh || scenario < 0 || scenario >= c->n_seg())
     return fail(c, SCDAT_ERR_ARG, "get_scenario_ledger: no such scenario");
   return guarded(c, [&] { fill_ledger(c, read_ledger(c), scenario, scenario + 1, out, n_groups); });
+}
+
+// ---- launch shapes ---------------------------------------------------------------
+
+int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (what != SCDAT_SHAPE_PUSH_CHUNKS || !(value == 1 || value == 2 || value == 4 || value == 8))
+    return fail(c, SCDAT_ERR_ARG, "set_launch_shape: push chunks must be 1, 2, 4 or 8");
+  c->push_chunks = value;
+  return SCDAT_OK;
+}
+
+int scdat_get_launch_shape(scdat_ctx* c, int32_t what, int32_t* value) {
+  if (!c || !value || what != SCDAT_SHAPE_PUSH_CHUNKS) return SCDAT_ERR_ARG;
+  *value = c->push_chunks;
+  return SCDAT_OK;
+}
+
+int scdat_autotune(scdat_ctx* c) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->initialized) return fail(c, SCDAT_ERR_STATE, "autotune: call scdat_initialize first");
+  return guarded(c, [&] {
+    if (c->is_fp64()) autotune_push<double>(c);
+    else autotune_push<float>(c);
+  });
 }
 
 // ---- discrete adjoint of the charging cycle ------------------------------------

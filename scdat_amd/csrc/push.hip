@@ -30,7 +30,9 @@ namespace scdat {
 // global accumulator, so any particle order is correct, just slower.
 constexpr int kHash = 512;
 constexpr int kProbe = 8;
-constexpr int kChunks = 4;  // particles per thread in the push kernel
+// Particles per thread of the push kernel (the deposit tile is chunks x
+// kBlock consecutive particles): 1, 2, 4 or 8, chosen per problem by
+// measurement (scdat_autotune); 4 by default.
 
 struct DepositTile {
   int32_t key[kHash];
@@ -101,8 +103,8 @@ __device__ __forceinline__ void tile_runs(int key, int32_t f[4], int64_t cs,
   }
 }
 
-// One block pushes kChunks x kBlock consecutive particles.
-template <typename T>
+// One block pushes CH x kBlock consecutive particles.
+template <typename T, int CH>
 __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
   const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
   __shared__ DepositTile tile;
@@ -117,8 +119,8 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
   double ke = 0.0;
   int alive = 0, started = 0;
 #pragma unroll 1
-  for (int ch = 0; ch < kChunks; ++ch) {
-    const int64_t i = ((int64_t)lb * kChunks + ch) * kBlock + threadIdx.x;
+  for (int ch = 0; ch < CH; ++ch) {
+    const int64_t i = ((int64_t)lb * CH + ch) * kBlock + threadIdx.x;
     int key = -1;
     int32_t f[4] = {0, 0, 0, 0};
     int c = (i < a.n) ? a.cell[i] : -1;
@@ -252,8 +254,8 @@ __global__ void __launch_bounds__(kBlock) deposit_kernel(PushArgs<T> a) {
   tile_clear(tile);
   __syncthreads();
 #pragma unroll 1
-  for (int ch = 0; ch < kChunks; ++ch) {
-    const int64_t i = ((int64_t)lb * kChunks + ch) * kBlock + threadIdx.x;
+  for (int ch = 0; ch < 4; ++ch) {
+    const int64_t i = ((int64_t)lb * 4 + ch) * kBlock + threadIdx.x;
     int key = -1;
     int32_t f[4] = {0, 0, 0, 0};
     const int c = (i < a.n) ? a.cell[i] : -1;
@@ -271,18 +273,25 @@ __global__ void __launch_bounds__(kBlock) deposit_kernel(PushArgs<T> a) {
   tile_flush(tile, a.rho);
 }
 
-int push_blocks(int64_t n) { return (int)((n + (int64_t)kBlock * kChunks - 1) / ((int64_t)kBlock * kChunks)); }
+int push_blocks(int64_t n, int chunks) {
+  return (int)((n + (int64_t)kBlock * chunks - 1) / ((int64_t)kBlock * chunks));
+}
 
 template <typename T>
 void launch_push(const PushArgs<T>& a, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   if (a.n <= 0) return;
-  hipExtLaunchKernelGGL(push_kernel<T>, dim3(push_blocks(a.n)), dim3(kBlock), 0, s, start, stop, 0,
-                        a);
+  const dim3 g(push_blocks(a.n, a.chunks)), b(kBlock);
+  switch (a.chunks) {
+    case 1: hipExtLaunchKernelGGL((push_kernel<T, 1>), g, b, 0, s, start, stop, 0, a); break;
+    case 2: hipExtLaunchKernelGGL((push_kernel<T, 2>), g, b, 0, s, start, stop, 0, a); break;
+    case 8: hipExtLaunchKernelGGL((push_kernel<T, 8>), g, b, 0, s, start, stop, 0, a); break;
+    default: hipExtLaunchKernelGGL((push_kernel<T, 4>), g, b, 0, s, start, stop, 0, a); break;
+  }
 }
 template <typename T>
 void launch_deposit(const PushArgs<T>& a, hipStream_t s) {
   if (a.n <= 0) return;
-  hipLaunchKernelGGL(deposit_kernel<T>, dim3(push_blocks(a.n)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(deposit_kernel<T>, dim3(push_blocks(a.n, 4)), dim3(kBlock), 0, s, a);
 }
 template void launch_push<float>(const PushArgs<float>&, hipStream_t, hipEvent_t, hipEvent_t);
 template void launch_push<double>(const PushArgs<double>&, hipStream_t, hipEvent_t, hipEvent_t);

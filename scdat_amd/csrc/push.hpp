@@ -58,10 +58,12 @@ struct PushArgs {
   const int32_t* tet_seg;
   const T* seg_dt;
   const double* seg_wscale;
+  int chunks;  // particles per thread of the push kernel: 1, 2, 4 or 8
 };
 
-// blocks of one push/deposit launch over n particle slots (one partial each)
-int push_blocks(int64_t n);
+// blocks of one push launch over n particle slots with `chunks` particles
+// per thread (one statistics partial each)
+int push_blocks(int64_t n, int chunks);
 template <typename T>
 // with start/stop non-null they receive the kernel's dispatch timestamps
 void launch_push(const PushArgs<T>& a, hipStream_t s, hipEvent_t start = nullptr,

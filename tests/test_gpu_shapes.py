@@ -1,0 +1,46 @@
+"""Launch shapes of the push kernel: every shape gives the same bits, and the
+tuner picks one without touching the simulation state."""
+import numpy as np
+import pytest
+
+import scdat_amd as S
+from helpers import make_ctx, plasma_setup
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_push_shapes_give_identical_results(precision):
+    st = plasma_setup(n=10, npart=20000, seed=5)
+    out = []
+    for chunks in (1, 2, 4, 8):
+        c = make_ctx(st, precision=precision, tol=1e-10, sort_interval=2)
+        c.push_chunks = chunks
+        assert c.push_chunks == chunks
+        c.initialize()
+        c.step(5)
+        out.append((c.fields(), c.get_particles_by_id(0, 20000), c.stats()))
+    for f, p, s in out[1:]:
+        for a, b in zip(f, out[0][0]):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(p[0], out[0][1][0])
+        assert s["floating_potential"] == out[0][2]["floating_potential"]
+        assert s["active_particles"] == out[0][2]["active_particles"]
+
+
+def test_autotune_keeps_state():
+    st = plasma_setup(n=10, npart=20000, seed=6)
+    c = make_ctx(st, precision=0, tol=1e-10, sort_interval=0)
+    c.initialize()
+    c.step(2)
+    before = (c.fields(), c.get_particles(0), c.stats(), c.ledger())
+    k = c.autotune()
+    assert k in (1, 2, 4, 8) and c.push_chunks == k
+    after = (c.fields(), c.get_particles(0), c.stats(), c.ledger())
+    for a, b in zip(before[0], after[0]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(before[1], after[1]):
+        np.testing.assert_array_equal(a, b)
+    assert before[2] == after[2] and before[3] == after[3]
+    with pytest.raises(S.ScdatError):
+        c.push_chunks = 3
