@@ -178,7 +178,8 @@ def run_sweep(args):
     swall, spush = 0.0, 0
     t0 = time.time()
     for k, (sc, st) in enumerate(zip(sweep, cases)):
-        c = B.standalone_context(sc, st, precision=0, tol=1e-8, sort_interval=args.sort_interval, autotune=True)
+        c = B.standalone_context(sc, st, precision=0, tol=1e-8, sort_interval=args.sort_interval, autotune=True,
+                                 like_batch=False)
         w, p = timed(c)
         swall += w
         spush += p

@@ -445,6 +445,41 @@ int scdat_get_scenario_stats(scdat_ctx* ctx, int32_t scenario, scdat_scenario_st
 /* getSurfaceCurrentLedger of one scenario (scdat_get_ledger sums them). */
 int scdat_get_scenario_ledger(scdat_ctx* ctx, int32_t scenario, scdat_ledger_entry* out, int32_t n_groups);
 
+/* ---- Internal node order -----------------------------------------------------
+ * SCDAT_ORDER_GIVEN (default) keeps the caller's node order;
+ * SCDAT_ORDER_MORTON renumbers the nodes internally in Morton order of their
+ * coordinates, so that consecutive 256-node row blocks are compact boxes
+ * (the coarse space of the two-level preconditioner needs this).  On a
+ * structured box mesh numbered (i, j, k) the caller's order is faster for
+ * the Jacobi iteration (neighbour gathers of consecutive rows coalesce:
+ * 15 us against 20 us per iteration at 64^3).  Every entry point takes and
+ * returns node arrays in the caller's order.  Batched scenarios always use
+ * the caller's order.  Call before scdat_set_mesh.                         */
+#define SCDAT_ORDER_MORTON 0
+#define SCDAT_ORDER_GIVEN 1
+int scdat_set_node_order(scdat_ctx* ctx, int32_t order);
+
+/* ---- Poisson preconditioner ------------------------------------------------
+ * SCDAT_PRECOND_JACOBI (default): diagonal (MultiBoundaryElectricFieldSolver
+ * .cpp:1432-1513), one fused kernel per iteration.  SCDAT_PRECOND_TWO_LEVEL:
+ * M^-1 = D^-1 + P Ac^-1 P^T, P the free nodes of each 256-node block of the
+ * internal order (use SCDAT_ORDER_MORTON, so that blocks are compact), Ac =
+ * P^T A P inverted once at scdat_set_mesh; it
+ * needs about half the iterations but a second (coarse) launch per
+ * iteration, which on one MI355X at 273k nodes costs more than it saves.
+ * Meshes under 4096 free nodes or over 2048 blocks, and batched scenarios,
+ * always use Jacobi.  No counterpart in the reference.                      */
+#define SCDAT_PRECOND_JACOBI 0
+#define SCDAT_PRECOND_TWO_LEVEL 1
+int scdat_set_preconditioner(scdat_ctx* ctx, int32_t kind);
+typedef struct {
+  int64_t rows, nonzeros;   /* operator rows (all nodes), stored non-zeros     */
+  int32_t ell_width;        /* ELL slots per row                              */
+  int32_t preconditioner;   /* in use for the next solve                       */
+  int32_t coarse_unknowns, reserved;  /* two-level: aggregates (row blocks); 0 */
+} scdat_solver_info;
+int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
+
 /* ---- Launch shapes -----------------------------------------------------------
  * SCDAT_SHAPE_PUSH_CHUNKS: particles per thread of the push kernel, whose
  * deposit tile is chunks x 256 consecutive (cell-sorted) particles: 1, 2, 4

@@ -103,6 +103,16 @@ class SurfaceEmission(C.Structure):
 
 
 SEGMENT_ALIGN = 256
+PRECOND_JACOBI, PRECOND_TWO_LEVEL = 0, 1
+ORDER_MORTON, ORDER_GIVEN = 0, 1
+
+
+class SolverInfo(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("nonzeros", C.c_int64), ("ell_width", C.c_int32),
+                ("preconditioner", C.c_int32), ("coarse_unknowns", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 SHAPE_PUSH_CHUNKS = 0
 
 
@@ -228,6 +238,9 @@ SIGNATURES = {
     "scdat_set_launch_shape": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "scdat_get_launch_shape": (C.c_int, [_P, C.c_int32, _I32]),
     "scdat_autotune": (C.c_int, [_P]),
+    "scdat_set_preconditioner": (C.c_int, [_P, C.c_int32]),
+    "scdat_set_node_order": (C.c_int, [_P, C.c_int32]),
+    "scdat_get_solver_info": (C.c_int, [_P, C.POINTER(SolverInfo)]),
     "scdat_charging_forward": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, _D]),
     "scdat_charging_adjoint": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, C.c_int32, _D, _D]),
     "scdat_set_scenario_emission": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(SurfaceEmission)]),
@@ -541,6 +554,17 @@ class Context:
         arr = (LedgerEntry * n_groups)()
         self._chk(lib().scdat_get_scenario_ledger(self.h, scenario, arr, n_groups))
         return [e.as_dict() for e in arr]
+
+    def set_node_order(self, order):
+        self._chk(lib().scdat_set_node_order(self.h, order))
+
+    def set_preconditioner(self, kind):
+        self._chk(lib().scdat_set_preconditioner(self.h, kind))
+
+    def solver_info(self):
+        i = SolverInfo()
+        self._chk(lib().scdat_get_solver_info(self.h, C.byref(i)))
+        return i.as_dict()
 
     def autotune(self):
         """Time the push kernel's launch shapes on a scratch copy and keep the

@@ -16,8 +16,8 @@ import math
 
 import numpy as np
 
-from . import (BC_CONDUCTOR, ELECTRON_MASS, ELEMENTARY_CHARGE, PROTON_MASS, SEE_STERNGLASS, SEGMENT_ALIGN,
-               VACUUM_PERMITTIVITY, Context, box_mesh, box_spec)
+from . import (BC_CONDUCTOR, ELECTRON_MASS, ELEMENTARY_CHARGE, ORDER_GIVEN, PRECOND_JACOBI, PROTON_MASS,
+               SEE_STERNGLASS, SEGMENT_ALIGN, VACUUM_PERMITTIVITY, Context, box_mesh, box_spec)
 from .loading import maxwellian_box, philox4x32
 
 CONDUCTOR = ((0.4, 0.4, 0.4), (0.6, 0.6, 0.6))
@@ -87,11 +87,18 @@ def _emission(ctx, scenario, sc):
 
 
 def standalone_context(sc, inputs=None, precision=0, tol=1e-8, sort_interval=20, see=True, capacity_factor=2,
-                       autotune=False):
-    """The single-case context that batched case `sc` reproduces."""
+                       autotune=False, like_batch=True):
+    """The single-case context of case `sc`.  like_batch: the settings a
+    batched case runs with (the caller's node order, Jacobi-PCG), under
+    which the batched case reproduces it bit for bit; otherwise the
+    context's defaults (Morton node order, two-level preconditioner on
+    large meshes)."""
     st = inputs or case_inputs(sc)
     m = st["mesh"]
     c = Context(precision=precision, dt=st["dt"], pcg_tol=tol, sort_interval=sort_interval)
+    if like_batch:
+        c.set_node_order(ORDER_GIVEN)
+        c.set_preconditioner(PRECOND_JACOBI)
     c.set_mesh(m["xyz"], m["tets"], m["kind"], group=m["group"])
     for g in CONDUCTOR_GROUPS:
         c.set_boundary(g, BC_CONDUCTOR)

@@ -89,7 +89,86 @@ struct HostMesh {
   std::vector<int32_t> kc_idx;
   std::vector<double> kc_val;
   bool has_cond = false;
+  // two-level preconditioner (nc = 0: none): row-block aggregates
+  int nc = 0;
+  std::vector<double> dco, aci;
 };
+
+// Coarse space of the two-level preconditioner: aggregate b = the free rows
+// of row block b (256 consecutive nodes of the internal order, a compact box
+// in Morton order), Ac = P^T A P on the eliminated operator, inverted once
+// by Cholesky (blocks without free rows keep zero rows), and the diagonal D
+// on free rows (0 on constrained rows) with which the iteration kernel forms
+// A P z from the column-scaled operator.  Leaves nc = 0 for meshes too small
+// to gain from it or too large for a dense coarse inverse.
+static void build_coarse(HostMesh& h) {
+  h.nc = 0;
+  h.dco.clear();
+  h.aci.clear();
+  const int64_t nn = h.nn;
+  int64_t nfree = 0;
+  for (int64_t i = 0; i < nn; ++i) nfree += h.kind[i] == SCDAT_NODE_FREE;
+  const int G = (int)((nn + kCoarseBlockRows - 1) / kCoarseBlockRows);
+  if (nfree < 4096 || G > kMaxCoarse) return;
+  const int W = h.W;
+  std::vector<double> Ac((size_t)G * G, 0.0);
+  std::vector<char> used((size_t)G, 0);
+  for (int64_t i = 0; i < nn; ++i) {
+    if (h.kind[i] != SCDAT_NODE_FREE) continue;
+    const int bi = (int)(i / kCoarseBlockRows);
+    used[bi] = 1;
+    for (int sl = 0; sl < W; ++sl) {
+      const size_t o = (size_t)sl * nn + i;
+      const double v = h.aval[o];
+      const int32_t j = h.col[o];
+      if (v == 0.0 || h.kind[j] != SCDAT_NODE_FREE) continue;
+      Ac[(size_t)bi * G + j / kCoarseBlockRows] += v;
+    }
+  }
+  std::vector<int> idx;  // used blocks
+  for (int b = 0; b < G; ++b)
+    if (used[b]) idx.push_back(b);
+  const int m = (int)idx.size();
+  std::vector<double> L((size_t)m * m);
+  for (int r = 0; r < m; ++r)
+    for (int q = 0; q < m; ++q) L[(size_t)r * m + q] = Ac[(size_t)idx[r] * G + idx[q]];
+  for (int j = 0; j < m; ++j) {  // Cholesky, lower triangle in place
+    double d = L[(size_t)j * m + j];
+    for (int k = 0; k < j; ++k) d -= L[(size_t)j * m + k] * L[(size_t)j * m + k];
+    if (!(d > 0.0)) return;
+    d = std::sqrt(d);
+    L[(size_t)j * m + j] = d;
+    for (int i = j + 1; i < m; ++i) {
+      double t = L[(size_t)i * m + j];
+      const double* li = &L[(size_t)i * m];
+      const double* lj = &L[(size_t)j * m];
+      for (int k = 0; k < j; ++k) t -= li[k] * lj[k];
+      L[(size_t)i * m + j] = t / d;
+    }
+  }
+  // inverse: Linv lower triangular, Ac^-1 = Linv^T Linv
+  std::vector<double> Li((size_t)m * m, 0.0);
+  for (int j = 0; j < m; ++j) {
+    Li[(size_t)j * m + j] = 1.0 / L[(size_t)j * m + j];
+    for (int i = j + 1; i < m; ++i) {
+      double t = 0.0;
+      for (int k = j; k < i; ++k) t -= L[(size_t)i * m + k] * Li[(size_t)k * m + j];
+      Li[(size_t)i * m + j] = t / L[(size_t)i * m + i];
+    }
+  }
+  h.aci.assign((size_t)G * G, 0.0);
+  for (int r = 0; r < m; ++r)
+    for (int q = 0; q <= r; ++q) {
+      double t = 0.0;
+      for (int k = r; k < m; ++k) t += Li[(size_t)k * m + r] * Li[(size_t)k * m + q];
+      h.aci[(size_t)idx[r] * G + idx[q]] = t;
+      h.aci[(size_t)idx[q] * G + idx[r]] = t;
+    }
+  h.dco.assign((size_t)nn, 0.0);
+  for (int64_t i = 0; i < nn; ++i)
+    if (h.kind[i] == SCDAT_NODE_FREE) h.dco[i] = 1.0 / h.dinv[i];
+  h.nc = G;
+}
 
 static bool solve3(double M[3][3], double r[3], double out[3]) {
   for (int p = 0; p < 3; ++p) {
@@ -394,6 +473,9 @@ struct scdat_ctx {
   DevBuf d_col, d_v32, d_v64, d_vh32, d_vh64, d_dinv32, d_dinv64, d_g[3];
   DevBuf d_kind, d_lift, d_dval, d_dscale, d_phi1, d_kc_idx, d_kc_val, d_b64, d_x64, d_r64, d_partial, d_sc;
   DevBuf d_inner[8];
+  // two-level preconditioner
+  DevBuf d_dco, d_aci, d_coarse[6];
+  int precond = SCDAT_PRECOND_JACOBI;
   // fields
   DevBuf d_E, d_phiT, d_phi64, d_rho[2];
   int rho_target = 0, rho_last = 0;
@@ -441,6 +523,8 @@ struct scdat_ctx {
   DevBuf d_tet_seg, d_blk_seg, d_seg_b0, d_seg_unit, d_seg_dt, d_seg_wscale, d_seg_qinit, d_seg_ccap, d_kc_off,
       d_seg_emit, d_seg_key;
   bool seg_emit_dirty = false;
+  int node_order = SCDAT_ORDER_GIVEN;
+  std::vector<int32_t> perm, iperm;  // internal node -> caller's node, and back (empty: same order)
   int push_chunks = 4;  // particles per thread of the push kernel (scdat_autotune)
   // charging adjoint: working steps, checkpoints, adjoint vectors, snapshot
   std::deque<AdjStep> adj_st, adj_ck;
@@ -576,6 +660,20 @@ void upload_mesh_device(scdat_ctx* c) {
       c->d_g[r].upload(*g[r], s);
     }
   }
+  if (h.nc > 0) {
+    if (sizeof(T) == 4) {
+      std::vector<float> a(h.dco.begin(), h.dco.end()), b(h.aci.begin(), h.aci.end());
+      c->d_dco.upload(a, s);
+      c->d_aci.upload(b, s);
+    } else {
+      c->d_dco.upload(h.dco, s);
+      c->d_aci.upload(h.aci, s);
+    }
+    for (auto& b : c->d_coarse) {
+      b.alloc(sizeof(T) * (size_t)h.nc);
+      HIPCHK(hipMemsetAsync(b.p, 0, b.bytes, s));
+    }
+  }
   c->d_kind.upload(h.kind, s);
   c->d_lift.upload(h.lift, s);
   c->d_dval.upload(h.dval, s);
@@ -651,6 +749,17 @@ EllDev ell(scdat_ctx* c) {
     A.gy32 = c->d_g[1].as<float>();
     A.gz32 = c->d_g[2].as<float>();
   }
+  // two-level preconditioner: one scenario only (the coarse space spans the mesh)
+  if (c->precond == SCDAT_PRECOND_TWO_LEVEL && c->h.nc > 0 && !c->segmented()) {
+    A.nc = c->h.nc;
+    if (c->is_fp64()) {
+      A.dco64 = c->d_dco.as<double>();
+      A.aci64 = c->d_aci.as<double>();
+    } else {
+      A.dco32 = c->d_dco.as<float>();
+      A.aci32 = c->d_aci.as<float>();
+    }
+  }
   return A;
 }
 
@@ -708,6 +817,11 @@ InnerBufs<T> ibufs(scdat_ctx* c) {
   }
   b.p = c->d_inner[6].as<T>();
   b.x = c->d_inner[7].as<T>();
+  for (int k = 0; k < 2; ++k) {
+    b.zr[k] = c->d_coarse[k].as<T>();
+    b.zs[k] = c->d_coarse[2 + k].as<T>();
+    b.zw[k] = c->d_coarse[4 + k].as<T>();
+  }
   return b;
 }
 
@@ -1613,6 +1727,97 @@ double node_unit(const scdat_ctx* c, int64_t i) {
   return kElem * (c->w_ref * c->seg_wscale[sg]) / 4294967296.0;
 }
 
+// ---- internal node order ------------------------------------------------------
+
+// 63-bit Morton key of a point in [lo, hi]^3 (21 bits per axis).
+static uint64_t morton3(const double* p, const double* lo, const double* hi) {
+  uint64_t key = 0;
+  uint32_t q[3];
+  for (int r = 0; r < 3; ++r) {
+    const double t = (p[r] - lo[r]) / (hi[r] - lo[r] + 1e-300);
+    q[r] = (uint32_t)std::min(2097151.0, std::max(0.0, t * 2097152.0));
+  }
+  for (int b = 20; b >= 0; --b)
+    for (int r = 0; r < 3; ++r) key = (key << 1) | ((q[r] >> b) & 1u);
+  return key;
+}
+
+// Mesh preparation with the nodes in the context's internal order: Morton
+// order of the coordinates (consecutive 256-node row blocks are compact
+// boxes: cache locality for the gathers and the coarse space of the
+// two-level preconditioner), or the caller's order.  perm[internal] = the
+// caller's node index.
+void mesh_setup(scdat_ctx* c, int64_t nn, const double* xyz, int64_t nt, const int32_t* tets,
+                const int32_t* kind, const double* dval, const int32_t* group, bool morton) {
+  HostMesh& h = c->h;
+  h = HostMesh();
+  h.nn = nn;
+  h.nt = nt;
+  c->perm.clear();
+  c->iperm.clear();
+  if (morton) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = 0; i < nn; ++i)
+      for (int r = 0; r < 3; ++r) {
+        lo[r] = std::min(lo[r], xyz[3 * i + r]);
+        hi[r] = std::max(hi[r], xyz[3 * i + r]);
+      }
+    std::vector<std::pair<uint64_t, int32_t>> key((size_t)nn);
+    for (int64_t i = 0; i < nn; ++i) key[i] = {morton3(&xyz[3 * i], lo, hi), (int32_t)i};
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
+                       return a.first < b.first;
+                     });
+    c->perm.resize((size_t)nn);
+    c->iperm.resize((size_t)nn);
+    for (int64_t k = 0; k < nn; ++k) {
+      c->perm[k] = key[k].second;
+      c->iperm[key[k].second] = (int32_t)k;
+    }
+  }
+  auto user = [&](int64_t k) { return c->perm.empty() ? k : (int64_t)c->perm[k]; };
+  h.xyz.resize(3 * (size_t)nn);
+  h.kind.resize((size_t)nn);
+  h.dval.resize((size_t)nn);
+  for (int64_t k = 0; k < nn; ++k) {
+    const int64_t u = user(k);
+    for (int r = 0; r < 3; ++r) h.xyz[3 * k + r] = xyz[3 * u + r];
+    h.kind[k] = kind[u];
+    h.dval[k] = dval[u];
+  }
+  h.tets.resize(4 * (size_t)nt);
+  for (int64_t k = 0; k < 4 * nt; ++k) h.tets[k] = c->iperm.empty() ? tets[k] : c->iperm[tets[k]];
+  h.group.assign(group, group + 4 * nt);
+  prepare_mesh(h, c->cfg.poisson_operator, c->cfg.permittivity);
+  build_coarse(h);
+  if (c->is_fp64()) upload_mesh_device<double>(c);
+  else upload_mesh_device<float>(c);
+  c->has_mesh = true;
+  c->initialized = false;
+}
+
+// Back to the caller's node order (set_scenarios: scenarios are ranges of
+// the caller's node numbering).
+void mesh_natural_order(scdat_ctx* c) {
+  if (c->perm.empty()) return;
+  const HostMesh h = c->h;
+  const int64_t nn = h.nn;
+  std::vector<double> xyz(3 * (size_t)nn), dv((size_t)nn);
+  std::vector<int32_t> kind((size_t)nn), tets(h.tets.size());
+  for (int64_t k = 0; k < nn; ++k) {
+    const int32_t u = c->perm[k];
+    for (int r = 0; r < 3; ++r) xyz[3 * (size_t)u + r] = h.xyz[3 * k + r];
+    kind[u] = h.kind[k];
+    dv[u] = h.dval[k];
+  }
+  for (size_t k = 0; k < tets.size(); ++k) tets[k] = c->perm[h.tets[k]];
+  std::vector<int32_t> group(h.group);
+  mesh_setup(c, nn, xyz.data(), h.nt, tets.data(), kind.data(), dv.data(), group.data(), false);
+}
+
+// node index conversions at the API boundary
+inline int64_t to_user(const scdat_ctx* c, int64_t k) { return c->perm.empty() ? k : c->perm[k]; }
+
 // ---- discrete adjoint of the charging cycle ---------------------------------
 
 struct ChargingRun {
@@ -2124,28 +2329,17 @@ int scdat_set_mesh(scdat_ctx* c, int64_t n_nodes, const double* xyz, int64_t n_t
     return fail(c, SCDAT_ERR_ARG, "set_mesh: empty mesh or null array");
   if (n_nodes > INT32_MAX || n_tets > INT32_MAX) return fail(c, SCDAT_ERR_ARG, "set_mesh: mesh too large");
   return guarded(c, [&] {
-    HostMesh& h = c->h;
-    h = HostMesh();
+    for (int64_t k = 0; k < 4 * n_tets; ++k)
+      if (tets[k] < 0 || tets[k] >= n_nodes) throw Error(SCDAT_ERR_ARG, "set_mesh: tet references a missing node");
+    for (int64_t i = 0; i < n_nodes; ++i)
+      if (node_kind[i] < 0 || node_kind[i] > 2) throw Error(SCDAT_ERR_ARG, "set_mesh: node kind must be 0, 1 or 2");
+    std::vector<double> dv(dirichlet ? std::vector<double>(dirichlet, dirichlet + n_nodes)
+                                     : std::vector<double>((size_t)n_nodes, 0.0));
+    std::vector<int32_t> fg(face_group ? std::vector<int32_t>(face_group, face_group + 4 * n_tets)
+                                       : std::vector<int32_t>(4 * (size_t)n_tets, 0));
     c->seg_node0.clear();
     c->seg_emit.clear();
-    h.nn = n_nodes;
-    h.nt = n_tets;
-    h.xyz.assign(xyz, xyz + 3 * n_nodes);
-    h.tets.assign(tets, tets + 4 * n_tets);
-    for (int32_t v : h.tets)
-      if (v < 0 || v >= n_nodes) throw Error(SCDAT_ERR_ARG, "set_mesh: tet references a missing node");
-    h.kind.assign(node_kind, node_kind + n_nodes);
-    for (int32_t k : h.kind)
-      if (k < 0 || k > 2) throw Error(SCDAT_ERR_ARG, "set_mesh: node kind must be 0, 1 or 2");
-    h.dval.assign(n_nodes, 0.0);
-    if (dirichlet) h.dval.assign(dirichlet, dirichlet + n_nodes);
-    h.group.assign(4 * n_tets, 0);
-    if (face_group) h.group.assign(face_group, face_group + 4 * n_tets);
-    prepare_mesh(h, c->cfg.poisson_operator, c->cfg.permittivity);
-    if (c->is_fp64()) upload_mesh_device<double>(c);
-    else upload_mesh_device<float>(c);
-    c->has_mesh = true;
-    c->initialized = false;
+    mesh_setup(c, n_nodes, xyz, n_tets, tets, node_kind, dv.data(), fg.data(), c->node_order == SCDAT_ORDER_MORTON);
   });
 }
 
@@ -2377,12 +2571,18 @@ int scdat_set_efield(scdat_ctx* c, const double* E) {
     const size_t nn = (size_t)c->h.nn;
     if (c->is_fp64()) {
       std::vector<Vec4<double>> e(nn);
-      for (size_t i = 0; i < nn; ++i) e[i] = {E[3 * i], E[3 * i + 1], E[3 * i + 2], 0.0};
+      for (size_t i = 0; i < nn; ++i) {
+        const int64_t u = to_user(c, (int64_t)i);
+        e[i] = {E[3 * u], E[3 * u + 1], E[3 * u + 2], 0.0};
+      }
       HIPCHK(hipMemcpyAsync(c->d_E.p, e.data(), sizeof(Vec4<double>) * nn, hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
     } else {
       std::vector<Vec4<float>> e(nn);
-      for (size_t i = 0; i < nn; ++i) e[i] = {(float)E[3 * i], (float)E[3 * i + 1], (float)E[3 * i + 2], 0.f};
+      for (size_t i = 0; i < nn; ++i) {
+        const int64_t u = to_user(c, (int64_t)i);
+        e[i] = {(float)E[3 * u], (float)E[3 * u + 1], (float)E[3 * u + 2], 0.f};
+      }
       HIPCHK(hipMemcpyAsync(c->d_E.p, e.data(), sizeof(Vec4<float>) * nn, hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
     }
@@ -2394,7 +2594,9 @@ int scdat_set_potential(scdat_ctx* c, const double* phi) {
   if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_potential: no mesh");
   const int rc = guarded(c, [&] {
     const size_t nn = (size_t)c->h.nn;
-    HIPCHK(hipMemcpyAsync(c->d_phi64.p, phi, 8 * nn, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> p(nn);
+    for (size_t i = 0; i < nn; ++i) p[i] = phi[to_user(c, (int64_t)i)];
+    HIPCHK(hipMemcpyAsync(c->d_phi64.p, p.data(), 8 * nn, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   });
   return rc != SCDAT_OK ? rc : scdat_compute_efield(c);
@@ -2406,7 +2608,7 @@ int scdat_set_rho(scdat_ctx* c, const double* rho) {
   return guarded(c, [&] {
     const size_t nn = (size_t)c->h.nn;
     std::vector<long long> fx(nn);
-    for (size_t i = 0; i < nn; ++i) fx[i] = std::llround(rho[i] / node_unit(c, (int64_t)i));
+    for (size_t i = 0; i < nn; ++i) fx[i] = std::llround(rho[to_user(c, (int64_t)i)] / node_unit(c, (int64_t)i));
     const int k = c->rho_target;
     HIPCHK(hipMemcpyAsync(c->d_rho[k].p, fx.data(), 8 * nn, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2425,11 +2627,13 @@ int scdat_get_fields(scdat_ctx* c, double* rho, double* phi, double* E) {
       std::vector<long long> fx(nn);
       HIPCHK(hipMemcpyAsync(fx.data(), c->d_rho[c->rho_last].p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
-      for (size_t i = 0; i < nn; ++i) rho[i] = (double)fx[i] * node_unit(c, (int64_t)i);
+      for (size_t i = 0; i < nn; ++i) rho[to_user(c, (int64_t)i)] = (double)fx[i] * node_unit(c, (int64_t)i);
     }
     if (phi) {
-      HIPCHK(hipMemcpyAsync(phi, c->d_phi64.p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
+      std::vector<double> p(nn);
+      HIPCHK(hipMemcpyAsync(p.data(), c->d_phi64.p, 8 * nn, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
+      for (size_t i = 0; i < nn; ++i) phi[to_user(c, (int64_t)i)] = p[i];
     }
     if (E) {
       if (c->is_fp64()) {
@@ -2437,18 +2641,20 @@ int scdat_get_fields(scdat_ctx* c, double* rho, double* phi, double* E) {
         HIPCHK(hipMemcpyAsync(e.data(), c->d_E.p, sizeof(Vec4<double>) * nn, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < nn; ++i) {
-          E[3 * i] = e[i].x;
-          E[3 * i + 1] = e[i].y;
-          E[3 * i + 2] = e[i].z;
+          const int64_t u = to_user(c, (int64_t)i);
+          E[3 * u] = e[i].x;
+          E[3 * u + 1] = e[i].y;
+          E[3 * u + 2] = e[i].z;
         }
       } else {
         std::vector<Vec4<float>> e(nn);
         HIPCHK(hipMemcpyAsync(e.data(), c->d_E.p, sizeof(Vec4<float>) * nn, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < nn; ++i) {
-          E[3 * i] = e[i].x;
-          E[3 * i + 1] = e[i].y;
-          E[3 * i + 2] = e[i].z;
+          const int64_t u = to_user(c, (int64_t)i);
+          E[3 * u] = e[i].x;
+          E[3 * u + 1] = e[i].y;
+          E[3 * u + 2] = e[i].z;
         }
       }
     }
@@ -2461,8 +2667,10 @@ int scdat_get_rho_fixed(scdat_ctx* c, int64_t* rho_fx, double* unit) {
   return guarded(c, [&] {
     if (unit) *unit = c->unit;
     if (rho_fx) {
-      HIPCHK(hipMemcpyAsync(rho_fx, c->d_rho[c->rho_last].p, 8 * (size_t)c->h.nn, hipMemcpyDeviceToHost, c->stream));
+      std::vector<int64_t> fx((size_t)c->h.nn);
+      HIPCHK(hipMemcpyAsync(fx.data(), c->d_rho[c->rho_last].p, 8 * fx.size(), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
+      for (int64_t i = 0; i < c->h.nn; ++i) rho_fx[to_user(c, i)] = fx[i];
     }
   });
 }
@@ -2699,6 +2907,7 @@ int scdat_set_scenarios(scdat_ctx* c, int32_t n, const int64_t* node_start, cons
     if (S.n > 0) return fail(c, SCDAT_ERR_STATE, "set_scenarios: call before set_particles");
   if (c->n_patch > 0) return fail(c, SCDAT_ERR_STATE, "set_scenarios: conductor patches need a single scenario");
   return guarded(c, [&] {
+    mesh_natural_order(c);  // scenarios are ranges of the caller's node numbering
     const HostMesh& h = c->h;
     const int64_t align = SCDAT_SEGMENT_ALIGN;
     static_assert(SCDAT_SEGMENT_ALIGN == kBlock, "scenario rows align with the row blocks");
@@ -2819,6 +3028,40 @@ int scdat_get_scenario_ledger(scdat_ctx* c, int32_t scenario, scdat_ledger_entry
 
 // ---- launch shapes ---------------------------------------------------------------
 
+int scdat_set_node_order(scdat_ctx* c, int32_t order) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (order != SCDAT_ORDER_MORTON && order != SCDAT_ORDER_GIVEN)
+    return fail(c, SCDAT_ERR_ARG, "set_node_order: unknown order");
+  if (c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_node_order: call before scdat_set_mesh");
+  c->node_order = order;
+  return SCDAT_OK;
+}
+
+int scdat_set_preconditioner(scdat_ctx* c, int32_t kind) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (kind != SCDAT_PRECOND_JACOBI && kind != SCDAT_PRECOND_TWO_LEVEL)
+    return fail(c, SCDAT_ERR_ARG, "set_preconditioner: unknown kind");
+  c->precond = kind;
+  return SCDAT_OK;
+}
+
+int scdat_get_solver_info(scdat_ctx* c, scdat_solver_info* out) {
+  if (!c || !out) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "get_solver_info: no mesh");
+  std::memset(out, 0, sizeof(*out));
+  const EllDev A = ell(c);
+  out->rows = A.n;
+  out->ell_width = A.w;
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < c->h.nn; ++i)
+    for (int sl = 0; sl < c->h.W; ++sl) nnz += c->h.aval[(size_t)sl * c->h.nn + i] != 0.0;
+  out->nonzeros = nnz;
+  out->coarse_unknowns = A.nc;
+  out->reserved = 0;
+  out->preconditioner = A.nc > 0 ? SCDAT_PRECOND_TWO_LEVEL : SCDAT_PRECOND_JACOBI;
+  return SCDAT_OK;
+}
+
 int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
   if (!c) return SCDAT_ERR_ARG;
   if (what != SCDAT_SHAPE_PUSH_CHUNKS || !(value == 1 || value == 2 || value == 4 || value == 8))
@@ -2899,7 +3142,7 @@ int scdat_set_conductor_circuit(scdat_ctx* c, int32_t n_patches, const int32_t* 
     const HostMesh& h = c->h;
     std::vector<int32_t> np((size_t)h.nn, -1);
     if (node_patch) {
-      np.assign(node_patch, node_patch + h.nn);
+      for (int64_t i = 0; i < h.nn; ++i) np[i] = node_patch[to_user(c, i)];
     } else {
       // six conductor faces (groups 6..11 -> patches 0..5) and the body
       // (patch 6: nodes on more than one face)
@@ -2951,9 +3194,11 @@ int scdat_get_patch_basis(scdat_ctx* c, int32_t patch, double* phi) {
   if (patch < 0 || patch >= c->n_patch || !c->initialized)
     return fail(c, SCDAT_ERR_STATE, "get_patch_basis: no such patch or not initialised");
   return guarded(c, [&] {
-    HIPCHK(hipMemcpyAsync(phi, c->d_phik.as<double>() + (size_t)patch * c->h.nn, 8 * (size_t)c->h.nn,
+    std::vector<double> b((size_t)c->h.nn);
+    HIPCHK(hipMemcpyAsync(b.data(), c->d_phik.as<double>() + (size_t)patch * c->h.nn, 8 * (size_t)c->h.nn,
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < c->h.nn; ++i) phi[to_user(c, i)] = b[i];
   });
 }
 

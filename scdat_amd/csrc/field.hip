@@ -18,6 +18,8 @@
 // order (wave shuffles, block partials summed by the next kernel), so a solve
 // is bitwise reproducible.  Per iteration: one kernel (Chronopoulos-Gear CG,
 // below).
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "field.hpp"
 
@@ -38,6 +40,13 @@ __device__ inline const float* dinv<float>(const EllDev& A) { return A.dinv32; }
 template <>
 __device__ inline const double* dinv<double>(const EllDev& A) { return A.dinv64; }
 
+template <typename T>
+__device__ inline const T* dco(const EllDev& A);
+template <>
+__device__ inline const float* dco<float>(const EllDev& A) { return A.dco32; }
+template <>
+__device__ inline const double* dco<double>(const EllDev& A) { return A.dco64; }
+
 inline uint32_t grid_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 // Reductions.  Every reducing kernel publishes one partial per block; the
@@ -49,7 +58,7 @@ inline uint32_t grid_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBloc
 constexpr int kPB = kBlock;           // PCG block: 4 waves
 constexpr int kRPT = 1;  // rows per thread in the row-parallel kernels (1: 14.7 us, 2: 19.7 us per iteration at 64^3)
 constexpr int kRowsPerBlock = kRPT * kPB;
-enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_COUNT };
+enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_WC, S_COUNT };
 
 __device__ inline double* slot(const FieldBufs& f, int s, int G) { return f.partial + (size_t)s * G; }
 
@@ -218,17 +227,93 @@ __global__ void __launch_bounds__(kPB) k_fin_resid(FieldBufs f, int G) {
 // from (r_i, w_i, s_{i-1}) with the same fused operations the owner uses, so
 // w_{i+1} = (A D^-1) r_{i+1} needs no second launch.  r, w and s are
 // double-buffered by iteration parity; p and x are owner-only, in place.
+static_assert(kRowsPerBlock == kCoarseBlockRows, "the two-level aggregates are the row blocks");
+constexpr int kCoarseRowsPerWave = 2;  // rows of Ac^-1 per wave in the coarse kernel
+
+// Restriction of the two-level preconditioner: block b's partial of P^T v is
+// the sum of v over its free rows, published like the dot products.
+__device__ inline void publish_restrict(double v, double* out, double* lds) {
+  const double t = block_sum(v, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kPB) k_inner_init(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   const PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
   if (sc->outer_done) return;
   const double inv = sc->inv_rnorm;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  double rv = 0.0;
   for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
-    ib.r[0][i] = (T)(f.r64[i] * inv);
+    const T ri = (T)(f.r64[i] * inv);
+    ib.r[0][i] = ri;
     ib.s[1][i] = T(0);
     ib.p[i] = T(0);
     ib.x[i] = T(0);
+    if (A.nc && dco<T>(A)[i] != T(0)) rv += (double)ri;
+  }
+  if (A.nc) {
+    __shared__ double lds[kPB / 64];
+    publish_restrict(rv, slot(f, S_WC, gridDim.x), lds);
+  }
+}
+
+// Coarse step of the two-level preconditioner (a few blocks, after the
+// kernel that published the restriction partials R = P^T v):
+//   mode 0 (after k_inner_init): zr[0] = Ac^-1 R, zs[1] = 0;
+//   mode 1 (after k_cg_start):   zw[0] = Ac^-1 R;
+//   mode 2 (after iteration it): zs_i = zw_i + b zs_{i-1}, zr_{i+1} = zr_i -
+//     a zs_i (the recurrences the iteration kernel evaluates on the fly, in
+//     the same fused operations) and zw_{i+1} = Ac^-1 R.
+template <typename T>
+__global__ void __launch_bounds__(kPB) k_coarse(EllDev A, FieldBufs f, InnerBufs<T> ib, int mode, int it, int G) {
+  const PcgScalars* sc = f.sc;
+  if (sc->outer_done || (mode == 2 && sc->inner_done)) return;
+  __shared__ T wc[kMaxCoarse];
+  const int nc = A.nc;
+  const double* part = slot(f, S_WC, G);
+  for (int c = threadIdx.x; c < nc; c += kPB) wc[c] = (T)part[c];
+  __syncthreads();
+  const int cur = it & 1, nxt = cur ^ 1;
+  T a = T(0), b = T(0);
+  if (mode == 2) {
+    const double g = sc->hist[cur];
+    a = (T)sc->hist[2 + cur];
+    b = it > 0 ? (T)(g / sc->hist[1 - cur]) : T(0);
+  }
+  const T* aci;
+  if constexpr (sizeof(T) == 4) aci = reinterpret_cast<const T*>(A.aci32);
+  else aci = reinterpret_cast<const T*>(A.aci64);
+  T* out = mode == 0 ? ib.zr[0] : (mode == 1 ? ib.zw[0] : ib.zw[nxt]);
+  constexpr int NW = kPB / 64;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // kCoarseRowsPerWave rows per wave, their loads interleaved
+  const int c0 = (blockIdx.x * NW + wv) * kCoarseRowsPerWave;
+  T t[kCoarseRowsPerWave];
+#pragma unroll
+  for (int q = 0; q < kCoarseRowsPerWave; ++q) t[q] = T(0);
+  for (int k = lane; k < nc; k += 64) {
+    const T x = wc[k];
+#pragma unroll
+    for (int q = 0; q < kCoarseRowsPerWave; ++q)
+      if (c0 + q < nc) t[q] = fma_t(aci[(size_t)(c0 + q) * nc + k], x, t[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < kCoarseRowsPerWave; ++q)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t[q] += __shfl_xor(t[q], off, 64);
+  for (int q = 0; q < kCoarseRowsPerWave; ++q) {
+    const int c = c0 + q;
+    if (c >= nc) break;
+    if (lane == 0) {
+      out[c] = t[q];
+      if (mode == 0) ib.zs[1][c] = T(0);
+      if (mode == 2) {
+        const T zs = fma_t(b, ib.zs[nxt][c], ib.zw[cur][c]);
+        ib.zs[cur][c] = zs;
+        ib.zr[nxt][c] = fma_t(-a, zs, ib.zr[cur][c]);
+      }
+    }
   }
 }
 
@@ -276,27 +361,37 @@ __global__ void __launch_bounds__(kPB) k_cg_start(EllDev A, FieldBufs f, InnerBu
   __shared__ double lds[3 * (kPB / 64)];
   const T* __restrict__ v = vhat<T>(A);
   const T* __restrict__ r = ib.r[0];
-  double g = 0.0, d = 0.0, rr = 0.0;
+  double g = 0.0, d = 0.0, rr = 0.0, wv = 0.0;
   const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
   for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
     T acc = T(0);
+    // w_0 = A u_0, u_0 = D^-1 r_0 + P zr_0: column j carries r_j + D_j zr_0[block of j]
     for (int k = 0; k < A.w; ++k) {
       const int64_t o = (int64_t)k * A.n + i;
-      acc = fma_t(v[o], r[A.col[o]], acc);
+      const int j = A.col[o];
+      const T uj = A.nc ? fma_t(dco<T>(A)[j], ib.zr[0][j / kRowsPerBlock], r[j]) : r[j];
+      acc = fma_t(v[o], uj, acc);
     }
     const T ri = r[i];
-    const T u = dinv<T>(A)[i] * ri;
+    T u = dinv<T>(A)[i] * ri;
+    if (A.nc && dco<T>(A)[i] != T(0)) {
+      u += ib.zr[0][blockIdx.x];
+      wv += (double)acc;
+    }
     ib.w[0][i] = acc;
     g += (double)ri * (double)u;
     d += (double)acc * (double)u;
     rr += (double)ri * (double)ri;
   }
   publish3(g, d, rr, slot(f, S_G0, G), slot(f, S_D0, G), slot(f, S_RR0, G), lds);
+  if (A.nc) publish_restrict(wv, slot(f, S_WC, G), lds);
 }
 
 // One iteration.  W > 0: compile-time ELL width, fully unrolled, with the 2W
 // coalesced slot loads issued before the scalar prologue; W = 0: runtime width.
-template <typename T, int W>
+// TL: two-level preconditioner (coarse terms from the coarse vectors of
+// this pass; restriction partials of w_{i+1} for the coarse kernel).
+template <typename T, int W, bool TL>
 __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBufs<T> ib, int it) {
   __shared__ double lds[3 * (kPB / 64)];
   __shared__ double tot[3];
@@ -384,36 +479,52 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
   const T* __restrict__ rc = ib.r[cur];
   const T* __restrict__ wc = ib.w[cur];
   const T* __restrict__ sp = ib.s[nxt];
-  double g1 = 0.0, d1 = 0.0, rr1 = 0.0;
+  double g1 = 0.0, d1 = 0.0, rr1 = 0.0, wrest = 0.0;
+  // zr_{i+1}[c] from this pass's coarse vectors (same operations as k_coarse)
+  auto zr1 = [&](int c) { return fma_t(-a, fma_t(b, ib.zs[nxt][c], ib.zw[cur][c]), ib.zr[cur][c]); };
 #pragma unroll
   for (int h = 0; h < kRPT; ++h) {
     const int64_t i = r0 + h * kPB;
     if (i >= n) continue;
     T acc = T(0);
+    // two-level: column j carries D_j u_j = r_j + D_j zr_{i+1}[block of j]
     if constexpr (W > 0) {
 #pragma unroll
       for (int k = 0; k < W; ++k) {
         const int j = cj[h][k];
-        acc = fma_t(vj[h][k], r_next(rc[j], wc[j], sp[j], a, b), acc);
+        T uj = r_next(rc[j], wc[j], sp[j], a, b);
+        if constexpr (TL) uj = fma_t(dco<T>(A)[j], zr1(j / kRowsPerBlock), uj);
+        acc = fma_t(vj[h][k], uj, acc);
       }
     } else {
       for (int k = 0; k < A.w; ++k) {
         const int64_t o = (int64_t)k * n + i;
         const int j = col[o];
-        acc = fma_t(v[o], r_next(rc[j], wc[j], sp[j], a, b), acc);
+        T uj = r_next(rc[j], wc[j], sp[j], a, b);
+        if constexpr (TL) uj = fma_t(dco<T>(A)[j], zr1(j / kRowsPerBlock), uj);
+        acc = fma_t(v[o], uj, acc);
       }
     }
     const T ri = rc[i], wi = wc[i], so = sp[i];
     const T si = fma_t(b, so, wi);
     const T rn = fma_t(-a, si, ri);
     const T di = dinv<T>(A)[i];
-    const T pi = fma_t(b, ib.p[i], di * ri);
+    T ui = di * ri, uc = T(0);
+    if constexpr (TL) {
+      // u = D^-1 r + P zr on free rows (this row's aggregate is its block)
+      if (dco<T>(A)[i] != T(0)) {
+        ui += ib.zr[cur][blockIdx.x];
+        uc = zr1(blockIdx.x);
+        wrest = (double)acc;
+      }
+    }
+    const T pi = fma_t(b, ib.p[i], ui);
     ib.x[i] = fma_t(a, pi, ib.x[i]);
     ib.p[i] = pi;
     ib.s[cur][i] = si;
     ib.r[nxt][i] = rn;
     ib.w[nxt][i] = acc;
-    const T un = di * rn;
+    const T un = di * rn + uc;
     g1 += (double)rn * (double)un;
     d1 += (double)acc * (double)un;
     rr1 += (double)rn * (double)rn;
@@ -425,6 +536,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
     sc->hist[2 + cur] = alpha;
     sc->inner_it = it + 1;
   }
+  if constexpr (TL) publish_restrict(wrest, slot(f, S_WC, G), lds);
 }
 
 // Closes an inner solve: records its iteration count and applies
@@ -580,6 +692,11 @@ __global__ void __launch_bounds__(kBlock) k_efield(EllDev A, const T* __restrict
 }  // namespace
 
 static inline int grid_rows(int64_t n) { return (int)((n + kRowsPerBlock - 1) / kRowsPerBlock); }
+// blocks of the coarse kernel: each recomputes the restricted vector, so few
+static inline int coarse_blocks(const EllDev& A) {
+  const int per_block = (kPB / 64) * kCoarseRowsPerWave;
+  return (A.nc + per_block - 1) / per_block;
+}
 
 void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* rho_fx,
                 double unit, int load_sign, hipStream_t s) {
@@ -602,23 +719,39 @@ template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_inner_init<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
+  if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 0, 0, grid_rows(A.n));
   hipLaunchKernelGGL(k_cg_start<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
+  if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 1, 0, grid_rows(A.n));
 }
 template <typename T>
 void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
                        hipStream_t s, hipEvent_t* ev) {
   const dim3 g(grid_rows(A.n)), b(kPB);
   hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
+  if (A.nc) {
+    switch (A.w) {
+#define SCDAT_W(w) \
+  case w:          \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w, true>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    break;
+      SCDAT_W(13) SCDAT_W(14) SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18)
+#undef SCDAT_W
+      default:
+        hipExtLaunchKernelGGL((k_cg_iter<T, 0, true>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
+    }
+    hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 2, it, grid_rows(A.n));
+    return;
+  }
   switch (A.w) {
 #define SCDAT_W(w) \
   case w:          \
-    hipExtLaunchKernelGGL((k_cg_iter<T, w>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
     break;
     SCDAT_W(7) SCDAT_W(8) SCDAT_W(9) SCDAT_W(10) SCDAT_W(11) SCDAT_W(12) SCDAT_W(13) SCDAT_W(14)
     SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18) SCDAT_W(19) SCDAT_W(20)
 #undef SCDAT_W
     default:
-      hipExtLaunchKernelGGL((k_cg_iter<T, 0>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
+      hipExtLaunchKernelGGL((k_cg_iter<T, 0, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
   }
 }
 template <typename T>

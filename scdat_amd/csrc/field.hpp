@@ -8,6 +8,9 @@
 
 namespace scdat {
 
+constexpr int kMaxCoarse = 2048;       // row blocks of the two-level preconditioner
+constexpr int kCoarseBlockRows = 256;  // rows per PCG block = nodes per aggregate
+
 // Scalars of one solve, resident on the device; the host polls `inner_done`
 // and `outer_done` between batches of iterations.
 struct PcgScalars {
@@ -40,6 +43,15 @@ struct EllDev {
   const double* gx64;
   const double* gy64;
   const double* gz64;
+  // Two-level preconditioner M^-1 = D^-1 + P Ac^-1 P^T (nc = 0: Jacobi only).
+  // The aggregates are the row blocks: P maps block b to its free rows (the
+  // context numbers nodes in Morton order, so a block is a compact box),
+  // Ac = P^T A P over the blocks (empty blocks: zero rows).
+  int nc;                  // row blocks (coarse vector length)
+  const float* dco32;      // 1 / D^-1 on free rows, 0 on constrained rows
+  const double* dco64;
+  const float* aci32;      // Ac^-1, nc x nc row-major
+  const double* aci64;
 };
 
 struct FieldBufs {
@@ -82,6 +94,11 @@ struct InnerBufs {
   T* s[2];
   T* p;
   T* x;
+  // two-level: coarse vectors Ac^-1 P^T r, Ac^-1 P^T s, Ac^-1 P^T w by
+  // iteration parity (one entry per row block)
+  T* zr[2];
+  T* zs[2];
+  T* zw[2];
 };
 
 void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* rho_fx,

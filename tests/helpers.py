@@ -40,10 +40,12 @@ def make_oracle(st, tol=1e-12, deposit_mode=0, adaptive=1, max_sub=64):
     return o
 
 
-def make_ctx(st, precision=0, tol=1e-8, sort_interval=20, adaptive=1):
+def make_ctx(st, precision=0, tol=1e-8, sort_interval=20, adaptive=1, order=None):
     m = st["mesh"]
     c = S.Context(precision=precision, dt=st["dt"], pcg_tol=tol, sort_interval=sort_interval,
                   adaptive_substep=adaptive)
+    if order is not None:
+        c.set_node_order(order)
     c.set_mesh(m["xyz"], m["tets"], m["kind"], group=m["group"])
     for g in range(6, 12):
         c.set_boundary(g, S.BC_CONDUCTOR)
