@@ -445,6 +445,52 @@ int scdat_get_scenario_stats(scdat_ctx* ctx, int32_t scenario, scdat_scenario_st
 /* getSurfaceCurrentLedger of one scenario (scdat_get_ledger sums them). */
 int scdat_get_scenario_ledger(scdat_ctx* ctx, int32_t scenario, scdat_ledger_entry* out, int32_t n_groups);
 
+/* ---- Mesh import and result export (host, no device) -----------------------
+ * Readers for the formats of GmshLoader / VTKLoader / STLLoader
+ * (Tools/Mesh/src/MeshParsing.cpp:651-1091): Gmsh ASCII 2.x and 4.x
+ * (triangles and tetrahedra with their physical tags), VTK legacy ASCII
+ * (CELL_TYPES 5 and 10), STL ASCII (triangles, vertices merged on a 1e-12
+ * grid).  SCDAT_MESH_AUTO picks the reader from the extension (msh, vtk,
+ * stl).  Errors: SCDAT_ERR_ARG with the message in scdat_io_last_error.   */
+#define SCDAT_MESH_AUTO 0
+#define SCDAT_MESH_GMSH 1
+#define SCDAT_MESH_VTK 2
+#define SCDAT_MESH_STL 3
+typedef struct scdat_mesh_data scdat_mesh_data;
+const char* scdat_io_last_error(void);
+int scdat_mesh_read(const char* path, int32_t format, scdat_mesh_data** out);
+void scdat_mesh_free(scdat_mesh_data* m);
+int scdat_mesh_sizes(const scdat_mesh_data* m, int64_t* n_nodes, int64_t* n_tets, int64_t* n_tris);
+/* xyz[3 n], tets[4 n_tets], tet_tag[n_tets], tris[3 n_tris], tri_tag[n_tris]; any may be NULL */
+int scdat_mesh_arrays(const scdat_mesh_data* m, double* xyz, int32_t* tets, int32_t* tet_tag, int32_t* tris,
+                      int32_t* tri_tag);
+/* face_group[4 n_tets] for scdat_set_mesh: -1 on interior faces; on a
+ * boundary face, groups[k] when the triangle with the same nodes has
+ * physical tag tags[k], else default_group.                               */
+int scdat_mesh_face_groups(const scdat_mesh_data* m, int32_t n_map, const int32_t* tags, const int32_t* groups,
+                           int32_t default_group, int32_t* face_group, int64_t* n_matched);
+/* Tetrahedral mesh with point fields (1 or 3 components) as a legacy ASCII
+ * VTK unstructured grid.                                                   */
+int scdat_write_vtk(const char* path, int64_t n_nodes, const double* xyz, int64_t n_tets, const int32_t* tets,
+                    int32_t n_fields, const char* const* names, const int32_t* components, const double* const* data);
+/* The columnar output contract of ResultExporter::exportDataSet
+ * (Tools/Output/src/ResultExporter.cpp:176-230): csv_path, and beside it
+ * <stem>.vtk (poly-data points along the axis) and <stem>.h5 (the text
+ * archive), each with a .metadata.json sidecar; series in name order.      */
+int scdat_export_columns(const char* csv_path, const char* module, const char* axis_name, int64_t n_rows,
+                         const double* axis, int32_t n_series, const char* const* names, const double* const* series);
+/* Layer profile of main_PICcore.cpp:230-310 (collectLayerProfile): nodes
+ * grouped by their coordinate along `axis` (0..2, rounded to 1e-9 m); per
+ * layer the mean potential and mean field component along the axis, and
+ * from the particles (binned by llround(clamp(x, 0, length) / (length /
+ * n_cells))) the electron and ion densities (weight / (area length /
+ * n_cells)) and the electron temperature (2/3 of the weighted mean kinetic
+ * energy, eV).  scdat_layer_count gives the number of layers.              */
+int scdat_layer_count(scdat_ctx* ctx, int32_t axis, int64_t* n_layers);
+int scdat_layer_profile(scdat_ctx* ctx, int32_t axis, double length, int32_t n_cells, double area, double* position,
+                        double* potential, double* field, double* electron_density, double* ion_density,
+                        double* electron_temperature_ev);
+
 /* ---- Internal node order -----------------------------------------------------
  * SCDAT_ORDER_GIVEN (default) keeps the caller's node order;
  * SCDAT_ORDER_MORTON renumbers the nodes internally in Morton order of their

@@ -14,6 +14,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "_ref", "libscdat_ref.so")
 _D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
 _I = C.POINTER(C.c_int32)
 _lib = None
 
@@ -179,3 +181,20 @@ def circuit_advance(nodes, branches, dt, current, didv, rhs=None, diag=None, ent
                                            _p(dg, C.c_double), len(en), _p(en, C.c_double), max_delta,
                                            _p(v, C.c_double), _p(i, C.c_double), _p(b, C.c_double))
     return rc, v, i, b[:len(br)]
+
+
+def mesh_read(path, fmt, cap_nodes=100000, cap_el=400000):
+    """The reference's GmshLoader (fmt 1) / VTKLoader (2) / STLLoader (3):
+    (xyz, element type, element nodes (4, -1 padded), material id)."""
+    L = lib()
+    L.ref_mesh_read.argtypes = [C.c_char_p, C.c_int, C.c_int64, C.c_int64, _I64, _I64, _D, _I32, _I32, _I32]
+    nn, ne = C.c_int64(), C.c_int64()
+    xyz = np.zeros((cap_nodes, 3))
+    et = np.zeros(cap_el, np.int32)
+    en = np.zeros((cap_el, 4), np.int32)
+    tg = np.zeros(cap_el, np.int32)
+    rc = L.ref_mesh_read(str(path).encode(), int(fmt), cap_nodes, cap_el, C.byref(nn), C.byref(ne),
+                         _p(xyz, C.c_double), _p(et, C.c_int32), _p(en, C.c_int32), _p(tg, C.c_int32))
+    if rc != 0:
+        raise RuntimeError("reference mesh loader failed (%d)" % rc)
+    return xyz[:nn.value], et[:ne.value], en[:ne.value], tg[:ne.value]

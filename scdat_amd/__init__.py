@@ -240,6 +240,18 @@ SIGNATURES = {
     "scdat_autotune": (C.c_int, [_P]),
     "scdat_set_preconditioner": (C.c_int, [_P, C.c_int32]),
     "scdat_set_node_order": (C.c_int, [_P, C.c_int32]),
+    "scdat_io_last_error": (C.c_char_p, []),
+    "scdat_mesh_read": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "scdat_mesh_free": (None, [_P]),
+    "scdat_mesh_sizes": (C.c_int, [_P, _I64, _I64, _I64]),
+    "scdat_mesh_arrays": (C.c_int, [_P, _D, _I32, _I32, _I32, _I32]),
+    "scdat_mesh_face_groups": (C.c_int, [_P, C.c_int32, _I32, _I32, C.c_int32, _I32, _I64]),
+    "scdat_write_vtk": (C.c_int, [C.c_char_p, C.c_int64, _D, C.c_int64, _I32, C.c_int32,
+                                  C.POINTER(C.c_char_p), _I32, C.POINTER(_D)]),
+    "scdat_export_columns": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64, _D, C.c_int32,
+                                       C.POINTER(C.c_char_p), C.POINTER(_D)]),
+    "scdat_layer_count": (C.c_int, [_P, C.c_int32, _I64]),
+    "scdat_layer_profile": (C.c_int, [_P, C.c_int32, C.c_double, C.c_int32, C.c_double, _D, _D, _D, _D, _D, _D]),
     "scdat_get_solver_info": (C.c_int, [_P, C.POINTER(SolverInfo)]),
     "scdat_charging_forward": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, _D]),
     "scdat_charging_adjoint": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, C.c_int32, _D, _D]),
@@ -554,6 +566,18 @@ class Context:
         arr = (LedgerEntry * n_groups)()
         self._chk(lib().scdat_get_scenario_ledger(self.h, scenario, arr, n_groups))
         return [e.as_dict() for e in arr]
+
+    def layer_profile(self, axis=2, length=1.0, n_cells=1, area=1.0):
+        """collectLayerProfile of the PIC harness: per node layer along `axis`
+        the mean potential and field component, electron and ion densities and
+        the electron temperature (eV)."""
+        n = C.c_int64()
+        self._chk(lib().scdat_layer_count(self.h, axis, C.byref(n)))
+        out = {k: np.zeros(n.value) for k in ("position", "potential_v", "electric_field_v_per_m",
+                                             "electron_density_m3", "ion_density_m3", "electron_temperature_ev")}
+        self._chk(lib().scdat_layer_profile(self.h, axis, float(length), int(n_cells), float(area),
+                                            *[_ptr(out[k], C.c_double) for k in out]))
+        return out
 
     def set_node_order(self, order):
         self._chk(lib().scdat_set_node_order(self.h, order))

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libscdat_amd.so")
 
 HIP_SOURCES = ["context.hip", "push.hip", "field.hip", "sort.hip", "emit.hip", "adjoint.hip"]
-CXX_SOURCES = ["mesh_host.cpp", "host_tools.cpp", "circuit.cpp", "sampler.cpp"]
+CXX_SOURCES = ["mesh_host.cpp", "host_tools.cpp", "circuit.cpp", "sampler.cpp", "mesh_io.cpp"]
 
 ARCH = "gfx950"
 HIPCC_FLAGS = [

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <limits>
+#include <map>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -2893,6 +2894,86 @@ int scdat_get_event_counts(scdat_ctx* c, scdat_event_counts* out) {
     out->photo_emitted = (int64_t)v[C_PHOTO];
     out->overflow = (int64_t)v[C_OVERFLOW];
     out->impacts_dropped = c->events_dropped;
+  });
+}
+
+// ---- layer profile (result export) ----------------------------------------------
+
+namespace {
+std::map<double, std::vector<int64_t>> layer_nodes(const scdat_ctx* c, int axis) {
+  std::map<double, std::vector<int64_t>> layers;
+  for (int64_t i = 0; i < c->h.nn; ++i)
+    layers[(double)std::llround(c->h.xyz[3 * i + axis] * 1.0e9) * 1.0e-9].push_back(i);
+  return layers;
+}
+}  // namespace
+
+int scdat_layer_count(scdat_ctx* c, int32_t axis, int64_t* n) {
+  if (!c || !n || axis < 0 || axis > 2) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "layer_count: no mesh");
+  return guarded(c, [&] { *n = (int64_t)layer_nodes(c, axis).size(); });
+}
+
+int scdat_layer_profile(scdat_ctx* c, int32_t axis, double length, int32_t n_cells, double area, double* position,
+                        double* potential, double* field, double* electron_density, double* ion_density,
+                        double* electron_temperature_ev) {
+  if (!c || axis < 0 || axis > 2 || !(length > 0.0) || n_cells < 1 || !(area > 0.0)) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "layer_profile: no mesh");
+  return guarded(c, [&] {
+    const auto layers = layer_nodes(c, axis);
+    const size_t nl = layers.size(), nn = (size_t)c->h.nn;
+    std::vector<double> phi(nn), E(3 * nn);
+    {
+      std::vector<double> pu(nn), eu(3 * nn);
+      const int rc = scdat_get_fields(c, nullptr, pu.data(), eu.data());
+      if (rc != SCDAT_OK) throw Error(rc, c->err);
+      for (size_t i = 0; i < nn; ++i) {  // back to the internal numbering of h
+        const int64_t u = to_user(c, (int64_t)i);
+        phi[i] = pu[u];
+        for (int r = 0; r < 3; ++r) E[3 * i + r] = eu[3 * u + r];
+      }
+    }
+    size_t k = 0;
+    for (const auto& L : layers) {
+      double ps = 0.0, fs = 0.0;
+      for (int64_t i : L.second) {
+        ps += phi[i];
+        fs += E[3 * i + axis];
+      }
+      if (position) position[k] = L.first;
+      if (potential) potential[k] = ps / (double)L.second.size();
+      if (field) field[k] = fs / (double)L.second.size();
+      ++k;
+    }
+    const double dz = length / (double)n_cells, vol = area * dz;
+    std::vector<double> ne(nl, 0.0), ni(nl, 0.0), acc(nl, 0.0);
+    for (size_t si = 0; si < c->sp.size(); ++si) {
+      const SpeciesState& S = c->sp[si];
+      if (S.n == 0 || S.q == 0.0) continue;
+      std::vector<double> x(3 * (size_t)S.n), v(3 * (size_t)S.n);
+      std::vector<int32_t> cell((size_t)S.n), id((size_t)S.n);
+      if (c->is_fp64()) get_particles_t<double>(c, (int)si, x.data(), v.data(), cell.data(), id.data());
+      else get_particles_t<float>(c, (int)si, x.data(), v.data(), cell.data(), id.data());
+      for (int64_t p = 0; p < S.n; ++p) {
+        if (cell[p] < 0) continue;
+        const double w = S.w * (c->segmented() ? c->seg_wscale[c->tet_seg_h[cell[p]]] : 1.0);
+        const double z = std::min(std::max(x[3 * p + axis], 0.0), length);
+        const long long b = std::min<long long>(std::max<long long>(std::llround(z / dz), 0), (long long)nl - 1);
+        if (S.q < 0.0) {
+          ne[b] += w / vol;
+          const double ke = 0.5 * S.m * (v[3 * p] * v[3 * p] + v[3 * p + 1] * v[3 * p + 1] + v[3 * p + 2] * v[3 * p + 2]);
+          acc[b] += w * ke / kElem;
+        } else {
+          ni[b] += w / vol;
+        }
+      }
+    }
+    for (size_t b = 0; b < nl; ++b) {
+      if (electron_density) electron_density[b] = ne[b];
+      if (ion_density) ion_density[b] = ni[b];
+      if (electron_temperature_ev)
+        electron_temperature_ev[b] = ne[b] > 0.0 ? acc[b] / std::max(1.0, ne[b] * vol) / 1.5 : 0.0;
+    }
   });
 }
 

@@ -22,7 +22,13 @@ the reference's outputs of one routine:
                         circuits (fixed nodes, biased branches, extra entries,
                         step limits)
 
-Usage: python tests/golden/gen_golden.py [field|surface|circuit]   (default: all)
+  mesh/box3_v2.msh, box3_v41.msh, box3.vtk, tet.stl and mesh_ref.npz
+                        small meshes in the Gmsh 2.2 / 4.1, VTK legacy and STL
+                        formats (written here from the box mesher, boundary
+                        triangles tagged 200 + group) and what the reference's
+                        GmshLoader / VTKLoader / STLLoader read from them
+
+Usage: python tests/golden/gen_golden.py [field|surface|circuit|mesh]   (default: all)
 """
 import os
 import sys
@@ -195,6 +201,101 @@ def gen_circuit():
     save("circuit_cases.npz", n_cases=ncase, **out)
 
 
+def boundary_triangles(m):
+    """(tri nodes, tag = 200 + group) of every boundary face of a box mesh."""
+    tris, tags = [], []
+    for t, k in zip(*np.nonzero(m["group"] >= 0)):
+        tris.append([m["tets"][t, q] for q in range(4) if q != k])
+        tags.append(200 + int(m["group"][t, k]))
+    return np.array(tris, np.int32), np.array(tags, np.int32)
+
+
+def write_meshes(d):
+    """The fixture meshes (also used by tests/test_mesh_io.py to rebuild them)."""
+    m = S.box_mesh(3, conductor=((1 / 3,) * 3, (2 / 3,) * 3))
+    tris, tags = boundary_triangles(m)
+    xyz, tets = m["xyz"], m["tets"]
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "box3_v2.msh"), "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(xyz))
+        for i, p in enumerate(xyz):
+            f.write("%d %.17g %.17g %.17g\n" % (i + 1, *p))
+        f.write("$EndNodes\n$Elements\n%d\n" % (len(tris) + len(tets)))
+        e = 1
+        for tri, tg in zip(tris, tags):
+            f.write("%d 2 2 %d %d %d %d %d\n" % (e, tg, tg, *(tri + 1)))
+            e += 1
+        for tet in tets:
+            f.write("%d 4 2 100 1 %d %d %d %d\n" % (e, *(tet + 1)))
+            e += 1
+        f.write("$EndElements\n")
+    # 4.1: one surface entity per tag, one volume; nodes in two blocks
+    utags = sorted(set(tags.tolist()))
+    with open(os.path.join(d, "box3_v41.msh"), "w") as f:
+        f.write("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Entities\n0 0 %d 1\n" % len(utags))
+        for k, tg in enumerate(utags):
+            f.write("%d 0 0 0 1 1 1 1 %d 0\n" % (k + 1, tg))
+        f.write("1 0 0 0 1 1 1 1 100 0\n$EndEntities\n")
+        half = len(xyz) // 2
+        blocks = [(0, half), (half, len(xyz))]
+        f.write("$Nodes\n2 %d 1 %d\n" % (len(xyz), len(xyz)))
+        for a, b in blocks:
+            f.write("3 1 0 %d\n" % (b - a))
+            for i in range(a, b):
+                f.write("%d\n" % (i + 1))
+            for i in range(a, b):
+                f.write("%.17g %.17g %.17g\n" % tuple(xyz[i]))
+        f.write("$EndNodes\n$Elements\n%d %d 1 %d\n" % (len(utags) + 1, len(tris) + len(tets),
+                                                          len(tris) + len(tets)))
+        e = 1
+        for k, tg in enumerate(utags):
+            sel = np.nonzero(tags == tg)[0]
+            f.write("2 %d 2 %d\n" % (k + 1, len(sel)))
+            for j in sel:
+                f.write("%d %d %d %d\n" % (e, *(tris[j] + 1)))
+                e += 1
+        f.write("3 1 4 %d\n" % len(tets))
+        for tet in tets:
+            f.write("%d %d %d %d %d\n" % (e, *(tet + 1)))
+            e += 1
+        f.write("$EndElements\n")
+    with open(os.path.join(d, "box3.vtk"), "w") as f:
+        f.write("# vtk DataFile Version 3.0\nbox\nASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS %d double\n" % len(xyz))
+        for p in xyz:
+            f.write("%.17g %.17g %.17g\n" % tuple(p))
+        f.write("CELLS %d %d\n" % (len(tets) + len(tris), 5 * len(tets) + 4 * len(tris)))
+        for tet in tets:
+            f.write("4 %d %d %d %d\n" % tuple(tet))
+        for tri in tris:
+            f.write("3 %d %d %d\n" % tuple(tri))
+        f.write("CELL_TYPES %d\n" % (len(tets) + len(tris)))
+        f.write("10\n" * len(tets) + "5\n" * len(tris))
+    with open(os.path.join(d, "tet.stl"), "w") as f:
+        p = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+        f.write("solid tet\n")
+        for tri in ([0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]):
+            f.write("  facet normal 0 0 0\n    outer loop\n")
+            for v in tri:
+                f.write("      vertex %.17g %.17g %.17g\n" % tuple(p[v]))
+            f.write("    endloop\n  endfacet\n")
+        f.write("endsolid tet\n")
+    return m, tris, tags
+
+
+def gen_mesh():
+    d = os.path.join(HERE, "mesh")
+    write_meshes(d)
+    out = {}
+    for name, fmt in (("box3_v2.msh", 1), ("box3_v41.msh", 1), ("box3.vtk", 2), ("tet.stl", 3)):
+        xyz, et, en, tg = R.mesh_read(os.path.join(d, name), fmt)
+        key = name.replace(".", "_")
+        out[key + "_xyz"] = xyz
+        out[key + "_type"] = et
+        out[key + "_nodes"] = en
+        out[key + "_tag"] = tg
+    save(os.path.join("mesh", "mesh_ref"), **out)
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     if what in ("all", "field"):
@@ -203,3 +304,5 @@ if __name__ == "__main__":
         gen_surface()
     if what in ("all", "circuit"):
         gen_circuit()
+    if what in ("all", "mesh"):
+        gen_mesh()
