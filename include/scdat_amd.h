@@ -183,7 +183,9 @@ int scdat_compute_efield(scdat_ctx* ctx);
 int scdat_sort(scdat_ctx* ctx);
 
 /* Inject a nodal field (V/m, 3 per node) / potential (V per node); the
- * latter mirrors PICCycle::setPotential (PICCycle.h:90-105) and recomputes E. */
+ * latter mirrors PICCycle::setPotential (PICCycle.h:90-105): it recomputes E,
+ * and later solves hold the Dirichlet nodes at the values given here
+ * (PICCycle.cpp:674-688 takes a DIRICHLET node's value from potential_).     */
 /* Replaces PICCycle::setTimeStep (PICCycle.h:86-92): ignored unless dt > 0. */
 int scdat_set_time_step(scdat_ctx* ctx, double dt);
 
@@ -444,6 +446,44 @@ typedef struct {
 int scdat_get_scenario_stats(scdat_ctx* ctx, int32_t scenario, scdat_scenario_stats* out);
 /* getSurfaceCurrentLedger of one scenario (scdat_get_ledger sums them). */
 int scdat_get_scenario_ledger(scdat_ctx* ctx, int32_t scenario, scdat_ledger_entry* out, int32_t n_groups);
+
+/* ---- Field-emission cathode (vacuum-arc coupling) ----------------------------
+ * FieldEmissionBoundaryCondition (Tools/Boundary/src/FieldEmissionBoundary
+ * Condition.cpp:23-178), as the vacuum-arc toolkit sets it on the cathode
+ * (SurfaceDischargeArcAlgorithm.cpp:441-458): an ion hitting the group is
+ * absorbed and adds J A dt / e expected field electrons and (when enabled)
+ * yield * w_ion / w_secondary expected secondaries to the group's running
+ * accumulators; their integer parts, at most max_*_per_event, are emitted
+ * along the face's inward normal at the given speeds, 1e-10 m apart off the
+ * impact point, into electron_species (which carries their weight).  Within
+ * a step, impacts are taken in particle-id order (the reference's is its
+ * container order).  Non-positive settings take the constructor's
+ * defaults.                                                                 */
+typedef struct {
+  int32_t electron_species;
+  double emission_current_density_a_per_m2;
+  double emission_area_m2;
+  double emitted_electron_speed_m_per_s;
+  double emitted_electron_weight;
+  int32_t max_emitted_particles_per_event;
+  int32_t enable_secondary_electron_emission;
+  double secondary_electron_yield_per_ion;
+  double secondary_electron_speed_m_per_s;
+  double secondary_electron_weight;
+  int32_t max_secondary_particles_per_event;
+} scdat_field_emission;
+/* FieldEmissionBoundaryCondition::Parameters defaults
+ * (FieldEmissionBoundaryCondition.h:17-31). */
+void scdat_field_emission_default(scdat_field_emission* p);
+int scdat_set_field_emission(scdat_ctx* ctx, int32_t group, const scdat_field_emission* e);
+/* setEmissionCurrentDensity (FieldEmissionBoundaryCondition.cpp:57-61) */
+int scdat_set_emission_current_density(scdat_ctx* ctx, int32_t group, double j_a_per_m2);
+/* Appends particles to a species between steps (ParticleManager::createIon /
+ * createElectron + PICCycle::addParticleLocation,
+ * SurfaceDischargeArcAlgorithm.cpp:527-557); cell NULL: located on the host;
+ * ids continue the species' count.                                         */
+int scdat_append_particles(scdat_ctx* ctx, int32_t species, int64_t n, const double* x, const double* v,
+                           const int32_t* cell);
 
 /* ---- Mesh import and result export (host, no device) -----------------------
  * Readers for the formats of GmshLoader / VTKLoader / STLLoader

@@ -129,6 +129,23 @@ class ScenarioStats(C.Structure):
 GRAD_DELTA_MAX, GRAD_PHOTO_J, GRAD_CONDUCTANCE, GRAD_FACE_YIELD, N_GRAD = 0, 1, 2, 3, 9
 
 
+class FieldEmission(C.Structure):
+    _fields_ = [("electron_species", C.c_int32), ("emission_current_density_a_per_m2", C.c_double),
+                ("emission_area_m2", C.c_double), ("emitted_electron_speed_m_per_s", C.c_double),
+                ("emitted_electron_weight", C.c_double), ("max_emitted_particles_per_event", C.c_int32),
+                ("enable_secondary_electron_emission", C.c_int32), ("secondary_electron_yield_per_ion", C.c_double),
+                ("secondary_electron_speed_m_per_s", C.c_double), ("secondary_electron_weight", C.c_double),
+                ("max_secondary_particles_per_event", C.c_int32)]
+
+
+def field_emission(**kw):
+    p = FieldEmission()
+    lib().scdat_field_emission_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 class ChargingParams(C.Structure):
     _fields_ = [("see_delta_max", C.c_double), ("see_energy_max_ev", C.c_double), ("face_yield", C.c_double * 6),
                 ("photo_current_density", C.c_double), ("photo_temperature_ev", C.c_double),
@@ -241,6 +258,10 @@ SIGNATURES = {
     "scdat_set_preconditioner": (C.c_int, [_P, C.c_int32]),
     "scdat_set_node_order": (C.c_int, [_P, C.c_int32]),
     "scdat_io_last_error": (C.c_char_p, []),
+    "scdat_field_emission_default": (None, [C.POINTER(FieldEmission)]),
+    "scdat_set_field_emission": (C.c_int, [_P, C.c_int32, C.POINTER(FieldEmission)]),
+    "scdat_set_emission_current_density": (C.c_int, [_P, C.c_int32, C.c_double]),
+    "scdat_append_particles": (C.c_int, [_P, C.c_int32, C.c_int64, _D, _D, _I32]),
     "scdat_mesh_read": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]),
     "scdat_mesh_free": (None, [_P]),
     "scdat_mesh_sizes": (C.c_int, [_P, _I64, _I64, _I64]),
@@ -566,6 +587,19 @@ class Context:
         arr = (LedgerEntry * n_groups)()
         self._chk(lib().scdat_get_scenario_ledger(self.h, scenario, arr, n_groups))
         return [e.as_dict() for e in arr]
+
+    def set_field_emission(self, group, params):
+        self._chk(lib().scdat_set_field_emission(self.h, group, C.byref(params)))
+
+    def set_emission_current_density(self, group, j):
+        self._chk(lib().scdat_set_emission_current_density(self.h, group, float(j)))
+
+    def append_particles(self, species, x, v, cell=None):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        cl = None if cell is None else np.ascontiguousarray(cell, np.int32)
+        self._chk(lib().scdat_append_particles(self.h, species, len(x), _ptr(x, C.c_double), _ptr(v, C.c_double),
+                                               _ptr(cl, C.c_int32)))
 
     def layer_profile(self, axis=2, length=1.0, n_cells=1, area=1.0):
         """collectLayerProfile of the PIC harness: per node layer along `axis`

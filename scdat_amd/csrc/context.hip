@@ -524,6 +524,11 @@ struct scdat_ctx {
   DevBuf d_tet_seg, d_blk_seg, d_seg_b0, d_seg_unit, d_seg_dt, d_seg_wscale, d_seg_qinit, d_seg_ccap, d_kc_off,
       d_seg_emit, d_seg_key;
   bool seg_emit_dirty = false;
+  // field-emission cathodes (ion-driven, FieldEmissionBoundaryCondition)
+  std::array<scdat_field_emission, kMaxGroups> fe{};
+  uint32_t fe_mask = 0;
+  std::array<double, kMaxGroups> fe_pending{}, fe_pending_sec{};
+  DevBuf d_fe_x, d_fe_v, d_fe_cell, d_fe_id;
   int node_order = SCDAT_ORDER_GIVEN;
   std::vector<int32_t> perm, iperm;  // internal node -> caller's node, and back (empty: same order)
   int push_chunks = 4;  // particles per thread of the push kernel (scdat_autotune)
@@ -1135,6 +1140,7 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   a.alive_partial = S.alive_partial.as<int32_t>();
   a.id = S.buf[S.cur][7].as<int32_t>();
   a.see_mask = 0;
+  if (S.q > 0 && c->event_cap > 0) a.see_mask = c->fe_mask;  // ion impacts drive field emission
   if (S.q < 0 && c->event_cap > 0)
     for (int g = 0; g < kMaxGroups; ++g) {
       bool on = c->emit[g].see_model != SCDAT_SEE_OFF;
@@ -1419,7 +1425,7 @@ void do_initialize(scdat_ctx* c) {
 }
 
 bool has_post_push(const scdat_ctx* c) {
-  if (!c->mcc.empty() || c->event_cap > 0) return true;
+  if (!c->mcc.empty() || c->event_cap > 0 || c->fe_mask) return true;
   for (const auto& e : c->emit)
     if (e.photo_current_density > 0.0) return true;
   return false;
@@ -1449,11 +1455,134 @@ SpeciesOut<T> species_out(scdat_ctx* c, int si) {
 // after the push and before the Poisson solve; new particles are deposited
 // into the charge of this step.  Species slot counts are updated once the
 // stream has synchronised (apply_appends).
+// Field-emission cathodes (FieldEmissionBoundaryCondition::processParticle,
+// Tools/Boundary/src/FieldEmissionBoundaryCondition.cpp:88-178): every ion
+// impact on such a group is absorbed and, in impact order (here: particle id
+// order), adds J A dt / e expected field electrons and yield * w_ion / w_sec
+// expected secondaries to the group's running accumulators; the integer
+// parts (at most max_*_per_event) are emitted along the inward normal at the
+// configured speeds, placed 1e-10 m apart off the impact point.  Host side:
+// the impacts of a step are few; the new electrons are appended and
+// deposited on the device.
+template <typename T>
+void field_emission(scdat_ctx* c, unsigned long long* counts) {
+  if (!c->fe_mask || c->event_cap <= 0) return;
+  hipStream_t s = c->stream;
+  unsigned long long ne = 0;
+  HIPCHK(hipMemcpyAsync(&ne, c->d_n_events.p, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int64_t n = std::min<int64_t>((int64_t)ne, c->event_cap);
+  if (n == 0) return;
+  std::vector<SeeEvent> ev((size_t)n);
+  HIPCHK(hipMemcpyAsync(ev.data(), c->d_events.p, sizeof(SeeEvent) * (size_t)n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  std::vector<const SeeEvent*> hits;
+  for (const auto& e : ev) {
+    const int g = e.group_face & 0xFF;
+    if (c->sp[e.species].q > 0 && ((c->fe_mask >> g) & 1u)) hits.push_back(&e);
+  }
+  std::sort(hits.begin(), hits.end(), [](const SeeEvent* a, const SeeEvent* b) {
+    return a->species != b->species ? a->species < b->species : a->id < b->id;
+  });
+  const HostMesh& h = c->h;
+  struct Out {
+    std::vector<double> xs, vs;
+    std::vector<int32_t> cells, ids;
+  };
+  std::array<Out, kMaxSpecies> out;
+  std::vector<unsigned long long> per((size_t)kMaxGroups * kMaxSpecies, 0ull);
+  for (const SeeEvent* e : hits) {
+    const int g = e->group_face & 0xFF, face = e->group_face >> 8;
+    const scdat_field_emission& p = c->fe[g];
+    const int es = p.electron_species;
+    std::vector<double>& xs = out[es].xs;
+    std::vector<double>& vs = out[es].vs;
+    std::vector<int32_t>& cells = out[es].cells;
+    std::vector<int32_t>& ids = out[es].ids;
+    // inward normal of the impact face: grad lambda_face of its tet
+    double nrm[3];
+    const double* A = &h.A[9 * (size_t)e->cell];
+    for (int r = 0; r < 3; ++r)
+      nrm[r] = face == 0 ? -(A[r] + A[3 + r] + A[6 + r]) : A[3 * (face - 1) + r];
+    const double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    for (int r = 0; r < 3; ++r) nrm[r] = len > 1e-12 ? nrm[r] / len : (r == 2 ? 1.0 : 0.0);
+    // The impact point projected onto the face's plane in FP64, as the
+    // reference projects the particle onto the boundary face
+    // (PICCycle.cpp:1262-1264): an FP32 walk's crossing point can sit a
+    // few 1e-11 m off the plane, comparable to the emission offsets.
+    const double* P = &h.xyz[3 * (size_t)h.tets[4 * (size_t)e->cell + ((face + 1) & 3)]];
+    const double dn = (e->x[0] - P[0]) * nrm[0] + (e->x[1] - P[1]) * nrm[1] + (e->x[2] - P[2]) * nrm[2];
+    const double hp[3] = {e->x[0] - dn * nrm[0], e->x[1] - dn * nrm[1], e->x[2] - dn * nrm[2]};
+    int made = 0;
+    auto emit = [&](int64_t count, double speed) {
+      for (int64_t k = 0; k < count; ++k) {
+        const double off = 1.0e-10 * (double)(made + 1);
+        for (int r = 0; r < 3; ++r) {
+          xs.push_back(hp[r] + nrm[r] * off);
+          vs.push_back(nrm[r] * speed);
+        }
+        cells.push_back(e->cell);
+        ids.push_back(c->sp[es].next_id++);
+        ++made;
+      }
+    };
+    const double dt = c->cfg.dt;
+    if (dt > 0.0 && p.emission_current_density_a_per_m2 > 0.0) {
+      const double expect = p.emission_current_density_a_per_m2 * p.emission_area_m2 * dt / kElem;
+      if (std::isfinite(expect) && expect > 0.0) {
+        c->fe_pending[g] += expect;
+        const int64_t cnt = std::min<int64_t>((int64_t)std::floor(c->fe_pending[g]), p.max_emitted_particles_per_event);
+        c->fe_pending[g] = std::max(0.0, c->fe_pending[g] - (double)cnt);
+        emit(cnt, p.emitted_electron_speed_m_per_s);
+      }
+    }
+    if (p.enable_secondary_electron_emission && p.secondary_electron_yield_per_ion > 0.0) {
+      const double expect = p.secondary_electron_yield_per_ion * std::max(0.0, c->sp[e->species].w) /
+                            std::max(1e-12, p.secondary_electron_weight);
+      if (std::isfinite(expect) && expect > 0.0) {
+        c->fe_pending_sec[g] += expect;
+        const int64_t cnt =
+            std::min<int64_t>((int64_t)std::floor(c->fe_pending_sec[g]), p.max_secondary_particles_per_event);
+        c->fe_pending_sec[g] = std::max(0.0, c->fe_pending_sec[g] - (double)cnt);
+        emit(cnt, p.secondary_electron_speed_m_per_s);
+      }
+    }
+    per[(size_t)g * kMaxSpecies + es] += (unsigned long long)made;
+  }
+  unsigned long long* rho = c->d_rho[c->rho_last].as<unsigned long long>();
+  bool any = false;
+  for (int es = 0; es < (int)c->sp.size(); ++es) {
+    const Out& o = out[es];
+    const int64_t m = (int64_t)o.cells.size();
+    if (m == 0) continue;
+    any = true;
+    if (c->sp[es].n + m > c->sp[es].cap) throw Error(SCDAT_ERR_NOMEM, "field emission: electron species capacity exceeded");
+    // staging buffers are reused per species: wait for the previous inject
+    HIPCHK(hipStreamSynchronize(s));
+    c->d_fe_x.upload(o.xs, s);
+    c->d_fe_v.upload(o.vs, s);
+    c->d_fe_cell.upload(o.cells, s);
+    c->d_fe_id.upload(o.ids, s);
+    launch_inject<T>(species_out<T>(c, es), m, c->d_fe_x.as<double>(), c->d_fe_v.as<double>(),
+                     c->d_fe_cell.as<int32_t>(), c->d_fe_id.as<int32_t>(), c->d_tets.as<TetMap<T>>(), rho, counts, s);
+    HIPCHK(hipGetLastError());
+  }
+  if (!any) return;
+  // emitted counts per group and species (ledger): read, add, write back
+  std::vector<unsigned long long> em((size_t)kMaxGroups * kMaxSpecies);
+  HIPCHK(hipMemcpyAsync(em.data(), c->d_emit_count.p, 8 * em.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (size_t k = 0; k < em.size(); ++k) em[k] += per[k];
+  HIPCHK(hipMemcpyAsync(c->d_emit_count.p, em.data(), 8 * em.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
 template <typename T>
 void post_push(scdat_ctx* c) {
   hipStream_t s = c->stream;
   unsigned long long* counts = c->d_counts.as<unsigned long long>();
   launch_fill_u64(c->d_append.as<unsigned long long>(), kMaxSpecies, s);
+  field_emission<T>(c, counts);
   const uint32_t step = (uint32_t)c->steps;
   unsigned long long* rho = c->d_rho[c->rho_last].as<unsigned long long>();
   const int ns = (int)c->sp.size();
@@ -1519,6 +1648,7 @@ void post_push(scdat_ctx* c) {
     e.out[o] = species_out<T>(c, o);
     e.q_abs_w[o] = std::abs(c->sp[o].q) * c->sp[o].w;
     e.mass[o] = c->sp[o].m;
+    if (c->sp[o].q < 0) e.electron_mask |= 1u << o;
   }
   e.tets = c->d_tets.as<TetMap<T>>();
   e.xyz = c->d_xyz64.as<double>();
@@ -2340,6 +2470,9 @@ int scdat_set_mesh(scdat_ctx* c, int64_t n_nodes, const double* xyz, int64_t n_t
                                        : std::vector<int32_t>(4 * (size_t)n_tets, 0));
     c->seg_node0.clear();
     c->seg_emit.clear();
+    c->fe_mask = 0;
+    c->fe_pending.fill(0.0);
+    c->fe_pending_sec.fill(0.0);
     mesh_setup(c, n_nodes, xyz, n_tets, tets, node_kind, dv.data(), fg.data(), c->node_order == SCDAT_ORDER_MORTON);
   });
 }
@@ -2351,6 +2484,7 @@ int scdat_set_boundary(scdat_ctx* c, int32_t group, int32_t type, int32_t partne
     return fail(c, SCDAT_ERR_ARG, "set_boundary: bad group or type");
   c->gtype[group] = type;
   c->gpartner[group] = partner_group;
+  c->fe_mask &= ~(1u << group);  // a new condition replaces field emission
   for (int r = 0; r < 3; ++r) c->gshift[group][r] = shift ? shift[r] : 0.0;
   c->initialized = false;
   return SCDAT_OK;
@@ -2598,6 +2732,30 @@ int scdat_set_potential(scdat_ctx* c, const double* phi) {
     std::vector<double> p(nn);
     for (size_t i = 0; i < nn; ++i) p[i] = phi[to_user(c, (int64_t)i)];
     HIPCHK(hipMemcpyAsync(c->d_phi64.p, p.data(), 8 * nn, hipMemcpyHostToDevice, c->stream));
+    // The next solve holds Dirichlet nodes at these values (PICCycle.cpp:
+    // 674-688 takes potential_[i] as the value of a DIRICHLET node): update
+    // the values and the lifting of the load on their free neighbours.
+    HostMesh& h = c->h;
+    bool changed = false;
+    for (size_t i = 0; i < nn; ++i)
+      if (h.kind[i] == SCDAT_NODE_DIRICHLET && h.dval[i] != p[i]) {
+        h.dval[i] = p[i];
+        changed = true;
+      }
+    if (changed) {
+      const int W = h.W;
+      for (size_t i = 0; i < nn; ++i) {
+        h.lift[i] = 0.0;
+        if (h.kind[i] != SCDAT_NODE_FREE) continue;
+        for (int sl = 1; sl < W; ++sl) {
+          const size_t o = (size_t)sl * nn + i;
+          const int32_t j = h.col[o];
+          if (h.kind[j] == SCDAT_NODE_DIRICHLET) h.lift[i] -= h.kfull[o] * h.dval[j];
+        }
+      }
+      c->d_lift.upload(h.lift, c->stream);
+      c->d_dval.upload(h.dval, c->stream);
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
   });
   return rc != SCDAT_OK ? rc : scdat_compute_efield(c);
@@ -2897,6 +3055,113 @@ int scdat_get_event_counts(scdat_ctx* c, scdat_event_counts* out) {
   });
 }
 
+// ---- field-emission cathodes ----------------------------------------------------
+
+void scdat_field_emission_default(scdat_field_emission* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->emission_area_m2 = 1.0e-8;
+  p->emitted_electron_speed_m_per_s = 2.0e6;
+  p->emitted_electron_weight = 1.0;
+  p->max_emitted_particles_per_event = 64;
+  p->secondary_electron_speed_m_per_s = 8.0e5;
+  p->secondary_electron_weight = 1.0;
+  p->max_secondary_particles_per_event = 64;
+}
+
+int scdat_set_field_emission(scdat_ctx* c, int32_t group, const scdat_field_emission* e) {
+  if (!c || !e) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_field_emission: set the mesh first");
+  if (c->segmented()) return fail(c, SCDAT_ERR_STATE, "set_field_emission: needs a single scenario");
+  if (group < 0 || group >= kMaxGroups || e->electron_species < 0 || e->electron_species >= (int)c->sp.size() ||
+      c->sp[e->electron_species].q >= 0)
+    return fail(c, SCDAT_ERR_ARG, "set_field_emission: bad group or electron species");
+  return guarded(c, [&] {
+    scdat_field_emission p = *e;
+    // the constructor's defaults for non-positive settings
+    // (FieldEmissionBoundaryCondition.cpp:23-55)
+    if (p.emission_area_m2 <= 0.0) p.emission_area_m2 = 1.0e-8;
+    if (p.emitted_electron_speed_m_per_s <= 0.0) p.emitted_electron_speed_m_per_s = 2.0e6;
+    if (p.emitted_electron_weight <= 0.0) p.emitted_electron_weight = 1.0;
+    if (p.max_emitted_particles_per_event <= 0) p.max_emitted_particles_per_event = 1;
+    if (p.secondary_electron_speed_m_per_s <= 0.0) p.secondary_electron_speed_m_per_s = 8.0e5;
+    if (p.secondary_electron_weight <= 0.0) p.secondary_electron_weight = 1.0;
+    if (p.max_secondary_particles_per_event <= 0) p.max_secondary_particles_per_event = 1;
+    // a species carries one weight: the emitted electrons take the species'
+    const double ws = c->sp[p.electron_species].w;
+    auto same = [&](double w) { return std::fabs(w - ws) <= 1e-12 * std::max(1.0, std::fabs(ws)); };
+    if (!same(p.emitted_electron_weight) || (p.enable_secondary_electron_emission && !same(p.secondary_electron_weight)))
+      throw Error(SCDAT_ERR_ARG, "set_field_emission: emitted electron weights must equal the electron species' weight");
+    p.emission_current_density_a_per_m2 = std::max(0.0, p.emission_current_density_a_per_m2);
+    // the emission condition replaces the group's particle condition and
+    // absorbs what hits it (FieldEmissionBoundaryCondition.cpp:94-95)
+    if (c->gtype[group] != SCDAT_BC_ABSORB) {
+      c->gtype[group] = SCDAT_BC_ABSORB;
+      c->gpartner[group] = -1;
+      c->initialized = false;
+    }
+    c->fe[group] = p;
+    c->fe_mask |= 1u << group;
+    c->fe_pending[group] = 0.0;
+    c->fe_pending_sec[group] = 0.0;
+    if (c->event_cap == 0) {
+      int64_t cap = 0;
+      for (const auto& S : c->sp) cap += S.cap;
+      c->event_cap = std::max<int64_t>(65536, cap / 64);
+      c->d_events.alloc(sizeof(SeeEvent) * (size_t)c->event_cap);
+    }
+  });
+}
+
+int scdat_set_emission_current_density(scdat_ctx* c, int32_t group, double j) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (group < 0 || group >= kMaxGroups || !((c->fe_mask >> group) & 1u))
+    return fail(c, SCDAT_ERR_ARG, "set_emission_current_density: group has no field emission");
+  c->fe[group].emission_current_density_a_per_m2 = std::max(0.0, j);
+  return SCDAT_OK;
+}
+
+int scdat_append_particles(scdat_ctx* c, int32_t species, int64_t n, const double* x, const double* v,
+                           const int32_t* cell) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (species < 0 || species >= (int)c->sp.size()) return fail(c, SCDAT_ERR_ARG, "append_particles: bad species");
+  if (n < 0 || (n > 0 && (!x || !v))) return fail(c, SCDAT_ERR_ARG, "append_particles: bad arrays");
+  if (n == 0) return SCDAT_OK;
+  return guarded(c, [&] {
+    SpeciesState& S = c->sp[species];
+    if (S.n + n > S.cap) throw Error(SCDAT_ERR_NOMEM, "append_particles: species capacity exceeded");
+    std::vector<int32_t> cl;
+    if (cell) cl.assign(cell, cell + n);
+    else host_locate(c->h, n, x, cl);
+    for (int64_t i = 0; i < n; ++i)
+      if (cl[i] < -1 || cl[i] >= c->h.nt) throw Error(SCDAT_ERR_ARG, "append_particles: cell index out of range");
+    const size_t ts = tsize(c);
+    std::vector<char> f[6];
+    for (int r = 0; r < 6; ++r) f[r].resize(ts * (size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+      for (int r = 0; r < 3; ++r) {
+        if (ts == 4) {
+          reinterpret_cast<float*>(f[r].data())[i] = (float)x[3 * i + r];
+          reinterpret_cast<float*>(f[3 + r].data())[i] = (float)v[3 * i + r];
+        } else {
+          reinterpret_cast<double*>(f[r].data())[i] = x[3 * i + r];
+          reinterpret_cast<double*>(f[3 + r].data())[i] = v[3 * i + r];
+        }
+      }
+    std::vector<int32_t> id((size_t)n);
+    for (int64_t i = 0; i < n; ++i) id[i] = S.next_id++;
+    for (int r = 0; r < 6; ++r)
+      HIPCHK(hipMemcpyAsync(static_cast<char*>(S.buf[S.cur][r].p) + ts * (size_t)S.n, f[r].data(), ts * (size_t)n,
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(S.buf[S.cur][6].as<int32_t>() + S.n, cl.data(), 4 * (size_t)n, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(S.buf[S.cur][7].as<int32_t>() + S.n, id.data(), 4 * (size_t)n, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    S.n += n;
+  });
+}
+
 // ---- layer profile (result export) ----------------------------------------------
 
 namespace {
@@ -2987,6 +3252,7 @@ int scdat_set_scenarios(scdat_ctx* c, int32_t n, const int64_t* node_start, cons
   for (const auto& S : c->sp)
     if (S.n > 0) return fail(c, SCDAT_ERR_STATE, "set_scenarios: call before set_particles");
   if (c->n_patch > 0) return fail(c, SCDAT_ERR_STATE, "set_scenarios: conductor patches need a single scenario");
+  if (c->fe_mask) return fail(c, SCDAT_ERR_STATE, "set_scenarios: field emission needs a single scenario");
   return guarded(c, [&] {
     mesh_natural_order(c);  // scenarios are ranges of the caller's node numbering
     const HostMesh& h = c->h;

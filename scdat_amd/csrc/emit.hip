@@ -260,6 +260,7 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
   const int64_t nev = (int64_t)ne < a.event_cap ? (int64_t)ne : a.event_cap;
   if (e >= nev) return;
   const SeeEvent ev = a.events[e];
+  if (!((a.electron_mask >> ev.species) & 1u)) return;  // ion impacts: field emission, on the host
   const int g = ev.group_face & 0xFF, face = ev.group_face >> 8;
   int sgm = 0;
   uint32_t k0 = a.key0, k1 = a.key1;
@@ -345,6 +346,19 @@ __global__ void __launch_bounds__(kBlock) k_photo(EmitArgs<T> a) {
   }
 }
 
+// Appends host-made particles (field emission) to a species and deposits
+// them into this step's charge.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_inject(SpeciesOut<T> o, int64_t n, const double* x, const double* v,
+                                                   const int32_t* cell, const int32_t* id, const TetMap<T>* tets,
+                                                   unsigned long long* rho, unsigned long long* counts) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T xt[3] = {(T)x[3 * i], (T)x[3 * i + 1], (T)x[3 * i + 2]};
+  const double vv[3] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+  append(o, xt, vv, cell[i], id[i], tets, rho, counts);
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -369,12 +383,22 @@ void launch_photo(const EmitArgs<T>& a, int64_t max_count, hipStream_t s) {
   if (max_count <= 0 || a.f1 <= a.f0) return;
   hipLaunchKernelGGL(k_photo<T>, dim3(blocks_for(max_count)), dim3(kBlock), 0, s, a);
 }
+template <typename T>
+void launch_inject(const SpeciesOut<T>& o, int64_t n, const double* x, const double* v, const int32_t* cell,
+                   const int32_t* id, const TetMap<T>* tets, unsigned long long* rho, unsigned long long* counts,
+                   hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_inject<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, o, n, x, v, cell, id, tets, rho, counts);
+}
 void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, n);
 }
 
 #define SCDAT_EMIT_INST(T)                                                        \
+  template void launch_inject<T>(const SpeciesOut<T>&, int64_t, const double*, const double*, const int32_t*, \
+                                 const int32_t*, const TetMap<T>*, unsigned long long*, unsigned long long*, \
+                                 hipStream_t);                                    \
   template void launch_mcc<T>(const MccArgs<T>&, hipStream_t);                    \
   template void launch_see<T>(const EmitArgs<T>&, int64_t, hipStream_t);          \
   template void launch_photo<T>(const EmitArgs<T>&, int64_t, hipStream_t);

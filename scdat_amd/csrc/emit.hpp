@@ -92,6 +92,7 @@ struct EmitArgs {
   const int32_t* tet_seg;
   const SurfEmitDev* gseg;
   const uint32_t* seg_key;
+  uint32_t electron_mask;  // species whose impacts are secondary-emission events
 };
 
 template <typename T>
@@ -102,5 +103,10 @@ void launch_see(const EmitArgs<T>& a, int64_t max_events, hipStream_t s);
 template <typename T>
 void launch_photo(const EmitArgs<T>& a, int64_t max_count, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s);
+// n particles (x, v: 3n FP64, cell, id) appended to o and deposited
+template <typename T>
+void launch_inject(const SpeciesOut<T>& o, int64_t n, const double* x, const double* v, const int32_t* cell,
+                   const int32_t* id, const TetMap<T>* tets, unsigned long long* rho, unsigned long long* counts,
+                   hipStream_t s);
 
 }  // namespace scdat
