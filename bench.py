@@ -83,7 +83,7 @@ def structural_nnz(tets, n_nodes):
     return int(2 * np.unique(key).size + n_nodes)
 
 
-def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True):
+def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns=True):
     import scdat_amd as S
     m = case["mesh"]
     t0 = time.time()
@@ -94,6 +94,7 @@ def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True):
     for (q, mass), (x, v, c) in zip(case["qm"], (case["e"], case["i"])):
         sid = ctx.add_species(q, mass, case["w"], len(x))
         ctx.set_particles(sid, x, v, c)
+    ctx.row_patterns = row_patterns
     ctx.initialize()
     chunks = ctx.autotune()
     log(f"gpu setup {time.time() - t0:.1f}s (push tile: {chunks} particles per thread); warmup {warmup} steps")
@@ -113,7 +114,8 @@ def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True):
     kt = {"push": ctx.kernel_time(S.KERNEL_PUSH), "pcg_iter": ctx.kernel_time(S.KERNEL_PCG)}
     pushes = st1["particle_pushes"] - st0["particle_pushes"]
     log(f"gpu: {steps} steps in {wall:.3f}s, pushes {pushes}, stages(ms) {stages}, kernels {kt}, stats {st1}")
-    return ctx, dict(wall=wall, pushes=pushes, stages=stages, kt=kt, stats=st1, stats0=st0, chunks=chunks)
+    return ctx, dict(wall=wall, pushes=pushes, stages=stages, kt=kt, stats=st1, stats0=st0, chunks=chunks,
+                     solver=ctx.solver_info())
 
 
 def run_cpu_baseline(case, steps, threads):
@@ -332,6 +334,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--scenarios", type=int, default=256)
     ap.add_argument("--sweep-seed", type=int, default=20261206)
+    ap.add_argument("--row-patterns", type=int, choices=(0, 1), default=1,
+                    help="1: the FP32 inner iteration may read the operator as row patterns")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -349,7 +353,8 @@ def main():
     import scdat_amd as S
     name, ncu = S.device_info()
     case = build_case(args.hex, args.particles, args.seed)
-    ctx, g = run_gpu(case, args.steps, args.warmup, args.sort_interval, not args.no_kernel_timing)
+    ctx, g = run_gpu(case, args.steps, args.warmup, args.sort_interval, not args.no_kernel_timing,
+                     bool(args.row_patterns))
 
     m = case["mesh"]
     n_nodes, n_tets = len(m["kind"]), len(m["tets"])
@@ -358,17 +363,25 @@ def main():
     # Algorithmic bytes per launch.
     #  push (one species): particle SoA read + write 2 x 28 B; tet record 64 B
     #    per tet; nodal E 16 B + integer charge 8 B per node
-    #  PCG iteration (one fused kernel): column + value 8 B per structural
-    #    entry; r, w, s_old, p, x, D^-1 read and r, w, s, p, x written, 4 B
-    #    each per row
+    #  PCG iteration (one fused kernel): r, w, s_old, p, x read and r, w, s,
+    #    p, x written, 4 B each per row; the operator either as ELL (column +
+    #    value 8 B per structural entry, D^-1 4 B per row) or as row patterns
+    #    (2-byte id per row, plus the table once: W offsets and values and
+    #    D^-1 per pattern)
     push_launch = g["pushes"] / max(1, kt["push"][1]) if kt["push"][1] else 0
+    sv = g["solver"]
+    npat, width = sv["row_patterns"], sv["ell_width"]
+    op_bytes = (2.0 * n_nodes + npat * (8.0 * width + 4.0)) if npat else (8.0 * nnz + 4.0 * n_nodes)
     bytes_k = {
         "push": 56.0 * push_launch + 64.0 * n_tets + 24.0 * n_nodes,
-        "pcg_iter": 8.0 * nnz + 44.0 * n_nodes,
+        "pcg_iter": op_bytes + 40.0 * n_nodes,
     }
     traffic = {}
     if os.path.exists(TRAFFIC_FILE):
-        traffic = json.load(open(TRAFFIC_FILE)).get("traffic_bytes_per_launch", {})
+        tf = json.load(open(TRAFFIC_FILE))
+        traffic = dict(tf.get("traffic_bytes_per_launch", {}))
+        if bool(tf.get("row_patterns", False)) != bool(npat):
+            traffic.pop("pcg_iter", None)  # measured with the other operator form
     # launches that did work in the timed region (PCG: one per inner iteration;
     # every 16th carries timing events)
     iters = g["stats"]["pcg_iterations_total"] - g["stats0"]["pcg_iterations_total"]
@@ -381,7 +394,8 @@ def main():
                       "launches": launches[k], "launches_timed": n,
                       "algorithmic_bytes_per_launch": bytes_k[k], "total_ms": ms * launches[k]}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    names = {"push": "push_kernel<float>", "pcg_iter": "k_cg_iter<float,W>"}
+    names = {"push": "push_kernel<float,CH>",
+             "pcg_iter": "k_cg_iter<float,W,false,%s>" % ("true" if npat else "false")}
     roof = dict(kernels[dominant])
     roof["kernel"] = names[dominant]
     st = g["stats"]
@@ -413,6 +427,8 @@ def main():
                "whole Poisson stage time (FP64 residuals, launches and host polls included)"},
         "stage_ms_total": g["stages"],
         "launch_shape": {"push_particles_per_thread": g["chunks"], "chosen_by": "scdat_autotune"},
+        "solver": {"rows": sv["rows"], "ell_width": width, "row_patterns": npat,
+                   "preconditioner": "jacobi" if sv["preconditioner"] == S.PRECOND_JACOBI else "two-level"},
         "floating_potential_v": st["floating_potential"],
         "pcg_initial_rel_residual": st["pcg_initial_rel_residual"],
     }

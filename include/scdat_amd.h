@@ -562,7 +562,8 @@ typedef struct {
   int64_t rows, nonzeros;   /* operator rows (all nodes), stored non-zeros     */
   int32_t ell_width;        /* ELL slots per row                              */
   int32_t preconditioner;   /* in use for the next solve                       */
-  int32_t coarse_unknowns, reserved;  /* two-level: aggregates (row blocks); 0 */
+  int32_t coarse_unknowns;  /* two-level: aggregates (row blocks); 0          */
+  int32_t row_patterns;     /* FP32 inner operator as row patterns (0: ELL)   */
 } scdat_solver_info;
 int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
 
@@ -575,6 +576,11 @@ int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
  * state is untouched.  Every shape gives the same bits.  No counterpart in
  * the reference.                                                           */
 #define SCDAT_SHAPE_PUSH_CHUNKS 0
+/* SCDAT_SHAPE_ROW_PATTERNS: 1 (default) lets the FP32 inner iteration read
+ * the operator as per-row pattern ids into a table of distinct rows (column
+ * offsets and values bit-identical), when the mesh has at most 4096 distinct
+ * rows; 0 keeps the ELL slots.  Same bits either way.                      */
+#define SCDAT_SHAPE_ROW_PATTERNS 1
 int scdat_set_launch_shape(scdat_ctx* ctx, int32_t what, int32_t value);
 int scdat_get_launch_shape(scdat_ctx* ctx, int32_t what, int32_t* value);
 int scdat_autotune(scdat_ctx* ctx);

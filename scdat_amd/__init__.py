@@ -109,11 +109,12 @@ ORDER_MORTON, ORDER_GIVEN = 0, 1
 
 class SolverInfo(C.Structure):
     _fields_ = [("rows", C.c_int64), ("nonzeros", C.c_int64), ("ell_width", C.c_int32),
-                ("preconditioner", C.c_int32), ("coarse_unknowns", C.c_int32), ("reserved", C.c_int32)]
+                ("preconditioner", C.c_int32), ("coarse_unknowns", C.c_int32), ("row_patterns", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 SHAPE_PUSH_CHUNKS = 0
+SHAPE_ROW_PATTERNS = 1
 
 
 class ScenarioStats(C.Structure):
@@ -639,6 +640,18 @@ class Context:
     @push_chunks.setter
     def push_chunks(self, n):
         self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_PUSH_CHUNKS, int(n)))
+
+    @property
+    def row_patterns(self):
+        """Whether the FP32 inner iteration may read the operator as row
+        patterns (it does when the mesh has few distinct rows)."""
+        v = C.c_int32()
+        self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_ROW_PATTERNS, C.byref(v)))
+        return bool(v.value)
+
+    @row_patterns.setter
+    def row_patterns(self, on):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_ROW_PATTERNS, int(bool(on))))
 
     def charging_forward(self, params, steps):
         """Floating potential after `steps` steps of the differentiable charging

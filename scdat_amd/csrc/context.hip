@@ -20,6 +20,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/scdat_amd.h"
@@ -472,6 +473,9 @@ struct scdat_ctx {
   DevBuf d_tets, d_nb, d_groups, d_partner, d_gshift;
   // ELL operator
   DevBuf d_col, d_v32, d_v64, d_vh32, d_vh64, d_dinv32, d_dinv64, d_g[3];
+  DevBuf d_pat, d_poff, d_pval, d_pdinv;  // row patterns of the FP32 inner operator
+  int npat = 0;                           // 0: none (too many, or FP64 inner solve)
+  bool use_patterns = true;
   DevBuf d_kind, d_lift, d_dval, d_dscale, d_phi1, d_kc_idx, d_kc_val, d_b64, d_x64, d_r64, d_partial, d_sc;
   DevBuf d_inner[8];
   // two-level preconditioner
@@ -584,6 +588,53 @@ int guarded(scdat_ctx* c, F&& fn) {
 
 size_t tsize(const scdat_ctx* c) { return c->is_fp64() ? 8 : 4; }
 
+// Row patterns of the FP32 inner operator A D^-1: rows whose column offsets
+// j - i, FP32 values and FP32 D^-1 agree bit for bit share one table entry.
+// Structured meshes in their natural order collapse to a few hundred
+// patterns (a 64^3 box with a conductor: under a thousand for 273k rows);
+// past kMaxPatterns the inner iteration keeps reading the ELL slots.
+constexpr int kMaxPatterns = 4096;
+void build_row_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::vector<float>& di) {
+  const HostMesh& h = c->h;
+  const int64_t n = h.nn;
+  const int W = h.W;
+  c->npat = 0;
+  if (W < 7 || W > 20 || n == 0) return;  // the pattern kernels exist for these widths
+  std::unordered_map<std::string, int> id;
+  id.reserve(2 * kMaxPatterns);
+  std::vector<uint16_t> pat((size_t)n);
+  std::vector<int32_t> poff;
+  std::vector<float> pval, pdinv;
+  std::string key((size_t)(8 * W + 4), '\0');
+  for (int64_t i = 0; i < n; ++i) {
+    char* k = key.data();
+    for (int sl = 0; sl < W; ++sl) {
+      const size_t o = (size_t)sl * n + i;
+      const int32_t off = h.col[o] - (int32_t)i;
+      std::memcpy(k + 8 * sl, &off, 4);
+      std::memcpy(k + 8 * sl + 4, &vh[o], 4);
+    }
+    std::memcpy(k + 8 * W, &di[i], 4);
+    auto it = id.find(key);
+    if (it == id.end()) {
+      if ((int)id.size() == kMaxPatterns) return;  // unstructured: stay on the ELL
+      it = id.emplace(key, (int)id.size()).first;
+      for (int sl = 0; sl < W; ++sl) {
+        const size_t o = (size_t)sl * n + i;
+        poff.push_back(h.col[o] - (int32_t)i);
+        pval.push_back(vh[o]);
+      }
+      pdinv.push_back(di[i]);
+    }
+    pat[(size_t)i] = (uint16_t)it->second;
+  }
+  c->d_pat.upload(pat, c->stream);
+  c->d_poff.upload(poff, c->stream);
+  c->d_pval.upload(pval, c->stream);
+  c->d_pdinv.upload(pdinv, c->stream);
+  c->npat = (int)id.size();
+}
+
 template <typename T>
 void upload_mesh_device(scdat_ctx* c) {
   HostMesh& h = c->h;
@@ -646,14 +697,16 @@ void upload_mesh_device(scdat_ctx* c) {
   // column-scaled operator A D^-1 of the inner solve (rounded once from FP64)
   std::vector<double> vh(h.aval.size());
   for (size_t o = 0; o < vh.size(); ++o) vh[o] = h.aval[o] * h.dinv[h.col[o]];
+  std::vector<float> di32(h.dinv.size());
+  for (size_t i = 0; i < di32.size(); ++i) di32[i] = (float)h.dinv[i];
+  c->npat = 0;
   if (sizeof(T) == 4) {
     for (size_t o = 0; o < vh.size(); ++o) f32[o] = (float)vh[o];
     c->d_vh32.upload(f32, s);
+    build_row_patterns(c, f32, di32);
   } else {
     c->d_vh64.upload(vh, s);
   }
-  std::vector<float> di32(h.dinv.size());
-  for (size_t i = 0; i < di32.size(); ++i) di32[i] = (float)h.dinv[i];
   c->d_dinv32.upload(di32, s);
   c->d_dinv64.upload(h.dinv, s);
   const std::vector<double>* g[3] = {&h.gx, &h.gy, &h.gz};
@@ -746,6 +799,13 @@ EllDev ell(scdat_ctx* c) {
   A.vh64 = c->d_vh64.as<double>();
   A.dinv32 = c->d_dinv32.as<float>();
   A.dinv64 = c->d_dinv64.as<double>();
+  if (c->use_patterns && c->npat > 0) {
+    A.npat = c->npat;
+    A.pat = c->d_pat.as<uint16_t>();
+    A.poff = c->d_poff.as<int32_t>();
+    A.pval32 = c->d_pval.as<float>();
+    A.pdinv32 = c->d_pdinv.as<float>();
+  }
   if (c->is_fp64()) {
     A.gx64 = c->d_g[0].as<double>();
     A.gy64 = c->d_g[1].as<double>();
@@ -3404,13 +3464,18 @@ int scdat_get_solver_info(scdat_ctx* c, scdat_solver_info* out) {
     for (int sl = 0; sl < c->h.W; ++sl) nnz += c->h.aval[(size_t)sl * c->h.nn + i] != 0.0;
   out->nonzeros = nnz;
   out->coarse_unknowns = A.nc;
-  out->reserved = 0;
+  out->row_patterns = A.npat;
   out->preconditioner = A.nc > 0 ? SCDAT_PRECOND_TWO_LEVEL : SCDAT_PRECOND_JACOBI;
   return SCDAT_OK;
 }
 
 int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
   if (!c) return SCDAT_ERR_ARG;
+  if (what == SCDAT_SHAPE_ROW_PATTERNS) {
+    if (value != 0 && value != 1) return fail(c, SCDAT_ERR_ARG, "set_launch_shape: row patterns must be 0 or 1");
+    c->use_patterns = value != 0;
+    return SCDAT_OK;
+  }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS || !(value == 1 || value == 2 || value == 4 || value == 8))
     return fail(c, SCDAT_ERR_ARG, "set_launch_shape: push chunks must be 1, 2, 4 or 8");
   c->push_chunks = value;
@@ -3418,7 +3483,12 @@ int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
 }
 
 int scdat_get_launch_shape(scdat_ctx* c, int32_t what, int32_t* value) {
-  if (!c || !value || what != SCDAT_SHAPE_PUSH_CHUNKS) return SCDAT_ERR_ARG;
+  if (!c || !value) return SCDAT_ERR_ARG;
+  if (what == SCDAT_SHAPE_ROW_PATTERNS) {
+    *value = c->use_patterns ? 1 : 0;
+    return SCDAT_OK;
+  }
+  if (what != SCDAT_SHAPE_PUSH_CHUNKS) return SCDAT_ERR_ARG;
   *value = c->push_chunks;
   return SCDAT_OK;
 }

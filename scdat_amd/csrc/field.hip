@@ -391,7 +391,12 @@ __global__ void __launch_bounds__(kPB) k_cg_start(EllDev A, FieldBufs f, InnerBu
 // coalesced slot loads issued before the scalar prologue; W = 0: runtime width.
 // TL: two-level preconditioner (coarse terms from the coarse vectors of
 // this pass; restriction partials of w_{i+1} for the coarse kernel).
-template <typename T, int W, bool TL>
+// PAT (FP32, W > 0, no TL): the row's slots and D^-1 come from its pattern;
+// the pattern id is loaded before the prologue and the table entries (hot
+// in L1/L2: neighbouring rows mostly share a pattern) right after it.
+// (Hoisting more loads above the prologue -- x, p, the scalars, unrolled
+// partials -- raised VGPRs and took the iteration from 11.7 to 12.3 us.)
+template <typename T, int W, bool TL, bool PAT>
 __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBufs<T> ib, int it) {
   __shared__ double lds[3 * (kPB / 64)];
   __shared__ double tot[3];
@@ -404,7 +409,14 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
   constexpr int WW = W > 0 ? W : 1;
   int32_t cj[kRPT][WW];
   T vj[kRPT][WW];
-  if constexpr (W > 0) {
+  int pj[kRPT];
+  if constexpr (PAT) {
+#pragma unroll
+    for (int h = 0; h < kRPT; ++h) {
+      const int64_t i = r0 + h * kPB;
+      pj[h] = i < n ? (int)A.pat[i] : 0;
+    }
+  } else if constexpr (W > 0) {
 #pragma unroll
     for (int h = 0; h < kRPT; ++h) {
       const int64_t i = r0 + h * kPB;
@@ -458,6 +470,19 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
     __syncthreads();
   }
   const double g = tot[0], d = tot[1], rr = tot[2];
+  if constexpr (PAT) {
+#pragma unroll
+    for (int h = 0; h < kRPT; ++h) {
+      const int64_t i = r0 + h * kPB;
+      const int32_t* po = A.poff + (size_t)pj[h] * W;
+      const float* pv = A.pval32 + (size_t)pj[h] * W;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        cj[h][k] = (int32_t)i + po[k];
+        vj[h][k] = (T)pv[k];
+      }
+    }
+  }
   PcgScalars* sc = f.sc + sg.s;
   const double tol = sc->inner_tol_eff;
   double beta = 0.0, alpha;
@@ -508,7 +533,9 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
     const T ri = rc[i], wi = wc[i], so = sp[i];
     const T si = fma_t(b, so, wi);
     const T rn = fma_t(-a, si, ri);
-    const T di = dinv<T>(A)[i];
+    T di;
+    if constexpr (PAT) di = (T)A.pdinv32[pj[h]];
+    else di = dinv<T>(A)[i];
     T ui = di * ri, uc = T(0);
     if constexpr (TL) {
       // u = D^-1 r + P zr on free rows (this row's aggregate is its block)
@@ -732,26 +759,39 @@ void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& 
     switch (A.w) {
 #define SCDAT_W(w) \
   case w:          \
-    hipExtLaunchKernelGGL((k_cg_iter<T, w, true>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w, true, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
     break;
       SCDAT_W(13) SCDAT_W(14) SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18)
 #undef SCDAT_W
       default:
-        hipExtLaunchKernelGGL((k_cg_iter<T, 0, true>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
+        hipExtLaunchKernelGGL((k_cg_iter<T, 0, true, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
     }
     hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 2, it, grid_rows(A.n));
     return;
   }
+  if constexpr (sizeof(T) == 4) {
+    if (A.npat > 0) switch (A.w) {
+#define SCDAT_W(w) \
+  case w:          \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w, false, true>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    return;
+      SCDAT_W(7) SCDAT_W(8) SCDAT_W(9) SCDAT_W(10) SCDAT_W(11) SCDAT_W(12) SCDAT_W(13) SCDAT_W(14)
+      SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18) SCDAT_W(19) SCDAT_W(20)
+#undef SCDAT_W
+      default:
+        break;  // no compile-time width: the ELL form below
+    }
+  }
   switch (A.w) {
 #define SCDAT_W(w) \
   case w:          \
-    hipExtLaunchKernelGGL((k_cg_iter<T, w, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w, false, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
     break;
     SCDAT_W(7) SCDAT_W(8) SCDAT_W(9) SCDAT_W(10) SCDAT_W(11) SCDAT_W(12) SCDAT_W(13) SCDAT_W(14)
     SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18) SCDAT_W(19) SCDAT_W(20)
 #undef SCDAT_W
     default:
-      hipExtLaunchKernelGGL((k_cg_iter<T, 0, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
+      hipExtLaunchKernelGGL((k_cg_iter<T, 0, false, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
   }
 }
 template <typename T>

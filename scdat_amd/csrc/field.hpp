@@ -52,6 +52,16 @@ struct EllDev {
   const double* dco64;
   const float* aci32;      // Ac^-1, nc x nc row-major
   const double* aci64;
+  // Row patterns of the FP32 inner operator (npat = 0: not used).  Rows with
+  // the same column offsets j - i, the same FP32 values of A D^-1 and the
+  // same FP32 D^-1 share one table entry, so an inner iteration reads a
+  // 2-byte pattern id per row instead of the row's ELL slots.  The values
+  // are the ELL's own, so both forms give the same bits.
+  int npat;
+  const uint16_t* pat;     // pattern of each row
+  const int32_t* poff;     // npat x w column offsets (pattern-major)
+  const float* pval32;     // npat x w values of A D^-1
+  const float* pdinv32;    // npat values of D^-1
 };
 
 struct FieldBufs {
