@@ -910,7 +910,7 @@ void read_scalars(scdat_ctx* c) {
 template <typename T>
 // `hist` (optional, >= max_outer + 1 entries) holds the inner iteration count
 // of each outer pass of the previous solve; the first batch of a pass is sized
-// from it so that a typical pass needs one or two host polls.
+// from it so that a typical pass needs one host poll.
 void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, int max_outer,
                int* hist) {
   const EllDev A = ell(c);
@@ -939,7 +939,9 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
     if (c->h_sc->outer_done) break;
     launch_inner_init<T>(A, f, ib, s);
     int it = 0, batch = 8;
-    if (hist && hist[outer] > 16) batch = std::min(1024, hist[outer] * 7 / 8);
+    // the previous solve's count + 2: most passes end in this batch, and the
+    // few launches past convergence exit at once (cheaper than a host poll)
+    if (hist && hist[outer] > 16) batch = std::min(1024, hist[outer] + 2);
     for (;;) {
       // timed launches of this batch (every kernel_timing-th iteration)
       std::vector<int> timed;
@@ -966,7 +968,7 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
         c->kt_n[1] += 1;
       }
       if (c->h_sc->inner_done) break;
-      batch = batch > 64 ? 8 : std::min(batch * 2, 64);  // after a predicted batch: small steps
+      batch = batch > 64 ? 4 : std::min(batch * 2, 64);  // after a predicted batch: small steps
     }
     if (hist) hist[outer] = c->h_sc->inner_it;
     launch_apply<T>(A, f, ib, s);
