@@ -107,7 +107,7 @@ typedef struct {
   int32_t load_sign;        /* +1 physical load (+rho), -1 reference sign         */
   double pcg_tol;           /* relative residual target ||r|| <= tol ||b||        */
   int32_t pcg_max_iter;     /* total inner iterations per solve                   */
-  double inner_tol;         /* mixed mode: FP32 inner PCG relative tolerance      */
+  double inner_tol;         /* mixed mode: FP32 inner PCG relative tolerance (1e-5) */
   int32_t sort_interval;    /* steps between cell sorts (0 = never)               */
   double bfield[3];         /* uniform magnetic field (T); reference B = 0        */
 } scdat_config;

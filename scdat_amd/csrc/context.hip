@@ -465,6 +465,8 @@ struct scdat_ctx {
   std::string err;
   hipStream_t stream = nullptr;
   bool has_mesh = false, initialized = false;
+  DevBuf d_xprev;          // previous step's solution (initial-guess extrapolation)
+  bool have_xprev = false;
   HostMesh h;
   std::array<int, kMaxGroups> gtype{};
   std::array<int, kMaxGroups> gpartner{};
@@ -744,6 +746,8 @@ void upload_mesh_device(scdat_ctx* c) {
   const size_t nn = (size_t)h.nn;
   c->d_b64.alloc(8 * nn);
   c->d_x64.alloc(8 * nn);
+  c->d_xprev.alloc(8 * nn);
+  c->have_xprev = false;
   c->d_r64.alloc(8 * nn);
   c->d_phi1.alloc(8 * nn);
   HIPCHK(hipMemsetAsync(c->d_x64.p, 0, 8 * nn, s));
@@ -1087,6 +1091,9 @@ void do_solve(scdat_ctx* c, int rho_idx) {
   const EllDev A = ell(c);
   FieldBufs f = fbufs(c);
   launch_rhs(A, f, c->d_rho[rho_idx].as<unsigned long long>(), c->unit, c->cfg.load_sign, c->stream);
+  // initial guess: 2 x_n - x_{n-1} on free rows (x_n alone from the second solve)
+  launch_extrapolate(f, c->d_xprev.as<double>(), A.n, !c->have_xprev, c->stream);
+  c->have_xprev = true;
   const bool fp64 = c->is_fp64();
   pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, c->inner_hist);
   if (c->h.has_cond && c->n_patch == 0) {
@@ -2415,7 +2422,7 @@ void scdat_config_default(scdat_config* c) {
   c->load_sign = 1;
   c->pcg_tol = 1e-8;
   c->pcg_max_iter = 20000;
-  c->inner_tol = 1e-6;
+  c->inner_tol = 1e-5;  // fewest iterations in total at 64^3 (1e-4..1e-5; 1e-6: +7 %)
   c->sort_interval = 20;
 }
 

@@ -804,6 +804,16 @@ size_t partial_doubles(int64_t n) { return (size_t)S_COUNT * (size_t)grid_rows(n
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s) {
   hipLaunchKernelGGL(k_float, dim3(f.blk_seg ? f.n_seg : 1), dim3(kBlock), 0, s, f, fa);
 }
+__global__ void __launch_bounds__(kBlock) k_extrapolate(FieldBufs f, double* prev, int64_t n, int first) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double t = f.x64[i];
+  if (!first && f.kind[i] == 0) f.x64[i] = 2.0 * t - prev[i];
+  prev[i] = t;
+}
+void launch_extrapolate(const FieldBufs& f, double* prev, int64_t n, int first, hipStream_t s) {
+  hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(n)), dim3(kBlock), 0, s, f, prev, n, first);
+}
 void launch_seg_summary(const FieldBufs& f, hipStream_t s) {
   if (!f.blk_seg) return;
   hipLaunchKernelGGL(k_seg_summary, dim3(1), dim3(kPB), 0, s, f);

@@ -145,6 +145,9 @@ struct FloatArgs {
   const double* seg_ccap;
 };
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s);
+// Initial guess of a time step's solve from the last two solutions: on free
+// rows x = 2 x_n - x_{n-1} (first = 1: x stays x_n); prev takes x_n.
+void launch_extrapolate(const FieldBufs& f, double* prev, int64_t n, int first, hipStream_t s);
 // Summary of the per-segment scalars into sc[n_seg] (no-op for one segment).
 void launch_seg_summary(const FieldBufs& f, hipStream_t s);
 template <typename T>
