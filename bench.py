@@ -367,12 +367,13 @@ def main():
     #  PCG iteration (one fused kernel): r, w, s_old, p, x read and r, w, s,
     #    p, x written, 4 B each per row; the operator either as ELL (column +
     #    value 8 B per structural entry, D^-1 4 B per row) or as row patterns
-    #    (2-byte id per row, plus the table once: W offsets and values and
-    #    D^-1 per pattern)
+    #    (2-byte id per row, 4-byte past 65536 patterns, plus the table once:
+    #    W offsets and values and D^-1 per pattern)
     push_launch = g["pushes"] / max(1, kt["push"][1]) if kt["push"][1] else 0
     sv = g["solver"]
     npat, width = sv["row_patterns"], sv["ell_width"]
-    op_bytes = (2.0 * n_nodes + npat * (8.0 * width + 4.0)) if npat else (8.0 * nnz + 4.0 * n_nodes)
+    id_bytes = 2.0 if npat <= 65536 else 4.0
+    op_bytes = (id_bytes * n_nodes + npat * (8.0 * width + 4.0)) if npat else (8.0 * nnz + 4.0 * n_nodes)
     bytes_k = {
         "push": 56.0 * push_launch + 128.0 * n_tets + 8.0 * n_nodes,
         "pcg_iter": op_bytes + 40.0 * n_nodes,

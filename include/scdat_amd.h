@@ -578,8 +578,11 @@ int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
 #define SCDAT_SHAPE_PUSH_CHUNKS 0
 /* SCDAT_SHAPE_ROW_PATTERNS: 1 (default) lets the FP32 inner iteration read
  * the operator as per-row pattern ids into a table of distinct rows (column
- * offsets and values bit-identical), when the mesh has at most 4096 distinct
- * rows; 0 keeps the ELL slots.  Same bits either way.                      */
+ * offsets and values bit-identical), when ids and table take at most half
+ * the ELL's bytes (structured meshes in their own node order); 0 keeps the
+ * ELL slots; 2 uses 4-byte pattern ids even where 2 bytes would do (the
+ * form a mesh with more than 65536 distinct rows gets).  Same bits in every
+ * form.                                                                    */
 #define SCDAT_SHAPE_ROW_PATTERNS 1
 int scdat_set_launch_shape(scdat_ctx* ctx, int32_t what, int32_t value);
 int scdat_get_launch_shape(scdat_ctx* ctx, int32_t what, int32_t* value);

@@ -643,15 +643,16 @@ class Context:
 
     @property
     def row_patterns(self):
-        """Whether the FP32 inner iteration may read the operator as row
-        patterns (it does when the mesh has few distinct rows)."""
+        """0: the FP32 inner iteration reads the ELL slots; 1 (default): row
+        patterns where they pay (few distinct rows); 2: row patterns with
+        4-byte ids."""
         v = C.c_int32()
         self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_ROW_PATTERNS, C.byref(v)))
-        return bool(v.value)
+        return v.value
 
     @row_patterns.setter
-    def row_patterns(self, on):
-        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_ROW_PATTERNS, int(bool(on))))
+    def row_patterns(self, mode):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_ROW_PATTERNS, int(mode)))
 
     def charging_forward(self, params, steps):
         """Floating potential after `steps` steps of the differentiable charging

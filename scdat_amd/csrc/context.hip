@@ -477,6 +477,8 @@ struct scdat_ctx {
   DevBuf d_col, d_v32, d_v64, d_vh32, d_vh64, d_dinv32, d_dinv64, d_g[3];
   DevBuf d_pat, d_poff, d_pval, d_pdinv;  // row patterns of the FP32 inner operator
   int npat = 0;                           // 0: none (too many, or FP64 inner solve)
+  bool pat_wide = false;                  // 4-byte pattern ids
+  bool pat_force_wide = false;            // 4-byte ids whatever the count (tests)
   bool use_patterns = true;
   DevBuf d_kind, d_lift, d_dval, d_dscale, d_phi1, d_kc_idx, d_kc_val, d_b64, d_x64, d_r64, d_partial, d_sc;
   DevBuf d_inner[8];
@@ -593,18 +595,21 @@ size_t tsize(const scdat_ctx* c) { return c->is_fp64() ? 8 : 4; }
 // Row patterns of the FP32 inner operator A D^-1: rows whose column offsets
 // j - i, FP32 values and FP32 D^-1 agree bit for bit share one table entry.
 // Structured meshes in their natural order collapse to a few hundred
-// patterns (a 64^3 box with a conductor: under a thousand for 273k rows);
-// past kMaxPatterns the inner iteration keeps reading the ELL slots.
-constexpr int kMaxPatterns = 4096;
+// patterns (a 64^3 box with a conductor: about a thousand for 273k rows; a
+// sweep of 256 boxes: about a thousand per box).  Used while the ids and
+// the table take at most half the ELL's bytes (8 W per row); 2-byte ids up
+// to 65536 patterns, 4-byte beyond.
 void build_row_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::vector<float>& di) {
   const HostMesh& h = c->h;
   const int64_t n = h.nn;
   const int W = h.W;
   c->npat = 0;
   if (W < 7 || W > 20 || n == 0) return;  // the pattern kernels exist for these widths
+  // table entries allowed: (npat (8 W + 4) + 4 n) <= 4 W n
+  const int64_t max_pat = std::max<int64_t>(0, (4 * (int64_t)W * n - 4 * n) / (8 * W + 4));
   std::unordered_map<std::string, int> id;
-  id.reserve(2 * kMaxPatterns);
-  std::vector<uint16_t> pat((size_t)n);
+  id.reserve(1024);
+  std::vector<uint32_t> pat((size_t)n);
   std::vector<int32_t> poff;
   std::vector<float> pval, pdinv;
   std::string key((size_t)(8 * W + 4), '\0');
@@ -619,7 +624,7 @@ void build_row_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::v
     std::memcpy(k + 8 * W, &di[i], 4);
     auto it = id.find(key);
     if (it == id.end()) {
-      if ((int)id.size() == kMaxPatterns) return;  // unstructured: stay on the ELL
+      if ((int64_t)id.size() >= max_pat) return;  // unstructured: stay on the ELL
       it = id.emplace(key, (int)id.size()).first;
       for (int sl = 0; sl < W; ++sl) {
         const size_t o = (size_t)sl * n + i;
@@ -628,12 +633,20 @@ void build_row_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::v
       }
       pdinv.push_back(di[i]);
     }
-    pat[(size_t)i] = (uint16_t)it->second;
+    pat[(size_t)i] = (uint32_t)it->second;
   }
-  c->d_pat.upload(pat, c->stream);
+  c->pat_wide = id.size() > 65536 || c->pat_force_wide;
+  if (c->pat_wide) {
+    c->d_pat.upload(pat, c->stream);
+  } else {
+    std::vector<uint16_t> p16(pat.begin(), pat.end());
+    c->d_pat.upload(p16, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));  // the staging vectors are locals
+  }
   c->d_poff.upload(poff, c->stream);
   c->d_pval.upload(pval, c->stream);
   c->d_pdinv.upload(pdinv, c->stream);
+  HIPCHK(hipStreamSynchronize(c->stream));  // the staging vectors are locals
   c->npat = (int)id.size();
 }
 
@@ -807,7 +820,8 @@ EllDev ell(scdat_ctx* c) {
   A.dinv64 = c->d_dinv64.as<double>();
   if (c->use_patterns && c->npat > 0) {
     A.npat = c->npat;
-    A.pat = c->d_pat.as<uint16_t>();
+    if (c->pat_wide) A.pat32 = c->d_pat.as<uint32_t>();
+    else A.pat = c->d_pat.as<uint16_t>();
     A.poff = c->d_poff.as<int32_t>();
     A.pval32 = c->d_pval.as<float>();
     A.pdinv32 = c->d_pdinv.as<float>();
@@ -3494,8 +3508,21 @@ int scdat_get_solver_info(scdat_ctx* c, scdat_solver_info* out) {
 int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
   if (!c) return SCDAT_ERR_ARG;
   if (what == SCDAT_SHAPE_ROW_PATTERNS) {
-    if (value != 0 && value != 1) return fail(c, SCDAT_ERR_ARG, "set_launch_shape: row patterns must be 0 or 1");
+    if (value < 0 || value > 2) return fail(c, SCDAT_ERR_ARG, "set_launch_shape: row patterns must be 0, 1 or 2");
     c->use_patterns = value != 0;
+    const bool wide = value == 2;
+    if (wide != c->pat_force_wide) {
+      c->pat_force_wide = wide;
+      if (c->has_mesh && !c->is_fp64())
+        return guarded(c, [&] {
+          // the FP32 values of A D^-1 and D^-1 as the upload rounds them
+          const HostMesh& h = c->h;
+          std::vector<float> vh(h.aval.size()), di(h.dinv.size());
+          for (size_t o = 0; o < vh.size(); ++o) vh[o] = (float)(h.aval[o] * h.dinv[h.col[o]]);
+          for (size_t i = 0; i < di.size(); ++i) di[i] = (float)h.dinv[i];
+          build_row_patterns(c, vh, di);
+        });
+    }
     return SCDAT_OK;
   }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS || !(value == 1 || value == 2 || value == 4 || value == 8))
@@ -3507,7 +3534,7 @@ int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
 int scdat_get_launch_shape(scdat_ctx* c, int32_t what, int32_t* value) {
   if (!c || !value) return SCDAT_ERR_ARG;
   if (what == SCDAT_SHAPE_ROW_PATTERNS) {
-    *value = c->use_patterns ? 1 : 0;
+    *value = c->use_patterns ? (c->pat_force_wide ? 2 : 1) : 0;
     return SCDAT_OK;
   }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS) return SCDAT_ERR_ARG;

@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
 #pragma unroll
     for (int h = 0; h < kRPT; ++h) {
       const int64_t i = r0 + h * kPB;
-      pj[h] = i < n ? (int)A.pat[i] : 0;
+      pj[h] = i < n ? (A.pat ? (int)A.pat[i] : (int)A.pat32[i]) : 0;
     }
   } else if constexpr (W > 0) {
 #pragma unroll

@@ -58,7 +58,8 @@ struct EllDev {
   // 2-byte pattern id per row instead of the row's ELL slots.  The values
   // are the ELL's own, so both forms give the same bits.
   int npat;
-  const uint16_t* pat;     // pattern of each row
+  const uint16_t* pat;     // pattern of each row (npat <= 65536), else
+  const uint32_t* pat32;   // 4-byte ids
   const int32_t* poff;     // npat x w column offsets (pattern-major)
   const float* pval32;     // npat x w values of A D^-1
   const float* pdinv32;    // npat values of D^-1
