@@ -429,7 +429,7 @@ int scdat_circuit_size(const scdat_circuit* c, int32_t* n_nodes, int32_t* n_bran
  * patches need a single scenario.  This batched form has no counterpart in
  * the reference, which runs one PICCycle per case
  * (PicMccSurfaceCurrentSampler.cpp:519-586 builds one per window).        */
-#define SCDAT_SEGMENT_ALIGN 256
+#define SCDAT_SEGMENT_ALIGN 512
 int scdat_set_scenarios(scdat_ctx* ctx, int32_t n, const int64_t* node_start, const double* weight_scale,
                         const double* time_step, const uint64_t* seed);
 /* Surface emission of one boundary group of one scenario
@@ -534,7 +534,7 @@ int scdat_layer_profile(scdat_ctx* ctx, int32_t axis, double length, int32_t n_c
 /* ---- Internal node order -----------------------------------------------------
  * SCDAT_ORDER_GIVEN (default) keeps the caller's node order;
  * SCDAT_ORDER_MORTON renumbers the nodes internally in Morton order of their
- * coordinates, so that consecutive 256-node row blocks are compact boxes
+ * coordinates, so that consecutive 512-node row blocks are compact boxes
  * (the coarse space of the two-level preconditioner needs this).  On a
  * structured box mesh numbered (i, j, k) the caller's order is faster for
  * the Jacobi iteration (neighbour gathers of consecutive rows coalesce:
@@ -548,7 +548,7 @@ int scdat_set_node_order(scdat_ctx* ctx, int32_t order);
 /* ---- Poisson preconditioner ------------------------------------------------
  * SCDAT_PRECOND_JACOBI (default): diagonal (MultiBoundaryElectricFieldSolver
  * .cpp:1432-1513), one fused kernel per iteration.  SCDAT_PRECOND_TWO_LEVEL:
- * M^-1 = D^-1 + P Ac^-1 P^T, P the free nodes of each 256-node block of the
+ * M^-1 = D^-1 + P Ac^-1 P^T, P the free nodes of each 512-node block of the
  * internal order (use SCDAT_ORDER_MORTON, so that blocks are compact), Ac =
  * P^T A P inverted once at scdat_set_mesh; it
  * needs about half the iterations but a second (coarse) launch per

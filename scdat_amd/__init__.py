@@ -102,7 +102,7 @@ class SurfaceEmission(C.Structure):
                 ("photo_reference_v", C.c_double)]
 
 
-SEGMENT_ALIGN = 256
+SEGMENT_ALIGN = 512
 PRECOND_JACOBI, PRECOND_TWO_LEVEL = 0, 1
 ORDER_MORTON, ORDER_GIVEN = 0, 1
 

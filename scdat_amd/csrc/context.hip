@@ -3353,12 +3353,13 @@ int scdat_set_scenarios(scdat_ctx* c, int32_t n, const int64_t* node_start, cons
     mesh_natural_order(c);  // scenarios are ranges of the caller's node numbering
     const HostMesh& h = c->h;
     const int64_t align = SCDAT_SEGMENT_ALIGN;
-    static_assert(SCDAT_SEGMENT_ALIGN == kBlock, "scenario rows align with the row blocks");
+    static_assert(SCDAT_SEGMENT_ALIGN == kPcgBlock, "scenario rows align with the PCG row blocks");
     if (node_start[0] != 0 || node_start[n] != h.nn)
       throw Error(SCDAT_ERR_ARG, "set_scenarios: node_start must run from 0 to the node count");
     for (int sg = 0; sg < n; ++sg) {
       if (node_start[sg + 1] <= node_start[sg]) throw Error(SCDAT_ERR_ARG, "set_scenarios: empty scenario");
-      if (node_start[sg] % align) throw Error(SCDAT_ERR_ARG, "set_scenarios: scenario starts must be multiples of 256");
+      if (node_start[sg] % align)
+        throw Error(SCDAT_ERR_ARG, "set_scenarios: scenario starts must be multiples of SCDAT_SEGMENT_ALIGN");
       if (weight_scale && !(weight_scale[sg] > 0.0)) throw Error(SCDAT_ERR_ARG, "set_scenarios: weight scale <= 0");
       if (time_step && !(time_step[sg] > 0.0 && std::isfinite(time_step[sg])))
         throw Error(SCDAT_ERR_ARG, "set_scenarios: bad time step");

@@ -55,7 +55,7 @@ inline uint32_t grid_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBloc
 // the same fixed order.  No device-scope fence or atomic is needed (kernel
 // boundaries order the memory), and every block obtains bit-identical
 // scalars, so a solve is bitwise reproducible.
-constexpr int kPB = kBlock;           // PCG block: 4 waves
+constexpr int kPB = kPcgBlock;        // PCG block: 8 waves
 constexpr int kRPT = 1;  // rows per thread in the row-parallel kernels (1: 14.7 us, 2: 19.7 us per iteration at 64^3)
 constexpr int kRowsPerBlock = kRPT * kPB;
 enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_WC, S_COUNT };
@@ -669,8 +669,7 @@ __global__ void __launch_bounds__(kBlock) k_phi(int64_t n, FieldBufs f, T* phiT,
   if (f.n_basis > 0) {
     for (int k = 0; k < f.n_basis; ++k) p = fma(f.coef[k], f.basis[(int64_t)k * n + i], p);
   } else if (f.phi1) {
-    // kBlock rows per block, the PCG row blocks
-    p = fma(f.sc[f.blk_seg ? f.blk_seg[lb] : 0].vfloat, f.phi1[i], p);
+    p = fma(f.sc[f.blk_seg ? f.blk_seg[i / kRowsPerBlock] : 0].vfloat, f.phi1[i], p);
   }
   phi64[i] = p;
   phiT[i] = (T)p;
