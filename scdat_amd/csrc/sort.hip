@@ -23,11 +23,27 @@ __global__ void __launch_bounds__(kBlock) k_zero_u32(uint32_t* p, int64_t n) {
   if (i < n) p[i] = 0u;
 }
 
+// Runs of equal cell inside a wavefront (particles are nearly cell-sorted):
+// first lane of this lane's run and the run's length.  Every lane of the
+// wavefront must call it.
+__device__ __forceinline__ void wave_run(int key, int& start, int& len) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const uint64_t heads = __ballot(head);
+  const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  start = 63 - __builtin_clzll(heads & le);
+  const uint64_t above = heads & ~le;
+  len = (above ? __builtin_ctzll(above) : 64) - start;
+}
+
+// Particles per cell: one atomic per run of equal cell.
 __global__ void __launch_bounds__(kBlock) k_hist(const int32_t* cell, int64_t n, uint32_t* cnt) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int c = cell[i];
-  if (c >= 0) atomicAdd(&cnt[c], 1u);
+  const int c = i < n ? cell[i] : -1;
+  int start, len;
+  wave_run(c, start, len);
+  if (c >= 0 && (int)(threadIdx.x & 63) == start) atomicAdd(&cnt[c], (uint32_t)len);
 }
 
 // Exclusive scan of one tile of 1024 counts; tile total to tot[blockIdx.x].
@@ -94,21 +110,50 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(uint32_t* out, int64_t n, c
   cursor[i] = 0u;
 }
 
+// Destination slot of every alive particle: the cell's offset plus a rank
+// taken one atomic per run; only the 4-byte source index is scattered.  The
+// particle is also packed into a record at its source slot (streamed).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_scatter(SortArgs<T> a) {
+__global__ void __launch_bounds__(kBlock) k_rank(SortArgs<T> a) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
-  const int c = a.src.cell[i];
-  if (c < 0) return;
-  const uint32_t d = a.offset[c] + atomicAdd(&a.cursor[c], 1u);
-  a.dst.x[d] = a.src.x[i];
-  a.dst.y[d] = a.src.y[i];
-  a.dst.z[d] = a.src.z[i];
-  a.dst.vx[d] = a.src.vx[i];
-  a.dst.vy[d] = a.src.vy[i];
-  a.dst.vz[d] = a.src.vz[i];
-  a.dst.cell[d] = c;
-  a.dst.id[d] = a.src.id[i];
+  const int c = i < a.n ? a.src.cell[i] : -1;
+  int start, len;
+  wave_run(c, start, len);
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0u;
+  if (c >= 0 && lane == start) base = atomicAdd(&a.cursor[c], (uint32_t)len);
+  base = __shfl(base, start, 64);
+  if (c >= 0) {
+    a.perm[a.offset[c] + base + (uint32_t)(lane - start)] = (uint32_t)i;
+    SortRec<T> r;
+    r.x = a.src.x[i];
+    r.y = a.src.y[i];
+    r.z = a.src.z[i];
+    r.vx = a.src.vx[i];
+    r.vy = a.src.vy[i];
+    r.vz = a.src.vz[i];
+    r.cell = c;
+    r.id = a.src.id[i];
+    a.stage[i] = r;
+  }
+}
+
+// The particles in destination order: one record read per particle (the
+// random side), eight streamed writes.  Scattering the eight arrays directly
+// wrote a partial line per field and particle.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gather(SortArgs<T> a) {
+  const int64_t d = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (d >= (int64_t)*a.total) return;
+  const SortRec<T> r = a.stage[a.perm[d]];
+  a.dst.x[d] = r.x;
+  a.dst.y[d] = r.y;
+  a.dst.z[d] = r.z;
+  a.dst.vx[d] = r.vx;
+  a.dst.vy[d] = r.vy;
+  a.dst.vz[d] = r.vz;
+  a.dst.cell[d] = r.cell;
+  a.dst.id[d] = r.id;
 }
 
 inline uint32_t blocks(int64_t n, int64_t per) { return (uint32_t)((n + per - 1) / per); }
@@ -124,7 +169,10 @@ void launch_sort(const SortArgs<T>& a, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kBlock), 0, s, a.tiles, (int64_t)nt, a.total);
   hipLaunchKernelGGL(k_scan_add, dim3(blocks(a.ncell, kBlock)), dim3(kBlock), 0, s, a.offset, a.ncell,
                      a.tiles, a.cursor);
-  if (a.n > 0) hipLaunchKernelGGL(k_scatter<T>, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a);
+  if (a.n > 0) {
+    hipLaunchKernelGGL(k_rank<T>, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_gather<T>, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a);
+  }
 }
 
 template void launch_sort<float>(const SortArgs<float>&, hipStream_t);

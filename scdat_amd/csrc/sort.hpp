@@ -13,6 +13,15 @@ struct SoA {
   int32_t* id;
 };
 
+// One particle as a record, for the sort's random-access pass: one 32-byte
+// (FP32) or 64-byte (FP64) read per particle instead of eight scattered
+// 4- or 8-byte reads.
+template <typename T>
+struct alignas(sizeof(T) == 4 ? 32 : 64) SortRec {
+  T x, y, z, vx, vy, vz;
+  int32_t cell, id;
+};
+
 template <typename T>
 struct SortArgs {
   SoA<T> src, dst;
@@ -23,6 +32,8 @@ struct SortArgs {
   uint32_t* cursor; // [ncell]
   uint32_t* tiles;  // [sort_tiles(ncell)]
   uint32_t* total;  // [1] alive count after the sort
+  uint32_t* perm;   // [n] source slot of each destination slot
+  SortRec<T>* stage;  // [n] the particles as records, in source order
 };
 
 template <typename T>
