@@ -242,9 +242,10 @@ def test_deposition_is_order_independent():
     assert res[0].sum() == 0  # equal numbers of electrons and ions, exact
 
 
-def test_sort_preserves_particles_and_removes_absorbed():
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sort_preserves_particles_and_removes_absorbed(precision):
     st = plasma_setup(n=8, npart=8000, seed=43)
-    c = make_ctx(st, precision=0, sort_interval=0)
+    c = make_ctx(st, precision=precision, sort_interval=0)
     c.initialize()
     c.step(3)
     x0, v0, c0, i0 = c.get_particles(0)
