@@ -108,7 +108,10 @@ typedef struct {
   double pcg_tol;           /* relative residual target ||r|| <= tol ||b||        */
   int32_t pcg_max_iter;     /* total inner iterations per solve                   */
   double inner_tol;         /* mixed mode: FP32 inner PCG relative tolerance (1e-5) */
-  int32_t sort_interval;    /* steps between cell sorts (0 = never)               */
+  int32_t sort_interval;    /* most steps between cell sorts of a species (0 =
+                               never); a species whose particles cross cells
+                               faster is sorted sooner (about every 8 cells of
+                               rms travel, at least every 2 steps)              */
   double bfield[3];         /* uniform magnetic field (T); reference B = 0        */
 } scdat_config;
 
