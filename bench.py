@@ -218,6 +218,80 @@ def run_sweep(args):
     print(json.dumps(res), flush=True)
 
 
+def run_sampler(args):
+    """Live-PIC surface-current sampler (PicMccSurfaceCurrentSampler::sample
+    and sampleWithDerivative) at the reference's default window: wall time
+    per call, one window and the three concurrent windows of dJ/dV."""
+    import scdat_amd as S
+    name, _ = S.device_info()
+    cfg = S.sampler_config()
+    _, ne, ni, _ = S.sampler_inputs(cfg)
+    S.sample_surface_current(cfg, 0.0)  # warm-up: first contexts, code objects
+    reps = max(1, args.steps // 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one = S.sample_surface_current(cfg, 0.0)
+    t_one = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        three = S.sample_surface_current(cfg, 1.0)
+    t_three = (time.perf_counter() - t0) / reps
+    pushes = (ne + ni) * cfg.window_steps
+    log(f"sampler: {ne}+{ni} macro-particles, {cfg.window_steps} steps; one window {t_one * 1e3:.1f} ms, "
+        f"three {t_three * 1e3:.1f} ms")
+    print(json.dumps({
+        "metric": "live-PIC surface-current sample with dJ/dV (three concurrent windows): wall time per call",
+        "value": t_three, "unit": "s", "n_gpus": 1, "steps": cfg.window_steps, "warmup": 1,
+        "ms_per_step": 1e3 * t_three / cfg.window_steps, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (particles) + f64 (field refinement, currents)",
+        "data": "synthetic (reference default sampler configuration, seeded loading)",
+        "config": {"workload": "PicMccSurfaceSamplerConfig defaults: %d macro-electrons + %d macro-ions, %d steps, "
+                               "argon MCC %s" % (ne, ni, cfg.window_steps, "on" if cfg.enable_mcc else "off")},
+        "device": name,
+        "one_window_s": t_one, "three_windows_s": t_three,
+        "concurrency_gain": 3.0 * t_one / t_three if t_three > 0 else None,
+        "particle_steps_per_s_three_windows": 3.0 * pushes / t_three if t_three > 0 else None,
+        "sample": {k: one[k] for k in ("net_collection_current_density_a_per_m2", "total_collisions")},
+        "derivative_a_per_m2_per_v": three["current_derivative_a_per_m2_per_v"],
+    }), flush=True)
+
+
+def run_arc(args):
+    """Vacuum-arc PIC boundary window: wall time per advance (set the
+    cathode current density, seed the probe protons, one PIC step, ledger
+    currents), 1e-10 s advances with about 4 field electrons per ion impact."""
+    import scdat_amd as S
+    from scdat_amd import arc
+    name, _ = S.device_info()
+    cfg = arc.WindowConfig(surface_potential_v=-200.0, enable_secondary_electron_emission=True,
+                           secondary_electron_yield_per_ion=0.5)
+    w = arc.BoundaryWindow(cfg)
+    dt = 1.0e-10
+    j = 4.0 * S.ELEMENTARY_CHARGE / (w.area * dt)
+    for _ in range(args.warmup):
+        w.advance(j, dt)
+    n = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        st = w.advance(j, dt)
+    wall = (time.perf_counter() - t0) / n
+    led = w.ctx.ledger(6)
+    log(f"arc window: {n} advances, {wall * 1e3:.3f} ms each, ledger {led[arc.ZMIN]}")
+    print(json.dumps({
+        "metric": "vacuum-arc PIC boundary window: wall time per advance",
+        "value": wall, "unit": "s", "n_gpus": 1, "steps": n, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (probe protons of the reference's seeding rule)",
+        "config": {"workload": "plate 1e-4 x 1e-4 x 5e-4 m, 1x1x6 hexes, cathode -200 V, J for 4 field electrons "
+                               "per impact, secondary yield 0.5, dt 1e-10 s"},
+        "device": name,
+        "cathode_ion_impacts": led[arc.ZMIN]["absorbed_ion_particles"],
+        "emitted_electrons": led[arc.ZMIN]["emitted_electron_particles"],
+        "last_status": st.__dict__,
+    }), flush=True)
+    w.close()
+
+
 def run_adjoint(args):
     import scdat_amd as S
     from scdat_amd.adjoint import central_difference
@@ -330,7 +404,7 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no per-launch events in the timed region (no roofline figures)")
     ap.add_argument("--copy-calibration", type=float, default=0.0)
-    ap.add_argument("--workload", choices=("box", "sweep", "adjoint"), default="box")
+    ap.add_argument("--workload", choices=("box", "sweep", "adjoint", "sampler", "arc"), default="box")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--scenarios", type=int, default=256)
     ap.add_argument("--sweep-seed", type=int, default=20261206)
@@ -348,6 +422,12 @@ def main():
         return
     if args.workload == "adjoint":
         run_adjoint(args)
+        return
+    if args.workload == "sampler":
+        run_sampler(args)
+        return
+    if args.workload == "arc":
+        run_arc(args)
         return
 
     import scdat_amd as S
