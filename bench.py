@@ -441,9 +441,8 @@ def main():
     nnz = structural_nnz(m["tets"], n_nodes)
     kt = g["kt"]
     # Algorithmic bytes per launch.
-    #  push (one species): particle SoA read + write 2 x 28 B; the 128 B tet
-    #    record (map, nodes, neighbours, E at the nodes) per tet; integer
-    #    charge 8 B per node
+    #  push (one species): particle SoA read + write 2 x 28 B; tet map 64 B
+    #    per tet; nodal E 16 B + integer charge 8 B per node
     #  PCG iteration (one fused kernel): r, w, s_old, p, x read and r, w, s,
     #    p, x written, 4 B each per row; the operator either as ELL (column +
     #    value 8 B per structural entry, D^-1 4 B per row) or as row patterns
@@ -455,7 +454,7 @@ def main():
     id_bytes = 2.0 if npat <= 65536 else 4.0
     op_bytes = (id_bytes * n_nodes + npat * (8.0 * width + 4.0)) if npat else (8.0 * nnz + 4.0 * n_nodes)
     bytes_k = {
-        "push": 56.0 * push_launch + 128.0 * n_tets + 8.0 * n_nodes,
+        "push": 56.0 * push_launch + 64.0 * n_tets + 24.0 * n_nodes,
         "pcg_iter": op_bytes + 40.0 * n_nodes,
     }
     traffic = {}

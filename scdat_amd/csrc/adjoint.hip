@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(AdjArgs<T> a) {
       T lam[4];
       bary(m, x, lam);
       T E[3];
-      gather_E(a.mesh, c, m, lam, E);
+      gather_E(a.mesh, m, lam, E);
       const T B0[3] = {T(0), T(0), T(0)};
       boris(v, E, B0, false, a.qm[s], a.dt);
       T xs[3] = {x[0], x[1], x[2]};

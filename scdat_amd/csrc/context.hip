@@ -663,8 +663,6 @@ void upload_mesh_device(scdat_ctx* c) {
     for (int k = 0; k < 9; ++k) m.a[k] = (T)h.A[9 * t + k];
     for (int k = 0; k < 3; ++k) m.x0[k] = (T)h.X0[3 * t + k];
     for (int k = 0; k < 4; ++k) m.node[k] = h.tets[4 * t + k];
-    if constexpr (sizeof(T) == 4)
-      for (int k = 0; k < 4; ++k) m.nb[k] = h.nb[4 * t + k];
   }
   c->d_tets.upload(tm, s);
   c->d_xyz64.upload(h.xyz, s);
@@ -1207,7 +1205,6 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   a.groups = c->d_groups.as<int4>();
   a.partner = c->d_partner.as<int4>();
   a.E = c->d_E.as<Vec4<T>>();
-  a.tet_e = sizeof(T) == 4 ? 1 : 0;  // the push's caller refreshes the records
   a.rho = c->d_rho[c->rho_target].as<unsigned long long>();
   a.periodic_mask = 0;
   a.reflect_mask = 0;
@@ -1268,13 +1265,6 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   return a;
 }
 
-// FP32 pushes gather E from the tet records: copy the nodal field into them.
-template <typename T>
-void refresh_tet_field(scdat_ctx* c) {
-  if constexpr (sizeof(T) == 4)
-    launch_tet_field(c->d_tets.as<TetMap<float>>(), c->d_E.as<Vec4<float>>(), c->h.nt, c->stream);
-}
-
 // Launch-shape tuner: for 1, 2, 4 and 8 particles per thread, pushes scratch
 // copies of every species for half a sort interval (2..10 pushes, so the
 // particle order is as disordered as it is on average between sorts) in the
@@ -1302,7 +1292,6 @@ void autotune_push(scdat_ctx* c) {
   HIPCHK(hipMemsetAsync(cnt.p, 0, 32, s));
   const int P = std::max(2, std::min(10, c->cfg.sort_interval / 2));
   const int cand[4] = {1, 2, 4, 8};
-  refresh_tet_field<T>(c);
   double best = 1e300;
   int pick = c->push_chunks;
   for (int k = 0; k < 4; ++k) {
@@ -1382,7 +1371,6 @@ hipEvent_t take_event(scdat_ctx* c, size_t idx) {
 template <typename T>
 void do_push(scdat_ctx* c, std::vector<std::pair<size_t, size_t>>* ev, size_t* evn) {
   zero_rho<T>(c, c->rho_target);
-  refresh_tet_field<T>(c);
   for (size_t si = 0; si < c->sp.size(); ++si) {
     if (c->sp[si].n == 0) continue;
     const PushArgs<T> a = push_args<T>(c, (int)si);
@@ -2225,7 +2213,6 @@ template <typename T>
 AdjArgs<T> adj_args(scdat_ctx* c, const ChargingRun& run) {
   AdjArgs<T> a{};
   a.mesh = push_args<T>(c, 0);
-  a.mesh.tet_e = 0;  // the adjoint gathers from its own stored fields
   a.sp = c->d_adj_sp.as<int32_t>();
   a.n = c->adj_np;
   for (size_t si = 0; si < c->sp.size(); ++si) {

@@ -16,22 +16,16 @@ constexpr int32_t kLambdaOne = 1 << kLambdaBits;
 enum : int { BC_ABSORB = 0, BC_PERIODIC = 1, BC_REFLECT = 2, BC_CONDUCTOR = 3 };
 
 // Affine barycentric map of one tet plus its node ids: lambda_{1..3} =
-// a (x - x0), lambda_0 = 1 - sum.  FP32: one 128-byte record per tet (one L2
-// line) that also holds the face neighbours and E at the four nodes
-// (refreshed from the nodal field before every push), so a walk step or a
-// field gather touches one line instead of the map, the neighbour table and
-// four node lines.
+// a (x - x0), lambda_0 = 1 - sum.  One 64-byte record per tet (FP32), read
+// once per particle-step and shared by all particles of the tet.
 template <typename T>
 struct TetMap;
 template <>
-struct alignas(128) TetMap<float> {
+struct alignas(64) TetMap<float> {
   float a[9];
   float x0[3];
   int32_t node[4];
-  int32_t nb[4];   // face neighbours (face k opposite vertex k), < 0 boundary
-  float e[12];     // E at node k: e[3k .. 3k+2]
 };
-static_assert(sizeof(TetMap<float>) == 128, "one L2 line per tet");
 template <>
 struct alignas(128) TetMap<double> {
   double a[9];

@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
         T lam[4];
         bary(m, x, lam);
         T E[3];
-        gather_E(a, c, m, lam, E);
+        gather_E(a, m, lam, E);
         const T h =
             a.adaptive ? adaptive_dt(v0, E, a.bmag, a.qm, remaining, a.floor_ratio) : remaining;
         boris(v, E, a.B, a.has_b != 0, a.qm, h);
@@ -293,22 +293,6 @@ void launch_deposit(const PushArgs<T>& a, hipStream_t s) {
   if (a.n <= 0) return;
   hipLaunchKernelGGL(deposit_kernel<T>, dim3(push_blocks(a.n, 4)), dim3(kBlock), 0, s, a);
 }
-// E at the four nodes of every tet into the FP32 tet records.
-__global__ void __launch_bounds__(kBlock) k_tet_field(TetMap<float>* tets, const Vec4<float>* E, int64_t nt) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nt) return;
-  const int4 nd = *reinterpret_cast<const int4*>(tets[t].node);
-  const Vec4<float> e0 = E[nd.x], e1 = E[nd.y], e2 = E[nd.z], e3 = E[nd.w];
-  float4* q = reinterpret_cast<float4*>(tets[t].e);
-  q[0] = make_float4(e0.x, e0.y, e0.z, e1.x);
-  q[1] = make_float4(e1.y, e1.z, e2.x, e2.y);
-  q[2] = make_float4(e2.z, e3.x, e3.y, e3.z);
-}
-void launch_tet_field(TetMap<float>* tets, const Vec4<float>* E, int64_t nt, hipStream_t s) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_tet_field, dim3((uint32_t)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, tets, E, nt);
-}
-
 template void launch_push<float>(const PushArgs<float>&, hipStream_t, hipEvent_t, hipEvent_t);
 template void launch_push<double>(const PushArgs<double>&, hipStream_t, hipEvent_t, hipEvent_t);
 template void launch_deposit<float>(const PushArgs<float>&, hipStream_t);

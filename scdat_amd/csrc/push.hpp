@@ -17,7 +17,6 @@ struct PushArgs {
   const int4* groups;   // boundary group per face
   const int4* partner;  // periodic partner tet per face
   const Vec4<T>* E;     // nodal E
-  int tet_e;            // FP32: 1 = gather E from the tet records (refreshed)
   unsigned long long* rho;  // int64 fixed-point nodal charge
   uint32_t periodic_mask;  // bit g set: group g periodic
   uint32_t reflect_mask;   // bit g set: group g specular
@@ -65,9 +64,6 @@ struct PushArgs {
 // blocks of one push launch over n particle slots with `chunks` particles
 // per thread (one statistics partial each)
 int push_blocks(int64_t n, int chunks);
-// E at the tets' nodes into the FP32 tet records (before a push that gathers
-// from them)
-void launch_tet_field(TetMap<float>* tets, const Vec4<float>* E, int64_t nt, hipStream_t s);
 template <typename T>
 // with start/stop non-null they receive the kernel's dispatch timestamps
 void launch_push(const PushArgs<T>& a, hipStream_t s, hipEvent_t start = nullptr,

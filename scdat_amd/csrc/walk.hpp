@@ -6,24 +6,8 @@
 namespace scdat {
 
 template <typename T>
-__device__ __forceinline__ void gather_E(const PushArgs<T>& a, int cell, const TetMap<T>& m, const T lam[4],
-                                         T E[3]) {
+__device__ __forceinline__ void gather_E(const PushArgs<T>& a, const TetMap<T>& m, const T lam[4], T E[3]) {
   E[0] = E[1] = E[2] = T(0);
-  if constexpr (sizeof(T) == 4) {
-    if (a.tet_e) {
-      // the same values and operation order as the nodal gather below
-      const float4* q = reinterpret_cast<const float4*>(a.tets[cell].e);
-      const float4 q0 = q[0], q1 = q[1], q2 = q[2];
-      const float e[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        E[0] = fma_t(lam[k], e[3 * k], E[0]);
-        E[1] = fma_t(lam[k], e[3 * k + 1], E[1]);
-        E[2] = fma_t(lam[k], e[3 * k + 2], E[2]);
-      }
-      return;
-    }
-  }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const Vec4<T> e = a.E[m.node[k]];
@@ -69,9 +53,7 @@ __device__ __forceinline__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1
       tb = T(0);
     }
     tb = tb < T(0) ? T(0) : (tb > T(1) ? T(1) : tb);
-    int4 nb4;
-    if constexpr (sizeof(T) == 4) nb4 = *reinterpret_cast<const int4*>(a.tets[cur].nb);
-    else nb4 = a.nb[cur];
+    const int4 nb4 = a.nb[cur];
     const int n = kb == 0 ? nb4.x : (kb == 1 ? nb4.y : (kb == 2 ? nb4.z : nb4.w));
     if (n >= 0) {
       cur = n;
