@@ -587,6 +587,13 @@ int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
  * form a mesh with more than 65536 distinct rows gets).  Same bits in every
  * form.                                                                    */
 #define SCDAT_SHAPE_ROW_PATTERNS 1
+/* SCDAT_SHAPE_PCG_BLOCK: rows (= threads) of a PCG row block.  0 (default):
+ * 512, or 256 for a single scenario under 131072 rows with the Jacobi
+ * preconditioner; 256 or 512 fix it (batched scenarios and the two-level
+ * preconditioner always use 512).  The block partition fixes the order of
+ * the solve's reductions: a batched case equals its stand-alone run bit for
+ * bit when both use 512.                                                   */
+#define SCDAT_SHAPE_PCG_BLOCK 2
 int scdat_set_launch_shape(scdat_ctx* ctx, int32_t what, int32_t value);
 int scdat_get_launch_shape(scdat_ctx* ctx, int32_t what, int32_t* value);
 int scdat_autotune(scdat_ctx* ctx);

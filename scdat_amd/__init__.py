@@ -115,6 +115,7 @@ class SolverInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 SHAPE_PUSH_CHUNKS = 0
 SHAPE_ROW_PATTERNS = 1
+SHAPE_PCG_BLOCK = 2
 
 
 class ScenarioStats(C.Structure):
@@ -653,6 +654,17 @@ class Context:
     @row_patterns.setter
     def row_patterns(self, mode):
         self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_ROW_PATTERNS, int(mode)))
+
+    @property
+    def pcg_block(self):
+        """Rows of a PCG row block: 0 (by size), 256 or 512."""
+        v = C.c_int32()
+        self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_PCG_BLOCK, C.byref(v)))
+        return v.value
+
+    @pcg_block.setter
+    def pcg_block(self, rows):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_PCG_BLOCK, int(rows)))
 
     def charging_forward(self, params, steps):
         """Floating potential after `steps` steps of the differentiable charging

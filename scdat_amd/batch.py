@@ -89,16 +89,16 @@ def _emission(ctx, scenario, sc):
 def standalone_context(sc, inputs=None, precision=0, tol=1e-8, sort_interval=20, see=True, capacity_factor=2,
                        autotune=False, like_batch=True):
     """The single-case context of case `sc`.  like_batch: the settings a
-    batched case runs with (the caller's node order, Jacobi-PCG), under
-    which the batched case reproduces it bit for bit; otherwise the
-    context's defaults (Morton node order, two-level preconditioner on
-    large meshes)."""
+    batched case runs with (the caller's node order, Jacobi-PCG, 512-row
+    PCG blocks), under which the batched case reproduces it bit for bit;
+    otherwise the context's defaults."""
     st = inputs or case_inputs(sc)
     m = st["mesh"]
     c = Context(precision=precision, dt=st["dt"], pcg_tol=tol, sort_interval=sort_interval)
     if like_batch:
         c.set_node_order(ORDER_GIVEN)
         c.set_preconditioner(PRECOND_JACOBI)
+        c.pcg_block = 512  # the batched solve's row blocks
     c.set_mesh(m["xyz"], m["tets"], m["kind"], group=m["group"])
     for g in CONDUCTOR_GROUPS:
         c.set_boundary(g, BC_CONDUCTOR)

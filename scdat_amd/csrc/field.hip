@@ -55,9 +55,11 @@ inline uint32_t grid_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBloc
 // the same fixed order.  No device-scope fence or atomic is needed (kernel
 // boundaries order the memory), and every block obtains bit-identical
 // scalars, so a solve is bitwise reproducible.
-constexpr int kPB = kPcgBlock;        // PCG block: 8 waves
-constexpr int kRPT = 1;  // rows per thread in the row-parallel kernels (1: 14.7 us, 2: 19.7 us per iteration at 64^3)
-constexpr int kRowsPerBlock = kRPT * kPB;
+// Row blocks of the PCG kernels: one row per thread, PB (256 or 512) rows
+// per block, chosen per context (EllDev::pb); kPB, the larger, is the
+// block of the one-block finishing kernels.
+constexpr int kPB = kPcgBlock;
+constexpr int kRPT = 1;
 enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_WC, S_COUNT };
 
 __device__ inline double* slot(const FieldBufs& f, int s, int G) { return f.partial + (size_t)s * G; }
@@ -96,15 +98,16 @@ __device__ inline void publish(double v, double* p, double* lds) {
   if (threadIdx.x == 0) p[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kPB) k_rhs(EllDev A, FieldBufs f,
+template <int PB>
+__global__ void __launch_bounds__(PB) k_rhs(EllDev A, FieldBufs f,
                                                  const unsigned long long* rho, double unit,
                                                  int load_sign) {
-  __shared__ double lds[kPB / 64];
+  __shared__ double lds[PB / 64];
   const int G = gridDim.x;
   if (f.seg_unit) unit = f.seg_unit[seg_of_block(f, blockIdx.x, G).s];
   double bb = 0.0, cl = 0.0;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
-  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
+  const int64_t r0 = (int64_t)blockIdx.x * PB;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + PB && i < A.n; i += PB) {
     const int k = f.kind[i];
     const double load = (double)load_sign * unit * (double)(long long)rho[i];
     double b;
@@ -124,13 +127,14 @@ __global__ void __launch_bounds__(kPB) k_rhs(EllDev A, FieldBufs f,
 // Right-hand side from an FP64 nodal vector: src as deposited charge (scaled
 // like k_rhs), or, homogeneous, the load on free rows and zero on
 // constrained rows (no lift): the adjoint solve A y = P load(src).
-__global__ void __launch_bounds__(kPB) k_rhs_real(EllDev A, FieldBufs f, const double* src, double scale,
+template <int PB>
+__global__ void __launch_bounds__(PB) k_rhs_real(EllDev A, FieldBufs f, const double* src, double scale,
                                                   int load_sign, int homogeneous) {
-  __shared__ double lds[kPB / 64];
+  __shared__ double lds[PB / 64];
   const int G = gridDim.x;
   double bb = 0.0, cl = 0.0;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
-  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
+  const int64_t r0 = (int64_t)blockIdx.x * PB;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + PB && i < A.n; i += PB) {
     const int k = f.kind[i];
     const double load = (double)load_sign * scale * src[i];
     double b;
@@ -169,13 +173,14 @@ __global__ void __launch_bounds__(kPB) k_fin_rhs(FieldBufs f, int G) {
   }
 }
 
-__global__ void __launch_bounds__(kPB) k_resid64(EllDev A, FieldBufs f) {
+template <int PB>
+__global__ void __launch_bounds__(PB) k_resid64(EllDev A, FieldBufs f) {
   const int G = gridDim.x;
   if (f.sc[seg_of_block(f, blockIdx.x, G).s].outer_done) return;
-  __shared__ double lds[kPB / 64];
+  __shared__ double lds[PB / 64];
   double rr = 0.0;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
-  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
+  const int64_t r0 = (int64_t)blockIdx.x * PB;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + PB && i < A.n; i += PB) {
     double s = 0.0;
     for (int k = 0; k < A.w; ++k) {
       const int64_t o = (int64_t)k * A.n + i;
@@ -227,7 +232,7 @@ __global__ void __launch_bounds__(kPB) k_fin_resid(FieldBufs f, int G) {
 // from (r_i, w_i, s_{i-1}) with the same fused operations the owner uses, so
 // w_{i+1} = (A D^-1) r_{i+1} needs no second launch.  r, w and s are
 // double-buffered by iteration parity; p and x are owner-only, in place.
-static_assert(kRowsPerBlock == kCoarseBlockRows, "the two-level aggregates are the row blocks");
+static_assert(kPcgBlock == kCoarseBlockRows, "the two-level aggregates are the 512-row blocks");
 constexpr int kCoarseRowsPerWave = 2;  // rows of Ac^-1 per wave in the coarse kernel
 
 // Restriction of the two-level preconditioner: block b's partial of P^T v is
@@ -237,14 +242,14 @@ __device__ inline void publish_restrict(double v, double* out, double* lds) {
   if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kPB) k_inner_init(EllDev A, FieldBufs f, InnerBufs<T> ib) {
+template <typename T, int PB>
+__global__ void __launch_bounds__(PB) k_inner_init(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   const PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
   if (sc->outer_done) return;
   const double inv = sc->inv_rnorm;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
+  const int64_t r0 = (int64_t)blockIdx.x * PB;
   double rv = 0.0;
-  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
+  for (int64_t i = r0 + threadIdx.x; i < r0 + PB && i < A.n; i += PB) {
     const T ri = (T)(f.r64[i] * inv);
     ib.r[0][i] = ri;
     ib.s[1][i] = T(0);
@@ -253,7 +258,7 @@ __global__ void __launch_bounds__(kPB) k_inner_init(EllDev A, FieldBufs f, Inner
     if (A.nc && dco<T>(A)[i] != T(0)) rv += (double)ri;
   }
   if (A.nc) {
-    __shared__ double lds[kPB / 64];
+    __shared__ double lds[PB / 64];
     publish_restrict(rv, slot(f, S_WC, gridDim.x), lds);
   }
 }
@@ -317,6 +322,7 @@ __global__ void __launch_bounds__(kPB) k_coarse(EllDev A, FieldBufs f, InnerBufs
   }
 }
 
+template <int PB>
 __device__ inline void publish3(double a, double b, double c, double* pa, double* pb, double* pc,
                                 double* lds) {
 #pragma unroll
@@ -325,7 +331,7 @@ __device__ inline void publish3(double a, double b, double c, double* pa, double
     b += __shfl_xor(b, off, 64);
     c += __shfl_xor(c, off, 64);
   }
-  constexpr int NW = kPB / 64;
+  constexpr int NW = PB / 64;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   __syncthreads();
   if (lane == 0) {
@@ -354,22 +360,22 @@ __device__ inline T r_next(T r, T w, T s, T a, T b) {
 }
 
 // w_0 = (A D^-1) r_0 and the first dot products.
-template <typename T>
-__global__ void __launch_bounds__(kPB) k_cg_start(EllDev A, FieldBufs f, InnerBufs<T> ib) {
+template <typename T, int PB>
+__global__ void __launch_bounds__(PB) k_cg_start(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   const int G = gridDim.x;
   if (f.sc[seg_of_block(f, blockIdx.x, G).s].outer_done) return;
-  __shared__ double lds[3 * (kPB / 64)];
+  __shared__ double lds[3 * (PB / 64)];
   const T* __restrict__ v = vhat<T>(A);
   const T* __restrict__ r = ib.r[0];
   double g = 0.0, d = 0.0, rr = 0.0, wv = 0.0;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
-  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB) {
+  const int64_t r0 = (int64_t)blockIdx.x * PB;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + PB && i < A.n; i += PB) {
     T acc = T(0);
     // w_0 = A u_0, u_0 = D^-1 r_0 + P zr_0: column j carries r_j + D_j zr_0[block of j]
     for (int k = 0; k < A.w; ++k) {
       const int64_t o = (int64_t)k * A.n + i;
       const int j = A.col[o];
-      const T uj = A.nc ? fma_t(dco<T>(A)[j], ib.zr[0][j / kRowsPerBlock], r[j]) : r[j];
+      const T uj = A.nc ? fma_t(dco<T>(A)[j], ib.zr[0][j / kCoarseBlockRows], r[j]) : r[j];
       acc = fma_t(v[o], uj, acc);
     }
     const T ri = r[i];
@@ -383,7 +389,7 @@ __global__ void __launch_bounds__(kPB) k_cg_start(EllDev A, FieldBufs f, InnerBu
     d += (double)acc * (double)u;
     rr += (double)ri * (double)ri;
   }
-  publish3(g, d, rr, slot(f, S_G0, G), slot(f, S_D0, G), slot(f, S_RR0, G), lds);
+  publish3<PB>(g, d, rr, slot(f, S_G0, G), slot(f, S_D0, G), slot(f, S_RR0, G), lds);
   if (A.nc) publish_restrict(wv, slot(f, S_WC, G), lds);
 }
 
@@ -396,16 +402,16 @@ __global__ void __launch_bounds__(kPB) k_cg_start(EllDev A, FieldBufs f, InnerBu
 // in L1/L2: neighbouring rows mostly share a pattern) right after it.
 // (Hoisting more loads above the prologue -- x, p, the scalars, unrolled
 // partials -- raised VGPRs and took the iteration from 11.7 to 12.3 us.)
-template <typename T, int W, bool TL, bool PAT>
-__global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBufs<T> ib, int it) {
-  __shared__ double lds[3 * (kPB / 64)];
+template <typename T, int W, bool TL, bool PAT, int PB>
+__global__ void __launch_bounds__(PB) k_cg_iter(EllDev A, FieldBufs f, InnerBufs<T> ib, int it) {
+  __shared__ double lds[3 * (PB / 64)];
   __shared__ double tot[3];
   const int G = gridDim.x;
   const int cur = it & 1, nxt = cur ^ 1;
   const T* __restrict__ v = vhat<T>(A);
   const int32_t* __restrict__ col = A.col;
   const int64_t n = A.n;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * PB + threadIdx.x;
   constexpr int WW = W > 0 ? W : 1;
   int32_t cj[kRPT][WW];
   T vj[kRPT][WW];
@@ -413,13 +419,13 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
   if constexpr (PAT) {
 #pragma unroll
     for (int h = 0; h < kRPT; ++h) {
-      const int64_t i = r0 + h * kPB;
+      const int64_t i = r0 + h * PB;
       pj[h] = i < n ? (A.pat ? (int)A.pat[i] : (int)A.pat32[i]) : 0;
     }
   } else if constexpr (W > 0) {
 #pragma unroll
     for (int h = 0; h < kRPT; ++h) {
-      const int64_t i = r0 + h * kPB;
+      const int64_t i = r0 + h * PB;
       const bool ok = i < n;
 #pragma unroll
       for (int k = 0; k < W; ++k) {
@@ -436,7 +442,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
     const double* pr = slot(f, cur ? S_RR1 : S_RR0, G) + sg.b0;
     const int nb = sg.b1 - sg.b0;
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int j = threadIdx.x; j < nb; j += kPB) {
+    for (int j = threadIdx.x; j < nb; j += PB) {
       a += pg[j];
       b += pd[j];
       c += pr[j];
@@ -447,7 +453,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
       b += __shfl_xor(b, off, 64);
       c += __shfl_xor(c, off, 64);
     }
-    constexpr int NW = kPB / 64;
+    constexpr int NW = PB / 64;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
       lds[wv] = a;
@@ -473,7 +479,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
   if constexpr (PAT) {
 #pragma unroll
     for (int h = 0; h < kRPT; ++h) {
-      const int64_t i = r0 + h * kPB;
+      const int64_t i = r0 + h * PB;
       const int32_t* po = A.poff + (size_t)pj[h] * W;
       const float* pv = A.pval32 + (size_t)pj[h] * W;
 #pragma unroll
@@ -509,7 +515,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
   auto zr1 = [&](int c) { return fma_t(-a, fma_t(b, ib.zs[nxt][c], ib.zw[cur][c]), ib.zr[cur][c]); };
 #pragma unroll
   for (int h = 0; h < kRPT; ++h) {
-    const int64_t i = r0 + h * kPB;
+    const int64_t i = r0 + h * PB;
     if (i >= n) continue;
     T acc = T(0);
     // two-level: column j carries D_j u_j = r_j + D_j zr_{i+1}[block of j]
@@ -518,7 +524,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
       for (int k = 0; k < W; ++k) {
         const int j = cj[h][k];
         T uj = r_next(rc[j], wc[j], sp[j], a, b);
-        if constexpr (TL) uj = fma_t(dco<T>(A)[j], zr1(j / kRowsPerBlock), uj);
+        if constexpr (TL) uj = fma_t(dco<T>(A)[j], zr1(j / kCoarseBlockRows), uj);
         acc = fma_t(vj[h][k], uj, acc);
       }
     } else {
@@ -526,7 +532,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
         const int64_t o = (int64_t)k * n + i;
         const int j = col[o];
         T uj = r_next(rc[j], wc[j], sp[j], a, b);
-        if constexpr (TL) uj = fma_t(dco<T>(A)[j], zr1(j / kRowsPerBlock), uj);
+        if constexpr (TL) uj = fma_t(dco<T>(A)[j], zr1(j / kCoarseBlockRows), uj);
         acc = fma_t(v[o], uj, acc);
       }
     }
@@ -556,7 +562,7 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
     d1 += (double)acc * (double)un;
     rr1 += (double)rn * (double)rn;
   }
-  publish3(g1, d1, rr1, slot(f, nxt ? S_G1 : S_G0, G), slot(f, nxt ? S_D1 : S_D0, G),
+  publish3<PB>(g1, d1, rr1, slot(f, nxt ? S_G1 : S_G0, G), slot(f, nxt ? S_D1 : S_D0, G),
            slot(f, nxt ? S_RR1 : S_RR0, G), lds);
   if ((int)blockIdx.x == sg.b0 && threadIdx.x == 0) {
     sc->hist[cur] = g;
@@ -568,13 +574,13 @@ __global__ void __launch_bounds__(kPB) k_cg_iter(EllDev A, FieldBufs f, InnerBuf
 
 // Closes an inner solve: records its iteration count and applies
 // x += |r| d (skipped when the outer loop has already converged).
-template <typename T>
-__global__ void __launch_bounds__(kPB) k_apply(EllDev A, FieldBufs f, InnerBufs<T> ib) {
+template <typename T, int PB>
+__global__ void __launch_bounds__(PB) k_apply(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   const PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
   if (sc->outer_done) return;
   const double rn = sc->rnorm;
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock;
-  for (int64_t i = r0 + threadIdx.x; i < r0 + kRowsPerBlock && i < A.n; i += kPB)
+  const int64_t r0 = (int64_t)blockIdx.x * PB;
+  for (int64_t i = r0 + threadIdx.x; i < r0 + PB && i < A.n; i += PB)
     f.x64[i] = fma(rn, (double)ib.x[i], f.x64[i]);
 }
 
@@ -669,7 +675,7 @@ __global__ void __launch_bounds__(kBlock) k_phi(int64_t n, FieldBufs f, T* phiT,
   if (f.n_basis > 0) {
     for (int k = 0; k < f.n_basis; ++k) p = fma(f.coef[k], f.basis[(int64_t)k * n + i], p);
   } else if (f.phi1) {
-    p = fma(f.sc[f.blk_seg ? f.blk_seg[i / kRowsPerBlock] : 0].vfloat, f.phi1[i], p);
+    p = fma(f.sc[f.blk_seg ? f.blk_seg[i / kPcgBlock] : 0].vfloat, f.phi1[i], p);
   }
   phi64[i] = p;
   phiT[i] = (T)p;
@@ -717,62 +723,61 @@ __global__ void __launch_bounds__(kBlock) k_efield(EllDev A, const T* __restrict
 
 }  // namespace
 
-static inline int grid_rows(int64_t n) { return (int)((n + kRowsPerBlock - 1) / kRowsPerBlock); }
+static inline int grid_rows(int64_t n, int pb) { return (int)((n + pb - 1) / pb); }
 // blocks of the coarse kernel: each recomputes the restricted vector, so few
 static inline int coarse_blocks(const EllDev& A) {
   const int per_block = (kPB / 64) * kCoarseRowsPerWave;
   return (A.nc + per_block - 1) / per_block;
 }
 
+// Row-block kernels by block size (EllDev::pb).
+#define SCDAT_PB(call) \
+  do {                 \
+    if (A.pb == 256) {  \
+      constexpr int PB = 256; \
+      call;            \
+    } else {           \
+      constexpr int PB = 512; \
+      call;            \
+    }                  \
+  } while (0)
+
 void launch_rhs(const EllDev& A, const FieldBufs& f, const unsigned long long* rho_fx,
                 double unit, int load_sign, hipStream_t s) {
-  const int G = grid_rows(A.n);
-  hipLaunchKernelGGL(k_rhs, dim3(G), dim3(kPB), 0, s, A, f, rho_fx, unit, load_sign);
+  const int G = grid_rows(A.n, A.pb);
+  SCDAT_PB(hipLaunchKernelGGL(k_rhs<PB>, dim3(G), dim3(PB), 0, s, A, f, rho_fx, unit, load_sign));
   hipLaunchKernelGGL(k_fin_rhs, dim3(f.blk_seg ? f.n_seg : 1), dim3(kPB), 0, s, f, G);
 }
 void launch_rhs_real(const EllDev& A, const FieldBufs& f, const double* src, double scale, int load_sign,
                      int homogeneous, hipStream_t s) {
-  const int G = grid_rows(A.n);
-  hipLaunchKernelGGL(k_rhs_real, dim3(G), dim3(kPB), 0, s, A, f, src, scale, load_sign, homogeneous);
+  const int G = grid_rows(A.n, A.pb);
+  SCDAT_PB(hipLaunchKernelGGL(k_rhs_real<PB>, dim3(G), dim3(PB), 0, s, A, f, src, scale, load_sign, homogeneous));
   hipLaunchKernelGGL(k_fin_rhs, dim3(f.blk_seg ? f.n_seg : 1), dim3(kPB), 0, s, f, G);
 }
 void launch_resid64(const EllDev& A, const FieldBufs& f, hipStream_t s) {
-  const int G = grid_rows(A.n);
-  hipLaunchKernelGGL(k_resid64, dim3(G), dim3(kPB), 0, s, A, f);
+  const int G = grid_rows(A.n, A.pb);
+  SCDAT_PB(hipLaunchKernelGGL(k_resid64<PB>, dim3(G), dim3(PB), 0, s, A, f));
   hipLaunchKernelGGL(k_fin_resid, dim3(f.blk_seg ? f.n_seg : 1), dim3(kPB), 0, s, f, G);
 }
 template <typename T>
 void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_inner_init<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
-  if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 0, 0, grid_rows(A.n));
-  hipLaunchKernelGGL(k_cg_start<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
-  if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 1, 0, grid_rows(A.n));
+  const int G = grid_rows(A.n, A.pb);
+  SCDAT_PB(hipLaunchKernelGGL((k_inner_init<T, PB>), dim3(G), dim3(PB), 0, s, A, f, ib));
+  if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 0, 0, G);
+  SCDAT_PB(hipLaunchKernelGGL((k_cg_start<T, PB>), dim3(G), dim3(PB), 0, s, A, f, ib));
+  if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 1, 0, G);
 }
-template <typename T>
-void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
-                       hipStream_t s, hipEvent_t* ev) {
-  const dim3 g(grid_rows(A.n)), b(kPB);
-  hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
-  if (A.nc) {
-    switch (A.w) {
-#define SCDAT_W(w) \
-  case w:          \
-    hipExtLaunchKernelGGL((k_cg_iter<T, w, true, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
-    break;
-      SCDAT_W(13) SCDAT_W(14) SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18)
-#undef SCDAT_W
-      default:
-        hipExtLaunchKernelGGL((k_cg_iter<T, 0, true, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
-    }
-    hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 2, it, grid_rows(A.n));
-    return;
-  }
+// One iteration at block size PB (compile-time ELL width where one exists).
+template <typename T, int PB>
+void launch_iter_pb(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it, hipStream_t s,
+                    hipEvent_t e0, hipEvent_t e1) {
+  const dim3 g(grid_rows(A.n, PB)), b(PB);
   if constexpr (sizeof(T) == 4) {
     if (A.npat > 0) switch (A.w) {
 #define SCDAT_W(w) \
   case w:          \
-    hipExtLaunchKernelGGL((k_cg_iter<T, w, false, true>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w, false, true, PB>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
     return;
       SCDAT_W(7) SCDAT_W(8) SCDAT_W(9) SCDAT_W(10) SCDAT_W(11) SCDAT_W(12) SCDAT_W(13) SCDAT_W(14)
       SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18) SCDAT_W(19) SCDAT_W(20)
@@ -784,22 +789,45 @@ void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& 
   switch (A.w) {
 #define SCDAT_W(w) \
   case w:          \
-    hipExtLaunchKernelGGL((k_cg_iter<T, w, false, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w, false, false, PB>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
     break;
     SCDAT_W(7) SCDAT_W(8) SCDAT_W(9) SCDAT_W(10) SCDAT_W(11) SCDAT_W(12) SCDAT_W(13) SCDAT_W(14)
     SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18) SCDAT_W(19) SCDAT_W(20)
 #undef SCDAT_W
     default:
-      hipExtLaunchKernelGGL((k_cg_iter<T, 0, false, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
+      hipExtLaunchKernelGGL((k_cg_iter<T, 0, false, false, PB>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
   }
 }
 template <typename T>
+void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
+                       hipStream_t s, hipEvent_t* ev) {
+  hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
+  if (A.nc) {  // two-level: 512-row blocks (the aggregates)
+    const dim3 g(grid_rows(A.n, kPcgBlock)), b(kPcgBlock);
+    switch (A.w) {
+#define SCDAT_W(w) \
+  case w:          \
+    hipExtLaunchKernelGGL((k_cg_iter<T, w, true, false, kPcgBlock>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    break;
+      SCDAT_W(13) SCDAT_W(14) SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18)
+#undef SCDAT_W
+      default:
+        hipExtLaunchKernelGGL((k_cg_iter<T, 0, true, false, kPcgBlock>), g, b, 0, s, e0, e1, 0, A, f, ib, it);
+    }
+    hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 2, it,
+                       grid_rows(A.n, kPcgBlock));
+    return;
+  }
+  if (A.pb == 256) launch_iter_pb<T, 256>(A, f, ib, it, s, e0, e1);
+  else launch_iter_pb<T, 512>(A, f, ib, it, s, e0, e1);
+}
+template <typename T>
 void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s) {
-  hipLaunchKernelGGL(k_apply<T>, dim3(grid_rows(A.n)), dim3(kPB), 0, s, A, f, ib);
+  SCDAT_PB(hipLaunchKernelGGL((k_apply<T, PB>), dim3(grid_rows(A.n, A.pb)), dim3(PB), 0, s, A, f, ib));
   const int ns = f.blk_seg ? f.n_seg : 1;
   hipLaunchKernelGGL(k_count_outer, dim3((ns + kPB - 1) / kPB), dim3(kPB), 0, s, f, ns);
 }
-size_t partial_doubles(int64_t n) { return (size_t)S_COUNT * (size_t)grid_rows(n) + 16; }
+size_t partial_doubles(int64_t n) { return (size_t)S_COUNT * (size_t)grid_rows(n, 256) + 16; }  // smallest block
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s) {
   hipLaunchKernelGGL(k_float, dim3(f.blk_seg ? f.n_seg : 1), dim3(kBlock), 0, s, f, fa);
 }

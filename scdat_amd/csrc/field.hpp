@@ -68,6 +68,10 @@ struct EllDev {
   const int32_t* poff;     // npat x w column offsets (pattern-major)
   const float* pval32;     // npat x w values of A D^-1
   const float* pdinv32;    // npat values of D^-1
+  // Threads (= rows) of a PCG row block: kPcgBlock, or 256 for a single
+  // scenario under about 131k rows with the Jacobi preconditioner (half the
+  // rows per block keeps the grid wide enough to fill the GPU).
+  int pb;
 };
 
 struct FieldBufs {
