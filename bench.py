@@ -476,7 +476,7 @@ def main():
                       "algorithmic_bytes_per_launch": bytes_k[k], "total_ms": ms * launches[k]}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
     names = {"push": "push_kernel<float,CH>",
-             "pcg_iter": "k_cg_iter<float,W,false,%s>" % ("true" if npat else "false")}
+             "pcg_iter": "k_cg_iter<float,W,false,%s,%d>" % ("true" if npat else "false", sv["pcg_block"])}
     roof = dict(kernels[dominant])
     roof["kernel"] = names[dominant]
     st = g["stats"]
@@ -508,7 +508,7 @@ def main():
                "whole Poisson stage time (FP64 residuals, launches and host polls included)"},
         "stage_ms_total": g["stages"],
         "launch_shape": {"push_particles_per_thread": g["chunks"], "chosen_by": "scdat_autotune"},
-        "solver": {"rows": sv["rows"], "ell_width": width, "row_patterns": npat,
+        "solver": {"rows": sv["rows"], "ell_width": width, "row_patterns": npat, "pcg_block": sv["pcg_block"],
                    "preconditioner": "jacobi" if sv["preconditioner"] == S.PRECOND_JACOBI else "two-level"},
         "floating_potential_v": st["floating_potential"],
         "pcg_initial_rel_residual": st["pcg_initial_rel_residual"],

@@ -567,6 +567,8 @@ typedef struct {
   int32_t preconditioner;   /* in use for the next solve                       */
   int32_t coarse_unknowns;  /* two-level: aggregates (row blocks); 0          */
   int32_t row_patterns;     /* FP32 inner operator as row patterns (0: ELL)   */
+  int32_t pcg_block;        /* rows (= threads) of a PCG row block            */
+  int32_t reserved;
 } scdat_solver_info;
 int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
 

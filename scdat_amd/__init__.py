@@ -109,7 +109,8 @@ ORDER_MORTON, ORDER_GIVEN = 0, 1
 
 class SolverInfo(C.Structure):
     _fields_ = [("rows", C.c_int64), ("nonzeros", C.c_int64), ("ell_width", C.c_int32),
-                ("preconditioner", C.c_int32), ("coarse_unknowns", C.c_int32), ("row_patterns", C.c_int32)]
+                ("preconditioner", C.c_int32), ("coarse_unknowns", C.c_int32), ("row_patterns", C.c_int32),
+                ("pcg_block", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

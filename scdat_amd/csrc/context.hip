@@ -3560,6 +3560,7 @@ int scdat_get_solver_info(scdat_ctx* c, scdat_solver_info* out) {
   out->nonzeros = nnz;
   out->coarse_unknowns = A.nc;
   out->row_patterns = A.npat;
+  out->pcg_block = A.nc > 0 ? kPcgBlock : A.pb;
   out->preconditioner = A.nc > 0 ? SCDAT_PRECOND_TWO_LEVEL : SCDAT_PRECOND_JACOBI;
   return SCDAT_OK;
 }
