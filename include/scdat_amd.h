@@ -171,7 +171,12 @@ int scdat_initialize(scdat_ctx* ctx);
 
 /* Replaces PICCycle::executeTimeStep (PICCycle.cpp:311-416), n times:
  * push + gather + walk + deposit with the previous E, boundary events,
- * Poisson (with floating conductor), nodal E.                               */
+ * Poisson (with floating conductor), nodal E.  A solve that misses the
+ * tolerance keeps the previous potential and field for that step
+ * (PICCycle::solvePoissonEquation, PICCycle.cpp:731-767) and the remaining
+ * steps still run; the call then returns SCDAT_ERR_SOLVER, naming the count
+ * and the first such step (scdat_stats.pcg_failed_solves counts them all).
+ * scdat_initialize and scdat_solve follow the same policy.                  */
 int scdat_step(scdat_ctx* ctx, int32_t n);
 
 /* Individual stages, for callers that drive the cycle themselves and tests.
@@ -191,6 +196,12 @@ int scdat_sort(scdat_ctx* ctx);
  * (PICCycle.cpp:674-688 takes a DIRICHLET node's value from potential_).     */
 /* Replaces PICCycle::setTimeStep (PICCycle.h:86-92): ignored unless dt > 0. */
 int scdat_set_time_step(scdat_ctx* ctx, double dt);
+
+/* Poisson stopping controls between steps: relative tolerance and inner
+ * iteration cap per solve (scdat_config.pcg_tol / pcg_max_iter); the
+ * tolerance and iteration arguments of LinearSystem::conjugateGradient
+ * (Tools/Solver/src/LinearSystem.cpp:258-327).  max_iter <= 0 keeps the cap. */
+int scdat_set_pcg_controls(scdat_ctx* ctx, double tol, int32_t max_iter);
 
 int scdat_set_efield(scdat_ctx* ctx, const double* E);
 int scdat_set_potential(scdat_ctx* ctx, const double* phi);
@@ -226,6 +237,9 @@ typedef struct {
                                   scdat_push (alive at the start of a push)  */
   double pcg_initial_rel_residual; /* last solve, warm-start ||r0|| / ||b||  */
   int64_t pcg_iterations_total;    /* inner PCG iterations of all solves     */
+  int64_t pcg_failed_solves;       /* solves that missed the tolerance; each kept
+                                      the previous potential and field
+                                      (PICCycle.cpp:731-767)                 */
 } scdat_stats;
 
 /* Replaces PICCycle::getStatistics (PICCycle.h:135-149). */

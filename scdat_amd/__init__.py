@@ -11,8 +11,11 @@ The class :class:`PICCycle` mirrors the reference orchestrator
 ``getChargeDensity`` / ``getPotential`` / ``getElectricField`` /
 ``getStatistics`` / ``getSurfaceCurrentLedger``.
 """
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -40,6 +43,12 @@ class ScdatError(RuntimeError):
         self.code = code
 
 
+class SolverFailure(ScdatError):
+    """A Poisson solve missed its tolerance (SCDAT_ERR_SOLVER).  The call
+    still completed: the failed steps kept the previous potential and field,
+    as PICCycle::solvePoissonEquation does (PICCycle.cpp:731-767)."""
+
+
 class BoxSpec(C.Structure):
     _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("n", C.c_int32 * 3),
                 ("conductor", C.c_int32), ("c_lo", C.c_double * 3), ("c_hi", C.c_double * 3)]
@@ -62,7 +71,8 @@ class Stats(C.Structure):
                 ("pcg_iterations", C.c_int32), ("pcg_outer", C.c_int32),
                 ("pcg_rel_residual", C.c_double), ("pcg_converged", C.c_int32),
                 ("lost_particles", C.c_int64), ("particle_pushes", C.c_int64),
-                ("pcg_initial_rel_residual", C.c_double), ("pcg_iterations_total", C.c_int64)]
+                ("pcg_initial_rel_residual", C.c_double), ("pcg_iterations_total", C.c_int64),
+                ("pcg_failed_solves", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -226,6 +236,7 @@ SIGNATURES = {
     "scdat_compute_efield": (C.c_int, [_P]),
     "scdat_sort": (C.c_int, [_P]),
     "scdat_set_time_step": (C.c_int, [_P, C.c_double]),
+    "scdat_set_pcg_controls": (C.c_int, [_P, C.c_double, C.c_int32]),
     "scdat_set_efield": (C.c_int, [_P, _D]),
     "scdat_set_potential": (C.c_int, [_P, _D]),
     "scdat_set_rho": (C.c_int, [_P, _D]),
@@ -309,6 +320,29 @@ def mcc_process(type, species, **kw):
     return p
 
 _lib = None
+
+# Live handles, released at interpreter exit in creation-reverse order while
+# the HIP runtime is still up: Python's atexit hooks run before the C
+# runtime's exit handlers (the HIP runtime's and a profiler's among them),
+# whereas a finalizer left to interpreter shutdown may run after them.
+_live = weakref.WeakValueDictionary()
+_live_seq = [0]
+
+
+def _track(obj):
+    _live_seq[0] += 1
+    _live[_live_seq[0]] = obj
+
+
+@atexit.register
+def _close_all():
+    for k in sorted(list(_live.keys()), reverse=True):
+        o = _live.get(k)
+        if o is not None:
+            try:
+                o.close()
+            except Exception:
+                pass
 
 
 def lib():
@@ -403,6 +437,7 @@ class Context:
         self.n_nodes = 0
         self.n_tets = 0
         self.species = []
+        _track(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -410,6 +445,10 @@ class Context:
             self.h = None
 
     def __del__(self):
+        # never during interpreter shutdown: the atexit hook has closed every
+        # live context by then, and the runtime may already be gone
+        if sys is None or sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -424,7 +463,8 @@ class Context:
     def _chk(self, rc):
         if rc != OK:
             msg = lib().scdat_last_error(self.h)
-            raise ScdatError(rc, msg.decode() if msg else "")
+            cls = SolverFailure if rc == ERR_SOLVER else ScdatError
+            raise cls(rc, msg.decode() if msg else "")
         return rc
 
     # -- setup --------------------------------------------------------------
@@ -714,6 +754,10 @@ class Context:
     def set_time_step(self, dt):
         self._chk(lib().scdat_set_time_step(self.h, dt))
 
+    def set_pcg_controls(self, tol, max_iter=0):
+        """Relative tolerance and inner-iteration cap of later solves."""
+        self._chk(lib().scdat_set_pcg_controls(self.h, float(tol), int(max_iter)))
+
     def stats(self):
         st = Stats()
         self._chk(lib().scdat_get_stats(self.h, C.byref(st)))
@@ -767,6 +811,7 @@ class Circuit:
         self.h = h
         self.n_nodes = 0
         self.n_branches = 0
+        _track(self)
 
     def close(self):
         if self.h:
@@ -774,6 +819,8 @@ class Circuit:
             self.h = None
 
     def __del__(self):
+        if sys is None or sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

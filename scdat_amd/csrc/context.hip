@@ -452,6 +452,40 @@ struct Engine {
   virtual ~Engine() = default;
 };
 
+// Host-side figures of the last solve (statistics and the next solve's batch
+// sizes), saved and restored around runs that must leave the context as it
+// was (the charging forward and adjoint).
+struct SolverFigures {
+  int last_iters = 0, last_outer = 0, last_conv = 0;
+  int inner_hist[16] = {};
+  int64_t iters_total = 0, failed_solves = 0;
+  double last_rel = 0.0, last_init_rel = 0.0;
+  template <typename C>
+  static SolverFigures of(const C* c) {
+    SolverFigures f;
+    f.last_iters = c->last_iters;
+    f.last_outer = c->last_outer;
+    f.last_conv = c->last_conv;
+    for (int k = 0; k < 16; ++k) f.inner_hist[k] = c->inner_hist[k];
+    f.iters_total = c->iters_total;
+    f.failed_solves = c->failed_solves;
+    f.last_rel = c->last_rel;
+    f.last_init_rel = c->last_init_rel;
+    return f;
+  }
+  template <typename C>
+  void restore(C* c) const {
+    c->last_iters = last_iters;
+    c->last_outer = last_outer;
+    c->last_conv = last_conv;
+    for (int k = 0; k < 16; ++k) c->inner_hist[k] = inner_hist[k];
+    c->iters_total = iters_total;
+    c->failed_solves = failed_solves;
+    c->last_rel = last_rel;
+    c->last_init_rel = last_init_rel;
+  }
+};
+
 // Particle state of one step of the charging adjoint plus that step's nodal E.
 struct AdjStep {
   DevBuf v[7];  // x, y, z, vx, vy, vz, cell
@@ -549,9 +583,15 @@ struct scdat_ctx {
   DevBuf d_adj_sp, d_adj_rho, d_adj_part, d_adj_xb, d_adj_vb, d_adj_phibar, d_adj_y, d_adj_rhobar, d_adj_Ebar[2],
       d_adj_dotp, d_snap_E, d_snap_x64, d_snap_phi64, d_snap_phiT, d_snap_sc;
   int64_t adj_np = 0;
+  SolverFigures snap_solver;
   int n_seg() const { return seg_node0.empty() ? 1 : (int)seg_node0.size() - 1; }
   bool segmented() const { return !seg_node0.empty(); }
   int last_iters = 0, last_outer = 0, last_conv = 0;
+  // Poisson solves that missed the tolerance (the previous potential was
+  // kept): over the context's life, and within the current scdat_step call
+  int64_t failed_solves = 0;
+  int batch_failed = 0;
+  int64_t first_failed_step = -1;
   int inner_hist[16] = {};  // inner iterations per outer pass of the last solve
   int64_t iters_total = 0;  // inner iterations of all solves
   double last_rel = 0.0, last_init_rel = 0.0;
@@ -1138,6 +1178,17 @@ void do_solve(scdat_ctx* c, int rho_idx) {
   c->have_xprev = true;
   const bool fp64 = c->is_fp64();
   pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, c->inner_hist);
+  if (!c->last_conv) {
+    // PICCycle::solvePoissonEquation keeps the previous potential when every
+    // solver fails (PICCycle.cpp:731-767) and updateElectricField then
+    // rebuilds the same E from it: the failed segments take back the
+    // previous solution x_n (the extrapolation saved it), k_float keeps their
+    // floating potential, so the field kernel reproduces the previous phi
+    // and E bit for bit.
+    launch_keep_failed(f, c->d_xprev.as<double>(), A.n, c->stream);
+    c->failed_solves++;
+    if (c->batch_failed++ == 0) c->first_failed_step = c->steps;
+  }
   if (c->h.has_cond && c->n_patch == 0) {
     if (!c->segmented())
       HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1859,6 +1910,7 @@ void do_step(scdat_ctx* c, int n) {
   std::vector<StageEv> sev;
   c->push_ms_sum = 0;
   c->push_launches = 0;
+  c->batch_failed = 0;
   for (int k = 0; k < n; ++k) {
     StageEv se;
     se.a = evn++;
@@ -1890,11 +1942,6 @@ void do_step(scdat_ctx* c, int n) {
     se.e = evn++;
     HIPCHK(hipEventRecord(take_event(c, se.e), c->stream));
     sev.push_back(se);
-    if (!c->last_conv) {
-      // PICCycle keeps the previous potential when the solver fails
-      // (PICCycle.cpp:731-767); here the field is still updated from the
-      // best iterate and the failure is reported in the statistics.
-    }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int k = 0; k < 5; ++k) c->stage_ms[k] = 0;
@@ -2203,6 +2250,7 @@ ChargingRun adj_begin(scdat_ctx* c, const scdat_charging_params* p, int steps_ke
   HIPCHK(hipMemcpyAsync(c->d_snap_phiT.p, c->d_phiT.p, c->d_phiT.bytes, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(c->d_snap_sc.p, c->d_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
+  c->snap_solver = SolverFigures::of(c);
   return run;
 }
 
@@ -2214,6 +2262,7 @@ void adj_end(scdat_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->d_phiT.p, c->d_snap_phiT.p, c->d_phiT.bytes, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(c->d_sc.p, c->d_snap_sc.p, sizeof(PcgScalars), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
+  c->snap_solver.restore(c);
 }
 
 template <typename T>
@@ -2741,6 +2790,14 @@ int scdat_set_time_step(scdat_ctx* c, double dt) {
   return SCDAT_OK;
 }
 
+int scdat_set_pcg_controls(scdat_ctx* c, double tol, int32_t max_iter) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!(tol > 0.0) || !std::isfinite(tol)) return fail(c, SCDAT_ERR_ARG, "set_pcg_controls: tolerance must be > 0");
+  c->cfg.pcg_tol = tol;
+  if (max_iter > 0) c->cfg.pcg_max_iter = max_iter;
+  return SCDAT_OK;
+}
+
 int scdat_set_conductor_charge(scdat_ctx* c, double q) {
   if (!c) return SCDAT_ERR_ARG;
   c->q_init = q;
@@ -2750,10 +2807,16 @@ int scdat_set_conductor_charge(scdat_ctx* c, double q) {
 int scdat_initialize(scdat_ctx* c) {
   if (!c) return SCDAT_ERR_ARG;
   if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "initialize: no mesh");
-  return guarded(c, [&] {
+  c->batch_failed = 0;
+  const int rc = guarded(c, [&] {
     if (c->is_fp64()) do_initialize<double>(c);
     else do_initialize<float>(c);
   });
+  if (rc == SCDAT_OK && c->batch_failed > 0) {
+    c->err = "initial Poisson solve did not reach the tolerance; the initial potential was kept";
+    return SCDAT_ERR_SOLVER;
+  }
+  return rc;
 }
 
 int scdat_step(scdat_ctx* c, int32_t n) {
@@ -2764,8 +2827,11 @@ int scdat_step(scdat_ctx* c, int32_t n) {
     if (c->is_fp64()) do_step<double>(c, n);
     else do_step<float>(c, n);
   });
-  if (rc == SCDAT_OK && !c->last_conv) {
-    c->err = "Poisson PCG did not reach the tolerance";
+  if (rc == SCDAT_OK && c->batch_failed > 0) {
+    c->err = std::to_string(c->batch_failed) + " of " + std::to_string(n) +
+             " Poisson solves did not reach the tolerance (first in step " +
+             std::to_string(c->first_failed_step + 1) +
+             "); those steps kept the previous potential and field, and every step ran";
     return SCDAT_ERR_SOLVER;
   }
   return rc;
@@ -2802,6 +2868,7 @@ int scdat_deposit(scdat_ctx* c) {
 int scdat_solve(scdat_ctx* c) {
   if (!c) return SCDAT_ERR_ARG;
   if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "solve: no mesh");
+  c->batch_failed = 0;
   const int rc = guarded(c, [&] {
     if (c->h.has_cond && !(c->ccap > 0)) {
       if (c->is_fp64()) solve_phi1<double>(c);
@@ -2816,8 +2883,8 @@ int scdat_solve(scdat_ctx* c) {
     }
     HIPCHK(hipStreamSynchronize(c->stream));
   });
-  if (rc == SCDAT_OK && !c->last_conv) {
-    c->err = "Poisson PCG did not reach the tolerance";
+  if (rc == SCDAT_OK && c->batch_failed > 0) {
+    c->err = "Poisson PCG did not reach the tolerance; the previous potential and field were kept";
     return SCDAT_ERR_SOLVER;
   }
   return rc;
@@ -3072,6 +3139,7 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
     st->pcg_converged = c->last_conv;
     st->pcg_initial_rel_residual = c->last_init_rel;
     st->pcg_iterations_total = c->iters_total;
+    st->pcg_failed_solves = c->failed_solves;
   });
 }
 

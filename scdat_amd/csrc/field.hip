@@ -659,6 +659,9 @@ __global__ void __launch_bounds__(kBlock) k_float(FieldBufs f, FloatArgs fa) {
         if (fa.emit_count) q -= (double)fa.emit_count[lo + g * kMaxSpecies + sp] * qw;
       }
     }
+    // a failed solve keeps the previous floating potential (its phi_0 is
+    // restored from the previous solution, PICCycle.cpp:731-767)
+    if (sc->failed) return;
     const double cap = fa.seg_ccap ? fa.seg_ccap[sg] : sc->ccap;
     sc->ccap = cap;
     sc->qcond = q;
@@ -840,6 +843,16 @@ __global__ void __launch_bounds__(kBlock) k_extrapolate(FieldBufs f, double* pre
 }
 void launch_extrapolate(const FieldBufs& f, double* prev, int64_t n, int first, hipStream_t s) {
   hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(n)), dim3(kBlock), 0, s, f, prev, n, first);
+}
+// Rows of segments whose solve failed take back the previous solution, so
+// that phi and E rebuilt from x64 equal the previous step's bit for bit.
+__global__ void __launch_bounds__(kBlock) k_keep_failed(FieldBufs f, const double* prev, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (f.sc[f.blk_seg ? f.blk_seg[i / kPcgBlock] : 0].failed) f.x64[i] = prev[i];
+}
+void launch_keep_failed(const FieldBufs& f, const double* prev, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_keep_failed, dim3(grid_for(n)), dim3(kBlock), 0, s, f, prev, n);
 }
 void launch_seg_summary(const FieldBufs& f, hipStream_t s) {
   if (!f.blk_seg) return;

@@ -158,6 +158,10 @@ void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s);
 // Initial guess of a time step's solve from the last two solutions: on free
 // rows x = 2 x_n - x_{n-1} (first = 1: x stays x_n); prev takes x_n.
 void launch_extrapolate(const FieldBufs& f, double* prev, int64_t n, int first, hipStream_t s);
+// After a solve: rows of every segment whose PCG failed get x64 = prev (the
+// previous step's solution), the reference's keep-the-previous-potential
+// policy (PICCycle.cpp:731-767).
+void launch_keep_failed(const FieldBufs& f, const double* prev, int64_t n, hipStream_t s);
 // Summary of the per-segment scalars into sc[n_seg] (no-op for one segment).
 void launch_seg_summary(const FieldBufs& f, hipStream_t s);
 template <typename T>

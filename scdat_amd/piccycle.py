@@ -9,7 +9,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import (BC_ABSORB, BC_CONDUCTOR, BC_PERIODIC, BC_REFLECT, Context, ELEMENTARY_CHARGE,
-               ELECTRON_MASS, PROTON_MASS, VACUUM_PERMITTIVITY, default_config)
+               ELECTRON_MASS, PROTON_MASS, SolverFailure, VACUUM_PERMITTIVITY, default_config)
 
 
 class BoundaryFace:
@@ -92,23 +92,33 @@ class PICCycle:
 
     # --- cycle (PICCycle.h:71-73) ---------------------------------------------
     def initialize(self):
-        self._ctx.initialize()
+        try:
+            self._ctx.initialize()
+        except SolverFailure:
+            self._initialized = True
+            return False
         self._initialized = True
         return True
 
     def executeTimeStep(self):
-        self._ctx.step(1)
-        return True
+        """One step.  The reference returns !checkConvergence(), which is
+        always true (PICCycle.cpp:410-416,968-972); here a step whose Poisson
+        solve missed the tolerance returns False.  That step still ran and
+        kept the previous potential and field (PICCycle.cpp:731-767);
+        getStatistics()["pcg_failed_solves"] counts such steps."""
+        return self.executeSteps(1)
 
     def executeSteps(self, n):
-        self._ctx.step(n)
+        """n steps; False when any of their solves missed the tolerance."""
+        try:
+            self._ctx.step(n)
+        except SolverFailure:
+            return False
         return True
 
     def executeSimulation(self, total_time):
         n = int(round(total_time / self.params.time_step))
-        if n > 0:
-            self._ctx.step(n)
-        return True
+        return self.executeSteps(n) if n > 0 else True
 
     # --- field access (PICCycle.h:73-84,90-117) ------------------------------
     def getChargeDensity(self):
