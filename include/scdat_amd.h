@@ -337,7 +337,16 @@ int scdat_set_surface_emission(scdat_ctx* ctx, int32_t group, const scdat_surfac
 /* Cumulative event counts (replaces MonteCarloCollisionHandler::getStatistics,
  * CollisionAlgorithm.h:176-226, and the CollisionStepResult counters).
  * overflow: products dropped at a full species; impacts_dropped: impacts
- * beyond the per-step impact buffer (no emission for them).                */
+ * beyond the per-step impact buffer (no emission for them).
+ * Collisions use the null-collision method (Vahedi & Surendra 1995): a
+ * constant majorant nu_max >= sum_p n_p sigma_p(E)|v| over the energies up
+ * to the cap (a speed twice the largest of the previous push) gives every
+ * particle one candidate draw with probability 1 - exp(-nu_max dt) inside
+ * the push, and only candidates evaluate cross sections, accepted with
+ * probability (1 - exp(-nu dt)) / (1 - exp(-nu_max dt)); particles faster
+ * than the cap are always candidates, accepted with 1 - exp(-nu dt).
+ * mcc_candidates: candidates tested; mcc_majorant_exceeded: candidates whose
+ * exact probability exceeded the majorant's (0 when the majorant holds).   */
 typedef struct {
   int64_t collisions[SCDAT_MAX_MCC_PROCESSES];
   int64_t created;
@@ -346,6 +355,8 @@ typedef struct {
   int64_t photo_emitted;
   int64_t overflow;
   int64_t impacts_dropped;
+  int64_t mcc_candidates;
+  int64_t mcc_majorant_exceeded;
 } scdat_event_counts;
 int scdat_get_event_counts(scdat_ctx* ctx, scdat_event_counts* out);
 

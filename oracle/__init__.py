@@ -392,10 +392,11 @@ class Oracle:
         lib().orc_post_push(self.h)
 
     def mcc_counts(self):
-        out = np.zeros(20, np.int64)
+        out = np.zeros(22, np.int64)
         lib().orc_mcc_counts(self.h, _p(out, C.c_int64))
         return dict(collisions=out[:16].copy(), created=int(out[16]), removed=int(out[17]),
-                    see_emitted=int(out[18]), photo_emitted=int(out[19]))
+                    see_emitted=int(out[18]), photo_emitted=int(out[19]), candidates=int(out[20]),
+                    majorant_exceeded=int(out[21]))
 
     def get_ids(self, s):
         n = lib().orc_get_particles(self.h, s, None, None, None)

@@ -231,6 +231,13 @@ struct Sim {
   uint32_t key[2] = {0, 0};
   int64_t mcc_count[16] = {};
   int64_t mcc_created = 0, mcc_removed = 0, see_emitted = 0, photo_emitted = 0;
+  // null-collision state: largest |v|^2 of the last push per species (from
+  // the particles themselves after a load), the majorant of the coming step
+  std::vector<double> v2max;
+  bool v2_have = false;
+  std::vector<std::array<double, 3>> mcc_ctl;  // P_null, v^2 cap, nu_max
+  bool ctl_ready = false;
+  int64_t mcc_candidates = 0, mcc_major = 0;
   std::array<SurfEmit, kMaxGroups> emit{};
   std::vector<SeeEvent> events;
 };
@@ -843,6 +850,67 @@ void deposit_point(Sim& s, int cell, const double* x, double qw) {
   for (int k = 0; k < 4; ++k) s.rho[s.tet[4 * (size_t)cell + k]] += qw * lam[k];
 }
 
+// Null-collision majorant (Vahedi & Surendra, Comput. Phys. Commun. 87
+// (1995) 179) of species si's processes: nu_max >= sum_p n_p sigma_p(E)
+// sqrt(2E/m) for E up to the energy of the cap v^2 = 4 v2max, sampled at
+// 1024 log-spaced energies over [1e-8, 1] x cap, just above each threshold,
+// and bounded below the lowest sample by sum n sigma_max v (sigma <= sigma_max
+// in every form); times 1.05.  Returns {P_null, v^2 cap, nu_max}.
+constexpr int kMajorantPoints = 1024;
+double nu_total(const Sim& s, const std::vector<int>& pr, double E, double m) {
+  double t = 0.0;
+  for (int j : pr) t += s.mcc[j].n * sigma_of(s.mcc[j], E);
+  return t * std::sqrt(2.0 * E * kE / m);
+}
+std::array<double, 3> majorant(const Sim& s, const std::vector<int>& pr, double m, double v2max) {
+  const double v2cap = 4.0 * v2max;
+  const double ecap = 0.5 * m * v2cap / kE;
+  double f = 0.0;
+  if (ecap > 0.0) {
+    const double elo = 1e-8 * ecap, la = std::log(elo), lb = std::log(ecap);
+    for (int k = 0; k < kMajorantPoints; ++k)
+      f = std::max(f, nu_total(s, pr, std::exp(la + (lb - la) * (double)k / (double)(kMajorantPoints - 1)), m));
+    for (int j : pr) {
+      const double thr = s.mcc[j].thr;
+      if (thr > elo && thr < ecap) f = std::max(f, nu_total(s, pr, thr * (1.0 + 1e-9), m));
+    }
+    double b = 0.0;
+    for (int j : pr) b += s.mcc[j].n * s.mcc[j].smax;
+    f = std::max(f, b * std::sqrt(2.0 * elo * kE / m));
+  }
+  const double numax = 1.05 * f;
+  return {-std::expm1(-numax * s.dt), v2cap, numax};
+}
+std::vector<int> procs_of(const Sim& s, size_t si) {
+  std::vector<int> pr;
+  for (size_t k = 0; k < s.mcc.size(); ++k)
+    if (s.mcc[k].species == (int)si) pr.push_back((int)k);
+  return pr;
+}
+// largest |v|^2 of the alive particles of each species
+void measure_v2max(Sim& s) {
+  s.v2max.assign(s.sp.size(), 0.0);
+  for (size_t si = 0; si < s.sp.size(); ++si) {
+    const Species& P = s.sp[si];
+    for (size_t i = 0; i < P.cell.size(); ++i) {
+      if (P.cell[i] < 0) continue;
+      const double* v = &P.v[3 * i];
+      s.v2max[si] = std::max(s.v2max[si], v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+  }
+  s.v2_have = true;
+}
+// The coming step's majorants (before its push).
+void mcc_prepare(Sim& s) {
+  if (!s.v2_have || s.v2max.size() != s.sp.size()) measure_v2max(s);
+  s.mcc_ctl.assign(s.sp.size(), {0.0, 0.0, 0.0});
+  for (size_t si = 0; si < s.sp.size(); ++si) {
+    const std::vector<int> pr = procs_of(s, si);
+    if (!pr.empty()) s.mcc_ctl[si] = majorant(s, pr, s.sp[si].m, s.v2max[si]);
+  }
+  s.ctl_ready = true;
+}
+
 // One background-collision step over the particles present before it
 // (replaces MonteCarloCollisionHandler::processCollisions,
 // CollisionAlgorithm.cpp:500-538, inside executeBackgroundMccStep,
@@ -855,8 +923,18 @@ void deposit_point(Sim& s, int cell, const double* x, double qw) {
 // (elastic, Coulomb), energy loss (inelastic, excitation above threshold),
 // ionisation with a uniform split of E - I and an ion at rest, removal
 // (charge exchange turns the ion neutral, recombination the electron).
+// Candidates by the null-collision method: a particle is a candidate when
+// its first uniform is below P_null of the step's majorant, or when it is
+// faster than the cap; a candidate collides when u2 P_c < 1 - exp(-nu dt)
+// (P_c = P_null, or 1 above the cap), so the collision probability stays
+// exactly 1 - exp(-nu dt), and R = (u2 P_c / p) nu picks the process.
 void mcc_step(Sim& s) {
   if (s.mcc.empty()) return;
+  if (!s.ctl_ready) {  // collisions without a push (checkers): majorant of the present particles
+    measure_v2max(s);
+    mcc_prepare(s);
+  }
+  s.ctl_ready = false;
   const uint32_t step = (uint32_t)s.steps;
   std::vector<size_t> n0(s.sp.size());
   for (size_t k = 0; k < s.sp.size(); ++k) n0[k] = s.sp[k].cell.size();
@@ -866,11 +944,18 @@ void mcc_step(Sim& s) {
       if (s.mcc[k].species == (int)si) pr.push_back((int)k);
     if (pr.empty()) continue;
     const double m = s.sp[si].m;
+    const std::array<double, 3> ctl = s.mcc_ctl[si];
     for (size_t i = 0; i < n0[si]; ++i) {
       Species& P = s.sp[si];
       if (P.cell[i] < 0) continue;
       double v[3] = {P.v[3 * i], P.v[3 * i + 1], P.v[3 * i + 2]};
       const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+      const uint32_t id = (uint32_t)P.id[i];
+      uint32_t a[4];
+      draw(s, id, step, (uint32_t)si, kTagMccA, a);
+      const bool forced = v2 > ctl[1];
+      if (!forced && !(u01(a[0]) < ctl[0])) continue;
+      s.mcc_candidates++;
       const double speed = std::sqrt(v2);
       const double E = 0.5 * m * v2 / kE;
       double nu[16], tot = 0.0;
@@ -879,11 +964,12 @@ void mcc_step(Sim& s) {
         tot += nu[j];
       }
       if (!(tot > 0.0)) continue;
-      const uint32_t id = (uint32_t)P.id[i];
-      uint32_t a[4];
-      draw(s, id, step, (uint32_t)si, kTagMccA, a);
-      if (!(u01(a[0]) < -std::expm1(-tot * s.dt))) continue;
-      const double R = u01(a[1]) * tot;
+      const double pc = forced ? 1.0 : ctl[0];
+      const double p = -std::expm1(-tot * s.dt);
+      if (p > pc) s.mcc_major++;
+      const double u = u01(a[1]) * pc;
+      if (!(u < p)) continue;
+      const double R = (u / p) * tot;
       int j = -1;
       double cum = 0.0;
       for (size_t q = 0; q < pr.size(); ++q) {
@@ -1245,6 +1331,7 @@ void orc_set_particles(void* h, int si, int64_t n, const double* x, const double
   p.cell.assign(cell, cell + n);
   p.id.resize(n);
   for (int64_t i = 0; i < n; ++i) p.id[i] = (int32_t)i;
+  s->v2_have = false;
 }
 
 int64_t orc_get_particles(void* h, int si, double* x, double* v, int32_t* cell) {
@@ -1328,7 +1415,9 @@ int orc_step(void* h, int nsteps) {
   auto* s = static_cast<Sim*>(h);
   int it = 0;
   for (int k = 0; k < nsteps; ++k) {
+    if (!s->mcc.empty()) mcc_prepare(*s);
     orc_push(h);
+    if (!s->mcc.empty()) measure_v2max(*s);  // the next step's majorant
     post_push(*s);
     it = solve_poisson(*s).iters;
     compute_efield(*s);
@@ -1477,6 +1566,7 @@ void orc_mcc_configure(void* h, int n, const double* prm) {
   auto* s = static_cast<Sim*>(h);
   s->mcc.clear();
   for (int k = 0; k < n && k < 16; ++k) s->mcc.push_back(unpack_proc(prm + 16 * k));
+  s->v2_have = false;
 }
 // 8 doubles: see_model, delta_max, energy_max, see_temperature, photo_j,
 // photo_temperature, photo_reference_v, electron_species.
@@ -1493,6 +1583,7 @@ void orc_mcc_counts(void* h, int64_t* out) {
   auto* s = static_cast<Sim*>(h);
   for (int k = 0; k < 16; ++k) out[k] = s->mcc_count[k];
   out[16] = s->mcc_created; out[17] = s->mcc_removed; out[18] = s->see_emitted; out[19] = s->photo_emitted;
+  out[20] = s->mcc_candidates; out[21] = s->mcc_major;
 }
 void orc_get_ids(void* h, int si, int32_t* ids) {
   auto* s = static_cast<Sim*>(h);

@@ -192,7 +192,8 @@ def charging_params(**kw):
 class EventCounts(C.Structure):
     _fields_ = [("collisions", C.c_int64 * MAX_MCC_PROCESSES), ("created", C.c_int64), ("removed", C.c_int64),
                 ("see_emitted", C.c_int64), ("photo_emitted", C.c_int64), ("overflow", C.c_int64),
-                ("impacts_dropped", C.c_int64)]
+                ("impacts_dropped", C.c_int64), ("mcc_candidates", C.c_int64),
+                ("mcc_majorant_exceeded", C.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "collisions"}

@@ -78,6 +78,14 @@ def case_inputs(sc):
     return dict(mesh=m, e=(xe, ve, ce), i=(xi, vi, ci), w=w, dt=dt)
 
 
+def all_case_inputs(scenarios, workers=16):
+    """case_inputs of every scenario, built on a thread pool (the loader and
+    the mesher run in the library, outside the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+        return list(ex.map(case_inputs, scenarios))
+
+
 def _emission(ctx, scenario, sc):
     for g in CONDUCTOR_GROUPS:
         if scenario is None:

@@ -446,6 +446,7 @@ struct SpeciesState {
   int64_t stat_blocks = 0;  // partials written by the last push
   double v2_mean = 0.0;     // mean |v|^2 of the uploaded particles (sort cadence)
   int sort_every = 0;       // steps between sorts of this species (0: cfg.sort_interval)
+  DevBuf mcc_cand;          // collision candidates the push flags (capacity entries)
 };
 
 struct Engine {
@@ -540,6 +541,11 @@ struct scdat_ctx {
   uint32_t key[2] = {0x5CDA7u, 0x2026u};
   std::array<scdat_surface_emission, kMaxGroups> emit{};
   DevBuf d_counts, d_append, d_emit_count, d_events, d_n_events, d_xyz64, d_tet4;
+  // null-collision state per species: majorant controls, max |v|^2 of the
+  // last push, candidate count; recomputed from the particles after a load
+  DevBuf d_mcc_ctl, d_mcc_v2max, d_mcc_ncand;
+  bool mcc_v2_stale = true;
+  bool push_mcc = false;  // the next push flags collision candidates
   DevBuf d_face_tet, d_face_k, d_face_cum;
   std::array<int64_t, kMaxGroups + 1> face_off{};
   std::array<double, kMaxGroups> face_area{};
@@ -1254,6 +1260,76 @@ void do_field(scdat_ctx* c, unsigned long long* rho_clear) {
                        c->d_E.as<Vec4<T>>(), rho_clear, c->stream);
 }
 
+bool species_has_mcc(const scdat_ctx* c, int si) {
+  for (const auto& q : c->mcc)
+    if (q.species == si) return true;
+  return false;
+}
+
+// Device form of the processes whose incident species is si; returns their count.
+int pack_procs(const scdat_ctx* c, int si, MccProcDev* out) {
+  int np = 0;
+  for (size_t k = 0; k < c->mcc.size(); ++k) {
+    const scdat_mcc_process& q = c->mcc[k];
+    if (q.species != si) continue;
+    MccProcDev& d = out[np++];
+    d.type = q.type;
+    d.e_species = q.product_electron_species;
+    d.i_species = q.product_ion_species;
+    d.slot = (int)k;
+    d.n = q.target_density;
+    d.thr = q.threshold_ev;
+    d.smax = q.sigma_max_m2;
+    d.loss = q.energy_loss_ev;
+    d.eref = q.reference_energy_ev;
+    d.pidx = q.power_index;
+    d.alpha = q.alpha;
+    d.beta = q.beta;
+    d.clog = q.coulomb_log;
+  }
+  return np;
+}
+
+// Before a step's push: the majorant of every colliding species from the
+// largest |v|^2 of the previous push (from the particles themselves after a
+// load), which also clears the push's accumulators.
+template <typename T>
+void mcc_prepare(scdat_ctx* c) {
+  hipStream_t s = c->stream;
+  const int ns = (int)c->sp.size();
+  if (c->d_mcc_ctl.bytes < 8 * (size_t)kMccCtl * kMaxSpecies) {
+    c->d_mcc_ctl.alloc(8 * (size_t)kMccCtl * kMaxSpecies);
+    c->d_mcc_v2max.alloc(8 * (size_t)kMaxSpecies);
+    c->d_mcc_ncand.alloc(4 * (size_t)kMaxSpecies);
+    c->mcc_v2_stale = true;
+  }
+  unsigned long long* v2 = c->d_mcc_v2max.as<unsigned long long>();
+  if (c->mcc_v2_stale) {
+    launch_fill_u64(v2, kMaxSpecies, s);
+    for (int si = 0; si < ns; ++si) {
+      SpeciesState& S = c->sp[si];
+      if (!species_has_mcc(c, si) || S.n == 0) continue;
+      launch_v2max<T>(S.buf[S.cur][3].as<T>(), S.buf[S.cur][4].as<T>(), S.buf[S.cur][5].as<T>(),
+                      S.buf[S.cur][6].as<int32_t>(), S.n, v2 + si, s);
+    }
+    c->mcc_v2_stale = false;
+  }
+  for (int si = 0; si < ns; ++si) {
+    if (!species_has_mcc(c, si)) continue;
+    SpeciesState& S = c->sp[si];
+    if (S.mcc_cand.bytes < 4 * (size_t)S.cap) S.mcc_cand.alloc(4 * (size_t)S.cap);
+    MccMajorantArgs m{};
+    m.np = pack_procs(c, si, m.proc);
+    m.mass = S.m;
+    m.dt = c->cfg.dt;
+    m.v2max = v2 + si;
+    m.n_cand = c->d_mcc_ncand.as<unsigned int>() + si;
+    m.ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si;
+    launch_mcc_majorant(m, s);
+  }
+  HIPCHK(hipGetLastError());
+}
+
 template <typename T>
 PushArgs<T> push_args(scdat_ctx* c, int si) {
   SpeciesState& S = c->sp[si];
@@ -1320,6 +1396,15 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
     a.seg_wscale = c->d_seg_wscale.as<double>();
   }
   a.chunks = c->push_chunks;
+  if (c->push_mcc && species_has_mcc(c, si)) {
+    a.mcc_ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si;
+    a.mcc_cand = S.mcc_cand.as<uint32_t>();
+    a.mcc_ncand = c->d_mcc_ncand.as<unsigned int>() + si;
+    a.mcc_v2max = c->d_mcc_v2max.as<unsigned long long>() + si;
+    a.mcc_step = (uint32_t)c->steps;
+    a.mcc_key0 = c->key[0];
+    a.mcc_key1 = c->key[1];
+  }
   return a;
 }
 
@@ -1614,6 +1699,7 @@ void do_initialize(scdat_ctx* c) {
   do_field<T>(c, nullptr);
   HIPCHK(hipStreamSynchronize(c->stream));
   set_sort_cadence(c);
+  c->mcc_v2_stale = true;
   c->initialized = true;
 }
 
@@ -1781,27 +1867,12 @@ void post_push(scdat_ctx* c) {
   const int ns = (int)c->sp.size();
   for (int si = 0; si < ns; ++si) {
     MccArgs<T> a{};
-    int np = 0;
-    for (size_t k = 0; k < c->mcc.size(); ++k) {
-      const scdat_mcc_process& q = c->mcc[k];
-      if (q.species != si) continue;
-      MccProcDev& d = a.proc[np++];
-      d.type = q.type;
-      d.e_species = q.product_electron_species;
-      d.i_species = q.product_ion_species;
-      d.slot = (int)k;
-      d.n = q.target_density;
-      d.thr = q.threshold_ev;
-      d.smax = q.sigma_max_m2;
-      d.loss = q.energy_loss_ev;
-      d.eref = q.reference_energy_ev;
-      d.pidx = q.power_index;
-      d.alpha = q.alpha;
-      d.beta = q.beta;
-      d.clog = q.coulomb_log;
-    }
+    const int np = pack_procs(c, si, a.proc);
     SpeciesState& S = c->sp[si];
     if (np == 0 || S.n == 0) continue;
+    a.cand = S.mcc_cand.as<uint32_t>();
+    a.n_cand = c->d_mcc_ncand.as<unsigned int>() + si;
+    a.ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si;
     a.x = S.buf[S.cur][0].as<T>();
     a.y = S.buf[S.cur][1].as<T>();
     a.z = S.buf[S.cur][2].as<T>();
@@ -1917,7 +1988,12 @@ void do_step(scdat_ctx* c, int n) {
     HIPCHK(hipEventRecord(take_event(c, se.a), c->stream));
     const bool post = has_post_push(c);
     if (c->event_cap > 0) launch_fill_u64(c->d_n_events.as<unsigned long long>(), 1, c->stream);
+    if (!c->mcc.empty()) {
+      mcc_prepare<T>(c);
+      c->push_mcc = true;
+    }
     do_push<T>(c, &pev, &evn);
+    c->push_mcc = false;
     if (post) post_push<T>(c);
     snapshot_ledger(c);
     se.b = evn++;
@@ -2749,6 +2825,7 @@ int scdat_set_particles(scdat_ctx* c, int32_t species, int64_t n, const double* 
   if (!c) return SCDAT_ERR_ARG;
   if (species < 0 || species >= (int)c->sp.size()) return fail(c, SCDAT_ERR_ARG, "set_particles: bad species");
   if (n < 0 || (n > 0 && (!x || !v))) return fail(c, SCDAT_ERR_ARG, "set_particles: bad arrays");
+  c->mcc_v2_stale = true;
   return guarded(c, [&] {
     if (c->is_fp64()) set_particles_t<double>(c, species, n, x, v, cell);
     else set_particles_t<float>(c, species, n, x, v, cell);
@@ -3223,6 +3300,7 @@ int scdat_mcc_configure(scdat_ctx* c, const scdat_mcc_process* procs, int32_t n)
       return fail(c, SCDAT_ERR_ARG, "mcc_configure: ionisation needs product species");
   }
   c->mcc.assign(procs, procs + n);
+  c->mcc_v2_stale = true;
   return SCDAT_OK;
 }
 
@@ -3274,6 +3352,8 @@ int scdat_get_event_counts(scdat_ctx* c, scdat_event_counts* out) {
     out->photo_emitted = (int64_t)v[C_PHOTO];
     out->overflow = (int64_t)v[C_OVERFLOW];
     out->impacts_dropped = c->events_dropped;
+    out->mcc_candidates = (int64_t)v[C_CAND];
+    out->mcc_majorant_exceeded = (int64_t)v[C_MAJOR];
   });
 }
 

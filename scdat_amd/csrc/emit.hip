@@ -6,8 +6,11 @@
 // executeBackgroundMccStep (CollisionPicAdapter.cpp:338-434).  Per particle:
 // nu_p = n_p sigma_p(E) |v| with the cross sections of CollisonTypes.cpp:56-176,
 // one collision with probability 1 - exp(-sum nu dt) and the process chosen
-// with probability nu_p / sum nu (null-collision thinning with the particle's
-// own total frequency as majorant).  Outcomes follow CollisionAlgorithm.cpp:
+// with probability nu_p / sum nu.  The null-collision method selects the
+// candidates: the push tests every particle against P_null = 1 -
+// exp(-nu_max dt) of a constant majorant nu_max (k_mcc_majorant) with one
+// Philox draw and no cross section, and only candidates reach k_mcc, which
+// accepts with probability (1 - exp(-nu dt)) / P_null.  Outcomes follow CollisionAlgorithm.cpp:
 // 189-460: isotropic scatter (elastic, Coulomb), energy loss (inelastic,
 // excitation above threshold), ionisation (uniform split of E - I, secondary
 // electron and ion at rest at the parent's position), removal (charge
@@ -19,6 +22,8 @@
 // step, species), so results do not depend on thread order.  New particles
 // are appended with one atomic per particle and deposited into the step's
 // charge at once (fixed point, so the order of deposits does not matter).
+#include <algorithm>
+
 #include "emit.hpp"
 #include "rng.hpp"
 
@@ -110,12 +115,75 @@ __device__ bool append(const SpeciesOut<T>& o, const T x[3], const double v[3], 
   return true;
 }
 
+// Total collision frequency sum_p n_p sigma_p(E) sqrt(2E/m) of a species.
+__device__ double nu_total(const MccProcDev* proc, int np, double E, double mass) {
+  double t = 0.0;
+  for (int j = 0; j < np; ++j) t += proc[j].n * sigma_of(proc[j], E);
+  return t * sqrt(2.0 * E * kE / mass);
+}
+
+// One block per species: nu_max over log-spaced energies up to the cap (and
+// just above every threshold, where a cross section may switch on), plus
+// the bound sum n sigma_max v over the energies below the lowest sample;
+// times kMccMajorantMargin.  Then P_null, and the push's accumulators reset.
+__global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
+  __shared__ double lds[kBlock / 64];
+  const double v2max = __longlong_as_double((long long)*a.v2max);
+  const double v2cap = kMccCapFactor * v2max;
+  const double ecap = 0.5 * a.mass * v2cap / kE;
+  double f = 0.0;
+  if (ecap > 0.0) {
+    const double elo = 1e-8 * ecap, la = log(elo), lb = log(ecap);
+    for (int k = threadIdx.x; k < kMccMajorantPoints; k += kBlock) {
+      const double E = exp(la + (lb - la) * (double)k / (double)(kMccMajorantPoints - 1));
+      f = fmax(f, nu_total(a.proc, a.np, E, a.mass));
+    }
+    if (threadIdx.x < a.np) {
+      const double thr = a.proc[threadIdx.x].thr;
+      if (thr > elo && thr < ecap) f = fmax(f, nu_total(a.proc, a.np, thr * (1.0 + 1e-9), a.mass));
+    }
+    if (threadIdx.x == 0) {
+      double b = 0.0;
+      for (int j = 0; j < a.np; ++j) b += a.proc[j].n * a.proc[j].smax;
+      f = fmax(f, b * sqrt(2.0 * elo * kE / a.mass));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) f = fmax(f, __shfl_xor(f, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = f;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) f = fmax(f, lds[w]);
+    f = fmax(f, lds[0]);
+    const double numax = kMccMajorantMargin * f;
+    a.ctl[0] = -expm1(-numax * a.dt);
+    a.ctl[1] = v2cap;
+    a.ctl[2] = numax;
+    *a.v2max = 0ull;
+    *a.n_cand = 0u;
+  }
+}
+
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_mcc(MccArgs<T> a) {
+__global__ void __launch_bounds__(kBlock) k_v2max(const T* vx, const T* vy, const T* vz, const int32_t* cell,
+                                                  int64_t n, unsigned long long* out) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
+  double v2 = 0.0;
+  if (i < n && cell[i] >= 0) v2 = (double)vx[i] * vx[i] + (double)vy[i] * vy[i] + (double)vz[i] * vz[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v2 = fmax(v2, __shfl_xor(v2, off, 64));
+  if ((threadIdx.x & 63) == 0 && v2 > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(v2));
+}
+
+// One candidate: the exact collision test given the pre-selection, process
+// choice, outcome.
+template <typename T>
+__device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
+  const bool forced = (entry & kMccForced) != 0u;
+  const int64_t i = (int64_t)(entry & ~kMccForced);
   const int cell = a.cell[i];
   if (cell < 0) return;
+  atomicAdd(&a.counts[C_CAND], 1ull);
   double v[3] = {(double)a.vx[i], (double)a.vy[i], (double)a.vz[i]};
   const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   const double speed = sqrt(v2);
@@ -130,10 +198,15 @@ __global__ void __launch_bounds__(kBlock) k_mcc(MccArgs<T> a) {
   if (!(tot > 0.0)) return;
   const uint32_t id = (uint32_t)a.id[i];
   const U4 r = philox4x32(id, a.step, (uint32_t)a.species, kTagMccA, a.key0, a.key1);
-  if (!(u01(r.x) < -expm1(-tot * a.dt))) return;
+  // accept with probability pex / P_c: overall probability pex = 1 - exp(-nu dt)
+  const double pc = forced ? 1.0 : a.ctl[0];
+  const double pex = -expm1(-tot * a.dt);
+  if (pex > pc) atomicAdd(&a.counts[C_MAJOR], 1ull);
+  const double u = u01(r.y) * pc;
+  if (!(u < pex)) return;
   // the first process whose cumulative frequency exceeds R (zero frequencies
-  // are never chosen), the checker's rule
-  const double R = u01(r.y) * tot;
+  // are never chosen); u / pex is uniform on (0, 1) given the collision
+  const double R = (u / pex) * tot;
   int j = -1;
   double cum = 0.0;
   bool stop = false;
@@ -195,6 +268,14 @@ __global__ void __launch_bounds__(kBlock) k_mcc(MccArgs<T> a) {
       atomicAdd(&a.counts[C_REMOVED], 1ull);
       break;
   }
+}
+
+// Candidates of one species (grid-stride over the count the push left).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mcc(MccArgs<T> a) {
+  const unsigned int nc = *a.n_cand;
+  for (unsigned int q = blockIdx.x * kBlock + threadIdx.x; q < nc; q += gridDim.x * kBlock)
+    mcc_candidate(a, a.cand[q]);
 }
 
 // ---- surface emission ------------------------------------------------------
@@ -371,7 +452,10 @@ inline uint32_t blocks_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBl
 template <typename T>
 void launch_mcc(const MccArgs<T>& a, hipStream_t s) {
   if (a.n <= 0 || a.np <= 0) return;
-  hipLaunchKernelGGL(k_mcc<T>, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+  // candidates are few (P_null n plus the particles above the cap): a fixed
+  // grid that loops over however many the push found
+  const int blocks = (int)std::min<int64_t>(256, std::max<int64_t>(1, blocks_for(a.n)));
+  hipLaunchKernelGGL(k_mcc<T>, dim3(blocks), dim3(kBlock), 0, s, a);
 }
 template <typename T>
 void launch_see(const EmitArgs<T>& a, int64_t max_events, hipStream_t s) {
@@ -395,7 +479,19 @@ void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, n);
 }
 
+void launch_mcc_majorant(const MccMajorantArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mcc_majorant, dim3(1), dim3(kBlock), 0, s, a);
+}
+template <typename T>
+void launch_v2max(const T* vx, const T* vy, const T* vz, const int32_t* cell, int64_t n, unsigned long long* v2max,
+                  hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_v2max<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, vx, vy, vz, cell, n, v2max);
+}
+
 #define SCDAT_EMIT_INST(T)                                                        \
+  template void launch_v2max<T>(const T*, const T*, const T*, const int32_t*, int64_t, unsigned long long*, \
+                                hipStream_t);                                     \
   template void launch_inject<T>(const SpeciesOut<T>&, int64_t, const double*, const double*, const int32_t*, \
                                  const int32_t*, const TetMap<T>*, unsigned long long*, unsigned long long*, \
                                  hipStream_t);                                    \

@@ -11,7 +11,28 @@ namespace scdat {
 constexpr int kMaxProc = 16;
 constexpr int kMaxEmit = 8;  // secondaries per impact
 // counter slots: [0, kMaxProc) collisions per process, then:
-enum CountSlot { C_CREATED = kMaxProc, C_REMOVED, C_SEE, C_PHOTO, C_OVERFLOW, C_EVENT_DROP, C_COUNT };
+enum CountSlot {
+  C_CREATED = kMaxProc, C_REMOVED, C_SEE, C_PHOTO, C_OVERFLOW, C_EVENT_DROP,
+  C_CAND,   // collision candidates tested (null-collision pre-selection)
+  C_MAJOR,  // candidates whose exact probability exceeded the majorant's
+  C_COUNT
+};
+
+// Null-collision majorant of one species (Vahedi & Surendra, Comput. Phys.
+// Commun. 87 (1995) 179): nu_max bounds sum_p n_p sigma_p(E) sqrt(2E/m) for
+// E up to the cap energy of a speed twice the largest speed of the previous
+// push (v^2 cap = kMccCapFactor x max v^2).  A particle is a candidate with
+// probability P_null = 1 - exp(-nu_max dt), or always when it is faster than
+// the cap; a candidate collides with probability (1 - exp(-nu dt)) / P_c
+// (P_c = P_null, or 1 for a forced candidate), so every particle collides
+// with probability exactly 1 - exp(-nu dt) while only candidates evaluate
+// cross sections.
+constexpr int kMccMajorantPoints = 1024;   // log-spaced energies over [1e-8, 1] x cap
+constexpr double kMccMajorantMargin = 1.05;
+constexpr double kMccCapFactor = 4.0;
+// ctl[species]: P_null, v^2 cap, nu_max, unused
+constexpr int kMccCtl = 4;
+constexpr uint32_t kMccForced = 0x80000000u;  // candidate entry flag: faster than the cap
 
 enum MccType { MCC_ELASTIC = 0, MCC_INELASTIC, MCC_IONIZATION, MCC_EXCITATION, MCC_CHARGE_EXCHANGE,
                MCC_RECOMBINATION, MCC_COULOMB };
@@ -45,6 +66,11 @@ struct MccArgs {
   int32_t* cell;
   const int32_t* id;
   int64_t n;
+  // candidates flagged by the push: slot | kMccForced, their count, and the
+  // species' majorant controls
+  const uint32_t* cand;
+  const unsigned int* n_cand;
+  const double* ctl;
   int species, np;
   double mass, dt;
   uint32_t step, key0, key1;
@@ -95,6 +121,21 @@ struct EmitArgs {
   uint32_t electron_mask;  // species whose impacts are secondary-emission events
 };
 
+// Majorant of one species from the largest |v|^2 of the previous push; also
+// clears that maximum and the candidate count for the coming push.
+struct MccMajorantArgs {
+  MccProcDev proc[kMaxProc];
+  int np;
+  double mass, dt;
+  unsigned long long* v2max;  // FP64 bits of max |v|^2 (non-negative: ordered as integers)
+  unsigned int* n_cand;
+  double* ctl;                // kMccCtl doubles
+};
+void launch_mcc_majorant(const MccMajorantArgs& a, hipStream_t s);
+// max |v|^2 over the alive particles of one species into *v2max (atomic max)
+template <typename T>
+void launch_v2max(const T* vx, const T* vy, const T* vz, const int32_t* cell, int64_t n, unsigned long long* v2max,
+                  hipStream_t s);
 template <typename T>
 void launch_mcc(const MccArgs<T>& a, hipStream_t s);
 template <typename T>

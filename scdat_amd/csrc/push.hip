@@ -15,6 +15,7 @@
 // result is independent of launch order and bit-exact.
 #include "device_common.hpp"
 #include "push.hpp"
+#include "rng.hpp"
 #include "walk.hpp"
 
 namespace scdat {
@@ -116,12 +117,14 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
     led_e[threadIdx.x] = 0.0;
   }
   __syncthreads();
-  double ke = 0.0;
+  double ke = 0.0, v2max = 0.0;
   int alive = 0, started = 0;
 #pragma unroll 1
   for (int ch = 0; ch < CH; ++ch) {
     const int64_t i = ((int64_t)lb * CH + ch) * kBlock + threadIdx.x;
     int key = -1;
+    bool is_cand = false;  // collision candidate: entry = slot | kMccForced when forced
+    uint32_t cand = 0;
     int32_t f[4] = {0, 0, 0, 0};
     int c = (i < a.n) ? a.cell[i] : -1;
     if (c >= 0) {
@@ -218,12 +221,44 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
         a.y[i] = x[1];
         a.z[i] = x[2];
         a.cell[i] = c;
+        if (a.mcc_ctl) {
+          // null-collision pre-selection on the post-push velocity
+          const double v2 = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+          v2max = fmax(v2max, v2);
+          if (v2 > a.mcc_ctl[1]) {
+            is_cand = true;
+            cand = (uint32_t)i | kMccForced;
+          } else {
+            const U4 r = philox4x32((uint32_t)a.id[i], a.mcc_step, (uint32_t)a.species, kTagMccA, a.mcc_key0,
+                                    a.mcc_key1);
+            is_cand = u01(r.x) < a.mcc_ctl[0];
+            cand = (uint32_t)i;
+          }
+        }
       }
       a.vx[i] = v[0];
       a.vy[i] = v[1];
       a.vz[i] = v[2];
     }
+    if (a.mcc_ctl) {
+      // wave-aggregated append of this chunk's candidates
+      const uint64_t m = __ballot(is_cand);
+      if (m) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(a.mcc_ncand, (unsigned int)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (is_cand) a.mcc_cand[base + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = cand;
+      }
+    }
     tile_runs(key, f, a.cs, a.tets, a.rho, tile);
+  }
+  if (a.mcc_ctl) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v2max = fmax(v2max, __shfl_xor(v2max, off, 64));
+    if ((threadIdx.x & 63) == 0 && v2max > 0.0)
+      atomicMax(a.mcc_v2max, (unsigned long long)__double_as_longlong(v2max));
   }
   __syncthreads();
   tile_flush(tile, a.rho);

@@ -59,6 +59,15 @@ struct PushArgs {
   const T* seg_dt;
   const double* seg_wscale;
   int chunks;  // particles per thread of the push kernel: 1, 2, 4 or 8
+  // Background collisions (null: none this push).  Every particle alive
+  // after the push draws its null-collision candidate test, Philox keyed by
+  // (id, step, species); candidates (and particles faster than the cap) are
+  // appended to mcc_cand, and the largest |v|^2 goes to mcc_v2max.
+  const double* mcc_ctl;  // P_null, v^2 cap (emit.hpp)
+  uint32_t* mcc_cand;
+  unsigned int* mcc_ncand;
+  unsigned long long* mcc_v2max;
+  uint32_t mcc_step, mcc_key0, mcc_key1;
 };
 
 // blocks of one push launch over n particle slots with `chunks` particles

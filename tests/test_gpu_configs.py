@@ -1,0 +1,246 @@
+"""The benchmark configurations at their own sizes, on the device.
+
+config[0]: 16^3 hexes (24,576 tets before the conductor cut), 2 x 10^5
+  particles, FP64, 100 collisionless steps, Jacobi-PCG to 1e-10, against the
+  FP64 oracle on identical seeded inputs (the loop of
+  Main/main_PICcore.cpp:487-515 around PICCycle::executeTimeStep,
+  Tools/PICcore/src/PICCycle.cpp:311-416).
+config[1]: 64^3 hexes, 2^24 particles: the mixed path (FP32 push, gather,
+  deposit and inner PCG, FP64 refinement to 1e-8) against the FP64 device
+  path over the configuration's 500 steps, and against the FP64 oracle over
+  the first steps.
+config[3]: the 256-case batched sweep.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import scdat_amd as S
+import scdat_amd.batch as B
+from helpers import QE, make_ctx, make_oracle, plasma_setup, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SEED0 = 20261203  # config[0] loading seed (Philox key)
+SEED1 = 20261204  # config[1]
+
+
+def _pos_err(c, o, n):
+    """Largest position difference (m) over particles alive on both sides,
+    and the particles alive on exactly one side, both species."""
+    worst, odd = 0.0, 0
+    for s in range(2):
+        xo, _, co = o.get_particles(s)
+        X, _, Cc = c.get_particles_by_id(s, n)
+        both = (co >= 0) & (Cc >= 0)
+        odd += int(((co >= 0) != (Cc >= 0)).sum())
+        if both.any():
+            worst = max(worst, float(np.max(np.abs(X[both] - xo[both]))))
+    return worst, odd
+
+
+def test_config0_fp64_100_steps_matches_oracle():
+    st = plasma_setup(n=16, npart=100_000, seed=SEED0)
+    n = len(st["e"][0])
+    o = make_oracle(st, tol=1e-10)
+    c = make_ctx(st, precision=1, tol=1e-10, sort_interval=20)
+    c.initialize()
+    o.deposit()
+    o.solve()
+    o.efield()
+    curve = []
+    for k in range(1, 101):
+        o.step(1)
+        c.step(1)
+        _, phi_o, E_o = o.fields()
+        _, phi_c, E_c = c.fields()
+        rec = dict(step=k, phi=rel_err(phi_c, phi_o), E=rel_err(E_c, E_o),
+                   vf_dev=c.stats()["floating_potential"], vf_orc=o.floating_potential())
+        if k % 10 == 0:
+            rec["pos"], rec["odd"] = _pos_err(c, o, n)
+        curve.append(rec)
+    for r in curve:
+        if r["step"] % 10 == 0 or r["step"] <= 3:
+            print(r)
+    led_o, led_c = o.ledger(12), c.ledger(12)
+    n_abs_o = sum(int(led_o[g, 4] + led_o[g, 5]) for g in range(12))
+    n_abs_c = sum(led_c[g]["absorbed_electron_particles"] + led_c[g]["absorbed_ion_particles"] for g in range(12))
+    print("absorbed: oracle", n_abs_o, "device", n_abs_c)
+    # north_star bounds: potential 1e-6 relative (FP64), trajectories 1e-5
+    # (relative to the 1 m domain) over the 100 steps
+    assert max(r["phi"] for r in curve) < 1e-6
+    assert max(r["E"] for r in curve) < 1e-6
+    assert max(r["pos"] for r in curve if "pos" in r) < 1e-5
+    assert all(r["odd"] == 0 for r in curve if "odd" in r)
+    assert n_abs_o == n_abs_c
+    vf = curve[-1]
+    assert abs(vf["vf_dev"] - vf["vf_orc"]) <= 1e-6 * abs(vf["vf_orc"])
+    c.close()
+    o.close()
+
+
+@pytest.fixture(scope="module")
+def case64():
+    return plasma_setup(n=64, npart=1 << 23, seed=SEED1)
+
+
+def _run_box(st, precision, steps, tol, every=10):
+    """Floating potential and particles absorbed on the conductor (groups
+    6..11, both species) every `every` steps."""
+    c = make_ctx(st, precision=precision, tol=tol, sort_interval=20)
+    c.initialize()
+    vf, nab = [], []
+    for _ in range(steps // every):
+        c.step(every)
+        vf.append(c.stats()["floating_potential"])
+        led = c.ledger(12)
+        nab.append(sum(led[g]["absorbed_electron_particles"] + led[g]["absorbed_ion_particles"] for g in range(6, 12)))
+    s = c.stats()
+    c.close()
+    return np.array(vf), np.array(nab, float), s
+
+
+def _rounded(st):
+    """The same plasma with positions and velocities rounded to FP32 (the
+    state the mixed path starts from), kept in FP64."""
+    r = dict(st)
+    for sp in ("e", "i"):
+        x, v, cell = st[sp]
+        r[sp] = (x.astype(np.float32).astype(np.float64), v.astype(np.float32).astype(np.float64), cell)
+    return r
+
+
+def test_config1_mixed_tracks_fp64_over_500_steps(case64):
+    """Floating potential of the mixed path against the FP64 device path over
+    the configuration's 500 steps (checked every 10).
+
+    A 1 eV plasma at this size is chaotic: two FP64 runs whose initial
+    states differ only by FP32 rounding (about 6e-8 relative) separate by
+    1e-9 V at step 100, 1e-7 V at step 200 and 1e-5 V at step 250, then stay
+    at the level of the shot noise of the charge the conductor collects.
+    The mixed run, whose every step rounds to FP32, separates from FP64
+    earlier.  So: (1) within 1e-4 relative through step 150 (about 4.8
+    plasma periods); (2) over all 500 steps, within 1e-4 relative plus three
+    standard deviations of the shot noise of the conductor's absorbed
+    macro-particle count, sqrt(N) q w / C, the spread of two independent
+    realizations of the same charging history (sqrt(2) times the shot noise
+    of one run); (3) the FP64-from-rounded-
+    state run does separate by orders of magnitude (the dynamics, not the
+    arithmetic, set the late difference)."""
+    vf64, nab, s64 = _run_box(case64, 1, 500, 1e-10)
+    vfr, _, sr = _run_box(_rounded(case64), 1, 500, 1e-10)
+    vf32, _, s32 = _run_box(case64, 0, 500, 1e-8)
+    steps = 10 * np.arange(1, len(vf64) + 1)
+    dev = np.abs(vf32 - vf64)
+    intrinsic = np.abs(vfr - vf64)
+    cap = abs(s64["conductor_charge"] / vf64[-1]) if vf64[-1] else 1.0
+    per_particle = QE * case64["w"] / cap
+    shot = np.sqrt(nab) * per_particle
+    for k in range(len(steps)):
+        print("step %3d  V_f64 %.9f  mixed-64 %.3e (rel %.2e)  rounded64-64 %.3e  shot %.3e"
+              % (steps[k], vf64[k], dev[k], dev[k] / abs(vf64[k]), intrinsic[k], shot[k]))
+    print("one macro-particle moves V_f by %.3e V" % per_particle)
+    assert s64["pcg_failed_solves"] == 0 and s32["pcg_failed_solves"] == 0 and sr["pcg_failed_solves"] == 0
+    early = steps <= 150
+    assert np.all(dev[early] <= 1e-4 * np.abs(vf64[early]))
+    assert np.all(dev <= 1e-4 * np.abs(vf64) + 3.0 * np.sqrt(2.0) * shot)
+    assert intrinsic[steps == 100][0] < 1e-7 * abs(vf64[steps == 100][0])
+    assert intrinsic[-10:].max() > 1e3 * intrinsic[steps == 100][0]
+
+
+def test_config1_mixed_field_vs_fp64_oracle(case64):
+    """The mixed path against the FP64 oracle (OpenMP, all host cores) over
+    the first 3 steps at full size: potential within 1e-4 relative."""
+    st = case64
+    o = make_oracle(st, tol=1e-10)
+    c = make_ctx(st, precision=0, tol=1e-8, sort_interval=20)
+    c.initialize()
+    o.deposit()
+    o.solve()
+    o.efield()
+    errs = []
+    for k in range(3):
+        o.step(1)
+        c.step(1)
+        _, phi_o, E_o = o.fields()
+        _, phi_c, E_c = c.fields()
+        errs.append((rel_err(phi_c, phi_o), rel_err(E_c, E_o),
+                     abs(c.stats()["floating_potential"] / o.floating_potential() - 1.0)))
+    print("mixed vs FP64 oracle (phi, E, V_f):", errs)
+    assert all(e[0] < 1e-4 and e[2] < 1e-4 for e in errs)
+    assert all(e[1] < 1e-3 for e in errs)
+    c.close()
+    o.close()
+
+
+SWEEP_SEED = 20261206  # config[3] sweep seed
+
+
+def test_config3_sweep_256_cases_in_one_context():
+    """The 256-case batched sweep (edge 8..32 hexes, 2e4..2e6 particles per
+    species, Te 1 eV..20 keV, n 1e6..1e11 m^-3, Sternglass delta_max 1..3,
+    FP32 with FP64 refinement to 1e-8) advanced together in one context.
+
+    Every case: the solve converged; its integer charge equals the exact
+    fixed-point charge of its alive particles (2^24 c_s per particle); no
+    case gains particles beyond its ledger (initial - absorbed + secondary
+    electrons), and over all cases the shortfall is exactly the particles
+    that left through the walls, which carry no boundary condition and so
+    no ledger entry (PICCycle.cpp:1254-1257).  A seeded subset of cases
+    equals the same case run alone (batched settings) bit for bit."""
+    sweep = B.sweep(256, SWEEP_SEED)
+    inputs = B.all_case_inputs(sweep)
+    steps = 6
+    bc, u = B.batch_context(sweep, inputs, precision=0, tol=1e-8, sort_interval=3)
+    bc.step(steps)
+    rho_fx, _ = bc.rho_fixed()
+    cs = [bc.species_charge_fixed(s) for s in (0, 1)]
+    parts = []
+    for s in (0, 1):
+        parts.append(B.split_particles(u, *bc.get_particles(s)))
+    n_emit = 0
+    short = [0, 0]
+    for k, sc in enumerate(sweep):
+        ss = bc.scenario_stats(k)
+        assert ss["pcg_converged"] == 1, (k, ss)
+        assert ss["completed_steps"] == steps
+        n0, n1 = int(u["node_start"][k]), int(u["node_start"][k + 1])
+        alive = [len(parts[s][k][0]) for s in (0, 1)]
+        q = int(rho_fx[n0:n1].sum())
+        assert q == (1 << 24) * (cs[0] * alive[0] + cs[1] * alive[1]), k
+        led = bc.scenario_ledger(k)
+        ab_e = sum(led[g]["absorbed_electron_particles"] for g in range(12))
+        ab_i = sum(led[g]["absorbed_ion_particles"] for g in range(12))
+        em = sum(led[g]["emitted_electron_particles"] for g in range(12))
+        n_emit += em
+        d = (sc["n_particles"] - ab_e + em - alive[0], sc["n_particles"] - ab_i - alive[1])
+        assert d[0] >= 0 and d[1] >= 0, (k, d)
+        short[0] += d[0]
+        short[1] += d[1]
+        assert np.isfinite(ss["floating_potential"])
+    assert n_emit > 0
+    assert bc.stats()["lost_particles"] == 0
+    assert short == [bc.escaped(0), bc.escaped(1)], (short, bc.escaped(0), bc.escaped(1))
+    ev = bc.event_counts()
+    assert ev["overflow"] == 0 and ev["impacts_dropped"] == 0
+    # seeded subset against stand-alone runs
+    key = [SWEEP_SEED & 0xFFFFFFFF, 7]
+    pick = sorted({int(S.philox4x32([j, 0, 0, 0x53554253], key)[0] % 256) for j in range(4)})
+    pick.append(max(range(256), key=lambda k: sweep[k]["n_particles"] * sweep[k]["edge"]))
+    _, phi_b, E_b = bc.fields()
+    for k in pick:
+        c = B.standalone_context(sweep[k], inputs[k], precision=0, tol=1e-8, sort_interval=3)
+        c.step(steps)
+        n0 = int(u["node_start"][k])
+        n1 = n0 + len(inputs[k]["mesh"]["xyz"])
+        _, phi, E = c.fields()
+        np.testing.assert_array_equal(phi, phi_b[n0:n1])
+        np.testing.assert_array_equal(E, E_b[n0:n1])
+        assert c.stats()["floating_potential"] == bc.scenario_stats(k)["floating_potential"]
+        lb, l1 = bc.scenario_ledger(k), c.ledger()
+        for g in range(12):
+            for f in ("absorbed_electron_particles", "absorbed_ion_particles", "emitted_electron_particles"):
+                assert lb[g][f] == l1[g][f], (k, g, f)
+        c.close()
+    print("checked cases", pick, "secondaries", n_emit)
+    bc.close()

@@ -113,6 +113,11 @@ def test_collisions_fp64_match_oracle_event_for_event():
     assert list(eo["collisions"][:4]) == ec["collisions"][:4]
     assert eo["collisions"][:3].sum() > 50 and eo["created"] > 10 and eo["removed"] > 10
     assert eo["created"] == ec["created"] and eo["removed"] == ec["removed"] and ec["overflow"] == 0
+    # null-collision pre-selection: the same candidates on both sides (this
+    # case is collisional, nu dt ~ 1, so most particles are candidates), and
+    # the majorant held for every one
+    assert ec["mcc_candidates"] == eo["candidates"] > eo["collisions"][:4].sum()
+    assert ec["mcc_majorant_exceeded"] == 0 and eo["majorant_exceeded"] == 0
     vth = np.sqrt(30 * QE / ME)
     # velocities carry the 1e-12 solver tolerance of three field solves
     # through the pushes (relative 1e-9); collision outcomes themselves agree
