@@ -30,6 +30,14 @@ photoemission J0, surface conductance and the six face yields, checked
 against central differences (h = 1e-4 relative) in FP64, with the FP32
 adjoint's error against the FP64 one.  value = wall time of one gradient.
 
+--workload geo runs the GEO charging case: 48^3 hexes, 2^23 particles
+(electrons and protons, Te = Ti = 10 keV, n = 1e6 m^-3), an oxygen
+background at 1e12 m^-3 (electron elastic and ionisation, proton elastic;
+null-collision MCC), Sternglass secondary emission (delta_max 2 at 400 eV)
+on the conductor and 20 uA/m^2 photoemission from its sunlit +x face, FP32
+with FP64 refinement, dt = 0.2 h / v_th,e, the configuration's 2000 steps
+timed after the warm-up (--geo-steps).  value = particle pushes / wall time.
+
 --copy-calibration MB runs only a plain device copy of MB megabytes (the
 reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
 """
@@ -50,6 +58,23 @@ TRAFFIC_FILE = os.path.join(HERE, "profiles", "pmc_traffic_64cube.json")
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def build_case(n_hex, n_part, seed):
@@ -378,6 +403,129 @@ def run_adjoint(args):
     print(json.dumps(res), flush=True)
 
 
+def _geo_oracle(st, threads):
+    """The FP64 oracle of a GEO case (CPU baseline leg only)."""
+    import oracle
+    import scdat_amd as S
+    import scdat_amd.geo as G
+    cfg = st["cfg"]
+    m = st["mesh"]
+    o = oracle.Oracle(m["xyz"], m["tets"], m["kind"], group=m["group"], threads=threads)
+    for g in G.CONDUCTOR_GROUPS:
+        o.set_group(g, S.BC_CONDUCTOR)
+    o.set_poisson(0, S.VACUUM_PERMITTIVITY, 1, 1e-8, 20000)
+    o.set_push(st["dt"], 1, 1e-3, 64, 0)
+    for q, mass in G.species():
+        o.add_species(q, mass, st["w"])
+    o.set_particles(G.ELECTRON, *st["e"])
+    o.set_particles(G.PROTON, *st["i"])
+    o.set_particles(G.OXYGEN_ION, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.int32))
+    o.mcc_configure(st["procs"])
+    o.set_seed(int(cfg["seed"]))
+    for g in G.CONDUCTOR_GROUPS:
+        o.set_emission(g, see_model=S.SEE_STERNGLASS, delta_max=cfg["delta_max"], energy_max=cfg["energy_max_ev"],
+                       see_temperature=cfg["see_temperature_ev"],
+                       photo_j=cfg["photo_current_density"] if g == G.SUNLIT_GROUP else 0.0,
+                       photo_temperature=cfg["photo_temperature_ev"], photo_reference_v=0.0,
+                       electron_species=G.ELECTRON)
+    return o
+
+
+def run_geo(args):
+    import scdat_amd as S
+    import scdat_amd.geo as G
+    name, _ = S.device_info()
+    t0 = time.time()
+    st = G.inputs()
+    m = st["mesh"]
+    n_nodes, n_tets = len(m["kind"]), len(m["tets"])
+    steps = args.geo_steps
+    log(f"geo: {st['cfg']['edge']}^3 hexes, {n_tets} tets, {n_nodes} nodes, {st['cfg']['n_particles']} particles, "
+        f"dt={st['dt']:.4e}s, inputs in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ctx = G.context(st, precision=0, tol=1e-8, sort_interval=args.sort_interval, steps=steps + args.warmup)
+    chunks = ctx.autotune()
+    log(f"geo setup {time.time() - t0:.1f}s (push tile {chunks}); warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        ctx.step(1)
+    ctx.sync()
+    st0, ev0 = ctx.stats(), ctx.event_counts()
+    ctx.set_kernel_timing(not args.no_kernel_timing, period=16)
+    t1 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    wall = time.perf_counter() - t1
+    st1, ev1 = ctx.stats(), ctx.event_counts()
+    stages = ctx.stage_times()
+    push_ms, push_n = ctx.kernel_time(S.KERNEL_PUSH)
+    pushes = st1["particle_pushes"] - st0["particle_pushes"]
+    led = ctx.ledger(12)
+    log(f"geo: {steps} steps in {wall:.3f}s, pushes {pushes}, stages {stages}, stats {st1}")
+    # push + gather + deposit bytes per launch: particle SoA read + write
+    # 2 x 28 B, the id 4 B for the collision draw, tet map 64 B per tet,
+    # nodal E 16 B + integer charge 8 B per node
+    per_launch = pushes / max(1, push_n)
+    push_bytes = 60.0 * per_launch + 64.0 * n_tets + 24.0 * n_nodes
+    gbs = push_bytes / (push_ms * 1e-3) / 1e9 if push_ms > 0 else 0.0
+    coll = [ev1["collisions"][k] - ev0["collisions"][k] for k in range(3)]
+    res = {
+        "metric": "particle-steps/s incl. Poisson, GEO charging case (MCC + SEE + photoemission) at 48^3/8M ptcl",
+        "value": pushes / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (push/SpMV) + f64 (refinement, dots, conductor, collisions, emission)",
+        "data": "synthetic Maxwellian plasma (Philox seed %d)" % st["cfg"]["seed"],
+        "config": {"workload": "GEO charging: box 1m, conductor 0.2m floating, %d^3 hex -> %d tets, %d nodes, "
+                               "%d particles (e-/H+), Te=Ti=10keV, n=1e6, O 1e12 (e elastic + ionisation, H+ "
+                               "elastic; null-collision), Sternglass SEE 2.0@400eV, photo 20uA/m^2 on +x, "
+                               "dt=%.3e s, PCG 1e-8" % (st["cfg"]["edge"], n_tets, n_nodes,
+                                                        st["cfg"]["n_particles"], st["dt"])},
+        "device": name,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": push_ms, "launches_timed": push_n,
+                     "algorithmic_bytes_per_launch": push_bytes, "kernel": "push_kernel<float,CH>"},
+        "stage_ms_total": stages,
+        "events": {"collisions_elastic_e": coll[0], "collisions_ionisation": coll[1], "collisions_elastic_h": coll[2],
+                   "mcc_candidates": ev1["mcc_candidates"] - ev0["mcc_candidates"],
+                   "mcc_majorant_exceeded": ev1["mcc_majorant_exceeded"],
+                   "see_emitted": ev1["see_emitted"] - ev0["see_emitted"],
+                   "photo_emitted": ev1["photo_emitted"] - ev0["photo_emitted"]},
+        "active_particles_end": st1["active_particles"],
+        "floating_potential_v": st1["floating_potential"],
+        "conductor_absorbed": {"electrons": sum(led[g]["absorbed_electron_particles"] for g in range(6, 12)),
+                               "ions": sum(led[g]["absorbed_ion_particles"] for g in range(6, 12))},
+        "pcg_failed_solves": st1["pcg_failed_solves"],
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        o = _geo_oracle(st, args.cpu_threads)
+        o.deposit()
+        o.solve()
+        o.efield()
+        o.step(1)
+        cw, cp = 0.0, 0
+        for _ in range(args.cpu_steps // 2):
+            alive = sum(int((o.get_particles(k)[2] >= 0).sum()) for k in range(3))
+            t1 = time.perf_counter()
+            o.step(1)
+            cw += time.perf_counter() - t1
+            cp += alive
+        import oracle
+        res["cpu_baseline"] = {"value": cp / cw, "unit": "particle-steps/s",
+                               "cores": oracle.lib().orc_num_threads(), "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "%d full steps of the same case (same inputs, after one warm step), FP64 "
+                                         "oracle with OpenMP" % (args.cpu_steps // 2)}
+        res["gpu_over_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        log(f"cpu baseline: {args.cpu_steps // 2} steps {cw:.2f}s")
+    print(json.dumps(res), flush=True)
+
+
 def copy_calibration(mb):
     import torch
     n = int(mb * 1e6) // 4
@@ -399,12 +547,15 @@ def main():
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--sort-interval", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=16)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=host_threads(),
+                    help="OpenMP threads of the CPU baseline (default: OMP_NUM_THREADS, else the CPUs this "
+                         "process may run on)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no per-launch events in the timed region (no roofline figures)")
     ap.add_argument("--copy-calibration", type=float, default=0.0)
-    ap.add_argument("--workload", choices=("box", "sweep", "adjoint", "sampler", "arc"), default="box")
+    ap.add_argument("--workload", choices=("box", "sweep", "adjoint", "sampler", "arc", "geo"), default="box")
+    ap.add_argument("--geo-steps", type=int, default=2000)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--scenarios", type=int, default=256)
     ap.add_argument("--sweep-seed", type=int, default=20261206)
@@ -428,6 +579,9 @@ def main():
         return
     if args.workload == "arc":
         run_arc(args)
+        return
+    if args.workload == "geo":
+        run_geo(args)
         return
 
     import scdat_amd as S
@@ -516,7 +670,7 @@ def main():
     if not args.no_cpu_baseline:
         c = run_cpu_baseline(case, args.cpu_steps, args.cpu_threads)
         res["cpu_baseline"] = {"value": c["pushes"] / c["wall"], "unit": "particle-steps/s",
-                               "cores": c["threads"], "kind": "port",
+                               "cores": c["threads"], "kind": "port", "cpu_model": cpu_model(),
                                "sample": "%d full steps of the same workload (same inputs, after one warm "
                                          "step), FP64 oracle with OpenMP, Jacobi-PCG to 1e-8" % args.cpu_steps}
         res["gpu_over_cpu"] = value / res["cpu_baseline"]["value"]

@@ -660,6 +660,25 @@ typedef struct {
 /* delta_max 2 at 400 eV, yields 1, 20 uA/m^2 at 2 eV on the XMax face, G 0 */
 void scdat_charging_params_default(scdat_charging_params* p);
 int scdat_charging_forward(scdat_ctx* ctx, const scdat_charging_params* p, int32_t steps, double* v_final);
+/* The same forward with its absorption events held fixed, for finite
+ * differences of the map the adjoint differentiates (each particle is
+ * absorbed at a fixed step through a fixed group; P1 gather and deposit are
+ * continuous across tet faces, so absorption is the map's only jump).
+ * SCDAT_EVENTS_RECORD runs the forward and keeps, per particle, the step and
+ * group of its absorption; SCDAT_EVENTS_REPLAY runs it with those events
+ * imposed: a particle is absorbed at its recorded step and group whatever
+ * its walk, and a particle the record kept whose walk leaves the mesh is
+ * relocated by a walk from the cell the recorded run reached to its
+ * unconstrained position (a residual overshoot beyond a boundary face is
+ * projected back onto the face's plane).  Both rules are continuous in the
+ * parameters and the identity where the events agree, so replaying at the
+ * recorded parameters reproduces the recorded run.  A record keeps 4 bytes
+ * per particle and step of device memory.                                 */
+#define SCDAT_EVENTS_FREE 0
+#define SCDAT_EVENTS_RECORD 1
+#define SCDAT_EVENTS_REPLAY 2
+int scdat_charging_forward_events(scdat_ctx* ctx, const scdat_charging_params* p, int32_t steps, int32_t events,
+                                  double* v_final);
 /* grad[SCDAT_N_GRAD]; states are kept every checkpoint_interval steps. */
 int scdat_charging_adjoint(scdat_ctx* ctx, const scdat_charging_params* p, int32_t steps, int32_t checkpoint_interval,
                            double* v_final, double* grad);

@@ -112,6 +112,7 @@ class SurfaceEmission(C.Structure):
                 ("photo_reference_v", C.c_double)]
 
 
+EVENTS_FREE, EVENTS_RECORD, EVENTS_REPLAY = 0, 1, 2
 SEGMENT_ALIGN = 512
 PRECOND_JACOBI, PRECOND_TWO_LEVEL = 0, 1
 ORDER_MORTON, ORDER_GIVEN = 0, 1
@@ -290,6 +291,7 @@ SIGNATURES = {
     "scdat_layer_profile": (C.c_int, [_P, C.c_int32, C.c_double, C.c_int32, C.c_double, _D, _D, _D, _D, _D, _D]),
     "scdat_get_solver_info": (C.c_int, [_P, C.POINTER(SolverInfo)]),
     "scdat_charging_forward": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, _D]),
+    "scdat_charging_forward_events": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, C.c_int32, _D]),
     "scdat_charging_adjoint": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, C.c_int32, _D, _D]),
     "scdat_set_scenario_emission": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(SurfaceEmission)]),
     "scdat_get_scenario_stats": (C.c_int, [_P, C.c_int32, C.POINTER(ScenarioStats)]),
@@ -708,11 +710,13 @@ class Context:
     def pcg_block(self, rows):
         self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_PCG_BLOCK, int(rows)))
 
-    def charging_forward(self, params, steps):
+    def charging_forward(self, params, steps, events=EVENTS_FREE):
         """Floating potential after `steps` steps of the differentiable charging
-        model, from the initialized state (the context is left unchanged)."""
+        model, from the initialized state (the context is left unchanged).
+        events: EVENTS_RECORD keeps each particle's absorption step and group,
+        EVENTS_REPLAY imposes the recorded ones (the event-frozen map)."""
         v = C.c_double()
-        self._chk(lib().scdat_charging_forward(self.h, C.byref(params), steps, C.byref(v)))
+        self._chk(lib().scdat_charging_forward_events(self.h, C.byref(params), steps, int(events), C.byref(v)))
         return v.value
 
     def charging_adjoint(self, params, steps, checkpoint_interval=20):

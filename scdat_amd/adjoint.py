@@ -9,7 +9,7 @@ gives the finite-difference gradient the adjoint is checked against.
 import numpy as np
 import torch
 
-from . import ChargingParams, charging_params
+from . import EVENTS_FREE, EVENTS_RECORD, EVENTS_REPLAY, ChargingParams, charging_params
 
 
 def _params(theta, base):
@@ -31,17 +31,27 @@ class FloatingPotential(torch.autograd.Function):
         return gout * g, None, None, None, None
 
 
-def central_difference(ctx, theta, steps, rel=1e-4, base=None, indices=None):
+def central_difference(ctx, theta, steps, rel=1e-4, base=None, indices=None, frozen=True):
     """dV/dtheta_i by (V(theta + h e_i) - V(theta - h e_i)) / 2h, h = rel |theta_i|
-    (rel when theta_i = 0), from the same initial state and streams."""
+    (rel when theta_i = 0), from the same initial state and streams.
+
+    frozen: both perturbed runs replay the absorption events of the run at
+    theta (EVENTS_RECORD, then EVENTS_REPLAY), so the quotient differences
+    the map the adjoint differentiates; otherwise each run has its own
+    events, and a macro-particle absorbed in one run and not in the other
+    moves V by q w / C whatever h is."""
     theta = np.asarray(theta, dtype=np.float64)
     out = np.full(len(theta), np.nan)
+    mode = EVENTS_FREE
+    if frozen:
+        ctx.charging_forward(_params(theta, base), steps, EVENTS_RECORD)
+        mode = EVENTS_REPLAY
     for i in (range(len(theta)) if indices is None else indices):
         h = rel * abs(theta[i]) if theta[i] != 0.0 else rel
         tp, tm = theta.copy(), theta.copy()
         tp[i] += h
         tm[i] -= h
-        vp = ctx.charging_forward(_params(tp, base), steps)
-        vm = ctx.charging_forward(_params(tm, base), steps)
+        vp = ctx.charging_forward(_params(tp, base), steps, mode)
+        vm = ctx.charging_forward(_params(tm, base), steps, mode)
         out[i] = (vp - vm) / (2.0 * h)
     return out

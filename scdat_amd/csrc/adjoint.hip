@@ -54,6 +54,23 @@ __device__ inline void publish_partials(double p[kAdjPartials], double* out) {
   }
 }
 
+// Centroid of a tet from its affine map: the point with lambda_1..3 = 1/4,
+// x = x0 + J (1/4, 1/4, 1/4), J the inverse of the stored map a.
+template <typename T>
+__device__ inline void tet_centroid(const TetMap<T>& m, T c[3]) {
+  const double a00 = m.a[0], a01 = m.a[1], a02 = m.a[2], a10 = m.a[3], a11 = m.a[4], a12 = m.a[5], a20 = m.a[6],
+               a21 = m.a[7], a22 = m.a[8];
+  const double det = a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20) + a02 * (a10 * a21 - a11 * a20);
+  const double q = 0.25 / det;
+  // J (1,1,1) / 4 = adj(a) (1,1,1) / (4 det)
+  const double y0 = (a11 * a22 - a12 * a21) - (a01 * a22 - a02 * a21) + (a01 * a12 - a02 * a11);
+  const double y1 = -(a10 * a22 - a12 * a20) + (a00 * a22 - a02 * a20) - (a00 * a12 - a02 * a10);
+  const double y2 = (a10 * a21 - a11 * a20) - (a00 * a21 - a01 * a20) + (a00 * a11 - a01 * a10);
+  c[0] = (T)((double)m.x0[0] + q * y0);
+  c[1] = (T)((double)m.x0[1] + q * y1);
+  c[2] = (T)((double)m.x0[2] + q * y2);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_adj_forward(AdjArgs<T> a) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -78,7 +95,71 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(AdjArgs<T> a) {
       T xn[3] = {fma_t(v[0], a.dt, x[0]), fma_t(v[1], a.dt, x[1]), fma_t(v[2], a.dt, x[2])};
       int group = -1, face = 0, last = c;
       T hit[3] = {x[0], x[1], x[2]};
-      const int r = walk(a.mesh, c, xs, xn, v, group, hit, m, face, last);
+      int r = walk(a.mesh, c, xs, xn, v, group, hit, m, face, last);
+      if (a.event_mode == 2) {
+        const int32_t ev = a.events[i];
+        if (ev >= 0 && (ev >> 8) == a.step) {
+          // absorbed here in the recorded run: absorbed here, same group
+          if (r >= 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) hit[q] = xn[q];
+          }
+          group = ev & 0xFF;
+          r = -1;
+        } else if (r == -1) {
+          // kept in the recorded run: relocate from the cell the recorded run
+          // reached (a straight path that clips a convex edge of the boundary
+          // would otherwise be stopped on the far side of the edge); a
+          // residual overshoot beyond a face is projected back onto the
+          // face's plane and the tangential remainder walked (a face, an
+          // edge, a corner).  Continuous, and the identity while the walk
+          // stays inside.
+          const int cb = a.path[(int64_t)a.step * a.n + i];
+          const TetMap<T> mb = a.mesh.tets[cb >= 0 ? cb : last];
+          T st[3];
+          tet_centroid(mb, st);
+          int cur = cb >= 0 ? cb : last;
+          m = mb;
+          {
+            T xe[3] = {xn[0], xn[1], xn[2]};
+            r = walk(a.mesh, cur, st, xe, v, group, hit, m, face, last);
+          }
+          for (int pass = 0; pass < 3 && r == -1; ++pass) {
+            T nrm[3];  // inward normal of the crossed face of `last`
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const T g1 = m.a[q], g2 = m.a[3 + q], g3 = m.a[6 + q];
+              nrm[q] = face == 0 ? -(g1 + g2 + g3) : (face == 1 ? g1 : (face == 2 ? g2 : g3));
+            }
+            const T inl = T(1) / sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            T d = T(0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              nrm[q] *= inl;
+              d += (xn[q] - hit[q]) * nrm[q];  // < 0: beyond the face
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              xn[q] -= d * nrm[q];
+              st[q] = hit[q];
+            }
+            cur = last;
+            T xe[3] = {xn[0], xn[1], xn[2]};
+            r = walk(a.mesh, cur, st, xe, v, group, hit, m, face, last);
+          }
+          if (r < 0) {
+            // not reached in three projections: stay at the last crossing
+            r = cur;
+            m = a.mesh.tets[cur];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) xn[q] = st[q];
+          }
+        }
+      }
+      if (a.event_mode == 1) {
+        if (r == -1) a.events[i] = (a.step << 8) | (group & 0xFF);
+        a.path[(int64_t)a.step * a.n + i] = r;
+      }
       if (r >= 0) {
         c1 = r;
         x[0] = xn[0];

@@ -40,6 +40,18 @@ struct AdjArgs {
   double emax;                // Sternglass E_max (eV)
   double* rho;                // forward: FP64 nodal charge
   double* partial;            // [blocks][kAdjPartials]
+  // Absorption events of the forward (the map the adjoint differentiates
+  // holds them fixed): mode 0 free; 1 record: events[i] = step << 8 | group
+  // of the step that absorbs particle i (-1 never); 2 replay a record: a
+  // particle is absorbed exactly at its recorded step and group, and one
+  // the record kept whose walk leaves the mesh is projected back onto the
+  // boundary face's plane and walks the tangential remainder.
+  // The record also keeps each particle's cell after every step (path[step
+  // x n + i]), where a replayed particle that the record kept is relocated.
+  int event_mode;
+  int step;
+  int32_t* events;
+  int32_t* path;
   // backward
   double* xb;                 // 3n position adjoints (in: step n+1, out: step n)
   double* vb;                 // 3n velocity adjoints

@@ -590,6 +590,10 @@ struct scdat_ctx {
       d_adj_dotp, d_snap_E, d_snap_x64, d_snap_phi64, d_snap_phiT, d_snap_sc;
   int64_t adj_np = 0;
   SolverFigures snap_solver;
+  DevBuf d_adj_events;  // recorded absorption events of a charging forward
+  DevBuf d_adj_path;    // and every particle's cell after each of its steps
+  int64_t adj_events_np = 0;
+  int adj_events_steps = 0;
   int n_seg() const { return seg_node0.empty() ? 1 : (int)seg_node0.size() - 1; }
   bool segmented() const { return !seg_node0.empty(); }
   int last_iters = 0, last_outer = 0, last_conv = 0;
@@ -2383,8 +2387,9 @@ std::array<double, kAdjPartials> adj_partials(scdat_ctx* c, int blocks) {
 // of step n + 1 out) + Poisson + E of step n + 1 into s1.E.
 template <typename T>
 void adj_forward_step(scdat_ctx* c, const ChargingRun& run, AdjArgs<T> a, AdjStep& s0, AdjStep& s1, double& Q,
-                      double& V) {
+                      double& V, int step = 0) {
   hipStream_t s = c->stream;
+  a.step = step;
   HIPCHK(hipMemsetAsync(c->d_adj_rho.p, 0, 8 * (size_t)c->h.nn, s));
   a.mesh.E = s0.E.as<Vec4<T>>();
   a.s0 = adj_state<T>(s0);
@@ -2414,16 +2419,31 @@ void adj_forward_step(scdat_ctx* c, const ChargingRun& run, AdjArgs<T> a, AdjSte
 }
 
 template <typename T>
-double charging_forward(scdat_ctx* c, const scdat_charging_params* p, int steps) {
+double charging_forward(scdat_ctx* c, const scdat_charging_params* p, int steps, int events) {
   const ChargingRun run = adj_begin<T>(c, p, 2);
   AdjArgs<T> a = adj_args<T>(c, run);
+  if (events == SCDAT_EVENTS_RECORD) {
+    c->d_adj_events.alloc(4 * (size_t)std::max<int64_t>(1, c->adj_np));
+    HIPCHK(hipMemsetAsync(c->d_adj_events.p, 0xFF, 4 * (size_t)c->adj_np, c->stream));  // -1: never absorbed
+    // each particle's cell after every step (4 B per particle-step)
+    c->d_adj_path.alloc(4 * (size_t)std::max<int64_t>(1, c->adj_np) * (size_t)std::max(1, steps));
+    c->adj_events_np = c->adj_np;
+    c->adj_events_steps = steps;
+  } else if (events == SCDAT_EVENTS_REPLAY) {
+    if (c->adj_events_np != c->adj_np || c->adj_events_steps < steps)
+      throw Error(SCDAT_ERR_STATE, "charging forward: no recorded events for these particles and steps");
+  }
+  a.event_mode = events;
+  a.events = c->d_adj_events.as<int32_t>();
+  a.path = c->d_adj_path.as<int32_t>();
   double Q = c->q_init, V = c->h_sc->vfloat;
   {
     PcgScalars q;
     HIPCHK(hipMemcpy(&q, c->d_sc.p, sizeof(q), hipMemcpyDeviceToHost));
     V = q.vfloat;
   }
-  for (int n = 0; n < steps; ++n) adj_forward_step<T>(c, run, a, c->adj_st[n & 1], c->adj_st[(n + 1) & 1], Q, V);
+  for (int n = 0; n < steps; ++n)
+    adj_forward_step<T>(c, run, a, c->adj_st[n & 1], c->adj_st[(n + 1) & 1], Q, V, n);
   adj_end(c);
   return V;
 }
@@ -2521,7 +2541,7 @@ double charging_adjoint(scdat_ctx* c, const scdat_charging_params* p, int steps,
     for (int j = 0; j < L; ++j) {
       AdjStep& from = c->adj_st[j & 1];
       AdjStep& to = c->adj_st[(j + 1) & 1];
-      adj_forward_step<T>(c, run, a, from, to, Q, V);
+      adj_forward_step<T>(c, run, a, from, to, Q, V, k * K + j);
     }
     if (L & 1) adj_copy_step<T>(c, c->adj_st[0], c->adj_st[1]);
   }
@@ -2549,7 +2569,7 @@ double charging_adjoint(scdat_ctx* c, const scdat_charging_params* p, int steps,
     V = ckV[k];
     Vs[0] = V;
     for (int j = 0; j < L; ++j) {
-      adj_forward_step<T>(c, run, a, c->adj_st[j], c->adj_st[j + 1], Q, V);
+      adj_forward_step<T>(c, run, a, c->adj_st[j], c->adj_st[j + 1], Q, V, k * K + j);
       Vs[j + 1] = V;
     }
     for (int j = L - 1; j >= 0; --j) {
@@ -3785,9 +3805,16 @@ void scdat_charging_params_default(scdat_charging_params* p) {
 }
 
 int scdat_charging_forward(scdat_ctx* c, const scdat_charging_params* p, int32_t steps, double* v_final) {
-  if (!c || !p || !v_final || steps < 0) return SCDAT_ERR_ARG;
+  return scdat_charging_forward_events(c, p, steps, SCDAT_EVENTS_FREE, v_final);
+}
+
+int scdat_charging_forward_events(scdat_ctx* c, const scdat_charging_params* p, int32_t steps, int32_t events,
+                                  double* v_final) {
+  if (!c || !p || !v_final || steps < 0 || events < SCDAT_EVENTS_FREE || events > SCDAT_EVENTS_REPLAY)
+    return SCDAT_ERR_ARG;
   return guarded(c, [&] {
-    *v_final = c->is_fp64() ? charging_forward<double>(c, p, steps) : charging_forward<float>(c, p, steps);
+    *v_final = c->is_fp64() ? charging_forward<double>(c, p, steps, events)
+                            : charging_forward<float>(c, p, steps, events);
   });
 }
 

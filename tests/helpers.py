@@ -85,3 +85,30 @@ def periodic_partners(m, pairs=((0, 1, 0), (2, 3, 1))):
             partner[t, ka] = tb
             partner[tb, kb] = t
     return partner
+
+
+def geo_oracle(st, tol=1e-10):
+    """The oracle of a GEO case built by scdat_amd.geo.inputs: same mesh,
+    species, processes, surface emission and seed as geo.context."""
+    import scdat_amd.geo as G
+    cfg = st["cfg"]
+    m = st["mesh"]
+    o = oracle.Oracle(m["xyz"], m["tets"], m["kind"], group=m["group"])
+    for g in G.CONDUCTOR_GROUPS:
+        o.set_group(g, S.BC_CONDUCTOR)
+    o.set_poisson(0, EPS0, 1, tol, 50000)
+    o.set_push(st["dt"], 1, 1e-3, 64, 0)
+    for q, mass in G.species():
+        o.add_species(q, mass, st["w"])
+    o.set_particles(G.ELECTRON, *st["e"])
+    o.set_particles(G.PROTON, *st["i"])
+    o.set_particles(G.OXYGEN_ION, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.int32))
+    o.mcc_configure(st["procs"])
+    o.set_seed(int(cfg["seed"]))
+    for g in G.CONDUCTOR_GROUPS:
+        o.set_emission(g, see_model=S.SEE_STERNGLASS, delta_max=cfg["delta_max"], energy_max=cfg["energy_max_ev"],
+                       see_temperature=cfg["see_temperature_ev"],
+                       photo_j=cfg["photo_current_density"] if g == G.SUNLIT_GROUP else 0.0,
+                       photo_temperature=cfg["photo_temperature_ev"], photo_reference_v=0.0,
+                       electron_species=G.ELECTRON)
+    return o

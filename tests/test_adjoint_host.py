@@ -18,7 +18,11 @@ class Analytic:
         t = p.vector()
         return float(self.a @ t + self.b * t[0] * t[1])
 
-    def charging_forward(self, p, steps):
+    def __init__(self):
+        self.modes = []
+
+    def charging_forward(self, p, steps, events=S.EVENTS_FREE):
+        self.modes.append(events)
         return self._v(p)
 
     def charging_adjoint(self, p, steps, interval):
@@ -49,5 +53,13 @@ def test_autograd_harness_and_central_difference():
     (2.0 * V).backward()
     _, g = ctx.charging_adjoint(p, 10, 5)
     np.testing.assert_allclose(theta.grad.numpy(), 2.0 * g, rtol=1e-15)
+    ctx.modes.clear()
     fd = central_difference(ctx, p.vector(), 10, rel=1e-4, base=p)
     np.testing.assert_allclose(fd, g, rtol=1e-7, atol=1e-9)
+    # the events of the run at theta are recorded once, then every perturbed
+    # run replays them
+    assert ctx.modes[0] == S.EVENTS_RECORD and set(ctx.modes[1:]) == {S.EVENTS_REPLAY}
+    assert len(ctx.modes) == 1 + 2 * len(g)
+    ctx.modes.clear()
+    central_difference(ctx, p.vector(), 10, rel=1e-4, base=p, frozen=False)
+    assert set(ctx.modes) == {S.EVENTS_FREE}

@@ -9,7 +9,12 @@ config[1]: 64^3 hexes, 2^24 particles: the mixed path (FP32 push, gather,
   deposit and inner PCG, FP64 refinement to 1e-8) against the FP64 device
   path over the configuration's 500 steps, and against the FP64 oracle over
   the first steps.
+config[2]: the GEO charging case, 48^3 hexes, 2^23 particles, 10 keV
+  plasma, oxygen background (null-collision MCC), Sternglass secondary
+  emission and photoemission, against the FP64 oracle event for event.
 config[3]: the 256-case batched sweep.
+config[4]: the discrete adjoint at 32^3 hexes, 2^20 particles, 200 steps,
+  checkpoint every 20, against central differences of the event-frozen map.
 """
 import numpy as np
 import pytest
@@ -17,7 +22,8 @@ import pytest
 import oracle as O
 import scdat_amd as S
 import scdat_amd.batch as B
-from helpers import QE, make_ctx, make_oracle, plasma_setup, rel_err
+import scdat_amd.geo as G
+from helpers import QE, geo_oracle, make_ctx, make_oracle, plasma_setup, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -244,3 +250,107 @@ def test_config3_sweep_256_cases_in_one_context():
         c.close()
     print("checked cases", pick, "secondaries", n_emit)
     bc.close()
+
+
+@pytest.mark.parametrize("neutral_density", [1e12, 1e18])
+def test_config2_geo_fp64_matches_oracle_event_for_event(neutral_density):
+    """The GEO case at full size (48^3 hexes, 2^22 electrons + 2^22 protons)
+    in FP64 against the oracle over 3 steps: the same collision candidates,
+    collisions per process, ionisation products, secondaries and
+    photoelectrons, the same particle counts and the potential to 1e-6.
+    At the configuration's 1e12 m^-3 of oxygen a 10 keV electron collides
+    about once in 1e10 steps, so the null-collision pre-selection examines
+    almost no particle; 1e18 m^-3 makes about a hundred collisions a step
+    to compare one for one."""
+    st = G.inputs(neutral_density=neutral_density)
+    c = G.context(st, precision=1, tol=1e-10, sort_interval=20, steps=10)
+    o = geo_oracle(st, tol=1e-10)
+    o.deposit()
+    o.solve()
+    o.efield()
+    for _ in range(3):
+        o.step(1)
+        c.step(1)
+    eo, ec = o.mcc_counts(), c.event_counts()
+    print("oracle", {k: (v[:3].tolist() if k == "collisions" else v) for k, v in eo.items()})
+    print("device", {k: (v[:3] if k == "collisions" else v) for k, v in ec.items()})
+    assert list(eo["collisions"][:3]) == ec["collisions"][:3]
+    assert eo["created"] == ec["created"] and eo["removed"] == ec["removed"]
+    assert eo["see_emitted"] == ec["see_emitted"] and eo["photo_emitted"] == ec["photo_emitted"]
+    assert eo["candidates"] == ec["mcc_candidates"]
+    assert ec["mcc_majorant_exceeded"] == 0 and ec["overflow"] == 0
+    assert ec["photo_emitted"] > 1000 and ec["see_emitted"] > 0
+    pushes = c.stats()["particle_pushes"]
+    if neutral_density < 1e15:
+        assert ec["mcc_candidates"] < 1e-6 * pushes
+    else:
+        assert sum(ec["collisions"][:3]) > 100 and ec["created"] > 0
+    for s in range(3):
+        n_o = int((o.get_particles(s)[2] >= 0).sum())
+        n_c = c.particle_count(s)[0]
+        assert n_o == n_c, (s, n_o, n_c)
+    _, phi_o, _ = o.fields()
+    _, phi_c, _ = c.fields()
+    print("phi rel", rel_err(phi_c, phi_o), "V_f", c.stats()["floating_potential"], o.floating_potential())
+    assert rel_err(phi_c, phi_o) < 1e-6
+    c.close()
+    o.close()
+
+
+def test_config4_adjoint_matches_central_differences():
+    """dV_float/d(delta_max, J0, G, face_yield[0..5]) by the discrete adjoint
+    at the configuration's size (32^3 hexes, 2^20 particles, 200
+    collisionless steps, checkpoint every 20) against central differences in
+    FP64.
+
+    The adjoint differentiates the map with every particle's absorption step
+    and group held fixed, and so do the differences: both perturbed runs
+    replay the events of the run at theta (without that, one macro-electron
+    absorbed in one run and not in the other moves V by q w / C ~ 2e-4 V,
+    which the quotient divides by 2h).  The event-frozen map is continuous
+    but only piecewise smooth (the P1 gather and deposit have kinks at tet
+    faces), so the quotient converges to the derivative as h shrinks: within
+    1e-3 at h = 1e-7 relative on all nine components, and within 1e-2 at
+    the configuration's h = 1e-4, where the kinks the perturbed particles
+    cross limit the quotient's own accuracy to about 2e-3.  The FP32
+    adjoint's error against the FP64 one is bounded too."""
+    from scdat_amd.adjoint import central_difference
+    st = plasma_setup(n=32, npart=1 << 19, seed=SEED1)
+    p = S.charging_params(see_delta_max=2.0, see_energy_max_ev=400.0,
+                          face_yield=[1.0, 0.8, 1.2, 0.9, 1.1, 1.05], photo_current_density=2e-5,
+                          photo_temperature_ev=2.0, photo_reference_v=0.0, sunlit_faces=0b000010,
+                          surface_conductance_s=1e-9)
+    theta = p.vector()
+    grads = {}
+    for prec in (1, 0):
+        c = make_ctx(st, precision=prec, tol=1e-12 if prec else 1e-8, sort_interval=0, adaptive=0)
+        c.initialize()
+        s0 = c.stats()
+        v, g = c.charging_adjoint(p, 200, 20)
+        grads[prec] = g
+        assert c.stats() == s0  # the context is left as it was
+        if prec == 1:
+            # replaying the recorded events at theta reproduces the free run
+            # (FP64 atomics in the deposit order the last bits differently)
+            v_rec = c.charging_forward(p, 200, S.EVENTS_RECORD)
+            v_rep = c.charging_forward(p, 200, S.EVENTS_REPLAY)
+            assert abs(v_rec - v) <= 1e-12 * abs(v) and abs(v_rep - v) <= 1e-12 * abs(v)
+            fd4 = central_difference(c, theta, 200, rel=1e-4, base=p)
+            fd7 = central_difference(c, theta, 200, rel=1e-7, base=p)
+            fd_free = central_difference(c, theta, 200, rel=1e-4, base=p, indices=[0], frozen=False)
+        c.close()
+    g64, g32 = grads[1], grads[0]
+    rel4 = np.abs(theta * (g64 - fd4)) / np.abs(theta * fd4)
+    rel7 = np.abs(theta * (g64 - fd7)) / np.abs(theta * fd7)
+    err32 = float(np.abs(theta * (g32 - g64)).max() / np.abs(theta * g64).max())
+    print("adjoint        ", g64.tolist())
+    print("FD frozen 1e-4 ", fd4.tolist())
+    print("FD frozen 1e-7 ", fd7.tolist())
+    print("relative differences h=1e-4", rel4.tolist())
+    print("relative differences h=1e-7", rel7.tolist())
+    print("FD free h=1e-4 (delta_max)", fd_free[0], "adjoint", g64[0])
+    print("FP32 adjoint error vs FP64: %.3e" % err32)
+    assert np.all(rel7 < 1e-3)
+    assert np.all(rel4 < 1e-2)
+    assert abs(fd_free[0] / g64[0] - 1.0) > 0.5  # the free map's events dominate at h = 1e-4
+    assert err32 < 5e-3
