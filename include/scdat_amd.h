@@ -244,6 +244,13 @@ typedef struct {
 
 /* Replaces PICCycle::getStatistics (PICCycle.h:135-149). */
 int scdat_get_stats(scdat_ctx* ctx, scdat_stats* st);
+/* The same statistics, bit for bit, with no allocation of any kind: no
+ * counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its device reductions use a static per-GPU scratch of
+ * 8 + 4096 words (at most 4096 scenarios; more is SCDAT_ERR_ARG); calls
+ * that overlap on one GPU race on that scratch.                           */
+int scdat_get_stats_noalloc(scdat_ctx* ctx, scdat_stats* st);
 
 typedef struct {
   double absorbed_electron_charge_c;

@@ -247,6 +247,7 @@ SIGNATURES = {
     "scdat_get_tet_maps": (C.c_int, [_P, _F]),
     "scdat_get_species_charge_fixed": (C.c_int, [_P, C.c_int32, _I64]),
     "scdat_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "scdat_get_stats_noalloc": (C.c_int, [_P, C.POINTER(Stats)]),
     "scdat_get_ledger": (C.c_int, [_P, C.POINTER(LedgerEntry), C.c_int32]),
     "scdat_get_stage_times": (C.c_int, [_P, _D]),
     "scdat_get_push_kernel_time": (C.c_int, [_P, _D, _I64]),
@@ -766,6 +767,12 @@ class Context:
     def stats(self):
         st = Stats()
         self._chk(lib().scdat_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def stats_noalloc(self):
+        """stats() through scdat_get_stats_noalloc (no allocation in the library)."""
+        st = Stats()
+        self._chk(lib().scdat_get_stats_noalloc(self.h, C.byref(st)))
         return st.as_dict()
 
     def ledger(self, n_groups=12):

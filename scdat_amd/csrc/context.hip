@@ -17,6 +17,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -31,6 +32,15 @@
 #include "emit.hpp"
 #include "sort.hpp"
 #include "adjoint.hpp"
+
+// Scratch of scdat_get_stats_noalloc: the results of its one reduction
+// kernel (8 header words, then the fixed-point charge sum of each scenario).
+// One copy per GPU, held until the process ends; every call overwrites
+// every word it reads back, so nothing needs clearing between calls, and
+// calls that overlap on one GPU race on it.
+constexpr int kStatsScratchHead = 8;
+constexpr int kStatsMaxSegments = 4096;
+static __device__ __align__(256) unsigned long long scratch[kStatsScratchHead + kStatsMaxSegments];
 
 namespace scdat {
 
@@ -453,6 +463,100 @@ struct Engine {
   virtual ~Engine() = default;
 };
 
+// Address of this process's statistics scratch on device `dev`, looked up
+// once per device with that device current (each GPU holds its own copy of
+// the array at its own address).  Null when the lookup fails.
+unsigned long long* stats_scratch(int dev) {
+  static std::mutex mu;
+  static std::array<unsigned long long*, 64> ptr{};
+  if (dev < 0 || dev >= (int)ptr.size()) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    ptr[dev] = static_cast<unsigned long long*>(p);
+  }
+  return ptr[dev];
+}
+
+// Inputs of the statistics reduction (scdat_get_stats_noalloc).
+struct StatsArgs {
+  const double* ke[kMaxSpecies];        // per-block kinetic-energy partials of the last push
+  const int32_t* alive[kMaxSpecies];    // per-block alive counts
+  int64_t nb[kMaxSpecies];
+  double q[kMaxSpecies], w[kMaxSpecies];
+  int ns;
+  const unsigned long long* rho;        // fixed-point charge of the last push
+  int64_t nn;
+  const int32_t* seg_b0;                // scenario row blocks (null: one scenario)
+  int n_seg;
+  const double* seg_wscale;
+  const unsigned long long* led_cnt;    // [scenario][group][species]
+  const unsigned long long* led_emi;
+  uint32_t surf_mask, cond_mask;        // groups with a condition; conductor groups
+  double q_init;
+  const unsigned long long* lost;
+  const unsigned long long* pushes;
+  const PcgScalars* sc;
+  unsigned long long* out;              // the scratch
+};
+
+// Block 0: the scalar statistics, in the order and with the roundings of
+// the host sums of scdat_get_stats (one thread, no fused multiply-adds);
+// block 1 + s: the exact integer charge of scenario s.
+__global__ void __launch_bounds__(kBlock) k_stats_noalloc(StatsArgs a) {
+  if (blockIdx.x == 0) {
+    if (threadIdx.x != 0) return;
+    double ke = 0.0;
+    long long alive = 0;
+    for (int s = 0; s < a.ns; ++s)
+      for (int64_t b = 0; b < a.nb[s]; ++b) {
+        ke = __dadd_rn(ke, a.ke[s][b]);
+        alive += a.alive[s][b];
+      }
+    double surf = 0.0, qc = a.q_init;
+    for (int sg = 0; sg < a.n_seg; ++sg) {
+      const double scale = a.seg_wscale ? a.seg_wscale[sg] : 1.0;
+      for (int g = 0; g < kMaxGroups; ++g) {
+        if (!((a.surf_mask >> g) & 1u)) continue;
+        for (int sp = 0; sp < a.ns; ++sp) {
+          const size_t o = ((size_t)sg * kMaxGroups + g) * kMaxSpecies + sp;
+          const double qw = __dmul_rn(a.q[sp], __dmul_rn(a.w[sp], scale));
+          const double qq = __dmul_rn((double)a.led_cnt[o], qw);
+          surf = __dadd_rn(surf, qq);
+          if (sg == 0 && ((a.cond_mask >> g) & 1u))
+            qc = __dadd_rn(qc, __dsub_rn(qq, __dmul_rn((double)a.led_emi[o], qw)));
+        }
+      }
+    }
+    a.out[0] = (unsigned long long)__double_as_longlong(ke);
+    a.out[1] = (unsigned long long)alive;
+    a.out[2] = (unsigned long long)__double_as_longlong(surf);
+    a.out[3] = (unsigned long long)__double_as_longlong(qc);
+    a.out[4] = *a.lost;
+    a.out[5] = *a.pushes;
+    a.out[6] = (unsigned long long)__double_as_longlong(a.sc->vfloat);
+    a.out[7] = 0ull;
+    return;
+  }
+  __shared__ long long lds[kBlock / 64];
+  const int sg = (int)blockIdx.x - 1;
+  const int64_t i0 = a.seg_b0 ? (int64_t)a.seg_b0[sg] * kPcgBlock : 0;
+  const int64_t e1 = a.seg_b0 ? (int64_t)a.seg_b0[sg + 1] * kPcgBlock : a.nn;
+  const int64_t i1 = e1 < a.nn ? e1 : a.nn;
+  long long t = 0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) t += (long long)a.rho[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += lds[w];
+    a.out[kStatsScratchHead + sg] = (unsigned long long)tot;
+  }
+}
+
 // Host-side figures of the last solve (statistics and the next solve's batch
 // sizes), saved and restored around runs that must leave the context as it
 // was (the charging forward and adjoint).
@@ -564,6 +668,9 @@ struct scdat_ctx {
   DevBuf d_phik, d_patch_v;
   double* h_patch_v = nullptr;               // pinned, kMaxPatches
   unsigned long long* h_ledger = nullptr;    // pinned, 2 kMaxGroups kMaxSpecies
+  unsigned long long* h_stats = nullptr;     // pinned mirror of the statistics scratch
+  unsigned long long* d_stats = nullptr;     // this device's statistics scratch (symbol address)
+  int device = 0;
   bool ledger_pending = false;
   // batched scenarios (segments); empty seg_node0: one scenario
   std::vector<int64_t> seg_node0;  // n_seg + 1 node offsets
@@ -622,6 +729,12 @@ namespace {
 
 int fail(scdat_ctx* c, int code, const std::string& m) {
   if (c) c->err = m;
+  return code;
+}
+// Without a temporary string: no allocation while the message fits the
+// context's reserved buffer.
+int fail(scdat_ctx* c, int code, const char* m) {
+  if (c) c->err.assign(m);
   return code;
 }
 
@@ -2707,6 +2820,11 @@ int scdat_create(const scdat_config* cfg, scdat_ctx** out) {
     delete c;
     return SCDAT_ERR_DEVICE;
   }
+  if (hipGetDevice(&c->device) != hipSuccess || !(c->d_stats = stats_scratch(c->device))) {
+    delete c;
+    return SCDAT_ERR_DEVICE;
+  }
+  c->err.reserve(512);  // messages fit without an allocation (the no-alloc entry point)
   if (hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return SCDAT_ERR_DEVICE;
@@ -2714,7 +2832,9 @@ int scdat_create(const scdat_config* cfg, scdat_ctx** out) {
   if (hipHostMalloc((void**)&c->h_sc, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_append, 8 * (kMaxSpecies + 1), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_patch_v, 8 * kMaxPatches, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_ledger, 16 * kMaxGroups * kMaxSpecies, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_ledger, 16 * kMaxGroups * kMaxSpecies, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_stats, sizeof(scratch), hipHostMallocDefault) != hipSuccess) {
+    if (c->h_ledger) (void)hipHostFree(c->h_ledger);
     if (c->h_append) (void)hipHostFree(c->h_append);
     if (c->h_patch_v) (void)hipHostFree(c->h_patch_v);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -2739,6 +2859,7 @@ void scdat_destroy(scdat_ctx* c) {
   if (c->h_append) (void)hipHostFree(c->h_append);
   if (c->h_patch_v) (void)hipHostFree(c->h_patch_v);
   if (c->h_ledger) (void)hipHostFree(c->h_ledger);
+  if (c->h_stats) (void)hipHostFree(c->h_stats);
   hipStream_t s = c->stream;
   delete c;  // DevBuf destructors free device memory
   if (s) (void)hipStreamDestroy(s);
@@ -3238,6 +3359,82 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
     st->pcg_iterations_total = c->iters_total;
     st->pcg_failed_solves = c->failed_solves;
   });
+}
+
+// The statistics of scdat_get_stats with no allocation: the device sums go
+// to this GPU's static scratch (one kernel, every result word written),
+// the host reads them back into the context's pinned mirror (allocated at
+// scdat_create) and fills *st.  Same values bit for bit.  No hipMalloc,
+// hipFree or host allocation: error messages fit the context's reserved
+// buffer.  At most kStatsMaxSegments scenarios.  Calls that overlap on one
+// GPU (two host threads, two contexts) race on the scratch.
+int scdat_get_stats_noalloc(scdat_ctx* c, scdat_stats* st) {
+  if (!c || !st) return SCDAT_ERR_ARG;
+  if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "get_stats_noalloc: no mesh");
+  if (c->n_seg() > kStatsMaxSegments) return fail(c, SCDAT_ERR_ARG, "get_stats_noalloc: more than 4096 scenarios");
+  if (c->sp.size() > (size_t)kMaxSpecies) return fail(c, SCDAT_ERR_ARG, "get_stats_noalloc: too many species");
+  StatsArgs a{};
+  a.ns = (int)c->sp.size();
+  for (int s = 0; s < a.ns; ++s) {
+    const SpeciesState& S = c->sp[s];
+    a.ke[s] = S.ke_partial.as<double>();
+    a.alive[s] = S.alive_partial.as<int32_t>();
+    a.nb[s] = S.stat_blocks;
+    a.q[s] = S.q;
+    a.w[s] = S.w;
+  }
+  a.rho = c->d_rho[c->rho_last].as<unsigned long long>();
+  a.nn = c->h.nn;
+  a.n_seg = c->n_seg();
+  if (c->segmented()) {
+    a.seg_b0 = c->d_seg_b0.as<int32_t>();
+    a.seg_wscale = c->d_seg_wscale.as<double>();
+  }
+  a.led_cnt = c->d_ledger_count.as<unsigned long long>();
+  a.led_emi = c->d_emit_count.as<unsigned long long>();
+  for (int g = 0; g < kMaxGroups; ++g) {
+    if (c->gtype[g] != SCDAT_BC_ESCAPE) a.surf_mask |= 1u << g;
+    if (c->gtype[g] == SCDAT_BC_CONDUCTOR) a.cond_mask |= 1u << g;
+  }
+  a.q_init = c->q_init;
+  a.lost = c->d_lost.as<unsigned long long>();
+  a.pushes = c->d_pushes.as<unsigned long long>();
+  a.sc = c->d_sc.as<PcgScalars>();
+  a.out = c->d_stats;
+  hipLaunchKernelGGL(k_stats_noalloc, dim3(1 + a.n_seg), dim3(kBlock), 0, c->stream, a);
+  const size_t words = (size_t)kStatsScratchHead + (size_t)a.n_seg;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(c->h_stats, c->d_stats, 8 * words, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(c, SCDAT_ERR_DEVICE, "get_stats_noalloc: device error");
+  const unsigned long long* h = c->h_stats;
+  auto as_double = [](unsigned long long u) {
+    double d;
+    std::memcpy(&d, &u, sizeof d);
+    return d;
+  };
+  std::memset(st, 0, sizeof(*st));
+  st->completed_steps = c->steps;
+  st->simulation_time = c->sim_time;
+  st->total_energy = as_double(h[0]);
+  st->active_particles = (int64_t)h[1];
+  long double tq = 0;
+  for (int sg = 0; sg < a.n_seg; ++sg)
+    tq += (long double)(long long)h[kStatsScratchHead + sg] * (long double)(c->unit * seg_scale(c, sg));
+  st->total_charge = (double)tq;
+  st->surface_deposited_charge = as_double(h[2]);
+  st->conductor_charge = as_double(h[3]);
+  st->floating_potential = c->h.has_cond ? as_double(h[6]) : 0.0;
+  st->lost_particles = (int64_t)h[4];
+  st->particle_pushes = (int64_t)h[5];
+  st->pcg_iterations = c->last_iters;
+  st->pcg_outer = c->last_outer;
+  st->pcg_rel_residual = c->last_rel;
+  st->pcg_converged = c->last_conv;
+  st->pcg_initial_rel_residual = c->last_init_rel;
+  st->pcg_iterations_total = c->iters_total;
+  st->pcg_failed_solves = c->failed_solves;
+  return SCDAT_OK;
 }
 
 int scdat_get_ledger(scdat_ctx* c, scdat_ledger_entry* out, int32_t n_groups) {
