@@ -108,11 +108,14 @@ def structural_nnz(tets, n_nodes):
     return int(2 * np.unique(key).size + n_nodes)
 
 
-def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns=True):
+def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns=True, precond="jacobi"):
     import scdat_amd as S
     m = case["mesh"]
     t0 = time.time()
     ctx = S.Context(precision=0, dt=case["dt"], pcg_tol=1e-8, sort_interval=sort_interval)
+    if precond == "two-level":
+        ctx.set_node_order(S.ORDER_TILED)
+        ctx.set_preconditioner(S.PRECOND_TWO_LEVEL)
     ctx.set_mesh(m["xyz"], m["tets"], m["kind"], group=m["group"])
     for g in range(6, 12):
         ctx.set_boundary(g, S.BC_CONDUCTOR)
@@ -561,6 +564,8 @@ def main():
     ap.add_argument("--sweep-seed", type=int, default=20261206)
     ap.add_argument("--row-patterns", type=int, choices=(0, 1), default=1,
                     help="1: the FP32 inner iteration may read the operator as row patterns")
+    ap.add_argument("--precond", choices=("jacobi", "two-level"), default="jacobi",
+                    help="Poisson preconditioner; two-level uses the tiled node order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -588,7 +593,7 @@ def main():
     name, ncu = S.device_info()
     case = build_case(args.hex, args.particles, args.seed)
     ctx, g = run_gpu(case, args.steps, args.warmup, args.sort_interval, not args.no_kernel_timing,
-                     bool(args.row_patterns))
+                     bool(args.row_patterns), args.precond)
 
     m = case["mesh"]
     n_nodes, n_tets = len(m["kind"]), len(m["tets"])

@@ -569,27 +569,36 @@ int scdat_layer_profile(scdat_ctx* ctx, int32_t axis, double length, int32_t n_c
 /* ---- Internal node order -----------------------------------------------------
  * SCDAT_ORDER_GIVEN (default) keeps the caller's node order;
  * SCDAT_ORDER_MORTON renumbers the nodes internally in Morton order of their
- * coordinates, so that consecutive 512-node row blocks are compact boxes
- * (the coarse space of the two-level preconditioner needs this).  On a
- * structured box mesh numbered (i, j, k) the caller's order is faster for
- * the Jacobi iteration (neighbour gathers of consecutive rows coalesce:
- * 15 us against 20 us per iteration at 64^3).  Every entry point takes and
- * returns node arrays in the caller's order.  Batched scenarios always use
- * the caller's order.  Call before scdat_set_mesh.                         */
+ * coordinates, so that consecutive 512-node row blocks are compact boxes;
+ * SCDAT_ORDER_TILED, for nodes on a lattice (uniform spacing per axis, such
+ * as scdat_box_mesh_build's, with or without the conductor box), makes every
+ * 512-node row block an 8 x 8 x 8 tile of lattice points (the index-0 layers
+ * fill the tiles' holes and follow the tiles); on other meshes it is Morton.
+ * The two-level preconditioner's aggregates are the row blocks, so it needs
+ * MORTON or TILED; with TILED and the mixed-precision solve its coarse step
+ * runs inside the iteration kernel.  On a structured box mesh numbered
+ * (i, j, k) the caller's order is faster for the Jacobi iteration than
+ * Morton (neighbour gathers of consecutive rows coalesce: 15 us against
+ * 20 us per iteration at 64^3).  Every entry point takes and returns node
+ * arrays in the caller's order.  Batched scenarios always use the caller's
+ * order.  Call before scdat_set_mesh.                                      */
 #define SCDAT_ORDER_MORTON 0
 #define SCDAT_ORDER_GIVEN 1
+#define SCDAT_ORDER_TILED 2
 int scdat_set_node_order(scdat_ctx* ctx, int32_t order);
 
 /* ---- Poisson preconditioner ------------------------------------------------
  * SCDAT_PRECOND_JACOBI (default): diagonal (MultiBoundaryElectricFieldSolver
  * .cpp:1432-1513), one fused kernel per iteration.  SCDAT_PRECOND_TWO_LEVEL:
  * M^-1 = D^-1 + P Ac^-1 P^T, P the free nodes of each 512-node block of the
- * internal order (use SCDAT_ORDER_MORTON, so that blocks are compact), Ac =
- * P^T A P inverted once at scdat_set_mesh; it
- * needs about half the iterations but a second (coarse) launch per
- * iteration, which on one MI355X at 273k nodes costs more than it saves.
- * Meshes under 4096 free nodes or over 2048 blocks, and batched scenarios,
- * always use Jacobi.  No counterpart in the reference.                      */
+ * internal order (use SCDAT_ORDER_TILED or SCDAT_ORDER_MORTON, so that
+ * blocks are compact), Ac = P^T A P inverted once at scdat_set_mesh.  It
+ * needs fewer iterations (64^3 box: about 0.4x).  In mixed precision with
+ * SCDAT_ORDER_TILED each iteration is still one kernel (every block applies
+ * the rows of Ac^-1 of the aggregates it touches); otherwise a coarse
+ * kernel runs after every iteration.  Meshes under 4096 free nodes or over
+ * 2048 blocks, and batched scenarios, always use Jacobi.  No counterpart in
+ * the reference.                                                           */
 #define SCDAT_PRECOND_JACOBI 0
 #define SCDAT_PRECOND_TWO_LEVEL 1
 int scdat_set_preconditioner(scdat_ctx* ctx, int32_t kind);
@@ -600,7 +609,7 @@ typedef struct {
   int32_t coarse_unknowns;  /* two-level: aggregates (row blocks); 0          */
   int32_t row_patterns;     /* FP32 inner operator as row patterns (0: ELL)   */
   int32_t pcg_block;        /* rows (= threads) of a PCG row block            */
-  int32_t reserved;
+  int32_t coarse_in_kernel; /* two-level coarse step inside the iteration kernel */
 } scdat_solver_info;
 int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
 

@@ -115,13 +115,13 @@ class SurfaceEmission(C.Structure):
 EVENTS_FREE, EVENTS_RECORD, EVENTS_REPLAY = 0, 1, 2
 SEGMENT_ALIGN = 512
 PRECOND_JACOBI, PRECOND_TWO_LEVEL = 0, 1
-ORDER_MORTON, ORDER_GIVEN = 0, 1
+ORDER_MORTON, ORDER_GIVEN, ORDER_TILED = 0, 1, 2
 
 
 class SolverInfo(C.Structure):
     _fields_ = [("rows", C.c_int64), ("nonzeros", C.c_int64), ("ell_width", C.c_int32),
                 ("preconditioner", C.c_int32), ("coarse_unknowns", C.c_int32), ("row_patterns", C.c_int32),
-                ("pcg_block", C.c_int32), ("reserved", C.c_int32)]
+                ("pcg_block", C.c_int32), ("coarse_in_kernel", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

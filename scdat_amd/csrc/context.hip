@@ -104,6 +104,9 @@ struct HostMesh {
   // two-level preconditioner (nc = 0: none): row-block aggregates
   int nc = 0;
   std::vector<double> dco, aci;
+  // aggregates each row block touches (its own first), CSR over the blocks
+  std::vector<int32_t> cnb_off, cnb;
+  int cnb_max = 0;
 };
 
 // Coarse space of the two-level preconditioner: aggregate b = the free rows
@@ -179,6 +182,29 @@ static void build_coarse(HostMesh& h) {
   h.dco.assign((size_t)nn, 0.0);
   for (int64_t i = 0; i < nn; ++i)
     if (h.kind[i] == SCDAT_NODE_FREE) h.dco[i] = 1.0 / h.dinv[i];
+  h.cnb_off.assign((size_t)G + 1, 0);
+  h.cnb.clear();
+  std::vector<int32_t> seen((size_t)G, -1);
+  for (int b = 0; b < G; ++b) {
+    h.cnb.push_back(b);
+    seen[b] = b;
+    const int64_t i1 = std::min<int64_t>(nn, (int64_t)(b + 1) * kCoarseBlockRows);
+    for (int64_t i = (int64_t)b * kCoarseBlockRows; i < i1; ++i) {
+      if (h.kind[i] != SCDAT_NODE_FREE) continue;
+      for (int sl = 0; sl < W; ++sl) {
+        const size_t o = (size_t)sl * nn + i;
+        const int32_t j = h.col[o];
+        if (h.aval[o] == 0.0 || h.kind[j] != SCDAT_NODE_FREE) continue;
+        const int c = j / kCoarseBlockRows;
+        if (seen[c] != b) {
+          seen[c] = b;
+          h.cnb.push_back(c);
+        }
+      }
+    }
+    h.cnb_off[b + 1] = (int32_t)h.cnb.size();
+    h.cnb_max = std::max(h.cnb_max, h.cnb_off[b + 1] - h.cnb_off[b]);
+  }
   h.nc = G;
 }
 
@@ -617,6 +643,9 @@ struct scdat_ctx {
   // ELL operator
   DevBuf d_col, d_v32, d_v64, d_vh32, d_vh64, d_dinv32, d_dinv64, d_g[3];
   DevBuf d_pat, d_poff, d_pval, d_pdinv;  // row patterns of the FP32 inner operator
+  DevBuf d_pap;                           // and their A P entries (two-level, tiled order)
+  bool pat_ap = false;
+  DevBuf d_cnb_off, d_cnb;                // aggregates each row block touches
   int npat = 0;                           // 0: none (too many, or FP64 inner solve)
   bool pat_wide = false;                  // 4-byte pattern ids
   bool pat_force_wide = false;            // 4-byte ids whatever the count (tests)
@@ -625,7 +654,7 @@ struct scdat_ctx {
   DevBuf d_kind, d_lift, d_dval, d_dscale, d_phi1, d_kc_idx, d_kc_val, d_b64, d_x64, d_r64, d_partial, d_sc;
   DevBuf d_inner[8];
   // two-level preconditioner
-  DevBuf d_dco, d_aci, d_coarse[6];
+  DevBuf d_dco, d_aci, d_acip, d_coarse[6];
   int precond = SCDAT_PRECOND_JACOBI;
   // fields
   DevBuf d_E, d_phiT, d_phi64, d_rho[2];
@@ -688,6 +717,7 @@ struct scdat_ctx {
   std::array<double, kMaxGroups> fe_pending{}, fe_pending_sec{};
   DevBuf d_fe_x, d_fe_v, d_fe_cell, d_fe_id;
   int node_order = SCDAT_ORDER_GIVEN;
+  int order_in_use = SCDAT_ORDER_GIVEN;  // the internal order of the current mesh
   std::vector<int32_t> perm, iperm;  // internal node -> caller's node, and back (empty: same order)
   int push_chunks = 4;  // particles per thread of the push kernel (scdat_autotune)
   // charging adjoint: working steps, checkpoints, adjoint vectors, snapshot
@@ -760,42 +790,59 @@ size_t tsize(const scdat_ctx* c) { return c->is_fp64() ? 8 : 4; }
 
 // Row patterns of the FP32 inner operator A D^-1: rows whose column offsets
 // j - i, FP32 values and FP32 D^-1 agree bit for bit share one table entry.
-// Structured meshes in their natural order collapse to a few hundred
+// A slot whose value is zero (a column eliminated by a constraint, or ELL
+// padding) is stored as offset 0 with value 0: it adds 0 either way, and its
+// column no longer splits rows that differ only in where their constrained
+// neighbours are numbered.  Structured meshes collapse to a few hundred
 // patterns (a 64^3 box with a conductor: about a thousand for 273k rows; a
 // sweep of 256 boxes: about a thousand per box).  Used while the ids and
 // the table take at most half the ELL's bytes (8 W per row); 2-byte ids up
-// to 65536 patterns, 4-byte beyond.
+// to 65536 patterns, 4-byte beyond.  In the tiled order with the two-level
+// coarse space the key also holds the row's FP32 entries of A P (A_ij on
+// free rows and free columns), which the fused two-level iteration reads.
 void build_row_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::vector<float>& di) {
   const HostMesh& h = c->h;
   const int64_t n = h.nn;
   const int W = h.W;
   c->npat = 0;
+  c->pat_ap = false;
   if (W < 7 || W > 20 || n == 0) return;  // the pattern kernels exist for these widths
-  // table entries allowed: (npat (8 W + 4) + 4 n) <= 4 W n
-  const int64_t max_pat = std::max<int64_t>(0, (4 * (int64_t)W * n - 4 * n) / (8 * W + 4));
+  const bool with_ap = h.nc > 0 && c->order_in_use == SCDAT_ORDER_TILED;
+  const int per = with_ap ? 12 : 8;  // key bytes per slot
+  // table entries allowed: (npat (per W + 4) + 4 n) <= 4 W n
+  const int64_t max_pat = std::max<int64_t>(0, (4 * (int64_t)W * n - 4 * n) / (per * W + 4));
   std::unordered_map<std::string, int> id;
   id.reserve(1024);
   std::vector<uint32_t> pat((size_t)n);
   std::vector<int32_t> poff;
-  std::vector<float> pval, pdinv;
-  std::string key((size_t)(8 * W + 4), '\0');
+  std::vector<float> pval, pdinv, pap;
+  std::string key((size_t)(per * W + 4), '\0');
   for (int64_t i = 0; i < n; ++i) {
     char* k = key.data();
+    const bool free_i = h.kind[i] == SCDAT_NODE_FREE;
     for (int sl = 0; sl < W; ++sl) {
       const size_t o = (size_t)sl * n + i;
-      const int32_t off = h.col[o] - (int32_t)i;
-      std::memcpy(k + 8 * sl, &off, 4);
-      std::memcpy(k + 8 * sl + 4, &vh[o], 4);
+      const int32_t j = h.col[o];
+      const float ap = with_ap && free_i && h.kind[j] == SCDAT_NODE_FREE ? (float)h.aval[o] : 0.0f;
+      const int32_t off = (vh[o] == 0.0f && ap == 0.0f) ? 0 : j - (int32_t)i;
+      std::memcpy(k + per * sl, &off, 4);
+      std::memcpy(k + per * sl + 4, &vh[o], 4);
+      if (with_ap) std::memcpy(k + per * sl + 8, &ap, 4);
     }
-    std::memcpy(k + 8 * W, &di[i], 4);
+    std::memcpy(k + per * W, &di[i], 4);
     auto it = id.find(key);
     if (it == id.end()) {
       if ((int64_t)id.size() >= max_pat) return;  // unstructured: stay on the ELL
       it = id.emplace(key, (int)id.size()).first;
       for (int sl = 0; sl < W; ++sl) {
-        const size_t o = (size_t)sl * n + i;
-        poff.push_back(h.col[o] - (int32_t)i);
-        pval.push_back(vh[o]);
+        int32_t off;
+        float v, ap = 0.0f;
+        std::memcpy(&off, k + per * sl, 4);
+        std::memcpy(&v, k + per * sl + 4, 4);
+        if (with_ap) std::memcpy(&ap, k + per * sl + 8, 4);
+        poff.push_back(off);
+        pval.push_back(v);
+        pap.push_back(ap);
       }
       pdinv.push_back(di[i]);
     }
@@ -812,8 +859,10 @@ void build_row_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::v
   c->d_poff.upload(poff, c->stream);
   c->d_pval.upload(pval, c->stream);
   c->d_pdinv.upload(pdinv, c->stream);
+  if (with_ap) c->d_pap.upload(pap, c->stream);
   HIPCHK(hipStreamSynchronize(c->stream));  // the staging vectors are locals
   c->npat = (int)id.size();
+  c->pat_ap = with_ap;
 }
 
 template <typename T>
@@ -905,6 +954,13 @@ void upload_mesh_device(scdat_ctx* c) {
       std::vector<float> a(h.dco.begin(), h.dco.end()), b(h.aci.begin(), h.aci.end());
       c->d_dco.upload(a, s);
       c->d_aci.upload(b, s);
+      // the fused iteration's copy: rows padded to a multiple of 64 columns
+      const size_t G = (size_t)h.nc, gp = (G + 63) / 64 * 64;
+      std::vector<float> bp(G * gp, 0.0f);
+      for (size_t r = 0; r < G; ++r)
+        for (size_t q = 0; q < G; ++q) bp[r * gp + q] = (float)h.aci[r * G + q];
+      c->d_acip.upload(bp, s);
+      HIPCHK(hipStreamSynchronize(s));  // the staging vectors are locals
     } else {
       c->d_dco.upload(h.dco, s);
       c->d_aci.upload(h.aci, s);
@@ -913,6 +969,8 @@ void upload_mesh_device(scdat_ctx* c) {
       b.alloc(sizeof(T) * (size_t)h.nc);
       HIPCHK(hipMemsetAsync(b.p, 0, b.bytes, s));
     }
+    c->d_cnb_off.upload(h.cnb_off, s);
+    c->d_cnb.upload(h.cnb, s);
   }
   c->d_kind.upload(h.kind, s);
   c->d_lift.upload(h.lift, s);
@@ -1009,6 +1067,16 @@ EllDev ell(scdat_ctx* c) {
     } else {
       A.dco32 = c->d_dco.as<float>();
       A.aci32 = c->d_aci.as<float>();
+      // the coarse solve runs inside the iteration kernel when the row
+      // patterns carry A P (tiled order) and the width has a fused kernel
+      if (A.npat > 0 && c->pat_ap && A.w >= 13 && A.w <= 18 && c->h.cnb_max <= kMaxTouched) {
+        A.fused = 1;
+        A.cnb_off = c->d_cnb_off.as<int32_t>();
+        A.cnb = c->d_cnb.as<int32_t>();
+        A.pap32 = c->d_pap.as<float>();
+        A.ncp = (A.nc + 63) / 64 * 64;
+        A.acip32 = c->d_acip.as<float>();
+      }
     }
   }
   // 512-row blocks halve the blocks whose partials every block sums (64^3:
@@ -2262,20 +2330,102 @@ static uint64_t morton3(const double* p, const double* lo, const double* hi) {
   return key;
 }
 
-// Mesh preparation with the nodes in the context's internal order: Morton
-// order of the coordinates (consecutive 256-node row blocks are compact
-// boxes: cache locality for the gathers and the coarse space of the
-// two-level preconditioner), or the caller's order.  perm[internal] = the
-// caller's node index.
+// Lattice indices of the nodes when they lie on a rectilinear grid (every
+// coordinate is one of a few grid-line values per axis, as on a structured
+// box mesh whose lines are graded to put a conductor's faces on them):
+// false when the grid lines would not be mostly occupied by nodes.
+static bool lattice_indices(int64_t nn, const double* xyz, std::vector<int32_t>& ijk, int32_t dims[3]) {
+  if (nn < 8) return false;
+  std::vector<double> line[3];
+  for (int r = 0; r < 3; ++r) {
+    std::vector<double> v((size_t)nn);
+    for (int64_t i = 0; i < nn; ++i) v[i] = xyz[3 * i + r];
+    std::sort(v.begin(), v.end());
+    const double ext = v.back() - v.front();
+    if (!(ext > 0.0)) return false;
+    const double eps = 1e-9 * ext;
+    for (double x : v)
+      if (line[r].empty() || x - line[r].back() > eps) line[r].push_back(x);
+    dims[r] = (int32_t)line[r].size();
+  }
+  if ((double)dims[0] * dims[1] * dims[2] > 2.0 * (double)nn) return false;
+  ijk.resize(3 * (size_t)nn);
+  for (int64_t i = 0; i < nn; ++i)
+    for (int r = 0; r < 3; ++r) {
+      const std::vector<double>& L = line[r];
+      const double x = xyz[3 * i + r];
+      // the grid line within eps below or at x
+      size_t k = (size_t)(std::upper_bound(L.begin(), L.end(), x + 1e-9 * (L.back() - L.front())) - L.begin());
+      ijk[3 * i + r] = (int32_t)(k - 1);
+    }
+  return true;
+}
+
+// Tiled order of a lattice (rectilinear grid) mesh: the lattice points with
+// every index >= 1 in tiles of 8 x 8 x 8 points (tiles in x, y, z order,
+// points inside a tile in x, y, z order), each tile filling one 512-row
+// block; a tile's missing
+// points (outside the mesh, or a removed box) are filled with nodes of the
+// index-0 layers, the rest of which follow the tiles.  A row block is then a
+// compact 8^3 box: the aggregate of the two-level coarse space.  perm[k] =
+// caller's node of internal row k.  False when the nodes are not a lattice.
+static bool tiled_perm(int64_t nn, const double* xyz, std::vector<int32_t>& perm) {
+  std::vector<int32_t> ijk;
+  int32_t dims[3];
+  if (!lattice_indices(nn, xyz, ijk, dims)) return false;
+  int32_t nt[3];
+  for (int r = 0; r < 3; ++r) nt[r] = (dims[r] - 1 + 7) / 8;
+  const int64_t ntiles = (int64_t)nt[0] * nt[1] * nt[2];
+  if (ntiles <= 0 || ntiles > (int64_t)1 << 22) return false;
+  std::vector<int32_t> slot((size_t)ntiles * 512, -1);
+  std::vector<int32_t> extra;
+  for (int64_t u = 0; u < nn; ++u) {
+    const int32_t* q = &ijk[3 * u];
+    if (q[0] == 0 || q[1] == 0 || q[2] == 0) {
+      extra.push_back((int32_t)u);
+      continue;
+    }
+    const int32_t a = q[0] - 1, b = q[1] - 1, cc = q[2] - 1;
+    const int64_t t = (a >> 3) + (int64_t)nt[0] * ((b >> 3) + (int64_t)nt[1] * (cc >> 3));
+    const int64_t k = t * 512 + (a & 7) + 8 * (b & 7) + 64 * (cc & 7);
+    if (slot[k] >= 0) extra.push_back((int32_t)u);  // two nodes at one point
+    else slot[k] = (int32_t)u;
+  }
+  perm.clear();
+  perm.reserve((size_t)nn);
+  size_t e = 0;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    bool any = false;
+    for (int k = 0; k < 512 && !any; ++k) any = slot[t * 512 + k] >= 0;
+    if (!any) continue;  // a tile with no node takes no block
+    for (int k = 0; k < 512; ++k) {
+      const int32_t u = slot[t * 512 + k];
+      if (u >= 0) perm.push_back(u);
+      else if (e < extra.size()) perm.push_back(extra[e++]);
+    }
+  }
+  while (e < extra.size()) perm.push_back(extra[e++]);
+  return (int64_t)perm.size() == nn;
+}
+
+// Mesh preparation with the nodes in the context's internal order: tiled
+// (lattice meshes; 512-row blocks are 8^3 boxes), Morton order of the
+// coordinates (consecutive row blocks are compact boxes on any mesh), or
+// the caller's order.  perm[internal] = the caller's node index.  The tiled
+// order falls back to Morton when the nodes are not a lattice.
 void mesh_setup(scdat_ctx* c, int64_t nn, const double* xyz, int64_t nt, const int32_t* tets,
-                const int32_t* kind, const double* dval, const int32_t* group, bool morton) {
+                const int32_t* kind, const double* dval, const int32_t* group, int order) {
   HostMesh& h = c->h;
   h = HostMesh();
   h.nn = nn;
   h.nt = nt;
   c->perm.clear();
   c->iperm.clear();
-  if (morton) {
+  if (order == SCDAT_ORDER_TILED && !tiled_perm(nn, xyz, c->perm)) {
+    c->perm.clear();
+    order = SCDAT_ORDER_MORTON;
+  }
+  if (order == SCDAT_ORDER_MORTON) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int64_t i = 0; i < nn; ++i)
       for (int r = 0; r < 3; ++r) {
@@ -2289,12 +2439,13 @@ void mesh_setup(scdat_ctx* c, int64_t nn, const double* xyz, int64_t nt, const i
                        return a.first < b.first;
                      });
     c->perm.resize((size_t)nn);
-    c->iperm.resize((size_t)nn);
-    for (int64_t k = 0; k < nn; ++k) {
-      c->perm[k] = key[k].second;
-      c->iperm[key[k].second] = (int32_t)k;
-    }
+    for (int64_t k = 0; k < nn; ++k) c->perm[k] = key[k].second;
   }
+  if (!c->perm.empty()) {
+    c->iperm.resize((size_t)nn);
+    for (int64_t k = 0; k < nn; ++k) c->iperm[c->perm[k]] = (int32_t)k;
+  }
+  c->order_in_use = order;
   auto user = [&](int64_t k) { return c->perm.empty() ? k : (int64_t)c->perm[k]; };
   h.xyz.resize(3 * (size_t)nn);
   h.kind.resize((size_t)nn);
@@ -2332,7 +2483,7 @@ void mesh_natural_order(scdat_ctx* c) {
   }
   for (size_t k = 0; k < tets.size(); ++k) tets[k] = c->perm[h.tets[k]];
   std::vector<int32_t> group(h.group);
-  mesh_setup(c, nn, xyz.data(), h.nt, tets.data(), kind.data(), dv.data(), group.data(), false);
+  mesh_setup(c, nn, xyz.data(), h.nt, tets.data(), kind.data(), dv.data(), group.data(), SCDAT_ORDER_GIVEN);
 }
 
 // node index conversions at the API boundary
@@ -2888,7 +3039,7 @@ int scdat_set_mesh(scdat_ctx* c, int64_t n_nodes, const double* xyz, int64_t n_t
     c->fe_mask = 0;
     c->fe_pending.fill(0.0);
     c->fe_pending_sec.fill(0.0);
-    mesh_setup(c, n_nodes, xyz, n_tets, tets, node_kind, dv.data(), fg.data(), c->node_order == SCDAT_ORDER_MORTON);
+    mesh_setup(c, n_nodes, xyz, n_tets, tets, node_kind, dv.data(), fg.data(), c->node_order);
   });
 }
 
@@ -3897,7 +4048,7 @@ int scdat_get_scenario_ledger(scdat_ctx* c, int32_t scenario, scdat_ledger_entry
 
 int scdat_set_node_order(scdat_ctx* c, int32_t order) {
   if (!c) return SCDAT_ERR_ARG;
-  if (order != SCDAT_ORDER_MORTON && order != SCDAT_ORDER_GIVEN)
+  if (order != SCDAT_ORDER_MORTON && order != SCDAT_ORDER_GIVEN && order != SCDAT_ORDER_TILED)
     return fail(c, SCDAT_ERR_ARG, "set_node_order: unknown order");
   if (c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_node_order: call before scdat_set_mesh");
   c->node_order = order;
@@ -3927,6 +4078,7 @@ int scdat_get_solver_info(scdat_ctx* c, scdat_solver_info* out) {
   out->row_patterns = A.npat;
   out->pcg_block = A.nc > 0 ? kPcgBlock : A.pb;
   out->preconditioner = A.nc > 0 ? SCDAT_PRECOND_TWO_LEVEL : SCDAT_PRECOND_JACOBI;
+  out->coarse_in_kernel = A.fused;
   return SCDAT_OK;
 }
 

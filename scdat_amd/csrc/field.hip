@@ -60,9 +60,12 @@ inline uint32_t grid_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBloc
 // block of the one-block finishing kernels.
 constexpr int kPB = kPcgBlock;
 constexpr int kRPT = 1;
-enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_WC, S_COUNT };
+enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_WC, S_WC1, S_COUNT };
 
-__device__ inline double* slot(const FieldBufs& f, int s, int G) { return f.partial + (size_t)s * G; }
+// slot s of the partials of a G-block grid; slots start 256-byte aligned
+__device__ inline double* slot(const FieldBufs& f, int s, int G) {
+  return f.partial + (size_t)s * (size_t)((G + 31) & ~31);
+}
 
 // Row blocks [b0, b1) of the segment that block `blk` belongs to, and its
 // scalars.  Sums over a segment run over its own blocks in the same order
@@ -354,6 +357,12 @@ __device__ inline void publish3(double a, double b, double c, double* pa, double
   }
 }
 
+// element at a 32-bit byte offset from a uniform base (the offset stays one
+// 32-bit register: the load takes the base from scalar registers)
+__device__ inline float at_byte(const float* __restrict__ base, uint32_t byte_off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 template <typename T>
 __device__ inline T r_next(T r, T w, T s, T a, T b) {
   return fma_t(-a, fma_t(b, s, w), r);
@@ -572,6 +581,264 @@ __global__ void __launch_bounds__(PB) k_cg_iter(EllDev A, FieldBufs f, InnerBufs
   if constexpr (TL) publish_restrict(wrest, slot(f, S_WC, G), lds);
 }
 
+// One iteration with the two-level preconditioner and the coarse solve
+// fused into it (FP32, row patterns, 512-row blocks = the aggregates).
+// The restriction R = P^T w_i of the previous kernel arrives with the dot
+// products as block partials (by parity).  Each block applies the rows of
+// Ac^-1 of the aggregates it touches (zw_i = Ac^-1 R, the same operations in
+// every block, so all blocks hold the same bits), advances zs_i = zw_i +
+// b zs_{i-1} and zr_{i+1} = zr_i - a zs_i for them, and the owner stores its
+// own.  The row then forms
+//   w_{i+1} = A u_{i+1} = (A D^-1) r_{i+1} + (A P) zr_{i+1}
+// from its pattern (column offsets, A D^-1, A P), with r_{i+1} of the
+// neighbours formed on the fly as in the Jacobi iteration.  One launch per
+// iteration, like Jacobi: no coarse kernel between iterations.
+template <int W>
+__global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_tl(EllDev A, FieldBufs f, InnerBufs<float> ib, int it) {
+  constexpr int PB = kPcgBlock;
+  constexpr int NW = PB / 64;
+  __shared__ double lds[4 * NW];
+  __shared__ double tot[3];
+  __shared__ __align__(16) float rl[kMaxCoarse];  // P^T w_i, zero past nc
+  __shared__ float zl[kMaxCoarse];  // zr_{i+1} of the aggregates this block touches, else 0
+  __shared__ int cl[kMaxTouched];   // the touched aggregates, own first
+  __shared__ float zown;            // zr_i of the block's own aggregate
+  const int G = gridDim.x;
+  const int nc = A.nc, ncp = A.ncp;
+  const int cur = it & 1, nxt = cur ^ 1;
+  const int64_t n = A.n;
+  const int bt = (int)blockIdx.x;
+  const int64_t i = (int64_t)bt * PB + threadIdx.x;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int pj = i < n ? (A.pat ? (int)A.pat[i] : (int)A.pat32[i]) : 0;
+  const int q0 = A.cnb_off[bt], K = A.cnb_off[bt + 1] - q0;
+  // the solve's scalars, read together up front (block 0 writes the other
+  // parity's history at the end; the flags are set by earlier launches)
+  PcgScalars* sc = f.sc;
+  const double g_prev = sc->hist[1 - cur], a_prev = sc->hist[2 + (1 - cur)];
+  const double tol = sc->inner_tol_eff;
+  const bool stopped = sc->inner_done || sc->outer_done || sc->total_it + it >= sc->max_it;
+  const int myc = (int)threadIdx.x < K ? A.cnb[q0 + threadIdx.x] : 0;  // in flight with the partials
+  {
+    // partials two per thread (16-byte loads; slots are 256-byte aligned, so
+    // the pair past G stays inside the slot), summed in index order; R = P^T
+    // w_i to LDS in FP32, zero up to a multiple of 512 columns (what the
+    // coarse rows' loop reads)
+    const double* pg = slot(f, cur ? S_G1 : S_G0, G);
+    const double* pd = slot(f, cur ? S_D1 : S_D0, G);
+    const double* pr = slot(f, cur ? S_RR1 : S_RR0, G);
+    const double* pw = slot(f, cur ? S_WC1 : S_WC, G);
+    const int r512 = (nc + 511) & ~511;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int j = 2 * (int)threadIdx.x; j < r512; j += 2 * PB) {
+      if (j < G) {
+        const double2 xg = *reinterpret_cast<const double2*>(pg + j);
+        const double2 xd = *reinterpret_cast<const double2*>(pd + j);
+        const double2 xr = *reinterpret_cast<const double2*>(pr + j);
+        const double2 xw = *reinterpret_cast<const double2*>(pw + j);
+        const bool two = j + 1 < G;
+        a += xg.x;
+        b += xd.x;
+        c += xr.x;
+        if (two) {
+          a += xg.y;
+          b += xd.y;
+          c += xr.y;
+        }
+        rl[j] = (float)xw.x;
+        rl[j + 1] = two ? (float)xw.y : 0.0f;
+        zl[j] = 0.0f;
+        zl[j + 1] = 0.0f;
+      } else {
+        rl[j] = 0.0f;
+        rl[j + 1] = 0.0f;
+      }
+    }
+    if ((int)threadIdx.x < K) cl[threadIdx.x] = myc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    if (lane == 0) {
+      lds[wv] = a;
+      lds[NW + wv] = b;
+      lds[2 * NW + wv] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        x += lds[k];
+        y += lds[NW + k];
+        z += lds[2 * NW + k];
+      }
+      tot[0] = x;
+      tot[1] = y;
+      tot[2] = z;
+    }
+    __syncthreads();
+  }
+  const double g = tot[0], d = tot[1], rr = tot[2];
+  double beta = 0.0, alpha;
+  if (it > 0) {
+    beta = g / g_prev;
+    alpha = g / (d - beta * g / a_prev);
+  } else {
+    alpha = g / d;
+  }
+  const bool done = stopped || (it > 0 && rr <= tol * tol) || !(alpha > 0.0) || !(alpha < 1e300);
+  if (done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->inner_done = 1;
+    return;
+  }
+  const float a = (float)alpha, b = (float)beta;
+  // zw_i = Ac^-1 R on the touched aggregates, then zs_i and zr_{i+1}: wave wv
+  // evaluates list entries wv + 2 NW m and wv + NW + 2 NW m, 16-byte loads
+  // of both rows (zero-padded to ncp columns; R is zero past nc)
+  for (int base = 0; base < K; base += 2 * NW) {
+    const int qa = base + wv, qb = qa + NW;
+    if (qa >= K) break;  // uniform in the wave
+    const bool hb = qb < K;
+    const int ca = cl[qa], cb = cl[hb ? qb : qa];
+    const float4* ra = reinterpret_cast<const float4*>(A.acip32 + (size_t)ca * ncp);
+    const float4* rb = reinterpret_cast<const float4*>(A.acip32 + (size_t)cb * ncp);
+    const float zra = ib.zr[cur][ca], zsa = ib.zs[nxt][ca];
+    const float zrb = ib.zr[cur][cb], zsb = ib.zs[nxt][cb];
+    float ta = 0.0f, tb = 0.0f;
+    for (int k0 = 0; k0 < ncp; k0 += 2 * 256) {
+      float4 xa[2], xb[2], xr[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int k = k0 + 256 * u + 4 * lane;  // first column of this lane's four
+        const int ks = k < ncp ? k / 4 : 0;     // past the row: any entry, times R = 0
+        xa[u] = ra[ks];
+        xb[u] = rb[ks];
+        xr[u] = *reinterpret_cast<const float4*>(&rl[k]);  // k < nc rounded up to 512
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        ta = fmaf(xa[u].x, xr[u].x, ta);
+        ta = fmaf(xa[u].y, xr[u].y, ta);
+        ta = fmaf(xa[u].z, xr[u].z, ta);
+        ta = fmaf(xa[u].w, xr[u].w, ta);
+        tb = fmaf(xb[u].x, xr[u].x, tb);
+        tb = fmaf(xb[u].y, xr[u].y, tb);
+        tb = fmaf(xb[u].z, xr[u].z, tb);
+        tb = fmaf(xb[u].w, xr[u].w, tb);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      ta += __shfl_xor(ta, off, 64);
+      tb += __shfl_xor(tb, off, 64);
+    }
+    if (lane == 0) {
+      const float sa = fmaf(b, zsa, ta), za = fmaf(-a, sa, zra);
+      zl[ca] = za;
+      if (ca == bt) {
+        ib.zs[cur][ca] = sa;
+        ib.zr[nxt][ca] = za;
+        zown = zra;
+      }
+      if (hb) {
+        const float sb = fmaf(b, zsb, tb), zb = fmaf(-a, sb, zrb);
+        zl[cb] = zb;
+        if (cb == bt) {
+          ib.zs[cur][cb] = sb;
+          ib.zr[nxt][cb] = zb;
+          zown = zrb;
+        }
+      }
+    }
+  }
+  const float* __restrict__ rc = ib.r[cur];
+  const float* __restrict__ wc = ib.w[cur];
+  const float* __restrict__ sp = ib.s[nxt];
+  const int32_t* po = A.poff + (size_t)pj * W;
+  // (A D^-1) r_{i+1}, r_{i+1} of the neighbours formed on the fly (32-bit
+  // byte offsets from the uniform bases: one address register per neighbour)
+  float acc = 0.0f, ri = 0.0f, wi = 0.0f, so = 0.0f, pold = 0.0f, xold = 0.0f, di = 0.0f;
+  if (i < n) {
+    const float* pv = A.pval32 + (size_t)pj * W;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const uint32_t jb = (uint32_t)((int)i + po[k]) * 4u;
+      acc = fmaf(pv[k], r_next(at_byte(rc, jb), at_byte(wc, jb), at_byte(sp, jb), a, b), acc);
+    }
+    // the row's own entries, in flight across the barrier
+    ri = rc[i];
+    wi = wc[i];
+    so = sp[i];
+    pold = ib.p[i];
+    xold = ib.x[i];
+    di = A.pdinv32[pj];
+  }
+  __syncthreads();  // zl
+  double g1 = 0.0, d1 = 0.0, rr1 = 0.0, wrest = 0.0;
+  if (i < n) {
+    // + (A P) zr_{i+1}: the table entries again (hot in L1)
+    const float* pa = A.pap32 + (size_t)pj * W;
+    float acz = 0.0f;
+#pragma unroll
+    for (int k = 0; k < W; ++k) acz = fmaf(pa[k], zl[((int)i + po[k]) / kCoarseBlockRows], acz);
+    acc += acz;
+    const bool free_row = pa[0] != 0.0f;  // slot 0 is self: A_ii on free rows
+    const float si = fmaf(b, so, wi);
+    const float rn = fmaf(-a, si, ri);
+    const float ui = free_row ? fmaf(di, ri, zown) : di * ri;
+    const float uc = free_row ? zl[bt] : 0.0f;
+    const float pi = fmaf(b, pold, ui);
+    ib.x[i] = fmaf(a, pi, xold);
+    ib.p[i] = pi;
+    ib.s[cur][i] = si;
+    ib.r[nxt][i] = rn;
+    ib.w[nxt][i] = acc;
+    const float un = fmaf(di, rn, uc);
+    g1 = (double)rn * (double)un;
+    d1 = (double)acc * (double)un;
+    rr1 = (double)rn * (double)rn;
+    if (free_row) wrest = (double)acc;
+  }
+  // block partials of the dots and of P^T w_{i+1}
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    g1 += __shfl_xor(g1, off, 64);
+    d1 += __shfl_xor(d1, off, 64);
+    rr1 += __shfl_xor(rr1, off, 64);
+    wrest += __shfl_xor(wrest, off, 64);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    lds[wv] = g1;
+    lds[NW + wv] = d1;
+    lds[2 * NW + wv] = rr1;
+    lds[3 * NW + wv] = wrest;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0, z = 0.0, q = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      x += lds[k];
+      y += lds[NW + k];
+      z += lds[2 * NW + k];
+      q += lds[3 * NW + k];
+    }
+    slot(f, nxt ? S_G1 : S_G0, G)[bt] = x;
+    slot(f, nxt ? S_D1 : S_D0, G)[bt] = y;
+    slot(f, nxt ? S_RR1 : S_RR0, G)[bt] = z;
+    slot(f, nxt ? S_WC1 : S_WC, G)[bt] = q;
+    if (blockIdx.x == 0) {
+      sc->hist[cur] = g;
+      sc->hist[2 + cur] = alpha;
+      sc->inner_it = it + 1;
+    }
+  }
+}
+
 // Closes an inner solve: records its iteration count and applies
 // x += |r| d (skipped when the outer loop has already converged).
 template <typename T, int PB>
@@ -769,7 +1036,9 @@ void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& 
   SCDAT_PB(hipLaunchKernelGGL((k_inner_init<T, PB>), dim3(G), dim3(PB), 0, s, A, f, ib));
   if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 0, 0, G);
   SCDAT_PB(hipLaunchKernelGGL((k_cg_start<T, PB>), dim3(G), dim3(PB), 0, s, A, f, ib));
-  if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 1, 0, G);
+  // fused: iteration 0 forms zw_0 from the restriction k_cg_start published
+  if (A.nc && !A.fused)
+    hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 1, 0, G);
 }
 // One iteration at block size PB (compile-time ELL width where one exists).
 template <typename T, int PB>
@@ -805,6 +1074,21 @@ template <typename T>
 void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, int it,
                        hipStream_t s, hipEvent_t* ev) {
   hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
+  if constexpr (sizeof(T) == 4) {
+    if (A.nc && A.fused) {  // two-level, coarse solve inside the iteration
+      const dim3 g(grid_rows(A.n, kPcgBlock)), b(kPcgBlock);
+      switch (A.w) {
+#define SCDAT_W(w) \
+  case w:          \
+    hipExtLaunchKernelGGL(k_cg_iter_tl<w>, g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    return;
+        SCDAT_W(13) SCDAT_W(14) SCDAT_W(15) SCDAT_W(16) SCDAT_W(17) SCDAT_W(18)
+#undef SCDAT_W
+        default:
+          break;  // ell() only fuses these widths
+      }
+    }
+  }
   if (A.nc) {  // two-level: 512-row blocks (the aggregates)
     const dim3 g(grid_rows(A.n, kPcgBlock)), b(kPcgBlock);
     switch (A.w) {
@@ -830,7 +1114,9 @@ void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, h
   const int ns = f.blk_seg ? f.n_seg : 1;
   hipLaunchKernelGGL(k_count_outer, dim3((ns + kPB - 1) / kPB), dim3(kPB), 0, s, f, ns);
 }
-size_t partial_doubles(int64_t n) { return (size_t)S_COUNT * (size_t)grid_rows(n, 256) + 16; }  // smallest block
+size_t partial_doubles(int64_t n) {  // smallest block, slots 256-byte aligned
+  return (size_t)S_COUNT * (size_t)((grid_rows(n, 256) + 31) & ~31) + 16;
+}
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s) {
   hipLaunchKernelGGL(k_float, dim3(f.blk_seg ? f.n_seg : 1), dim3(kBlock), 0, s, f, fa);
 }

@@ -15,6 +15,7 @@ namespace scdat {
 constexpr int kPcgBlock = 512;
 constexpr int kMaxCoarse = 2048;               // row blocks of the two-level preconditioner
 constexpr int kCoarseBlockRows = kPcgBlock;    // nodes per aggregate
+constexpr int kMaxTouched = 32;                // aggregates one row block may touch (fused two-level)
 
 // Scalars of one solve, resident on the device; the host polls `inner_done`
 // and `outer_done` between batches of iterations.
@@ -57,6 +58,16 @@ struct EllDev {
   const double* dco64;
   const float* aci32;      // Ac^-1, nc x nc row-major
   const double* aci64;
+  // Fused coarse solve (FP32 row patterns): block b applies the rows of
+  // Ac^-1 of the aggregates its rows touch, cnb[cnb_off[b] .. cnb_off[b+1])
+  // (its own first), inside the iteration kernel; pap32 holds per pattern
+  // slot the FP32 entry of A P (A_ij on free rows and free columns, else 0).
+  int fused;
+  const int32_t* cnb_off;
+  const int32_t* cnb;
+  const float* pap32;
+  int ncp;                 // row stride of acip32: nc rounded up to 64, zero-padded
+  const float* acip32;     // Ac^-1 in FP32 with that stride
   // Row patterns of the FP32 inner operator (npat = 0: not used).  Rows with
   // the same column offsets j - i, the same FP32 values of A D^-1 and the
   // same FP32 D^-1 share one table entry, so an inner iteration reads a

@@ -1,5 +1,6 @@
-"""Two-level Poisson preconditioner (row-block aggregates of the Morton node
-order) through the C ABI: the same fields as Jacobi-PCG and as the oracle's
+"""Two-level Poisson preconditioner (row-block aggregates of the Morton or
+tiled node order; in mixed precision with the tiled order the coarse step
+runs inside the iteration kernel) through the C ABI: the same fields as Jacobi-PCG and as the oracle's
 FP64 cycle, fewer iterations, bitwise reproducible solves; node arrays come
 back in the caller's order under the internal renumbering."""
 import numpy as np
@@ -11,12 +12,13 @@ from helpers import make_ctx, make_oracle, plasma_setup, rel_err
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("order", [S.ORDER_MORTON, S.ORDER_TILED])
 @pytest.mark.parametrize("precision", [1, 0])
-def test_two_level_matches_jacobi_and_needs_fewer_iterations(precision):
+def test_two_level_matches_jacobi_and_needs_fewer_iterations(precision, order):
     st = plasma_setup(n=24, npart=40000, seed=8)
     res = {}
     for pc in (S.PRECOND_JACOBI, S.PRECOND_TWO_LEVEL):
-        c = make_ctx(st, precision=precision, tol=1e-10, sort_interval=0, order=S.ORDER_MORTON)
+        c = make_ctx(st, precision=precision, tol=1e-10, sort_interval=0, order=order)
         c.set_preconditioner(pc)
         c.initialize()
         info = c.solver_info()
@@ -29,6 +31,8 @@ def test_two_level_matches_jacobi_and_needs_fewer_iterations(precision):
     print("coarse", two[2], "iterations jacobi", jac[3], "two-level", two[3])
     assert jac[2]["preconditioner"] == S.PRECOND_JACOBI
     assert two[2]["preconditioner"] == S.PRECOND_TWO_LEVEL and two[2]["coarse_unknowns"] >= 8
+    # mixed precision in the tiled order: the coarse step inside the iteration kernel
+    assert two[2]["coarse_in_kernel"] == int(precision == 0 and order == S.ORDER_TILED)
     assert rel_err(two[0][1], jac[0][1]) < 1e-7
     assert two[1]["floating_potential"] == pytest.approx(jac[1]["floating_potential"], rel=1e-7)
     assert all(c == 1 for c in (two[1]["pcg_converged"], jac[1]["pcg_converged"]))
