@@ -108,10 +108,10 @@ def structural_nnz(tets, n_nodes):
     return int(2 * np.unique(key).size + n_nodes)
 
 
-def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns=True, precond="jacobi"):
+def box_context(case, sort_interval, row_patterns=True, precond="two-level"):
+    """The box workload's mixed-precision context, initialized."""
     import scdat_amd as S
     m = case["mesh"]
-    t0 = time.time()
     ctx = S.Context(precision=0, dt=case["dt"], pcg_tol=1e-8, sort_interval=sort_interval)
     if precond == "two-level":
         ctx.set_node_order(S.ORDER_TILED)
@@ -124,15 +124,22 @@ def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns
         ctx.set_particles(sid, x, v, c)
     ctx.row_patterns = row_patterns
     ctx.initialize()
+    return ctx
+
+
+def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns=True, precond="two-level"):
+    import scdat_amd as S
+    t0 = time.time()
+    ctx = box_context(case, sort_interval, row_patterns, precond)
     chunks = ctx.autotune()
     log(f"gpu setup {time.time() - t0:.1f}s (push tile: {chunks} particles per thread); warmup {warmup} steps")
     for _ in range(warmup):
         ctx.step(1)
     ctx.sync()
     st0 = ctx.stats()
-    # every push launch and every 16th PCG iteration carry dispatch-timestamp
+    # every push launch and every 64th PCG iteration carry dispatch-timestamp
     # events (timing every launch would slow the host's launch loop)
-    ctx.set_kernel_timing(kernel_timing, period=16)
+    ctx.set_kernel_timing(kernel_timing, period=64)
     t1 = time.perf_counter()
     ctx.step(steps)
     ctx.sync()
@@ -172,7 +179,9 @@ def run_cpu_baseline(case, steps, threads):
         alive.append(sum(int((o.get_particles(s)[2] >= 0).sum()) for s in range(2)))
     pushes = sum(alive[:-1])
     log(f"cpu baseline: setup {setup:.1f}s, {steps} steps {wall:.2f}s, threads {oracle.lib().orc_num_threads()}")
-    return dict(wall=wall, pushes=pushes, threads=oracle.lib().orc_num_threads())
+    _, phi, _ = o.fields()
+    return dict(wall=wall, pushes=pushes, threads=oracle.lib().orc_num_threads(), steps=1 + steps, phi=phi,
+                vfloat=o.floating_potential())
 
 
 def run_sweep(args):
@@ -564,7 +573,7 @@ def main():
     ap.add_argument("--sweep-seed", type=int, default=20261206)
     ap.add_argument("--row-patterns", type=int, choices=(0, 1), default=1,
                     help="1: the FP32 inner iteration may read the operator as row patterns")
-    ap.add_argument("--precond", choices=("jacobi", "two-level"), default="jacobi",
+    ap.add_argument("--precond", choices=("jacobi", "two-level"), default="two-level",
                     help="Poisson preconditioner; two-level uses the tiled node order")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -607,21 +616,45 @@ def main():
     #    value 8 B per structural entry, D^-1 4 B per row) or as row patterns
     #    (2-byte id per row, 4-byte past 65536 patterns, plus the table once:
     #    W offsets and values and D^-1 per pattern)
+    #  tile-staged PCG iteration: a 2-byte box pattern id per row, the table
+    #    once (16 direction codes, W values of A D^-1 and of A P, D^-1 per
+    #    pattern) and 27 neighbour blocks per row block
+    #  two-level coarse step (inside the iteration): Ac^-1 once (nc rows of
+    #    ncp FP32 columns), zr and zs, the 4 G partials
     push_launch = g["pushes"] / max(1, kt["push"][1]) if kt["push"][1] else 0
     sv = g["solver"]
     npat, width = sv["row_patterns"], sv["ell_width"]
-    id_bytes = 2.0 if npat <= 65536 else 4.0
-    op_bytes = (id_bytes * n_nodes + npat * (8.0 * width + 4.0)) if npat else (8.0 * nnz + 4.0 * n_nodes)
+    nblk = -(-n_nodes // 512)
+    if sv["tile_staged"]:
+        op_bytes = 2.0 * n_nodes + sv["box_patterns"] * (16.0 + 8.0 * width + 4.0) + 27 * 4.0 * nblk
+    elif npat:
+        id_bytes = 2.0 if npat <= 65536 else 4.0
+        op_bytes = id_bytes * n_nodes + npat * ((12.0 if sv["coarse_in_kernel"] else 8.0) * width + 4.0)
+    else:
+        op_bytes = 8.0 * nnz + 4.0 * n_nodes
+    nc = sv["coarse_unknowns"]
+    coarse_bytes = (4.0 * nc * (-(-nc // 64) * 64) + 8.0 * nc + 32.0 * nblk) if sv["coarse_in_kernel"] else 0.0
     bytes_k = {
         "push": 56.0 * push_launch + 64.0 * n_tets + 24.0 * n_nodes,
-        "pcg_iter": op_bytes + 40.0 * n_nodes,
+        "pcg_iter": op_bytes + 40.0 * n_nodes + coarse_bytes,
     }
-    traffic = {}
+    # the reference's CSR iteration at this size (8 B per structural entry,
+    # row pointers, seven vectors of 8 B): the bytes the kernel replaces
+    csr_bytes = 8.0 * nnz + 4.0 * (n_nodes + 1) + 56.0 * n_nodes
+    pcg_kernel = ("k_cg_iter_box<%d,%s>" % (width, "true" if sv["coarse_in_kernel"] else "false")
+                  if sv["tile_staged"] else
+                  "k_cg_iter<float,W,false,%s,%d>" % ("true" if npat else "false", sv["pcg_block"]))
+    traffic, traffic_src = {}, {}
     if os.path.exists(TRAFFIC_FILE):
         tf = json.load(open(TRAFFIC_FILE))
-        traffic = dict(tf.get("traffic_bytes_per_launch", {}))
-        if bool(tf.get("row_patterns", False)) != bool(npat):
-            traffic.pop("pcg_iter", None)  # measured with the other operator form
+        kn = tf.get("kernel_names", {})
+        for k, v in tf.get("traffic_bytes_per_launch", {}).items():
+            if k == "pcg_iter" and kn.get(k) != pcg_kernel:
+                continue  # measured on another form of the iteration
+            traffic[k] = v
+            traffic_src[k] = ("profiles/pmc_traffic_64cube.json: FETCH_SIZE + WRITE_SIZE per launch of %s, "
+                              "scaled by a device copy (commands in profiles/pmc_traffic_64cube_16M.md)"
+                              % kn.get(k, k))
     # launches that did work in the timed region (PCG: one per inner iteration;
     # every 16th carries timing events)
     iters = g["stats"]["pcg_iterations_total"] - g["stats0"]["pcg_iterations_total"]
@@ -630,12 +663,16 @@ def main():
     for k, (ms, n) in kt.items():
         gbs = bytes_k[k] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         kernels[k] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": gbs / HBM_PEAK_GBS, "traffic": traffic.get(k), "avg_launch_ms": ms,
-                      "launches": launches[k], "launches_timed": n,
+                      "frac": gbs / HBM_PEAK_GBS, "traffic": traffic.get(k), "traffic_source": traffic_src.get(k),
+                      "avg_launch_ms": ms, "launches": launches[k], "launches_timed": n,
                       "algorithmic_bytes_per_launch": bytes_k[k], "total_ms": ms * launches[k]}
+        if traffic.get(k) and ms > 0:
+            kernels[k]["moved_frac"] = traffic[k] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    if kt["pcg_iter"][0] > 0:
+        kernels["pcg_iter"]["csr_bytes_per_launch"] = csr_bytes
+        kernels["pcg_iter"]["csr_frac"] = csr_bytes / (kt["pcg_iter"][0] * 1e-3) / 1e9 / HBM_PEAK_GBS
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    names = {"push": "push_kernel<float,CH>",
-             "pcg_iter": "k_cg_iter<float,W,false,%s,%d>" % ("true" if npat else "false", sv["pcg_block"])}
+    names = {"push": "push_kernel<float,CH>", "pcg_iter": pcg_kernel}
     roof = dict(kernels[dominant])
     roof["kernel"] = names[dominant]
     st = g["stats"]
@@ -668,7 +705,9 @@ def main():
         "stage_ms_total": g["stages"],
         "launch_shape": {"push_particles_per_thread": g["chunks"], "chosen_by": "scdat_autotune"},
         "solver": {"rows": sv["rows"], "ell_width": width, "row_patterns": npat, "pcg_block": sv["pcg_block"],
-                   "preconditioner": "jacobi" if sv["preconditioner"] == S.PRECOND_JACOBI else "two-level"},
+                   "preconditioner": "jacobi" if sv["preconditioner"] == S.PRECOND_JACOBI else "two-level",
+                   "coarse_unknowns": nc, "coarse_in_kernel": sv["coarse_in_kernel"],
+                   "tile_staged": sv["tile_staged"], "box_patterns": sv["box_patterns"]},
         "floating_potential_v": st["floating_potential"],
         "pcg_initial_rel_residual": st["pcg_initial_rel_residual"],
     }
@@ -679,6 +718,20 @@ def main():
                                "sample": "%d full steps of the same workload (same inputs, after one warm "
                                          "step), FP64 oracle with OpenMP, Jacobi-PCG to 1e-8" % args.cpu_steps}
         res["gpu_over_cpu"] = value / res["cpu_baseline"]["value"]
+        # this path against the FP64 oracle over the oracle's own steps
+        ctx.close()
+        chk = box_context(case, args.sort_interval, bool(args.row_patterns), args.precond)
+        chk.step(c["steps"])
+        _, phi_g, _ = chk.fields()
+        vf_g = chk.stats()["floating_potential"]
+        chk.close()
+        res["error_vs_fp64_oracle"] = {
+            "steps": c["steps"],
+            "phi_rel_l2": float(np.linalg.norm(phi_g - c["phi"]) / np.linalg.norm(c["phi"])),
+            "floating_potential_v": vf_g, "oracle_floating_potential_v": c["vfloat"],
+            "floating_potential_rel": abs(vf_g - c["vfloat"]) / abs(c["vfloat"]),
+            "note": "same inputs, this path (mixed precision) and the FP64 oracle from the initial state; "
+                    "plasma trajectories decorrelate slowly, so the figure grows with the step count"}
     print(json.dumps(res), flush=True)
 
 

@@ -610,6 +610,8 @@ typedef struct {
   int32_t row_patterns;     /* FP32 inner operator as row patterns (0: ELL)   */
   int32_t pcg_block;        /* rows (= threads) of a PCG row block            */
   int32_t coarse_in_kernel; /* two-level coarse step inside the iteration kernel */
+  int32_t tile_staged;      /* iteration stages tiles in LDS (box patterns)   */
+  int32_t box_patterns;     /* distinct rows of the tile-staged iteration     */
 } scdat_solver_info;
 int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
 
@@ -637,6 +639,13 @@ int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
  * the solve's reductions: a batched case equals its stand-alone run bit for
  * bit when both use 512.                                                   */
 #define SCDAT_SHAPE_PCG_BLOCK 2
+/* SCDAT_SHAPE_TILE_STAGING: 1 (default) lets the mixed-precision iteration
+ * in SCDAT_ORDER_TILED stage each 512-row tile's neighbour values in LDS (a
+ * 10^3 box per tile, read by direction codes) instead of gathering W
+ * neighbours per row, where every tile fills its row block and every
+ * operator entry joins lattice neighbours; 0 keeps the row patterns.  Same
+ * bits either way.                                                        */
+#define SCDAT_SHAPE_TILE_STAGING 3
 int scdat_set_launch_shape(scdat_ctx* ctx, int32_t what, int32_t value);
 int scdat_get_launch_shape(scdat_ctx* ctx, int32_t what, int32_t* value);
 int scdat_autotune(scdat_ctx* ctx);
@@ -743,8 +752,9 @@ int scdat_get_push_kernel_time(scdat_ctx* ctx, double* avg_ms, int64_t* launches
 /* Per-kernel timing from the kernels' own dispatch timestamps (HIP events
  * attached to the launches on the context's stream; off by default, enabling
  * resets the sums).  period 0: off; n > 0: every push launch and every n-th
- * PCG iteration launch is timed (attached events cost host time, so a
- * period of 8-16 keeps the measured loop close to the untimed one).
+ * PCG iteration launch (counted across solves, skipping the first five of
+ * each batch) is timed; a timed launch leaves about 5 us gaps on both sides,
+ * so a period of 32-64 keeps the measured loop close to the untimed one.
  * which: 0 push, 1 PCG iteration (fused SpMV + vector update); averages over
  * timed launches that did work since timing was enabled.                   */
 #define SCDAT_KERNEL_PUSH 0

@@ -121,13 +121,15 @@ ORDER_MORTON, ORDER_GIVEN, ORDER_TILED = 0, 1, 2
 class SolverInfo(C.Structure):
     _fields_ = [("rows", C.c_int64), ("nonzeros", C.c_int64), ("ell_width", C.c_int32),
                 ("preconditioner", C.c_int32), ("coarse_unknowns", C.c_int32), ("row_patterns", C.c_int32),
-                ("pcg_block", C.c_int32), ("coarse_in_kernel", C.c_int32)]
+                ("pcg_block", C.c_int32), ("coarse_in_kernel", C.c_int32),
+                ("tile_staged", C.c_int32), ("box_patterns", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 SHAPE_PUSH_CHUNKS = 0
 SHAPE_ROW_PATTERNS = 1
 SHAPE_PCG_BLOCK = 2
+SHAPE_TILE_STAGING = 3
 
 
 class ScenarioStats(C.Structure):
@@ -710,6 +712,19 @@ class Context:
     @pcg_block.setter
     def pcg_block(self, rows):
         self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_PCG_BLOCK, int(rows)))
+
+    @property
+    def tile_staging(self):
+        """1 (default): in the tiled order the mixed-precision iteration stages
+        each tile's neighbour values in LDS where the mesh allows; 0: row
+        patterns.  Same bits either way."""
+        v = C.c_int32()
+        self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_TILE_STAGING, C.byref(v)))
+        return v.value
+
+    @tile_staging.setter
+    def tile_staging(self, mode):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_TILE_STAGING, int(mode)))
 
     def charging_forward(self, params, steps, events=EVENTS_FREE):
         """Floating potential after `steps` steps of the differentiable charging

@@ -107,6 +107,13 @@ struct HostMesh {
   // aggregates each row block touches (its own first), CSR over the blocks
   std::vector<int32_t> cnb_off, cnb;
   int cnb_max = 0;
+  // tiled order: tile (x + nx (y + ny z)) of each of the first tile_blocks
+  // row blocks, the tile grid, and whether every tile fills its block slot
+  // for slot (a tile's point (x, y, z) at row x + 8 y + 64 z of its block)
+  std::vector<int32_t> tile_of_block;
+  int32_t tdim[3] = {0, 0, 0};
+  int64_t tile_blocks = 0;
+  bool tiles_aligned = false;
 };
 
 // Coarse space of the two-level preconditioner: aggregate b = the free rows
@@ -646,6 +653,12 @@ struct scdat_ctx {
   DevBuf d_pap;                           // and their A P entries (two-level, tiled order)
   bool pat_ap = false;
   DevBuf d_cnb_off, d_cnb;                // aggregates each row block touches
+  // tile-staged iteration (tiled order): per-row box pattern ids, the
+  // table (neighbour directions, A D^-1, A P, D^-1) and each block's 27
+  // neighbour blocks
+  DevBuf d_bpat, d_bd, d_bv, d_bap, d_bdi, d_nbb;
+  int nbpat = 0;
+  bool use_box = true;
   int npat = 0;                           // 0: none (too many, or FP64 inner solve)
   bool pat_wide = false;                  // 4-byte pattern ids
   bool pat_force_wide = false;            // 4-byte ids whatever the count (tests)
@@ -748,7 +761,8 @@ struct scdat_ctx {
   std::vector<hipEvent_t> ev_pool;
   // per-kernel timing (HIP events on the context stream), enabled on request:
   // [0] push, [1] PCG iteration; sums in ms over executed launches
-  int kernel_timing = 0;  // 0 off, else time every kernel_timing-th PCG iteration
+  int kernel_timing = 0;  // 0 off, else time every kernel_timing-th PCG iteration launch
+  int64_t kt_seq = 0;     // PCG iteration launches so far (the sampling count)
   std::vector<hipEvent_t> kt_pool;
   double kt_sum[2] = {0, 0};
   int64_t kt_n[2] = {0, 0};
@@ -865,6 +879,117 @@ void build_row_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::v
   c->pat_ap = with_ap;
 }
 
+// Box patterns of the tile-staged iteration (tiled order, every tile in its
+// block slot).  Row i of block b sits at point (x, y, z) of b's tile (slot
+// x + 8 y + 64 z); every column it reads with a non-zero entry must be a
+// lattice neighbour p + delta, delta in {-1, 0, 1}^3, so the iteration can
+// take r_{i+1} of all of them from the block's 10^3 box of staged values.
+// Its pattern holds, per ELL slot, the direction code (delta + 1 in two bits
+// per axis, x lowest; 21 = the row itself, also for zero entries), the FP32
+// A D^-1 entry, the
+// FP32 A P entry (two-level coarse space; 0 otherwise) and D^-1: the same
+// values in the same slot order as the row patterns, so both iterations give
+// the same bits.  Rows of blocks past the tiles (constrained nodes of the
+// index-0 layers) must have only their diagonal.  Leaves nbpat = 0 when a
+// condition fails or the table would need more than 65536 entries.
+void build_box_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::vector<float>& di) {
+  const HostMesh& h = c->h;
+  c->nbpat = 0;
+  const int64_t n = h.nn;
+  const int W = h.W;
+  if (c->order_in_use != SCDAT_ORDER_TILED || !h.tiles_aligned || h.tile_blocks == 0 || W > 16 || W < 7) return;
+  const int64_t G = (n + kPcgBlock - 1) / kPcgBlock;
+  const int64_t nx = h.tdim[0], ny = h.tdim[1], nz = h.tdim[2];
+  std::vector<int32_t> block_of_tile((size_t)(nx * ny * nz), -1);
+  for (int64_t b = 0; b < h.tile_blocks; ++b) block_of_tile[h.tile_of_block[b]] = (int32_t)b;
+  std::vector<int32_t> nbb((size_t)G * 27, -1);
+  for (int64_t b = 0; b < G; ++b) {
+    nbb[b * 27 + 13] = (int32_t)b;
+    if (b >= h.tile_blocks) continue;
+    const int64_t t = h.tile_of_block[b];
+    const int64_t tx = t % nx, ty = (t / nx) % ny, tz = t / (nx * ny);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int64_t ux = tx + dx, uy = ty + dy, uz = tz + dz;
+          if (ux < 0 || uy < 0 || uz < 0 || ux >= nx || uy >= ny || uz >= nz) continue;
+          nbb[b * 27 + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = block_of_tile[ux + nx * (uy + ny * uz)];
+        }
+  }
+  auto tile_xyz = [&](int64_t b, int64_t* q) {
+    const int64_t t = h.tile_of_block[b];
+    q[0] = t % nx;
+    q[1] = (t / nx) % ny;
+    q[2] = t / (nx * ny);
+  };
+  const bool with_ap = h.nc > 0;
+  std::unordered_map<std::string, int> id;
+  id.reserve(1024);
+  std::vector<uint16_t> pat((size_t)n);
+  std::vector<uint8_t> pd;
+  std::vector<float> pv, pap, pdi;
+  const int per = 9;  // key bytes per slot: direction, A D^-1, A P
+  std::string key((size_t)(per * W + 4), '\0');
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t b = i / kPcgBlock, sl_i = i % kPcgBlock;
+    const bool tile_row = b < h.tile_blocks;
+    int64_t ti[3] = {0, 0, 0};
+    if (tile_row) tile_xyz(b, ti);
+    const int64_t pi[3] = {8 * ti[0] + (sl_i & 7), 8 * ti[1] + ((sl_i >> 3) & 7), 8 * ti[2] + (sl_i >> 6)};
+    const bool free_i = h.kind[i] == SCDAT_NODE_FREE;
+    char* k = key.data();
+    for (int sl = 0; sl < W; ++sl) {
+      const size_t o = (size_t)sl * n + i;
+      const int32_t j = h.col[o];
+      const float ap = with_ap && free_i && h.kind[j] == SCDAT_NODE_FREE ? (float)h.aval[o] : 0.0f;
+      uint8_t code = 21;
+      if (!(vh[o] == 0.0f && ap == 0.0f) && j != i) {
+        const int64_t bj = j / kPcgBlock, sj = j % kPcgBlock;
+        if (!tile_row || bj >= h.tile_blocks) return;  // only the diagonal off the tiles
+        int64_t tj[3];
+        tile_xyz(bj, tj);
+        const int64_t pj[3] = {8 * tj[0] + (sj & 7), 8 * tj[1] + ((sj >> 3) & 7), 8 * tj[2] + (sj >> 6)};
+        int dcode = 0;
+        for (int r = 0; r < 3; ++r) {
+          const int64_t d = pj[r] - pi[r];
+          if (d < -1 || d > 1) return;  // not a lattice neighbour
+          dcode |= (int)(d + 1) << (2 * r);
+        }
+        code = (uint8_t)dcode;
+      }
+      const float v = vh[o];
+      k[per * sl] = (char)code;
+      std::memcpy(k + per * sl + 1, &v, 4);
+      std::memcpy(k + per * sl + 5, &ap, 4);
+    }
+    std::memcpy(k + per * W, &di[i], 4);
+    auto it = id.find(key);
+    if (it == id.end()) {
+      if (id.size() >= 65536) return;
+      it = id.emplace(key, (int)id.size()).first;
+      for (int sl = 0; sl < 16; ++sl) pd.push_back(sl < W ? (uint8_t)k[per * sl] : (uint8_t)21);
+      for (int sl = 0; sl < W; ++sl) {
+        float v, ap;
+        std::memcpy(&v, k + per * sl + 1, 4);
+        std::memcpy(&ap, k + per * sl + 5, 4);
+        pv.push_back(v);
+        pap.push_back(ap);
+      }
+      pdi.push_back(di[i]);
+    }
+    pat[(size_t)i] = (uint16_t)it->second;
+  }
+  hipStream_t s = c->stream;
+  c->d_bpat.upload(pat, s);
+  c->d_bd.upload(pd, s);
+  c->d_bv.upload(pv, s);
+  c->d_bap.upload(pap, s);
+  c->d_bdi.upload(pdi, s);
+  c->d_nbb.upload(nbb, s);
+  HIPCHK(hipStreamSynchronize(s));  // the staging vectors are locals
+  c->nbpat = (int)id.size();
+}
+
 template <typename T>
 void upload_mesh_device(scdat_ctx* c) {
   HostMesh& h = c->h;
@@ -934,6 +1059,7 @@ void upload_mesh_device(scdat_ctx* c) {
     for (size_t o = 0; o < vh.size(); ++o) f32[o] = (float)vh[o];
     c->d_vh32.upload(f32, s);
     build_row_patterns(c, f32, di32);
+    build_box_patterns(c, f32, di32);
   } else {
     c->d_vh64.upload(vh, s);
   }
@@ -1058,6 +1184,15 @@ EllDev ell(scdat_ctx* c) {
     A.gy32 = c->d_g[1].as<float>();
     A.gz32 = c->d_g[2].as<float>();
   }
+  if (c->use_box && c->nbpat > 0 && !c->segmented() && !c->is_fp64() && A.w >= 13 && A.w <= 16) {
+    A.box = 1;
+    A.bpat = c->d_bpat.as<uint16_t>();
+    A.bd = c->d_bd.as<uint8_t>();
+    A.bv = c->d_bv.as<float>();
+    A.bap = c->d_bap.as<float>();
+    A.bdi = c->d_bdi.as<float>();
+    A.nbb = c->d_nbb.as<int32_t>();
+  }
   // two-level preconditioner: one scenario only (the coarse space spans the mesh)
   if (c->precond == SCDAT_PRECOND_TWO_LEVEL && c->h.nc > 0 && !c->segmented()) {
     A.nc = c->h.nc;
@@ -1068,12 +1203,14 @@ EllDev ell(scdat_ctx* c) {
       A.dco32 = c->d_dco.as<float>();
       A.aci32 = c->d_aci.as<float>();
       // the coarse solve runs inside the iteration kernel when the row
-      // patterns carry A P (tiled order) and the width has a fused kernel
-      if (A.npat > 0 && c->pat_ap && A.w >= 13 && A.w <= 18 && c->h.cnb_max <= kMaxTouched) {
+      // patterns or the box patterns carry A P (tiled order) and the width
+      // has a fused kernel
+      const bool rows_ap = A.npat > 0 && c->pat_ap && A.w >= 13 && A.w <= 18;
+      if ((rows_ap || A.box) && c->h.cnb_max <= kMaxTouched) {
         A.fused = 1;
         A.cnb_off = c->d_cnb_off.as<int32_t>();
         A.cnb = c->d_cnb.as<int32_t>();
-        A.pap32 = c->d_pap.as<float>();
+        if (rows_ap) A.pap32 = c->d_pap.as<float>();
         A.ncp = (A.nc + 63) / 64 * 64;
         A.acip32 = c->d_acip.as<float>();
       }
@@ -1082,8 +1219,10 @@ EllDev ell(scdat_ctx* c) {
   // 512-row blocks halve the blocks whose partials every block sums (64^3:
   // 12.0 -> 11.0 us per iteration) but leave a small mesh's grid too narrow
   // (32^3, 36k rows: the charging adjoint 1.09 -> 1.63 s)
+  // the tile-staged iteration needs the tiles' 512-row blocks
+  if (A.box && c->pcg_block == 256) A.box = 0;
   const bool small = A.n < (int64_t)kPcgBlock * 256;
-  if (!c->segmented() && A.nc == 0 && (c->pcg_block == 256 || (c->pcg_block == 0 && small))) A.pb = 256;
+  if (!c->segmented() && A.nc == 0 && !A.box && (c->pcg_block == 256 || (c->pcg_block == 0 && small))) A.pb = 256;
   return A;
 }
 
@@ -1161,81 +1300,112 @@ void read_scalars(scdat_ctx* c) {
 }
 
 // PCG driver: b64 and the scalars' bnorm2 are already set (k_rhs or host).
-// Iterations are launched in growing batches; kernels after convergence
-// return immediately, the host polls the done flags between batches.
-template <typename T>
 // `hist` (optional, >= max_outer + 1 entries) holds the inner iteration count
-// of each outer pass of the previous solve; the first batch of a pass is sized
-// from it so that a typical pass needs one host poll.
+// of each outer pass of the previous solve.  A pass it predicts (more than 16
+// iterations) is launched as one batch of that count + 2 followed by its
+// apply, with no host poll: the launches past convergence exit at once, and
+// an inner solve cut short by the batch is an inexact inner solve that the
+// FP64 outer loop corrects.  The host reads the flags when the outer loop
+// runs past the predicted passes, then polls batch by batch as before; a
+// typical solve costs one host poll.  With segments (batched scenarios) a
+// predicted pass is launched without reading the residual first, but its
+// inner loop is polled.
+template <typename T>
 void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, int max_outer,
                int* hist) {
   const EllDev A = ell(c);
   const InnerBufs<T> ib = ibufs<T>(c);
   hipStream_t s = c->stream;
-  // scalar controls (every segment)
-  {
-    std::vector<PcgScalars> ctl(n_scalars(c));
-    HIPCHK(hipMemcpyAsync(ctl.data(), f.sc, sizeof(PcgScalars) * ctl.size(), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    for (auto& q : ctl) {
-      q.tol = tol;
-      q.inner_tol = inner_tol;
-      q.max_it = c->cfg.pcg_max_iter;
-      q.max_outer = max_outer;
+  launch_set_controls(f, (int)n_scalars(c), tol, inner_tol, c->cfg.pcg_max_iter, max_outer, s);
+  const bool chain = hist && !c->segmented();
+  // kernel-timing samples: (pass, iteration, event pair); a sample counts if
+  // its launch ran an iteration (below that pass's count)
+  struct Sample {
+    int pass, it;
+  };
+  std::vector<Sample> samples;
+  size_t judged = 0;
+  auto judge = [&](size_t upto, int pass, int count) {
+    for (; judged < upto; ++judged) {
+      const Sample& q = samples[judged];
+      if (q.pass != pass || q.it >= count) continue;
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->kt_pool[2 * judged], c->kt_pool[2 * judged + 1]));
+      c->kt_sum[1] += ms;
+      c->kt_n[1] += 1;
     }
-    HIPCHK(hipMemcpyAsync(f.sc, ctl.data(), sizeof(PcgScalars) * ctl.size(), hipMemcpyHostToDevice, s));
-  }
+  };
+  bool fresh = false;  // h_sc was read after the last launch
   int passes = 0;
   for (int outer = 0; outer <= max_outer; ++outer) {
     launch_resid64(A, f, s);
-    // Where the previous solve ran an inner pass at this index, this one
-    // most likely does too: launch it without waiting for the residual's
-    // read-back (its kernels exit at once if the outer loop has converged).
+    fresh = false;
     const bool spec = hist && hist[outer] > 16;
     if (!spec) {
       read_scalars(c);
-      if (outer == 0) {
+      fresh = true;
+      if (chain && c->h_sc->stall) {
+        // a predicted pass ran out of its batch before its inner solve
+        // stopped (the launches after it did nothing): resume it where it
+        // stands, as a polled pass would have gone on
+        const int pass = c->h_sc->outer_it;
+        int it = c->h_sc->inner_it, batch = 4;
+        launch_clear_stall(f, s);
+        for (;;) {
+          for (int k = 0; k < batch; ++k, ++it) launch_inner_iter<T>(A, f, ib, it, s, nullptr);
+          read_scalars(c);
+          if (c->h_sc->inner_done) break;
+          batch = std::min(batch * 2, 64);
+        }
+        launch_apply<T>(A, f, ib, s);
+        fresh = false;
+        outer = pass;  // the loop goes on with the next pass's residual
+        continue;
+      }
+      if (outer == 0 && !chain) {
         const double bn0 = std::sqrt(c->h_sc->bnorm2);
         c->last_init_rel = bn0 > 0 ? c->h_sc->rnorm / bn0 : 0.0;
       }
       if (c->h_sc->outer_done) break;
     }
     launch_inner_init<T>(A, f, ib, s);
+    fresh = false;
     int it = 0, batch = 8;
-    // the previous solve's count + 2: most passes end in this batch, and the
-    // few launches past convergence exit at once (cheaper than a host poll)
-    if (hist && hist[outer] > 16) batch = std::min(1024, hist[outer] + 2);
-    for (;;) {
-      // timed launches of this batch (every kernel_timing-th iteration)
-      std::vector<int> timed;
-      for (int k = 0; k < batch; ++k, ++it) {
-        // sampled away from batch starts (the first launch after a host poll
-        // finds an idle GPU and is not representative)
-        if (c->kernel_timing > 0 && (it + 5) % c->kernel_timing == 0) {
-          const size_t j = timed.size();
+    // the previous solve's count + 2 (polled) or + a sixteenth, 3 to 8
+    // (launched without a poll: a pass that runs out of its batch stalls the
+    // solve until the host resumes it, and the next pass's launches run
+    // idle): most passes end in this batch
+    if (spec) batch = std::min(1024, hist[outer] + (chain ? std::max(3, std::min(8, hist[outer] / 16)) : 2));
+    auto launch_batch = [&](int count) {
+      for (int k = 0; k < count; ++k, ++it) {
+        // every kernel_timing-th launch, away from batch starts (the first
+        // launch after a host poll finds an idle GPU and is not representative)
+        const bool sample = c->kernel_timing > 0 && it >= 5 && ++c->kt_seq % c->kernel_timing == 0;
+        if (sample) {
+          const size_t j = samples.size();
           hipEvent_t ev[2] = {take_kt_event(c, 2 * j), take_kt_event(c, 2 * j + 1)};
           launch_inner_iter<T>(A, f, ib, it, s, ev);
-          timed.push_back(it);
+          samples.push_back({outer, it});
         } else {
           launch_inner_iter<T>(A, f, ib, it, s, nullptr);
         }
       }
+    };
+    if (chain && spec) {
+      launch_batch(batch);
+      launch_apply<T>(A, f, ib, s);
+      continue;
+    }
+    for (;;) {
+      launch_batch(batch);
       read_scalars(c);
+      fresh = true;
       if (spec && it == batch) {
         if (outer == 0) {
           const double bn0 = std::sqrt(c->h_sc->bnorm2);
           c->last_init_rel = bn0 > 0 ? c->h_sc->rnorm / bn0 : 0.0;
         }
         if (c->h_sc->outer_done) break;  // converged before this pass: nothing ran
-      }
-      // iterations below inner_it ran; later launches of the batch exited
-      // early and are not counted
-      for (size_t j = 0; j < timed.size(); ++j) {
-        if (timed[j] >= c->h_sc->inner_it) break;
-        float a = 0;
-        HIPCHK(hipEventElapsedTime(&a, c->kt_pool[2 * j], c->kt_pool[2 * j + 1]));
-        c->kt_sum[1] += a;
-        c->kt_n[1] += 1;
       }
       if (c->h_sc->inner_done) break;
       batch = batch > 64 ? 4 : std::min(batch * 2, 64);  // after a predicted batch: small steps
@@ -1244,19 +1414,36 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
       if (hist) hist[outer] = 0;  // no speculation at this index next time
       break;
     }
+    if (!chain) judge(samples.size(), outer, c->h_sc->inner_it);
     ++passes;
     if (hist) hist[outer] = c->h_sc->inner_it;
     launch_apply<T>(A, f, ib, s);
+    fresh = false;
   }
-  if (hist)
+  if (!fresh) read_scalars(c);
+  const PcgScalars& q = *c->h_sc;
+  if (chain) {
+    // every pass's count from the device: the samples and the next prediction
+    for (size_t j = 0; j < samples.size(); ++j) {
+      const Sample& sm = samples[j];
+      if (sm.pass >= 16 || sm.pass >= q.outer_it || sm.it >= q.pass_it[sm.pass]) continue;
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->kt_pool[2 * j], c->kt_pool[2 * j + 1]));
+      c->kt_sum[1] += ms;
+      c->kt_n[1] += 1;
+    }
+    for (int k = 0; k <= max_outer; ++k) hist[k] = k < q.outer_it && k < 16 ? q.pass_it[k] : 0;
+    const double bn0 = std::sqrt(q.bnorm2);
+    c->last_init_rel = bn0 > 0 ? q.init_rnorm / bn0 : 0.0;
+  } else if (hist) {
     for (int k = passes + 1; k <= max_outer; ++k) hist[k] = 0;
-  read_scalars(c);
-  c->last_iters = c->h_sc->total_it;
-  c->iters_total += c->h_sc->total_it;
-  c->last_outer = c->h_sc->outer_it;
-  const double bn = std::sqrt(c->h_sc->bnorm2);
-  c->last_rel = bn > 0 ? c->h_sc->rnorm / bn : c->h_sc->rnorm;
-  c->last_conv = c->h_sc->failed ? 0 : 1;
+  }
+  c->last_iters = q.total_it;
+  c->iters_total += q.total_it;
+  c->last_outer = q.outer_it;
+  const double bn = std::sqrt(q.bnorm2);
+  c->last_rel = bn > 0 ? q.rnorm / bn : q.rnorm;
+  c->last_conv = q.failed ? 0 : 1;
 }
 
 // Unit-potential solution: the nodes with in_set[i] at 1 V, every other
@@ -1345,6 +1532,7 @@ void solve_phi1(scdat_ctx* c) {
 FloatArgs float_args(scdat_ctx* c) {
   FloatArgs fa{};
   fa.q_init = c->q_init;
+  fa.ccap = c->ccap;
   fa.ledger_count = c->d_ledger_count.as<unsigned long long>();
   fa.emit_count = c->d_emit_count.as<unsigned long long>();
   for (size_t s = 0; s < c->sp.size() && s < (size_t)kMaxSpecies; ++s) fa.qw[s] = c->sp[s].q * c->sp[s].w;
@@ -1380,11 +1568,7 @@ void do_solve(scdat_ctx* c, int rho_idx) {
     c->failed_solves++;
     if (c->batch_failed++ == 0) c->first_failed_step = c->steps;
   }
-  if (c->h.has_cond && c->n_patch == 0) {
-    if (!c->segmented())
-      HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    launch_float(f, float_args(c), c->stream);
-  }
+  if (c->h.has_cond && c->n_patch == 0) launch_float(f, float_args(c), c->stream);
 }
 
 // Snapshot of the cumulative ledger and emission counts (read after the next
@@ -2369,7 +2553,7 @@ static bool lattice_indices(int64_t nn, const double* xyz, std::vector<int32_t>&
 // index-0 layers, the rest of which follow the tiles.  A row block is then a
 // compact 8^3 box: the aggregate of the two-level coarse space.  perm[k] =
 // caller's node of internal row k.  False when the nodes are not a lattice.
-static bool tiled_perm(int64_t nn, const double* xyz, std::vector<int32_t>& perm) {
+static bool tiled_perm(int64_t nn, const double* xyz, std::vector<int32_t>& perm, HostMesh& h) {
   std::vector<int32_t> ijk;
   int32_t dims[3];
   if (!lattice_indices(nn, xyz, ijk, dims)) return false;
@@ -2394,17 +2578,24 @@ static bool tiled_perm(int64_t nn, const double* xyz, std::vector<int32_t>& perm
   perm.clear();
   perm.reserve((size_t)nn);
   size_t e = 0;
+  bool aligned = true;
+  h.tile_of_block.clear();
   for (int64_t t = 0; t < ntiles; ++t) {
     bool any = false;
     for (int k = 0; k < 512 && !any; ++k) any = slot[t * 512 + k] >= 0;
     if (!any) continue;  // a tile with no node takes no block
+    h.tile_of_block.push_back((int32_t)t);
     for (int k = 0; k < 512; ++k) {
       const int32_t u = slot[t * 512 + k];
       if (u >= 0) perm.push_back(u);
       else if (e < extra.size()) perm.push_back(extra[e++]);
+      else aligned = false;  // the following tiles slide off their blocks
     }
   }
   while (e < extra.size()) perm.push_back(extra[e++]);
+  for (int r = 0; r < 3; ++r) h.tdim[r] = nt[r];
+  h.tile_blocks = (int64_t)h.tile_of_block.size();
+  h.tiles_aligned = aligned;
   return (int64_t)perm.size() == nn;
 }
 
@@ -2421,8 +2612,11 @@ void mesh_setup(scdat_ctx* c, int64_t nn, const double* xyz, int64_t nt, const i
   h.nt = nt;
   c->perm.clear();
   c->iperm.clear();
-  if (order == SCDAT_ORDER_TILED && !tiled_perm(nn, xyz, c->perm)) {
+  if (order == SCDAT_ORDER_TILED && !tiled_perm(nn, xyz, c->perm, h)) {
     c->perm.clear();
+    h.tile_of_block.clear();
+    h.tile_blocks = 0;
+    h.tiles_aligned = false;
     order = SCDAT_ORDER_MORTON;
   }
   if (order == SCDAT_ORDER_MORTON) {
@@ -2671,7 +2865,6 @@ void adj_forward_step(scdat_ctx* c, const ChargingRun& run, AdjArgs<T> a, AdjSte
   const bool fp64 = c->is_fp64();
   pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, c->inner_hist);
   if (!c->last_conv) throw Error(SCDAT_ERR_SOLVER, "charging adjoint: forward Poisson solve did not converge");
-  HIPCHK(hipMemcpyAsync(&f.sc->ccap, &c->ccap, sizeof(double), hipMemcpyHostToDevice, s));
   FloatArgs fa = float_args(c);
   fa.q_init = Q;
   fa.cond_mask = 0;  // the charge is Q itself
@@ -4079,6 +4272,8 @@ int scdat_get_solver_info(scdat_ctx* c, scdat_solver_info* out) {
   out->pcg_block = A.nc > 0 ? kPcgBlock : A.pb;
   out->preconditioner = A.nc > 0 ? SCDAT_PRECOND_TWO_LEVEL : SCDAT_PRECOND_JACOBI;
   out->coarse_in_kernel = A.fused;
+  out->tile_staged = A.box && (A.nc == 0 || A.fused);
+  out->box_patterns = c->nbpat;
   return SCDAT_OK;
 }
 
@@ -4108,6 +4303,11 @@ int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
     }
     return SCDAT_OK;
   }
+  if (what == SCDAT_SHAPE_TILE_STAGING) {
+    if (value != 0 && value != 1) return fail(c, SCDAT_ERR_ARG, "set_launch_shape: tile staging must be 0 or 1");
+    c->use_box = value != 0;
+    return SCDAT_OK;
+  }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS || !(value == 1 || value == 2 || value == 4 || value == 8))
     return fail(c, SCDAT_ERR_ARG, "set_launch_shape: push chunks must be 1, 2, 4 or 8");
   c->push_chunks = value;
@@ -4122,6 +4322,10 @@ int scdat_get_launch_shape(scdat_ctx* c, int32_t what, int32_t* value) {
   }
   if (what == SCDAT_SHAPE_ROW_PATTERNS) {
     *value = c->use_patterns ? (c->pat_force_wide ? 2 : 1) : 0;
+    return SCDAT_OK;
+  }
+  if (what == SCDAT_SHAPE_TILE_STAGING) {
+    *value = c->use_box ? 1 : 0;
     return SCDAT_OK;
   }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS) return SCDAT_ERR_ARG;

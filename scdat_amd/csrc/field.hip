@@ -63,6 +63,10 @@ constexpr int kRPT = 1;
 enum Slot { S_BB = 0, S_CL, S_RR, S_G0, S_G1, S_D0, S_D1, S_RR0, S_RR1, S_WC, S_WC1, S_COUNT };
 
 // slot s of the partials of a G-block grid; slots start 256-byte aligned
+// Nothing to do for this solve: it has converged (or given up), or it waits
+// for the host to resume a pass.
+__device__ inline bool idle(const PcgScalars* s) { return s->outer_done || s->stall; }
+
 __device__ inline double* slot(const FieldBufs& f, int s, int G) {
   return f.partial + (size_t)s * (size_t)((G + 31) & ~31);
 }
@@ -173,13 +177,14 @@ __global__ void __launch_bounds__(kPB) k_fin_rhs(FieldBufs f, int G) {
     sc->outer_it = 0;
     sc->total_it = 0;
     sc->failed = 0;
+    sc->stall = 0;
   }
 }
 
 template <int PB>
 __global__ void __launch_bounds__(PB) k_resid64(EllDev A, FieldBufs f) {
   const int G = gridDim.x;
-  if (f.sc[seg_of_block(f, blockIdx.x, G).s].outer_done) return;
+  if (idle(f.sc + seg_of_block(f, blockIdx.x, G).s)) return;
   __shared__ double lds[PB / 64];
   double rr = 0.0;
   const int64_t r0 = (int64_t)blockIdx.x * PB;
@@ -199,7 +204,7 @@ __global__ void __launch_bounds__(PB) k_resid64(EllDev A, FieldBufs f) {
 // one block per segment
 __global__ void __launch_bounds__(kPB) k_fin_resid(FieldBufs f, int G) {
   const SegRange sg = seg_by_index(f, blockIdx.x, G);
-  if (f.sc[sg.s].outer_done) return;
+  if (idle(f.sc + sg.s)) return;
   __shared__ double lds[kPB / 64];
   const double rr = sum_partials(slot(f, S_RR, G) + sg.b0, sg.b1 - sg.b0, lds);
   if (threadIdx.x == 0) {
@@ -209,6 +214,7 @@ __global__ void __launch_bounds__(kPB) k_fin_resid(FieldBufs f, int G) {
     sc->rnorm = rn;
     sc->inv_rnorm = rn > 0.0 ? 1.0 / rn : 0.0;
     const bool ok = rn <= sc->tol * (bn > 0.0 ? bn : 1.0) || rn == 0.0;
+    if (sc->outer_it == 0) sc->init_rnorm = rn;
     const bool give_up = sc->outer_it >= sc->max_outer || sc->total_it >= sc->max_it;
     if (ok || give_up) {
       sc->outer_done = 1;
@@ -248,7 +254,7 @@ __device__ inline void publish_restrict(double v, double* out, double* lds) {
 template <typename T, int PB>
 __global__ void __launch_bounds__(PB) k_inner_init(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   const PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
-  if (sc->outer_done) return;
+  if (idle(sc)) return;
   const double inv = sc->inv_rnorm;
   const int64_t r0 = (int64_t)blockIdx.x * PB;
   double rv = 0.0;
@@ -276,7 +282,7 @@ __global__ void __launch_bounds__(PB) k_inner_init(EllDev A, FieldBufs f, InnerB
 template <typename T>
 __global__ void __launch_bounds__(kPB) k_coarse(EllDev A, FieldBufs f, InnerBufs<T> ib, int mode, int it, int G) {
   const PcgScalars* sc = f.sc;
-  if (sc->outer_done || (mode == 2 && sc->inner_done)) return;
+  if (idle(sc) || (mode == 2 && sc->inner_done)) return;
   __shared__ T wc[kMaxCoarse];
   const int nc = A.nc;
   const double* part = slot(f, S_WC, G);
@@ -372,7 +378,7 @@ __device__ inline T r_next(T r, T w, T s, T a, T b) {
 template <typename T, int PB>
 __global__ void __launch_bounds__(PB) k_cg_start(EllDev A, FieldBufs f, InnerBufs<T> ib) {
   const int G = gridDim.x;
-  if (f.sc[seg_of_block(f, blockIdx.x, G).s].outer_done) return;
+  if (idle(f.sc + seg_of_block(f, blockIdx.x, G).s)) return;
   __shared__ double lds[3 * (PB / 64)];
   const T* __restrict__ v = vhat<T>(A);
   const T* __restrict__ r = ib.r[0];
@@ -509,7 +515,8 @@ __global__ void __launch_bounds__(PB) k_cg_iter(EllDev A, FieldBufs f, InnerBufs
   }
   // inner_done: an earlier launch of this batch already stopped the solve;
   // the partials of this parity are then stale and must not be used
-  const bool done = sc->inner_done || sc->outer_done || (it > 0 && rr <= tol * tol) ||
+  if (idle(sc)) return;  // converged, or a stalled solve waits for the host
+  const bool done = sc->inner_done || (it > 0 && rr <= tol * tol) ||
                     (sc->total_it + it >= sc->max_it) || !(alpha > 0.0) || !(alpha < 1e300);
   if (done) {
     if ((int)blockIdx.x == sg.b0 && threadIdx.x == 0) sc->inner_done = 1;
@@ -615,9 +622,10 @@ __global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_tl(EllDev A, FieldBufs
   // the solve's scalars, read together up front (block 0 writes the other
   // parity's history at the end; the flags are set by earlier launches)
   PcgScalars* sc = f.sc;
+  const bool idle_now = idle(sc);  // converged, or a stalled solve waits for the host
   const double g_prev = sc->hist[1 - cur], a_prev = sc->hist[2 + (1 - cur)];
   const double tol = sc->inner_tol_eff;
-  const bool stopped = sc->inner_done || sc->outer_done || sc->total_it + it >= sc->max_it;
+  const bool stopped = sc->inner_done || sc->total_it + it >= sc->max_it;
   const int myc = (int)threadIdx.x < K ? A.cnb[q0 + threadIdx.x] : 0;  // in flight with the partials
   {
     // partials two per thread (16-byte loads; slots are 256-byte aligned, so
@@ -689,6 +697,7 @@ __global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_tl(EllDev A, FieldBufs
   } else {
     alpha = g / d;
   }
+  if (idle_now) return;
   const bool done = stopped || (it > 0 && rr <= tol * tol) || !(alpha > 0.0) || !(alpha < 1e300);
   if (done) {
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->inner_done = 1;
@@ -839,12 +848,328 @@ __global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_tl(EllDev A, FieldBufs
   }
 }
 
+// One iteration in the tiled order with the neighbours' r_{i+1} staged in
+// LDS (Jacobi, or TL: two-level with the coarse step as in k_cg_iter_tl).
+// Block b stages r_{i+1} = r - a (w + b s) once for every point of the 10^3
+// box around its tile (its tile and the one-point halo from the 26
+// neighbour tiles; points with no node hold 0), then each row reads its
+// neighbours from the box by direction code.  Per row this replaces W
+// three-vector gathers by about two staged loads; the values, their order
+// and the fused operations are those of the row-pattern iteration, so the
+// two give the same bits.
+template <int W, bool TL>
+__global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_box(EllDev A, FieldBufs f, InnerBufs<float> ib, int it) {
+  constexpr int PB = kPcgBlock;
+  constexpr int NW = PB / 64;
+  constexpr int NC = TL ? kMaxCoarse : 1;
+  __shared__ double lds[4 * NW];
+  __shared__ double tot[3];
+  __shared__ __align__(16) float rl[NC];  // P^T w_i, zero past nc
+  __shared__ float zl[NC];                // zr_{i+1} of the aggregates this block touches, else 0
+  __shared__ int cl[kMaxTouched];
+  __shared__ float zown;
+  __shared__ float box[1000];             // r_{i+1} at (x, y, z) in [-1, 8]^3: (x+1) + 10 (y+1) + 100 (z+1)
+  __shared__ int nbl[27];                 // neighbour blocks; missing ones: this block (their entries are 0)
+  const int G = gridDim.x;
+  const int cur = it & 1, nxt = cur ^ 1;
+  const int64_t n = A.n;
+  const int bt = (int)blockIdx.x;
+  const int64_t i = (int64_t)bt * PB + threadIdx.x;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int pj = i < n ? (int)A.bpat[i] : 0;
+  const int mynb = threadIdx.x < 27 ? A.nbb[(size_t)bt * 27 + threadIdx.x] : 0;
+  PcgScalars* sc = f.sc;
+  const bool idle_now = idle(sc);  // converged, or a stalled solve waits for the host
+  const double g_prev = sc->hist[1 - cur], a_prev = sc->hist[2 + (1 - cur)];
+  const double tol = sc->inner_tol_eff;
+  const bool stopped = sc->inner_done || sc->total_it + it >= sc->max_it;
+  int K = 0;
+  if constexpr (TL) {
+    const int nc = A.nc;
+    const int q0 = A.cnb_off[bt];
+    K = A.cnb_off[bt + 1] - q0;
+    const int myc = (int)threadIdx.x < K ? A.cnb[q0 + threadIdx.x] : 0;
+    // as k_cg_iter_tl: partials two per thread, R to LDS in FP32
+    const double* pg = slot(f, cur ? S_G1 : S_G0, G);
+    const double* pd = slot(f, cur ? S_D1 : S_D0, G);
+    const double* pr = slot(f, cur ? S_RR1 : S_RR0, G);
+    const double* pw = slot(f, cur ? S_WC1 : S_WC, G);
+    const int r512 = (nc + 511) & ~511;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int j = 2 * (int)threadIdx.x; j < r512; j += 2 * PB) {
+      if (j < G) {
+        const double2 xg = *reinterpret_cast<const double2*>(pg + j);
+        const double2 xd = *reinterpret_cast<const double2*>(pd + j);
+        const double2 xr = *reinterpret_cast<const double2*>(pr + j);
+        const double2 xw = *reinterpret_cast<const double2*>(pw + j);
+        const bool two = j + 1 < G;
+        a += xg.x;
+        b += xd.x;
+        c += xr.x;
+        if (two) {
+          a += xg.y;
+          b += xd.y;
+          c += xr.y;
+        }
+        rl[j] = (float)xw.x;
+        rl[j + 1] = two ? (float)xw.y : 0.0f;
+        zl[j] = 0.0f;
+        zl[j + 1] = 0.0f;
+      } else {
+        rl[j] = 0.0f;
+        rl[j + 1] = 0.0f;
+      }
+    }
+    if ((int)threadIdx.x < K) cl[threadIdx.x] = myc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    if (lane == 0) {
+      lds[wv] = a;
+      lds[NW + wv] = b;
+      lds[2 * NW + wv] = c;
+    }
+  } else {
+    // as k_cg_iter: partials in block order, one per thread and pass
+    const double* pg = slot(f, cur ? S_G1 : S_G0, G);
+    const double* pd = slot(f, cur ? S_D1 : S_D0, G);
+    const double* pr = slot(f, cur ? S_RR1 : S_RR0, G);
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int j = threadIdx.x; j < G; j += PB) {
+      a += pg[j];
+      b += pd[j];
+      c += pr[j];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    if (lane == 0) {
+      lds[wv] = a;
+      lds[NW + wv] = b;
+      lds[2 * NW + wv] = c;
+    }
+  }
+  if (threadIdx.x < 27) nbl[threadIdx.x] = mynb >= 0 ? mynb : bt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      x += lds[k];
+      y += lds[NW + k];
+      z += lds[2 * NW + k];
+    }
+    tot[0] = x;
+    tot[1] = y;
+    tot[2] = z;
+  }
+  __syncthreads();
+  const double g = tot[0], d = tot[1], rr = tot[2];
+  double beta = 0.0, alpha;
+  if (it > 0) {
+    beta = g / g_prev;
+    alpha = g / (d - beta * g / a_prev);
+  } else {
+    alpha = g / d;
+  }
+  if (idle_now) return;
+  const bool done = stopped || (it > 0 && rr <= tol * tol) || !(alpha > 0.0) || !(alpha < 1e300);
+  if (done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->inner_done = 1;
+    return;
+  }
+  const float a = (float)alpha, b = (float)beta;
+  if constexpr (TL) {
+    const int ncp = A.ncp;
+    for (int base = 0; base < K; base += 2 * NW) {
+      const int qa = base + wv, qb = qa + NW;
+      if (qa >= K) break;  // uniform in the wave
+      const bool hb = qb < K;
+      const int ca = cl[qa], cb = cl[hb ? qb : qa];
+      const float4* ra = reinterpret_cast<const float4*>(A.acip32 + (size_t)ca * ncp);
+      const float4* rb = reinterpret_cast<const float4*>(A.acip32 + (size_t)cb * ncp);
+      const float zra = ib.zr[cur][ca], zsa = ib.zs[nxt][ca];
+      const float zrb = ib.zr[cur][cb], zsb = ib.zs[nxt][cb];
+      float ta = 0.0f, tb = 0.0f;
+      for (int k0 = 0; k0 < ncp; k0 += 2 * 256) {
+        float4 xa[2], xb[2], xr[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int k = k0 + 256 * u + 4 * lane;
+          const int ks = k < ncp ? k / 4 : 0;
+          xa[u] = ra[ks];
+          xb[u] = rb[ks];
+          xr[u] = *reinterpret_cast<const float4*>(&rl[k]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          ta = fmaf(xa[u].x, xr[u].x, ta);
+          ta = fmaf(xa[u].y, xr[u].y, ta);
+          ta = fmaf(xa[u].z, xr[u].z, ta);
+          ta = fmaf(xa[u].w, xr[u].w, ta);
+          tb = fmaf(xb[u].x, xr[u].x, tb);
+          tb = fmaf(xb[u].y, xr[u].y, tb);
+          tb = fmaf(xb[u].z, xr[u].z, tb);
+          tb = fmaf(xb[u].w, xr[u].w, tb);
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        ta += __shfl_xor(ta, off, 64);
+        tb += __shfl_xor(tb, off, 64);
+      }
+      if (lane == 0) {
+        const float sa = fmaf(b, zsa, ta), za = fmaf(-a, sa, zra);
+        zl[ca] = za;
+        if (ca == bt) {
+          ib.zs[cur][ca] = sa;
+          ib.zr[nxt][ca] = za;
+          zown = zra;
+        }
+        if (hb) {
+          const float sb = fmaf(b, zsb, tb), zb = fmaf(-a, sb, zrb);
+          zl[cb] = zb;
+          if (cb == bt) {
+            ib.zs[cur][cb] = sb;
+            ib.zr[nxt][cb] = zb;
+            zown = zrb;
+          }
+        }
+      }
+    }
+  }
+  const float* __restrict__ rc = ib.r[cur];
+  const float* __restrict__ wc = ib.w[cur];
+  const float* __restrict__ sp = ib.s[nxt];
+  // stage r_{i+1} over the box
+  for (int q = threadIdx.x; q < 1000; q += PB) {
+    const int bx = q % 10 - 1, by = (q / 10) % 10 - 1, bz = q / 100 - 1;
+    const int code = ((bx >> 3) + 1) + 3 * ((by >> 3) + 1) + 9 * ((bz >> 3) + 1);
+    const int nb = nbl[code];
+    const int64_t row = (int64_t)nb * PB + (bx & 7) + 8 * (by & 7) + 64 * (bz & 7);
+    float v = 0.0f;
+    if ((code == 13 || nb != bt) && row < n) v = r_next(rc[row], wc[row], sp[row], a, b);
+    box[q] = v;
+  }
+  // the row's own entries, in flight across the barrier
+  float ri = 0.0f, wi = 0.0f, so = 0.0f, pold = 0.0f, xold = 0.0f;
+  if (i < n) {
+    ri = rc[i];
+    wi = wc[i];
+    so = sp[i];
+    pold = ib.p[i];
+    xold = ib.x[i];
+  }
+  __syncthreads();  // box, zl, zown
+  double g1 = 0.0, d1 = 0.0, rr1 = 0.0, wrest = 0.0;
+  if (i < n) {
+    const int x = threadIdx.x & 7, y = (threadIdx.x >> 3) & 7, z = threadIdx.x >> 6;
+    const int cpos = (x + 1) + 10 * (y + 1) + 100 * (z + 1);
+    const uint4 dq = *reinterpret_cast<const uint4*>(A.bd + (size_t)pj * 16);
+    const uint32_t dw[4] = {dq.x, dq.y, dq.z, dq.w};
+    const float* pv = A.bv + (size_t)pj * W;
+    const float* pa = A.bap + (size_t)pj * W;
+    // (A D^-1) r_{i+1} in acc and, two-level, (A P) zr_{i+1} in acz, each in
+    // slot order
+    float acc = 0.0f, acz = 0.0f;
+    // (in slices of 5: the table entries are L1 hits, and loading all 2 W of
+    // them at once would spill)
+#pragma unroll 5
+    for (int k = 0; k < W; ++k) {
+      const int dc = (int)((dw[k >> 2] >> (8 * (k & 3))) & 0xffu);
+      const int dx = (dc & 3) - 1, dy = ((dc >> 2) & 3) - 1, dz = (dc >> 4) - 1;
+      acc = fmaf(pv[k], box[cpos + dx + 10 * dy + 100 * dz], acc);
+      if constexpr (TL) {
+        const int tc = (((x + dx) >> 3) + 1) + 3 * (((y + dy) >> 3) + 1) + 9 * (((z + dz) >> 3) + 1);
+        acz = fmaf(pa[k], zl[nbl[tc]], acz);
+      }
+    }
+    const float di = A.bdi[pj];
+    bool free_row = false;
+    float uc = 0.0f, ui;
+    if constexpr (TL) {
+      acc += acz;
+      free_row = pa[0] != 0.0f;  // slot 0 is self: A_ii on free rows
+      ui = free_row ? fmaf(di, ri, zown) : di * ri;
+      uc = free_row ? zl[bt] : 0.0f;
+    } else {
+      ui = di * ri;
+    }
+    const float si = fmaf(b, so, wi);
+    const float rn = fmaf(-a, si, ri);
+    const float pi = fmaf(b, pold, ui);
+    ib.x[i] = fmaf(a, pi, xold);
+    ib.p[i] = pi;
+    ib.s[cur][i] = si;
+    ib.r[nxt][i] = rn;
+    ib.w[nxt][i] = acc;
+    float un;
+    if constexpr (TL) un = fmaf(di, rn, uc);
+    else un = di * rn + uc;
+    g1 += (double)rn * (double)un;
+    d1 += (double)acc * (double)un;
+    rr1 += (double)rn * (double)rn;
+    if (free_row) wrest = (double)acc;
+  }
+  if constexpr (TL) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      g1 += __shfl_xor(g1, off, 64);
+      d1 += __shfl_xor(d1, off, 64);
+      rr1 += __shfl_xor(rr1, off, 64);
+      wrest += __shfl_xor(wrest, off, 64);
+    }
+    __syncthreads();
+    if (lane == 0) {
+      lds[wv] = g1;
+      lds[NW + wv] = d1;
+      lds[2 * NW + wv] = rr1;
+      lds[3 * NW + wv] = wrest;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double x = 0.0, y = 0.0, z = 0.0, q = 0.0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        x += lds[k];
+        y += lds[NW + k];
+        z += lds[2 * NW + k];
+        q += lds[3 * NW + k];
+      }
+      slot(f, nxt ? S_G1 : S_G0, G)[bt] = x;
+      slot(f, nxt ? S_D1 : S_D0, G)[bt] = y;
+      slot(f, nxt ? S_RR1 : S_RR0, G)[bt] = z;
+      slot(f, nxt ? S_WC1 : S_WC, G)[bt] = q;
+    }
+  } else {
+    publish3<PB>(g1, d1, rr1, slot(f, nxt ? S_G1 : S_G0, G), slot(f, nxt ? S_D1 : S_D0, G),
+                 slot(f, nxt ? S_RR1 : S_RR0, G), lds);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->hist[cur] = g;
+    sc->hist[2 + cur] = alpha;
+    sc->inner_it = it + 1;
+  }
+}
+
 // Closes an inner solve: records its iteration count and applies
 // x += |r| d (skipped when the outer loop has already converged).
 template <typename T, int PB>
 __global__ void __launch_bounds__(PB) k_apply(EllDev A, FieldBufs f, InnerBufs<T> ib) {
-  const PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
-  if (sc->outer_done) return;
+  PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
+  if (idle(sc)) return;
+  if (!sc->inner_done) {
+    // a pass launched without a host poll ran out of its batch: wait for
+    // the host to resume it (every block sees the same flags)
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->stall = 1;
+    return;
+  }
   const double rn = sc->rnorm;
   const int64_t r0 = (int64_t)blockIdx.x * PB;
   for (int64_t i = r0 + threadIdx.x; i < r0 + PB && i < A.n; i += PB)
@@ -856,9 +1181,21 @@ __global__ void k_count_outer(FieldBufs f, int n_seg) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_seg) return;
   PcgScalars* sc = f.sc + s;
-  if (sc->outer_done) return;
+  if (idle(sc)) return;
+  if (sc->outer_it < 16) sc->pass_it[sc->outer_it] = sc->inner_it;
   sc->outer_it += 1;
   sc->total_it += sc->inner_it;
+}
+
+__global__ void k_clear_stall(PcgScalars* sc) { sc->stall = 0; }
+
+__global__ void k_set_controls(PcgScalars* sc, int n, double tol, double inner_tol, int max_it, int max_outer) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  sc[k].tol = tol;
+  sc[k].inner_tol = inner_tol;
+  sc[k].max_it = max_it;
+  sc[k].max_outer = max_outer;
 }
 
 // Host-polled summary of the segments: done when every segment is done,
@@ -929,7 +1266,7 @@ __global__ void __launch_bounds__(kBlock) k_float(FieldBufs f, FloatArgs fa) {
     // a failed solve keeps the previous floating potential (its phi_0 is
     // restored from the previous solution, PICCycle.cpp:731-767)
     if (sc->failed) return;
-    const double cap = fa.seg_ccap ? fa.seg_ccap[sg] : sc->ccap;
+    const double cap = fa.seg_ccap ? fa.seg_ccap[sg] : fa.ccap;
     sc->ccap = cap;
     sc->qcond = q;
     sc->vfloat = cap > 0.0 ? (q + sc->cond_load - tot) / cap : 0.0;
@@ -1075,7 +1412,23 @@ void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& 
                        hipStream_t s, hipEvent_t* ev) {
   hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
   if constexpr (sizeof(T) == 4) {
-    if (A.nc && A.fused) {  // two-level, coarse solve inside the iteration
+    if (A.box && (A.nc == 0 || A.fused)) {  // tile-staged (Jacobi or fused two-level)
+      const dim3 g(grid_rows(A.n, kPcgBlock)), b(kPcgBlock);
+      switch (A.w * 2 + (A.nc ? 1 : 0)) {
+#define SCDAT_W(w) \
+  case 2 * w:      \
+    hipExtLaunchKernelGGL((k_cg_iter_box<w, false>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    return;        \
+  case 2 * w + 1:  \
+    hipExtLaunchKernelGGL((k_cg_iter_box<w, true>), g, b, 0, s, e0, e1, 0, A, f, ib, it); \
+    return;
+        SCDAT_W(13) SCDAT_W(14) SCDAT_W(15) SCDAT_W(16)
+#undef SCDAT_W
+        default:
+          break;  // ell() only stages these widths
+      }
+    }
+    if (A.nc && A.fused && A.pap32) {  // two-level, coarse solve inside the iteration
       const dim3 g(grid_rows(A.n, kPcgBlock)), b(kPcgBlock);
       switch (A.w) {
 #define SCDAT_W(w) \
@@ -1116,6 +1469,14 @@ void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, h
 }
 size_t partial_doubles(int64_t n) {  // smallest block, slots 256-byte aligned
   return (size_t)S_COUNT * (size_t)((grid_rows(n, 256) + 31) & ~31) + 16;
+}
+void launch_set_controls(const FieldBufs& f, int n_scalars, double tol, double inner_tol, int max_it,
+                         int max_outer, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_controls, dim3((n_scalars + 63) / 64), dim3(64), 0, s, f.sc, n_scalars, tol, inner_tol,
+                     max_it, max_outer);
+}
+void launch_clear_stall(const FieldBufs& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_stall, dim3(1), dim3(1), 0, s, f.sc);
 }
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s) {
   hipLaunchKernelGGL(k_float, dim3(f.blk_seg ? f.n_seg : 1), dim3(kBlock), 0, s, f, fa);

@@ -27,6 +27,12 @@ struct PcgScalars {
   double hist[4];  // (g_i, alpha_i) of the last inner iteration, by parity
   int32_t outer_done, inner_done, inner_it, outer_it, total_it, max_it, max_outer, failed;
   unsigned int ticket[4];
+  double init_rnorm;      // residual norm of the solve's initial guess
+  int32_t pass_it[16];    // inner iterations of each outer pass (the first 16)
+  // A pass launched without a host poll whose batch ended before its inner
+  // solve stopped: the apply sets stall instead of applying, and every
+  // later kernel of the solve does nothing until the host resumes the pass.
+  int32_t stall;
 };
 
 // ELL operator over all nodes, slot-major (slot s of row i at s*n + i), self
@@ -68,6 +74,17 @@ struct EllDev {
   const float* pap32;
   int ncp;                 // row stride of acip32: nc rounded up to 64, zero-padded
   const float* acip32;     // Ac^-1 in FP32 with that stride
+  // Tile-staged iteration (tiled order, FP32; box = 0: not used).  Block b
+  // stages r_{i+1} of its tile and the one-point halo (a 10^3 box) in LDS
+  // once; row i of b then reads its neighbours there by direction code
+  // (delta + 1 in two bits per axis, x lowest) from its box pattern.
+  int box;
+  const uint16_t* bpat;    // box pattern of each row
+  const uint8_t* bd;       // 16 direction codes per pattern (21: the row itself)
+  const float* bv;         // A D^-1 entries per pattern (w per pattern)
+  const float* bap;        // A P entries per pattern (two-level; 0 otherwise)
+  const float* bdi;        // D^-1 per pattern
+  const int32_t* nbb;      // 27 neighbour blocks per block (13: itself; -1: none)
   // Row patterns of the FP32 inner operator (npat = 0: not used).  Rows with
   // the same column offsets j - i, the same FP32 values of A D^-1 and the
   // same FP32 D^-1 share one table entry, so an inner iteration reads a
@@ -164,8 +181,14 @@ struct FloatArgs {
   const double* seg_qinit;
   const double* seg_wscale;
   const double* seg_ccap;
+  double ccap;  // without segments: the capacitance sum_C K phi_1
 };
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s);
+// Lets a stalled solve (one scenario) go on: the host resumes the stalled pass.
+void launch_clear_stall(const FieldBufs& f, hipStream_t s);
+// The stopping controls of the solve(s) on the device, without a host round trip.
+void launch_set_controls(const FieldBufs& f, int n_scalars, double tol, double inner_tol, int max_it,
+                         int max_outer, hipStream_t s);
 // Initial guess of a time step's solve from the last two solutions: on free
 // rows x = 2 x_n - x_{n-1} (first = 1: x stays x_n); prev takes x_n.
 void launch_extrapolate(const FieldBufs& f, double* prev, int64_t n, int first, hipStream_t s);

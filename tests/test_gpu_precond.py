@@ -61,3 +61,29 @@ def test_two_level_full_cycle_fp64_matches_oracle_and_is_reproducible():
     assert rel_err(phi_c, phi_o) < 1e-6 and rel_err(E_c, E_o) < 1e-6
     for a, b in zip(runs[0], runs[1]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precond", [S.PRECOND_JACOBI, S.PRECOND_TWO_LEVEL])
+def test_tile_staging_gives_the_row_pattern_bits(precond):
+    """In the tiled order the iteration that stages each tile's neighbour
+    values in LDS reads the same operator entries in the same slot order as
+    the row-pattern iteration: same bits, Jacobi and two-level."""
+    st = plasma_setup(n=24, npart=40000, seed=8)
+    res = {}
+    for staged in (1, 0):
+        c = make_ctx(st, precision=0, tol=1e-10, sort_interval=0, order=S.ORDER_TILED)
+        c.set_preconditioner(precond)
+        c.tile_staging = staged
+        c.initialize()
+        info = c.solver_info()
+        c.step(3)
+        res[staged] = (c.fields(), c.stats(), info)
+    on, off = res[1], res[0]
+    print("staged", on[2], "row patterns", off[2])
+    assert on[2]["tile_staged"] == 1 and on[2]["box_patterns"] > 0
+    assert off[2]["tile_staged"] == 0
+    assert on[2]["coarse_in_kernel"] == int(precond == S.PRECOND_TWO_LEVEL)
+    for a, b in zip(on[0], off[0]):
+        np.testing.assert_array_equal(a, b)
+    assert on[1]["floating_potential"] == off[1]["floating_potential"]
+    assert on[1]["pcg_iterations_total"] == off[1]["pcg_iterations_total"]
