@@ -238,3 +238,50 @@ def test_photoemission_fp64_matches_oracle():
     _compare_species(o, c, 0, np.sqrt(2 * QE / ME), 1e-9, 1e-7)
     # same bound as the FP64 full-cycle parity test (1e-12 solves, three steps)
     assert abs(c.stats()["floating_potential"] - o.floating_potential()) <= 1e-6 * abs(o.floating_potential()) + 1e-12
+
+
+def test_collision_handler_mirror_reports_the_device_steps():
+    """MonteCarloCollisionHandler + executeBackgroundMccStep after each
+    Context.step (the reference main loop's order): per-step counts are the
+    device's event-count differences, and a re-initialisation sets the
+    effective densities on the device's processes."""
+    from scdat_amd import collisions as CL
+    st = _setup(n=6, npart=3000, te_ev=30.0, seed=33)
+    m = st["mesh"]
+    c = S.Context(precision=0, dt=st["dt"], pcg_tol=1e-10, sort_interval=4)
+    c.set_mesh(m["xyz"], m["tets"], m["kind"], group=m["group"])
+    for g in range(6, 12):
+        c.set_boundary(g, S.BC_CONDUCTOR)
+    c.add_species(-QE, ME, st["w"], 20000)
+    c.add_species(QE, MP, st["w"], 20000)
+    c.set_particles(0, *st["e"])
+    c.set_particles(1, *st["i"])
+    h = CL.MonteCarloCollisionHandler(random_seed=SEED, electron_species=0, ion_species=1)
+    h.initializeDefaultConfiguration(neutral_density=3e19, ion_density=1e16, electron_density=1e16)
+    h.bind(c)
+    c.initialize()
+    cfg = CL.BackgroundMccRuntimeConfig(CL.CollisionBackgroundDensities(3e19, 1e16, 1e16))
+    tot = 0
+    for k in range(4):
+        before = c.event_counts()
+        c.step(1)
+        after = c.event_counts()
+        if k == 2:
+            cfg.reinitialize_configuration = True
+        r = CL.executeBackgroundMccStep(c, h, st["dt"], 1.0, cfg)
+        names = h.processNames()
+        d = [a - b for a, b in zip(after["collisions"], before["collisions"])]
+        assert r.collision_events == sum(d[:len(names)])
+        assert r.ionization_events == d[names.index("ionization_e_n")]
+        assert r.excitation_events == d[names.index("excitation_e_n")]
+        assert r.charge_exchange_events == d[names.index("charge_exchange_i_n")]
+        assert r.generated_particles == after["created"] - before["created"]
+        assert r.active_particles == sum(c.particle_count(s)[0] for s in range(2))
+        tot += r.collision_events
+    assert tot > 0
+    stt = h.getStatistics()
+    assert stt.total_collisions == tot
+    # the re-initialisation moved the electron-ion processes to the effective densities
+    ev = CL.estimateCollisionBackgroundDensities(c, 1.0, cfg.floor_densities)
+    assert h._procs["recombination_e_i"].process.target_density == pytest.approx(ev.ion_density_m3, rel=0.2)
+    c.close()

@@ -71,3 +71,40 @@ def test_row_patterns_give_identical_results():
         np.testing.assert_array_equal(p0[0], p1[0])
         assert s0["floating_potential"] == s1["floating_potential"]
         assert s0["pcg_iterations"] == s1["pcg_iterations"]
+
+
+def test_pcg_block_sizes_agree_and_overrides_are_reported():
+    """256- and 512-row PCG blocks solve the same system to the tolerance
+    (their reductions differ in order, so the bits may not); other sizes are
+    refused; the two-level preconditioner reports the 512 it runs on."""
+    st = plasma_setup(n=12, npart=20000, seed=8)
+    res = {}
+    for pb in (256, 512):
+        c = make_ctx(st, precision=0, tol=1e-10, sort_interval=2)
+        c.pcg_block = pb
+        assert c.pcg_block == pb
+        c.initialize()
+        assert c.solver_info()["pcg_block"] == pb
+        c.step(4)
+        res[pb] = (c.fields(), c.stats())
+        c.close()
+    (_, phi_a, E_a), sa = res[256]
+    (_, phi_b, E_b), sb = res[512]
+    scale = np.abs(phi_a).max()
+    assert np.abs(phi_a - phi_b).max() <= 1e-6 * scale
+    assert sa["floating_potential"] == pytest.approx(sb["floating_potential"], rel=1e-6, abs=1e-9)
+    assert sa["active_particles"] == sb["active_particles"]
+    c = make_ctx(st, precision=0, tol=1e-10, sort_interval=2)
+    with pytest.raises(S.ScdatError):
+        c.pcg_block = 384
+    c.close()
+    st2 = plasma_setup(n=18, npart=8000, seed=9)
+    c = make_ctx(st2, precision=0, tol=1e-10, sort_interval=2, order=S.ORDER_TILED)
+    c.pcg_block = 256
+    c.set_preconditioner(S.PRECOND_TWO_LEVEL)
+    c.initialize()
+    info = c.solver_info()
+    assert info["coarse_unknowns"] > 0 and info["pcg_block"] == 512
+    c.step(2)
+    assert c.stats()["pcg_converged"] == 1
+    c.close()
