@@ -647,6 +647,7 @@ struct scdat_ctx {
   std::array<std::array<double, 3>, kMaxGroups> gshift{};
   // device mesh
   DevBuf d_tets, d_nb, d_groups, d_partner, d_gshift;
+  DevBuf d_wmap;  // FP32 walk records: tet map + face neighbours
   // ELL operator
   DevBuf d_col, d_v32, d_v64, d_vh32, d_vh64, d_dinv32, d_dinv64, d_g[3];
   DevBuf d_pat, d_poff, d_pval, d_pdinv;  // row patterns of the FP32 inner operator
@@ -1042,6 +1043,17 @@ void upload_mesh_device(scdat_ctx* c) {
     c->d_face_cum.upload(fc, s);
   }
   c->d_nb.upload(h.nb, s);
+  if constexpr (sizeof(T) == 4) {
+    std::vector<WalkMap<float>> wm((size_t)h.nt);
+    for (int64_t t = 0; t < h.nt; ++t) {
+      WalkMap<float>& m = wm[t];
+      for (int k = 0; k < 9; ++k) m.a[k] = (float)h.A[9 * t + k];
+      for (int k = 0; k < 3; ++k) m.x0[k] = (float)h.X0[3 * t + k];
+      for (int k = 0; k < 4; ++k) m.nb[k] = h.nb[4 * t + k];
+    }
+    c->d_wmap.upload(wm, s);
+    HIPCHK(hipStreamSynchronize(s));  // the staging vector is local
+  }
   c->d_groups.upload(h.group, s);
   c->d_partner.upload(h.partner, s);
   c->d_col.upload(h.col, s);
@@ -1704,6 +1716,7 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   SpeciesState& S = c->sp[si];
   PushArgs<T> a{};
   a.tets = c->d_tets.as<TetMap<T>>();
+  if constexpr (sizeof(T) == 4) a.wmap = c->d_wmap.as<WalkMap<T>>();
   a.nb = c->d_nb.as<int4>();
   a.groups = c->d_groups.as<int4>();
   a.partner = c->d_partner.as<int4>();

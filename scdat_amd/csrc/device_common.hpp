@@ -34,6 +34,19 @@ struct alignas(128) TetMap<double> {
   int32_t pad[4];
 };
 
+// Walk record of the FP32 push: the tet's barycentric map with its four face
+// neighbours instead of its nodes, one 64-byte line, so a face crossing is a
+// single dependent load (the next tet's record) rather than two (neighbour
+// ids, then the next tet's map).
+template <typename T>
+struct WalkMap;
+template <>
+struct alignas(64) WalkMap<float> {
+  float a[9];
+  float x0[3];
+  int32_t nb[4];
+};
+
 template <typename T>
 struct Vec4;
 template <>
@@ -59,8 +72,8 @@ __host__ __device__ inline double fma_t<double>(double a, double b, double c) {
 // Barycentric coordinates of p in tet m.  The fma form is part of the
 // fixed-point deposition contract: the host checker evaluates the identical
 // expression, so the integer weights agree bit for bit.
-template <typename T>
-__device__ __forceinline__ void bary(const TetMap<T>& m, const T p[3], T lam[4]) {
+template <typename M, typename T>
+__device__ __forceinline__ void bary(const M& m, const T p[3], T lam[4]) {
   const T d0 = p[0] - m.x0[0], d1 = p[1] - m.x0[1], d2 = p[2] - m.x0[2];
   lam[1] = fma_t(m.a[0], d0, fma_t(m.a[1], d1, m.a[2] * d2));
   lam[2] = fma_t(m.a[3], d0, fma_t(m.a[4], d1, m.a[5] * d2));

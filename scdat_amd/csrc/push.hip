@@ -13,6 +13,8 @@
 // (segmented scan on 32-bit integers), then one 64-bit atomic per (run, node)
 // lands in the node accumulator.  Integer addition is associative, so the
 // result is independent of launch order and bit-exact.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "push.hpp"
 #include "rng.hpp"
@@ -132,7 +134,11 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
       T x[3] = {a.x[i], a.y[i], a.z[i]};
       T v[3] = {a.vx[i], a.vy[i], a.vz[i]};
       const T v0[3] = {v[0], v[1], v[2]};
-      TetMap<T> m = a.tets[c];
+      // FP32: walk records (one dependent load per face crossing); the nodes
+      // of the sub-step's start cell come from the tet map
+      using Map = typename std::conditional<sizeof(T) == 4, WalkMap<float>, TetMap<T>>::type;
+      Map m;
+      load_map(a, c, m);
       int sg = 0;
       T remaining = a.dt;
       double wt = a.weight;
@@ -151,7 +157,13 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
         T lam[4];
         bary(m, x, lam);
         T E[3];
-        gather_E(a, m, lam, E);
+        if constexpr (sizeof(T) == 4) {
+          const int4 nd4 = *reinterpret_cast<const int4*>(a.tets[c].node);
+          const int32_t nd[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
+          gather_E(a, nd, lam, E);
+        } else {
+          gather_E(a, m, lam, E);
+        }
         const T h =
             a.adaptive ? adaptive_dt(v0, E, a.bmag, a.qm, remaining, a.floor_ratio) : remaining;
         boris(v, E, a.B, a.has_b != 0, a.qm, h);

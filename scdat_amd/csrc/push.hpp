@@ -13,6 +13,7 @@ template <typename T>
 struct PushArgs {
   // mesh
   const TetMap<T>* tets;
+  const WalkMap<T>* wmap;  // FP32: walk records (map + face neighbours) per tet
   const int4* nb;       // face neighbours (face k opposite vertex k), <0 boundary
   const int4* groups;   // boundary group per face
   const int4* partner;  // periodic partner tet per face

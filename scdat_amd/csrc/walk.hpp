@@ -5,24 +5,42 @@
 
 namespace scdat {
 
+// P1 gather of nodal E at barycentric lam over the tet's nodes nd.
 template <typename T>
-__device__ __forceinline__ void gather_E(const PushArgs<T>& a, const TetMap<T>& m, const T lam[4], T E[3]) {
+__device__ __forceinline__ void gather_E(const PushArgs<T>& a, const int32_t nd[4], const T lam[4], T E[3]) {
   E[0] = E[1] = E[2] = T(0);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const Vec4<T> e = a.E[m.node[k]];
+    const Vec4<T> e = a.E[nd[k]];
     E[0] = fma_t(lam[k], e.x, E[0]);
     E[1] = fma_t(lam[k], e.y, E[1]);
     E[2] = fma_t(lam[k], e.z, E[2]);
   }
 }
+template <typename T>
+__device__ __forceinline__ void gather_E(const PushArgs<T>& a, const TetMap<T>& m, const T lam[4], T E[3]) {
+  gather_E(a, m.node, lam, E);
+}
+
+// The walk reads a tet's map and its face neighbours either from the tet map
+// and the neighbour array, or from the walk record that holds both.
+template <typename T>
+__device__ __forceinline__ void load_map(const PushArgs<T>& a, int t, TetMap<T>& m) { m = a.tets[t]; }
+template <typename T>
+__device__ __forceinline__ void load_map(const PushArgs<T>& a, int t, WalkMap<T>& m) { m = a.wmap[t]; }
+template <typename T>
+__device__ __forceinline__ int4 face_nb(const PushArgs<T>& a, int t, const TetMap<T>&) { return a.nb[t]; }
+template <typename T>
+__device__ __forceinline__ int4 face_nb(const PushArgs<T>&, int, const WalkMap<T>& m) {
+  return make_int4(m.nb[0], m.nb[1], m.nb[2], m.nb[3]);
+}
 
 // Straight-segment walk from x0 (inside cell `cur`) to x1.  Returns the final
 // cell, -1 when the particle left through an absorbing/conductor face (group,
 // hit point, exit face and its tet set), -2 when the walk limit was exceeded.
-template <typename T>
+template <typename T, typename M>
 __device__ __forceinline__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1[3], T v[3], int& group,
-                    T hit[3], TetMap<T>& m, int& face, int& last) {
+                    T hit[3], M& m, int& face, int& last) {
   const T tol = a.tol;
   for (int it = 0; it < a.max_walk; ++it) {
     T l1[4];
@@ -53,11 +71,11 @@ __device__ __forceinline__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1
       tb = T(0);
     }
     tb = tb < T(0) ? T(0) : (tb > T(1) ? T(1) : tb);
-    const int4 nb4 = a.nb[cur];
+    const int4 nb4 = face_nb(a, cur, m);
     const int n = kb == 0 ? nb4.x : (kb == 1 ? nb4.y : (kb == 2 ? nb4.z : nb4.w));
     if (n >= 0) {
       cur = n;
-      m = a.tets[cur];
+      load_map(a, cur, m);
       continue;
     }
     const int4 g4 = a.groups[cur];
@@ -76,7 +94,7 @@ __device__ __forceinline__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1
           x1[r] += sh;
         }
         cur = pt;
-        m = a.tets[cur];
+        load_map(a, cur, m);
         continue;
       }
     } else if (a.reflect_mask & gbit) {
