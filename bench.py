@@ -405,10 +405,12 @@ def run_adjoint(args):
         "rel_diff_h1e-8": dict(zip(names, rel8.tolist())),
         "sensitivity_rel_diff_max_h1e-8": float(rel8.max()),
         "sensitivity_rel_diff_max_best_h": float(np.minimum(rel, rel8).max()),
-        "central_difference_note": "at h = 1e-4 the perturbed runs differ by discrete events (a particle absorbed "
-                                   "in one run and not in the other moves V by q w / C ~ 2e-4 V), which the "
-                                   "difference quotient divides by 2h; at h = 1e-8 no event changes and the "
-                                   "quotient measures the derivative the adjoint computes",
+        "central_difference_note": "both perturbed runs replay the absorption events of the run at theta, so "
+                                   "the quotient differences the map the adjoint differentiates; what remains "
+                                   "shrinks with h (the cell weights are piecewise linear in position: a "
+                                   "perturbed trajectory that enters another tet changes the slope the quotient "
+                                   "averages); the conductance's h = 1e-8 step moves V by ~1e-12 V, at the PCG "
+                                   "tolerance",
         "central_difference_s": t_fd,
         "mixed_precision": {"floating_potential_v": v32, "gradient_s": t32, "sensitivity_err_vs_fp64": err32},
     }
@@ -455,14 +457,15 @@ def run_geo(args):
     log(f"geo: {st['cfg']['edge']}^3 hexes, {n_tets} tets, {n_nodes} nodes, {st['cfg']['n_particles']} particles, "
         f"dt={st['dt']:.4e}s, inputs in {time.time() - t0:.1f}s")
     t0 = time.time()
-    ctx = G.context(st, precision=0, tol=1e-8, sort_interval=args.sort_interval, steps=steps + args.warmup)
+    ctx = G.context(st, precision=0, tol=1e-8, sort_interval=args.sort_interval, steps=steps + args.warmup,
+                    two_level=args.precond == "two-level")
     chunks = ctx.autotune()
     log(f"geo setup {time.time() - t0:.1f}s (push tile {chunks}); warmup {args.warmup} steps")
     for _ in range(args.warmup):
         ctx.step(1)
     ctx.sync()
     st0, ev0 = ctx.stats(), ctx.event_counts()
-    ctx.set_kernel_timing(not args.no_kernel_timing, period=16)
+    ctx.set_kernel_timing(not args.no_kernel_timing, period=64)
     t1 = time.perf_counter()
     ctx.step(steps)
     ctx.sync()

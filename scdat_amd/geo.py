@@ -23,8 +23,8 @@ oxygen; the forms and their meaning are the reference's.
 """
 import math
 
-from . import (BC_CONDUCTOR, ELECTRON_MASS, ELEMENTARY_CHARGE, MCC_ELASTIC, MCC_IONIZATION, PROTON_MASS,
-               SEE_STERNGLASS, Context, box_mesh, box_spec, mcc_process)
+from . import (BC_CONDUCTOR, ELECTRON_MASS, ELEMENTARY_CHARGE, MCC_ELASTIC, MCC_IONIZATION, ORDER_TILED,
+               PRECOND_TWO_LEVEL, PROTON_MASS, SEE_STERNGLASS, Context, box_mesh, box_spec, mcc_process)
 from .loading import maxwellian_box
 
 CONDUCTOR = ((0.4, 0.4, 0.4), (0.6, 0.6, 0.6))
@@ -105,11 +105,15 @@ def capacities(st, steps):
     return [half + extra, half, max(4096, half // 8)]
 
 
-def context(st, precision=0, tol=1e-8, sort_interval=20, steps=2000, seed=None):
-    """The case's device context, initialized."""
+def context(st, precision=0, tol=1e-8, sort_interval=20, steps=2000, seed=None, two_level=False):
+    """The case's device context, initialized; two_level: the two-level
+    Poisson preconditioner in the tiled node order."""
     cfg = st["cfg"]
     m = st["mesh"]
     c = Context(precision=precision, dt=st["dt"], pcg_tol=tol, sort_interval=sort_interval)
+    if two_level:
+        c.set_node_order(ORDER_TILED)
+        c.set_preconditioner(PRECOND_TWO_LEVEL)
     c.set_mesh(m["xyz"], m["tets"], m["kind"], group=m["group"])
     for g in CONDUCTOR_GROUPS:
         c.set_boundary(g, BC_CONDUCTOR)
