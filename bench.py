@@ -108,7 +108,7 @@ def structural_nnz(tets, n_nodes):
     return int(2 * np.unique(key).size + n_nodes)
 
 
-def box_context(case, sort_interval, row_patterns=True, precond="two-level"):
+def box_context(case, sort_interval, row_patterns=True, precond="two-level", cell_order=0):
     """The box workload's mixed-precision context, initialized."""
     import scdat_amd as S
     m = case["mesh"]
@@ -123,14 +123,16 @@ def box_context(case, sort_interval, row_patterns=True, precond="two-level"):
         sid = ctx.add_species(q, mass, case["w"], len(x))
         ctx.set_particles(sid, x, v, c)
     ctx.row_patterns = row_patterns
+    ctx.cell_order = cell_order
     ctx.initialize()
     return ctx
 
 
-def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns=True, precond="two-level"):
+def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns=True, precond="two-level",
+            cell_order=0):
     import scdat_amd as S
     t0 = time.time()
-    ctx = box_context(case, sort_interval, row_patterns, precond)
+    ctx = box_context(case, sort_interval, row_patterns, precond, cell_order)
     chunks = ctx.autotune()
     log(f"gpu setup {time.time() - t0:.1f}s (push tile: {chunks} particles per thread); warmup {warmup} steps")
     for _ in range(warmup):
@@ -578,6 +580,8 @@ def main():
                     help="1: the FP32 inner iteration may read the operator as row patterns")
     ap.add_argument("--precond", choices=("jacobi", "two-level"), default="two-level",
                     help="Poisson preconditioner; two-level uses the tiled node order")
+    ap.add_argument("--cell-order", type=int, choices=(0, 1), default=0,
+                    help="order of the cells along the sorted particles: 0 tet index, 1 Morton")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -605,7 +609,7 @@ def main():
     name, ncu = S.device_info()
     case = build_case(args.hex, args.particles, args.seed)
     ctx, g = run_gpu(case, args.steps, args.warmup, args.sort_interval, not args.no_kernel_timing,
-                     bool(args.row_patterns), args.precond)
+                     bool(args.row_patterns), args.precond, args.cell_order)
 
     m = case["mesh"]
     n_nodes, n_tets = len(m["kind"]), len(m["tets"])
@@ -706,7 +710,8 @@ def main():
                if pcg_ms > 0 else 0.0, "note": "algorithmic bytes of the PCG iteration kernel per inner iteration over the "
                "whole Poisson stage time (FP64 residuals, launches and host polls included)"},
         "stage_ms_total": g["stages"],
-        "launch_shape": {"push_particles_per_thread": g["chunks"], "chosen_by": "scdat_autotune"},
+        "launch_shape": {"push_particles_per_thread": g["chunks"], "chosen_by": "scdat_autotune",
+                         "cell_order": "morton" if args.cell_order == 1 else "tet index"},
         "solver": {"rows": sv["rows"], "ell_width": width, "row_patterns": npat, "pcg_block": sv["pcg_block"],
                    "preconditioner": "jacobi" if sv["preconditioner"] == S.PRECOND_JACOBI else "two-level",
                    "coarse_unknowns": nc, "coarse_in_kernel": sv["coarse_in_kernel"],
@@ -723,7 +728,7 @@ def main():
         res["gpu_over_cpu"] = value / res["cpu_baseline"]["value"]
         # this path against the FP64 oracle over the oracle's own steps
         ctx.close()
-        chk = box_context(case, args.sort_interval, bool(args.row_patterns), args.precond)
+        chk = box_context(case, args.sort_interval, bool(args.row_patterns), args.precond, args.cell_order)
         chk.step(c["steps"])
         _, phi_g, _ = chk.fields()
         vf_g = chk.stats()["floating_potential"]

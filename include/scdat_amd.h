@@ -619,10 +619,11 @@ int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
  * SCDAT_SHAPE_PUSH_CHUNKS: particles per thread of the push kernel, whose
  * deposit tile is chunks x 256 consecutive (cell-sorted) particles: 1, 2, 4
  * (default) or 8.  scdat_autotune (after scdat_initialize) times the push at
- * every shape on scratch copies of all species, after half a sort interval
- * of pushes in the current field, and keeps the fastest; the simulation
- * state is untouched.  Every shape gives the same bits.  No counterpart in
- * the reference.                                                           */
+ * every shape on cell-sorted scratch copies of all species, every push of
+ * one sort cycle of each species, in the current field, and
+ * keeps the shape with the least summed average; the simulation state is
+ * untouched.  Every shape gives the same bits.  No counterpart in the
+ * reference.                                                               */
 #define SCDAT_SHAPE_PUSH_CHUNKS 0
 /* SCDAT_SHAPE_ROW_PATTERNS: 1 (default) lets the FP32 inner iteration read
  * the operator as per-row pattern ids into a table of distinct rows (column
@@ -646,6 +647,15 @@ int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
  * operator entry joins lattice neighbours; 0 keeps the row patterns.  Same
  * bits either way.                                                        */
 #define SCDAT_SHAPE_TILE_STAGING 3
+/* SCDAT_SHAPE_CELL_ORDER: order of the cells along the sorted particle
+ * arrays.  0 (default): tet index order, in which a block's particles read
+ * consecutive tet records; 1: the cells' centroids along a Morton curve
+ * (inside each scenario), so that a stretch of the push covers a compact
+ * region while its particles drift between sorts.  Equal cells stay
+ * contiguous either way; fields, particles and ledger counts are the same
+ * bit for bit (the ledger's FP64 energy sums are atomics in block order:
+ * equal to rounding).                                                     */
+#define SCDAT_SHAPE_CELL_ORDER 4
 int scdat_set_launch_shape(scdat_ctx* ctx, int32_t what, int32_t value);
 int scdat_get_launch_shape(scdat_ctx* ctx, int32_t what, int32_t* value);
 int scdat_autotune(scdat_ctx* ctx);

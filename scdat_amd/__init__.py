@@ -130,6 +130,7 @@ SHAPE_PUSH_CHUNKS = 0
 SHAPE_ROW_PATTERNS = 1
 SHAPE_PCG_BLOCK = 2
 SHAPE_TILE_STAGING = 3
+SHAPE_CELL_ORDER = 4
 
 
 class ScenarioStats(C.Structure):
@@ -725,6 +726,19 @@ class Context:
     @tile_staging.setter
     def tile_staging(self, mode):
         self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_TILE_STAGING, int(mode)))
+
+    @property
+    def cell_order(self):
+        """Order of the cells along the sorted particle arrays: 0 (default)
+        the tet index, 1 the centroids' Morton order.  Same fields, particles
+        and ledger counts either way."""
+        v = C.c_int32()
+        self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_CELL_ORDER, C.byref(v)))
+        return v.value
+
+    @cell_order.setter
+    def cell_order(self, mode):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_CELL_ORDER, int(mode)))
 
     def charging_forward(self, params, steps, events=EVENTS_FREE):
         """Floating potential after `steps` steps of the differentiable charging

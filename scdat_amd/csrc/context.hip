@@ -678,6 +678,10 @@ struct scdat_ctx {
   DevBuf d_ledger_count, d_ledger_energy, d_lost, d_pushes;
   // sort scratch
   DevBuf d_count, d_offset, d_cursor, d_tiles, d_total, d_perm, d_stage;
+  // rank of every cell in the sorted particle order (SCDAT_SHAPE_CELL_ORDER)
+  int cell_order = 0;
+  bool cell_key_ready = false;
+  DevBuf d_cell_key;
   std::deque<SpeciesState> sp;
   double unit = 0.0, w_ref = 0.0, q_init = 0.0, ccap = 0.0;
   int64_t steps = 0;
@@ -1790,11 +1794,16 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   return a;
 }
 
-// Launch-shape tuner: for 1, 2, 4 and 8 particles per thread, pushes scratch
-// copies of every species for half a sort interval (2..10 pushes, so the
-// particle order is as disordered as it is on average between sorts) in the
-// current field and times the last two pushes; keeps the fastest.  The
+// Launch-shape tuner: for 1, 2, 4 and 8 particles per thread, pushes
+// cell-sorted scratch copies of every species for one sort cycle of that
+// species (its sort cadence, at most 20 pushes; 10 without sorting) in the
+// current field and times every push: a species' push slows as its
+// particles drift from cell order, so the cycle's average is what the
+// stepping loop sees.  Keeps the shape with the least summed average.  The
 // context's particles, charge, ledger and statistics are untouched.
+template <typename T>
+uint32_t sort_species_copy(scdat_ctx* c, int si);
+
 template <typename T>
 void autotune_push(scdat_ctx* c) {
   const int ns = (int)c->sp.size();
@@ -1815,27 +1824,30 @@ void autotune_push(scdat_ctx* c) {
   HIPCHK(hipMemsetAsync(led_c.p, 0, led_c.bytes, s));
   HIPCHK(hipMemsetAsync(led_e.p, 0, led_e.bytes, s));
   HIPCHK(hipMemsetAsync(cnt.p, 0, 32, s));
-  const int P = std::max(2, std::min(10, c->cfg.sort_interval / 2));
+  std::vector<int> P((size_t)ns);
+  int Pmax = 1;
+  for (int si = 0; si < ns; ++si) {
+    const int every = c->sp[si].sort_every > 0 ? c->sp[si].sort_every : c->cfg.sort_interval;
+    P[si] = every > 0 ? std::min(every, 20) : 10;
+    Pmax = std::max(Pmax, P[si]);
+  }
   const int cand[4] = {1, 2, 4, 8};
   double best = 1e300;
   int pick = c->push_chunks;
+  std::vector<int64_t> nsc((size_t)ns, 0);
   for (int k = 0; k < 4; ++k) {
-    for (int si = 0; si < ns; ++si) {
-      SpeciesState& S = c->sp[si];
-      for (int r = 0; r < 8 && S.n; ++r) {
-        const size_t es = r < 6 ? sizeof(T) : 4;
-        HIPCHK(hipMemcpyAsync(S.buf[1 - S.cur][r].p, S.buf[S.cur][r].p, es * (size_t)S.n, hipMemcpyDeviceToDevice,
-                              s));
-      }
-    }
+    // scratch copies in cell order, as the stepping loop has them after a sort
+    for (int si = 0; si < ns; ++si)
+      if (c->sp[si].n) nsc[si] = sort_species_copy<T>(c, si);
     double tsum = 0.0;
-    for (int p = 0; p < P; ++p) {
+    for (int p = 0; p < Pmax; ++p) {
       HIPCHK(hipMemsetAsync(rho.p, 0, rho.bytes, s));
       for (int si = 0; si < ns; ++si) {
         SpeciesState& S = c->sp[si];
-        if (!S.n) continue;
+        if (!nsc[si] || p >= P[si]) continue;
         const int dst = 1 - S.cur;
         PushArgs<T> a = push_args<T>(c, si);
+        a.n = nsc[si];
         a.x = S.buf[dst][0].as<T>();
         a.y = S.buf[dst][1].as<T>();
         a.z = S.buf[dst][2].as<T>();
@@ -1859,12 +1871,11 @@ void autotune_push(scdat_ctx* c) {
         HIPCHK(hipGetLastError());
       }
       HIPCHK(hipStreamSynchronize(s));
-      if (p < P - 2) continue;  // time the last two pushes
       for (int si = 0; si < ns; ++si) {
-        if (!c->sp[si].n) continue;
+        if (!nsc[si] || p >= P[si]) continue;
         float t = 0.f;
         HIPCHK(hipEventElapsedTime(&t, c->kt_pool[2 * si], c->kt_pool[2 * si + 1]));
-        tsum += t;
+        tsum += t / P[si];
       }
     }
     if (tsum < best) {
@@ -1914,10 +1925,56 @@ void do_push(scdat_ctx* c, std::vector<std::pair<size_t, size_t>>* ev, size_t* e
   c->rho_target = 1 - c->rho_target;
 }
 
+static uint64_t morton3(const double* p, const double* lo, const double* hi);
+
+// Sort rank of every cell: its centroid's position along a Morton curve over
+// the bounding box of its scenario (scenario ranges kept, ties by index), so
+// that a stretch of sorted particles covers a compact region, in every
+// direction, and keeps it while the particles drift between sorts.
+void build_cell_key(scdat_ctx* c) {
+  const HostMesh& h = c->h;
+  const int64_t nt = h.nt;
+  const int ns = c->segmented() ? c->n_seg() : 1;
+  auto seg_of = [&](int64_t t) { return c->segmented() ? c->tet_seg_h[t] : 0; };
+  std::vector<double> cen(3 * (size_t)nt);
+  std::vector<double> lo(3 * (size_t)ns, 1e300), hi(3 * (size_t)ns, -1e300);
+  for (int64_t t = 0; t < nt; ++t) {
+    const int sg = seg_of(t);
+    for (int r = 0; r < 3; ++r) {
+      double x = 0.0;
+      for (int v = 0; v < 4; ++v) x += h.xyz[3 * (size_t)h.tets[4 * t + v] + r];
+      cen[3 * t + r] = 0.25 * x;
+      lo[3 * sg + r] = std::min(lo[3 * sg + r], cen[3 * t + r]);
+      hi[3 * sg + r] = std::max(hi[3 * sg + r], cen[3 * t + r]);
+    }
+  }
+  std::vector<std::pair<uint64_t, int32_t>> key((size_t)nt);
+  for (int64_t t = 0; t < nt; ++t) {
+    const int sg = seg_of(t);
+    key[t] = {morton3(&cen[3 * t], &lo[3 * sg], &hi[3 * sg]), (int32_t)t};
+  }
+  // scenarios are consecutive tet ranges: sort inside each
+  int64_t b = 0;
+  while (b < nt) {
+    int64_t e = b + 1;
+    while (e < nt && seg_of(e) == seg_of(b)) ++e;
+    std::sort(key.begin() + b, key.begin() + e);
+    b = e;
+  }
+  std::vector<int32_t> rank((size_t)nt);
+  for (int64_t k = 0; k < nt; ++k) rank[key[k].second] = (int32_t)k;
+  c->d_cell_key.upload(rank, c->stream);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->cell_key_ready = true;
+}
+
+// Sorts species si from its current buffer into the other one and returns
+// the alive count; the caller decides whether the sorted copy becomes the
+// species' state.
 template <typename T>
-void do_sort_species(scdat_ctx* c, int si) {
+uint32_t sort_species_copy(scdat_ctx* c, int si) {
   SpeciesState& S = c->sp[si];
-  if (S.n == 0) return;
+  if (c->cell_order == 1 && !c->cell_key_ready) build_cell_key(c);
   SortArgs<T> a{};
   const int s0 = S.cur, s1 = 1 - S.cur;
   a.src = {S.buf[s0][0].as<T>(), S.buf[s0][1].as<T>(), S.buf[s0][2].as<T>(), S.buf[s0][3].as<T>(),
@@ -1928,6 +1985,7 @@ void do_sort_species(scdat_ctx* c, int si) {
            S.buf[s1][7].as<int32_t>()};
   a.n = S.n;
   a.ncell = c->h.nt;
+  a.key = c->cell_order == 1 ? c->d_cell_key.as<int32_t>() : nullptr;
   a.count = c->d_count.as<uint32_t>();
   a.offset = c->d_offset.as<uint32_t>();
   a.cursor = c->d_cursor.as<uint32_t>();
@@ -1942,7 +2000,15 @@ void do_sort_species(scdat_ctx* c, int si) {
   uint32_t tot = 0;
   HIPCHK(hipMemcpyAsync(&tot, a.total, 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  S.cur = s1;
+  return tot;
+}
+
+template <typename T>
+void do_sort_species(scdat_ctx* c, int si) {
+  SpeciesState& S = c->sp[si];
+  if (S.n == 0) return;
+  const uint32_t tot = sort_species_copy<T>(c, si);
+  S.cur = 1 - S.cur;
   S.n = tot;
 }
 
@@ -2672,6 +2738,7 @@ void mesh_setup(scdat_ctx* c, int64_t nn, const double* xyz, int64_t nt, const i
   else upload_mesh_device<float>(c);
   c->has_mesh = true;
   c->initialized = false;
+  c->cell_key_ready = false;
 }
 
 // Back to the caller's node order (set_scenarios: scenarios are ranges of
@@ -4321,6 +4388,12 @@ int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
     c->use_box = value != 0;
     return SCDAT_OK;
   }
+  if (what == SCDAT_SHAPE_CELL_ORDER) {
+    if (value != 0 && value != 1) return fail(c, SCDAT_ERR_ARG, "set_launch_shape: cell order must be 0 or 1");
+    if (value != c->cell_order) c->cell_key_ready = false;
+    c->cell_order = value;
+    return SCDAT_OK;
+  }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS || !(value == 1 || value == 2 || value == 4 || value == 8))
     return fail(c, SCDAT_ERR_ARG, "set_launch_shape: push chunks must be 1, 2, 4 or 8");
   c->push_chunks = value;
@@ -4339,6 +4412,10 @@ int scdat_get_launch_shape(scdat_ctx* c, int32_t what, int32_t* value) {
   }
   if (what == SCDAT_SHAPE_TILE_STAGING) {
     *value = c->use_box ? 1 : 0;
+    return SCDAT_OK;
+  }
+  if (what == SCDAT_SHAPE_CELL_ORDER) {
+    *value = c->cell_order;
     return SCDAT_OK;
   }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS) return SCDAT_ERR_ARG;

@@ -1,11 +1,13 @@
 // Counting sort of one species by cell, removing absorbed particles.
 // Replaces ParticleContainer::removeInactiveParticles
 // (Tools/Particle/src/ParticleManager.cpp:157-177) and keeps the SoA arrays in
-// tet order, which is what lets the fused push kernel share tet records and
-// reduce deposition within a wavefront.
-//   1. histogram of alive particles per cell (32-bit atomics)
+// cell order, which is what lets the fused push kernel share tet records and
+// reduce deposition within a wavefront.  The cells are ranked by a key table
+// (key[cell], a permutation of the cells; null: the tet index), so the order
+// of the cells along the sorted arrays can follow a space-filling curve.
+//   1. histogram of alive particles per cell rank (32-bit atomics)
 //   2. exclusive scan of the histogram (block scan + block totals + add)
-//   3. scatter of every SoA field to its slot (offset[cell] + atomic cursor)
+//   3. scatter of every SoA field to its slot (offset[rank] + atomic cursor)
 // The order inside one cell depends on atomic arrival; no result depends on
 // it (per-particle updates are independent, deposition is integer), and the
 // particle id travels with the particle.
@@ -37,13 +39,16 @@ __device__ __forceinline__ void wave_run(int key, int& start, int& len) {
   len = (above ? __builtin_ctzll(above) : 64) - start;
 }
 
+__device__ __forceinline__ int cell_rank(const int32_t* key, int c) { return key ? key[c] : c; }
+
 // Particles per cell: one atomic per run of equal cell.
-__global__ void __launch_bounds__(kBlock) k_hist(const int32_t* cell, int64_t n, uint32_t* cnt) {
+__global__ void __launch_bounds__(kBlock) k_hist(const int32_t* cell, const int32_t* key, int64_t n,
+                                                  uint32_t* cnt) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int c = i < n ? cell[i] : -1;
   int start, len;
   wave_run(c, start, len);
-  if (c >= 0 && (int)(threadIdx.x & 63) == start) atomicAdd(&cnt[c], (uint32_t)len);
+  if (c >= 0 && (int)(threadIdx.x & 63) == start) atomicAdd(&cnt[cell_rank(key, c)], (uint32_t)len);
 }
 
 // Exclusive scan of one tile of 1024 counts; tile total to tot[blockIdx.x].
@@ -120,11 +125,12 @@ __global__ void __launch_bounds__(kBlock) k_rank(SortArgs<T> a) {
   int start, len;
   wave_run(c, start, len);
   const int lane = threadIdx.x & 63;
+  const int k = c >= 0 ? cell_rank(a.key, c) : -1;
   uint32_t base = 0u;
-  if (c >= 0 && lane == start) base = atomicAdd(&a.cursor[c], (uint32_t)len);
+  if (c >= 0 && lane == start) base = atomicAdd(&a.cursor[k], (uint32_t)len);
   base = __shfl(base, start, 64);
   if (c >= 0) {
-    a.perm[a.offset[c] + base + (uint32_t)(lane - start)] = (uint32_t)i;
+    a.perm[a.offset[k] + base + (uint32_t)(lane - start)] = (uint32_t)i;
     SortRec<T> r;
     r.x = a.src.x[i];
     r.y = a.src.y[i];
@@ -163,7 +169,8 @@ template <typename T>
 void launch_sort(const SortArgs<T>& a, hipStream_t s) {
   // zero-fill by kernel (no memset nodes, so the sort can be graph-captured)
   hipLaunchKernelGGL(k_zero_u32, dim3(blocks(a.ncell, kBlock)), dim3(kBlock), 0, s, a.count, a.ncell);
-  if (a.n > 0) hipLaunchKernelGGL(k_hist, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a.src.cell, a.n, a.count);
+  if (a.n > 0) hipLaunchKernelGGL(k_hist, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a.src.cell, a.key, a.n,
+                                a.count);
   const uint32_t nt = blocks(a.ncell, kScanTile);
   hipLaunchKernelGGL(k_scan_tile, dim3(nt), dim3(kBlock), 0, s, a.count, a.offset, a.ncell, a.tiles);
   hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kBlock), 0, s, a.tiles, (int64_t)nt, a.total);

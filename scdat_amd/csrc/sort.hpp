@@ -27,6 +27,7 @@ struct SortArgs {
   SoA<T> src, dst;
   int64_t n;        // source slots
   int64_t ncell;    // tets
+  const int32_t* key;  // [ncell] rank of each cell in the sorted order (null: the cell index)
   uint32_t* count;  // [ncell]
   uint32_t* offset; // [ncell]
   uint32_t* cursor; // [ncell]
