@@ -460,8 +460,10 @@ int scdat_circuit_size(const scdat_circuit* c, int32_t* n_nodes, int32_t* n_bran
  * results equal those of the same scenario run alone in its own context.
  * Any array may be NULL (cfg.dt, 1, the context seed).  Call after
  * scdat_set_mesh and before scdat_set_particles; particle ids then count
- * within each scenario.  Background collisions, photoemission and conductor
- * patches need a single scenario.  This batched form has no counterpart in
+ * within each scenario.  Background collisions (each scenario with its own
+ * target densities, scdat_set_scenario_mcc_density) and photoemission (per
+ * scenario through scdat_set_scenario_emission) run per scenario; conductor
+ * patches and field emission need a single scenario.  This batched form has no counterpart in
  * the reference, which runs one PICCycle per case
  * (PicMccSurfaceCurrentSampler.cpp:519-586 builds one per window).        */
 #define SCDAT_SEGMENT_ALIGN 512
@@ -470,6 +472,13 @@ int scdat_set_scenarios(scdat_ctx* ctx, int32_t n, const int64_t* node_start, co
 /* Surface emission of one boundary group of one scenario
  * (scdat_set_surface_emission sets the group in every scenario). */
 int scdat_set_scenario_emission(scdat_ctx* ctx, int32_t scenario, int32_t group, const scdat_surface_emission* e);
+/* Target density of background-collision process `process` (an index into
+ * the scdat_mcc_configure list) in one scenario; every scenario starts at the
+ * process's own density, and scdat_mcc_configure resets them.  The
+ * scenario's majorant, collision probabilities and random key are its own,
+ * so its collisions equal those of the scenario run alone with that
+ * density (executeBackgroundMccStep per case, CollisionPicAdapter.cpp:338-434). */
+int scdat_set_scenario_mcc_density(scdat_ctx* ctx, int32_t scenario, int32_t process, double density);
 typedef struct {
   int64_t completed_steps;
   double simulation_time, time_step, weight_scale;

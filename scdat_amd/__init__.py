@@ -298,6 +298,7 @@ SIGNATURES = {
     "scdat_charging_forward_events": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, C.c_int32, _D]),
     "scdat_charging_adjoint": (C.c_int, [_P, C.POINTER(ChargingParams), C.c_int32, C.c_int32, _D, _D]),
     "scdat_set_scenario_emission": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(SurfaceEmission)]),
+    "scdat_set_scenario_mcc_density": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double]),
     "scdat_get_scenario_stats": (C.c_int, [_P, C.c_int32, C.POINTER(ScenarioStats)]),
     "scdat_get_scenario_ledger": (C.c_int, [_P, C.c_int32, C.POINTER(LedgerEntry), C.c_int32]),
     "scdat_set_patch_didv": (C.c_int, [_P, _D]),
@@ -624,9 +625,15 @@ class Context:
         self.n_scenarios = len(ns) - 1
 
     def set_scenario_emission(self, scenario, group, see_model=SEE_OFF, electron_species=0, delta_max=0.0,
-                              energy_max=400.0, see_temperature=2.0):
-        e = SurfaceEmission(see_model, electron_species, delta_max, energy_max, see_temperature, 0.0, 2.0, 0.0)
+                              energy_max=400.0, see_temperature=2.0, photo_j=0.0, photo_temperature=2.0,
+                              photo_reference_v=0.0):
+        e = SurfaceEmission(see_model, electron_species, delta_max, energy_max, see_temperature, photo_j,
+                            photo_temperature, photo_reference_v)
         self._chk(lib().scdat_set_scenario_emission(self.h, scenario, group, C.byref(e)))
+
+    def set_scenario_mcc_density(self, scenario, process, density):
+        """Target density of collision process `process` in one scenario."""
+        self._chk(lib().scdat_set_scenario_mcc_density(self.h, scenario, process, density))
 
     def scenario_stats(self, scenario):
         st = ScenarioStats()

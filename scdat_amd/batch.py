@@ -86,20 +86,38 @@ def all_case_inputs(scenarios, workers=16):
         return list(ex.map(case_inputs, scenarios))
 
 
+SUNLIT_GROUP = 7  # the conductor's +x face
+OXYGEN_ION = 2
+
+
 def _emission(ctx, scenario, sc):
+    """Sternglass secondary emission on the conductor and, when the case has
+    `photo_j` (A/m^2), photoemission of 2 eV electrons from its +x face."""
     for g in CONDUCTOR_GROUPS:
+        pj = float(sc.get("photo_j", 0.0)) if g == SUNLIT_GROUP else 0.0
         if scenario is None:
-            ctx.set_surface_emission(g, SEE_STERNGLASS, 0, sc["delta_max"], 400.0, 2.0)
+            ctx.set_surface_emission(g, SEE_STERNGLASS, 0, sc["delta_max"], 400.0, 2.0, photo_j=pj)
         else:
-            ctx.set_scenario_emission(scenario, g, SEE_STERNGLASS, 0, sc["delta_max"], 400.0, 2.0)
+            ctx.set_scenario_emission(scenario, g, SEE_STERNGLASS, 0, sc["delta_max"], 400.0, 2.0, photo_j=pj)
+
+
+def _has_mcc(scenarios):
+    return any(float(sc.get("neutral_density", 0.0)) > 0.0 for sc in scenarios)
+
+
+def _processes(density):
+    from .geo import device_process, processes
+    return [device_process(p) for p in processes(density)]
 
 
 def standalone_context(sc, inputs=None, precision=0, tol=1e-8, sort_interval=20, see=True, capacity_factor=2,
-                       autotune=False, like_batch=True):
+                       autotune=False, like_batch=True, mcc=None):
     """The single-case context of case `sc`.  like_batch: the settings a
     batched case runs with (the caller's node order, Jacobi-PCG, 512-row
     PCG blocks), under which the batched case reproduces it bit for bit;
-    otherwise the context's defaults."""
+    otherwise the context's defaults.  mcc: the oxygen background of the
+    GEO case at the case's `neutral_density` (with an O+ species); None: when
+    that density is positive."""
     st = inputs or case_inputs(sc)
     m = st["mesh"]
     c = Context(precision=precision, dt=st["dt"], pcg_tol=tol, sort_interval=sort_interval)
@@ -114,6 +132,11 @@ def standalone_context(sc, inputs=None, precision=0, tol=1e-8, sort_interval=20,
     n = len(st["e"][0])
     c.add_species(-ELEMENTARY_CHARGE, ELECTRON_MASS, st["w"], capacity_factor * n)
     c.add_species(ELEMENTARY_CHARGE, PROTON_MASS, st["w"], capacity_factor * n)
+    if mcc is None:
+        mcc = _has_mcc([sc])
+    if mcc:
+        c.add_species(ELEMENTARY_CHARGE, 16.0 * PROTON_MASS, st["w"], n)
+        c.mcc_configure(_processes(float(sc.get("neutral_density", 0.0))))
     if see:
         _emission(c, None, sc)
     c.set_particles(0, *st["e"])
@@ -170,6 +193,13 @@ def batch_context(scenarios, inputs=None, precision=0, tol=1e-8, sort_interval=2
     n = len(u["e"][0])
     c.add_species(-ELEMENTARY_CHARGE, ELECTRON_MASS, 1.0, capacity_factor * n)
     c.add_species(ELEMENTARY_CHARGE, PROTON_MASS, 1.0, capacity_factor * n)
+    if _has_mcc(scenarios):
+        # one process list; each case's background density is its own
+        c.add_species(ELEMENTARY_CHARGE, 16.0 * PROTON_MASS, 1.0, n)
+        c.mcc_configure(_processes(0.0))
+        for k, sc in enumerate(scenarios):
+            for p in range(3):
+                c.set_scenario_mcc_density(k, p, float(sc.get("neutral_density", 0.0)))
     if see:
         for k, sc in enumerate(scenarios):
             _emission(c, k, sc)

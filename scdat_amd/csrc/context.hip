@@ -729,6 +729,15 @@ struct scdat_ctx {
   DevBuf d_tet_seg, d_blk_seg, d_seg_b0, d_seg_unit, d_seg_dt, d_seg_wscale, d_seg_qinit, d_seg_ccap, d_kc_off,
       d_seg_emit, d_seg_key;
   bool seg_emit_dirty = false;
+  // per-scenario background-collision target densities [segment][kMaxProc]
+  // (NaN: the process's own density) and FP64 time steps; photoemission face
+  // ranges [group][segment + 1] of the per-scenario face table and their
+  // areas [group][segment]
+  std::vector<double> seg_mcc_n;
+  bool seg_mcc_dirty = true;
+  DevBuf d_seg_mcc_n, d_seg_dt64, d_seg_face_off;
+  std::vector<int64_t> seg_face_off_h;
+  std::vector<double> seg_face_area_h;
   // field-emission cathodes (ion-driven, FieldEmissionBoundaryCondition)
   std::array<scdat_field_emission, kMaxGroups> fe{};
   uint32_t fe_mask = 0;
@@ -995,6 +1004,68 @@ void build_box_patterns(scdat_ctx* c, const std::vector<float>& vh, const std::v
   c->nbpat = (int)id.size();
 }
 
+// Boundary faces per group, and within a group per scenario, in (tet, face)
+// order with cumulative areas restarting at each (group, scenario): the
+// photoemission source table.  A scenario's range is the table a context of
+// that scenario alone builds for the group, so its face choice is the same.
+void build_face_table(scdat_ctx* c) {
+  const HostMesh& h = c->h;
+  const int ns = c->segmented() ? c->n_seg() : 1;
+  struct F {
+    int32_t g, sg, t, k;
+  };
+  std::vector<F> all;
+  for (int64_t t = 0; t < h.nt; ++t)
+    for (int k = 0; k < 4; ++k) {
+      const int g = h.group[4 * t + k];
+      if (g < 0 || g >= kMaxGroups || h.nb[4 * t + k] >= 0) continue;
+      all.push_back({g, c->segmented() ? c->tet_seg_h[t] : 0, (int32_t)t, k});
+    }
+  std::stable_sort(all.begin(), all.end(), [](const F& a, const F& b) { return a.g != b.g ? a.g < b.g : a.sg < b.sg; });
+  std::vector<int32_t> ft, fk;
+  std::vector<double> fc;
+  c->seg_face_off_h.assign((size_t)kMaxGroups * (ns + 1), 0);
+  c->seg_face_area_h.assign((size_t)kMaxGroups * ns, 0.0);
+  size_t q = 0;
+  for (int g = 0; g < kMaxGroups; ++g) {
+    c->face_off[g] = (int64_t)ft.size();
+    c->face_area[g] = 0.0;
+    for (int sg = 0; sg < ns; ++sg) {
+      c->seg_face_off_h[(size_t)g * (ns + 1) + sg] = (int64_t)ft.size();
+      double acc = 0.0;
+      for (; q < all.size() && all[q].g == g && all[q].sg == sg; ++q) {
+        const int64_t t = all[q].t;
+        const int k = all[q].k;
+        const double* p[3];
+        int m = 0;
+        for (int r = 0; r < 4; ++r)
+          if (r != k) p[m++] = &h.xyz[3 * (size_t)h.tets[4 * t + r]];
+        const double u[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+        const double w[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+        const double cr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+        acc += 0.5 * std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        ft.push_back((int32_t)t);
+        fk.push_back(k);
+        fc.push_back(acc);
+      }
+      c->seg_face_area_h[(size_t)g * ns + sg] = acc;
+      c->face_area[g] += acc;
+    }
+    c->seg_face_off_h[(size_t)g * (ns + 1) + ns] = (int64_t)ft.size();
+  }
+  c->face_off[kMaxGroups] = (int64_t)ft.size();
+  if (ft.empty()) {
+    ft.push_back(0);
+    fk.push_back(0);
+    fc.push_back(0.0);
+  }
+  hipStream_t s = c->stream;
+  c->d_face_tet.upload(ft, s);
+  c->d_face_k.upload(fk, s);
+  c->d_face_cum.upload(fc, s);
+  c->d_seg_face_off.upload(c->seg_face_off_h, s);
+}
+
 template <typename T>
 void upload_mesh_device(scdat_ctx* c) {
   HostMesh& h = c->h;
@@ -1010,42 +1081,7 @@ void upload_mesh_device(scdat_ctx* c) {
   c->d_tets.upload(tm, s);
   c->d_xyz64.upload(h.xyz, s);
   c->d_tet4.upload(h.tets, s);
-  {
-    // boundary faces per group in (tet, face) order with cumulative areas,
-    // the photoemission source table
-    std::vector<int32_t> ft, fk;
-    std::vector<double> fc;
-    for (int g = 0; g < kMaxGroups; ++g) {
-      c->face_off[g] = (int64_t)ft.size();
-      double acc = 0.0;
-      c->face_area[g] = 0.0;
-      for (int64_t t = 0; t < h.nt; ++t)
-        for (int k = 0; k < 4; ++k) {
-          if (h.group[4 * t + k] != g || h.nb[4 * t + k] >= 0) continue;
-          const double* q[3];
-          int m = 0;
-          for (int r = 0; r < 4; ++r)
-            if (r != k) q[m++] = &h.xyz[3 * (size_t)h.tets[4 * t + r]];
-          const double u[3] = {q[1][0] - q[0][0], q[1][1] - q[0][1], q[1][2] - q[0][2]};
-          const double w[3] = {q[2][0] - q[0][0], q[2][1] - q[0][1], q[2][2] - q[0][2]};
-          const double cr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
-          acc += 0.5 * std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
-          ft.push_back((int32_t)t);
-          fk.push_back(k);
-          fc.push_back(acc);
-          c->face_area[g] = acc;
-        }
-    }
-    c->face_off[kMaxGroups] = (int64_t)ft.size();
-    if (ft.empty()) {
-      ft.push_back(0);
-      fk.push_back(0);
-      fc.push_back(0.0);
-    }
-    c->d_face_tet.upload(ft, s);
-    c->d_face_k.upload(fk, s);
-    c->d_face_cum.upload(fc, s);
-  }
+  build_face_table(c);
   c->d_nb.upload(h.nb, s);
   if constexpr (sizeof(T) == 4) {
     std::vector<WalkMap<float>> wm((size_t)h.nt);
@@ -1678,24 +1714,30 @@ int pack_procs(const scdat_ctx* c, int si, MccProcDev* out) {
 // Before a step's push: the majorant of every colliding species from the
 // largest |v|^2 of the previous push (from the particles themselves after a
 // load), which also clears the push's accumulators.
+void upload_seg_mcc(scdat_ctx* c);
+
 template <typename T>
 void mcc_prepare(scdat_ctx* c) {
   hipStream_t s = c->stream;
   const int ns = (int)c->sp.size();
-  if (c->d_mcc_ctl.bytes < 8 * (size_t)kMccCtl * kMaxSpecies) {
-    c->d_mcc_ctl.alloc(8 * (size_t)kMccCtl * kMaxSpecies);
-    c->d_mcc_v2max.alloc(8 * (size_t)kMaxSpecies);
+  // controls and |v|^2 maxima per (species, scenario)
+  const size_t nseg = c->segmented() ? (size_t)c->n_seg() : 1;
+  if (c->d_mcc_ctl.bytes < 8 * (size_t)kMccCtl * kMaxSpecies * nseg) {
+    c->d_mcc_ctl.alloc(8 * (size_t)kMccCtl * kMaxSpecies * nseg);
+    c->d_mcc_v2max.alloc(8 * (size_t)kMaxSpecies * nseg);
     c->d_mcc_ncand.alloc(4 * (size_t)kMaxSpecies);
     c->mcc_v2_stale = true;
   }
+  if (c->segmented() && c->seg_mcc_dirty) upload_seg_mcc(c);
   unsigned long long* v2 = c->d_mcc_v2max.as<unsigned long long>();
   if (c->mcc_v2_stale) {
-    launch_fill_u64(v2, kMaxSpecies, s);
+    launch_fill_u64(v2, (int64_t)(kMaxSpecies * nseg), s);
     for (int si = 0; si < ns; ++si) {
       SpeciesState& S = c->sp[si];
       if (!species_has_mcc(c, si) || S.n == 0) continue;
       launch_v2max<T>(S.buf[S.cur][3].as<T>(), S.buf[S.cur][4].as<T>(), S.buf[S.cur][5].as<T>(),
-                      S.buf[S.cur][6].as<int32_t>(), S.n, v2 + si, s);
+                      S.buf[S.cur][6].as<int32_t>(), S.n, v2 + si * nseg,
+                      c->segmented() ? c->d_tet_seg.as<int32_t>() : nullptr, s);
     }
     c->mcc_v2_stale = false;
   }
@@ -1707,9 +1749,14 @@ void mcc_prepare(scdat_ctx* c) {
     m.np = pack_procs(c, si, m.proc);
     m.mass = S.m;
     m.dt = c->cfg.dt;
-    m.v2max = v2 + si;
+    m.v2max = v2 + si * nseg;
     m.n_cand = c->d_mcc_ncand.as<unsigned int>() + si;
-    m.ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si;
+    m.ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si * nseg;
+    if (c->segmented()) {
+      m.seg_n = c->d_seg_mcc_n.as<double>();
+      m.seg_dt = c->d_seg_dt64.as<double>();
+      m.n_seg = (int)nseg;
+    }
     launch_mcc_majorant(m, s);
   }
   HIPCHK(hipGetLastError());
@@ -1783,10 +1830,12 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   }
   a.chunks = c->push_chunks;
   if (c->push_mcc && species_has_mcc(c, si)) {
-    a.mcc_ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si;
+    const size_t nseg = c->segmented() ? (size_t)c->n_seg() : 1;
+    a.mcc_ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si * nseg;
     a.mcc_cand = S.mcc_cand.as<uint32_t>();
     a.mcc_ncand = c->d_mcc_ncand.as<unsigned int>() + si;
-    a.mcc_v2max = c->d_mcc_v2max.as<unsigned long long>() + si;
+    a.mcc_v2max = c->d_mcc_v2max.as<unsigned long long>() + si * nseg;
+    if (c->segmented()) a.mcc_seg_key = c->d_seg_key.as<uint32_t>();
     a.mcc_step = (uint32_t)c->steps;
     a.mcc_key0 = c->key[0];
     a.mcc_key1 = c->key[1];
@@ -2044,14 +2093,7 @@ void upload_seg_emit(scdat_ctx* c) {
 template <typename T>
 void upload_segments(scdat_ctx* c) {
   if (!c->segmented()) return;
-  for (const auto& q : c->mcc)
-    if (q.target_density > 0.0) throw Error(SCDAT_ERR_STATE, "background collisions need a single scenario");
   if (c->n_patch > 0) throw Error(SCDAT_ERR_STATE, "conductor patches need a single scenario");
-  for (const auto& q : c->emit)
-    if (q.photo_current_density > 0.0) throw Error(SCDAT_ERR_STATE, "photoemission needs a single scenario");
-  for (const auto& se : c->seg_emit)
-    for (const auto& q : se)
-      if (q.photo_current_density > 0.0) throw Error(SCDAT_ERR_STATE, "photoemission needs a single scenario");
   const int ns = c->n_seg();
   hipStream_t s = c->stream;
   c->d_tet_seg.upload(c->tet_seg_h, s);
@@ -2076,7 +2118,25 @@ void upload_segments(scdat_ctx* c) {
   c->d_kc_off.upload(c->kc_off_h, s);
   c->seg_ccap.assign((size_t)ns, 0.0);
   c->d_seg_ccap.upload(c->seg_ccap, s);
+  c->d_seg_dt64.upload(c->seg_dt, s);
+  build_face_table(c);
+  c->seg_mcc_dirty = true;
   upload_seg_emit(c);
+}
+
+// Per-scenario collision densities on the device: each scenario's override,
+// else the process's own target density.
+void upload_seg_mcc(scdat_ctx* c) {
+  const int ns = c->n_seg();
+  std::vector<double> d((size_t)ns * kMaxProc, 0.0);
+  for (int sg = 0; sg < ns; ++sg)
+    for (size_t k = 0; k < c->mcc.size() && k < (size_t)kMaxProc; ++k) {
+      const double o = c->seg_mcc_n.size() == d.size() ? c->seg_mcc_n[(size_t)sg * kMaxProc + k] : NAN;
+      d[(size_t)sg * kMaxProc + k] = std::isnan(o) ? c->mcc[k].target_density : o;
+    }
+  c->d_seg_mcc_n.upload(d, c->stream);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->seg_mcc_dirty = false;
 }
 
 // Sort cadence per species: a species is re-sorted by cell once its
@@ -2155,6 +2215,9 @@ bool has_post_push(const scdat_ctx* c) {
   if (!c->mcc.empty() || c->event_cap > 0 || c->fe_mask) return true;
   for (const auto& e : c->emit)
     if (e.photo_current_density > 0.0) return true;
+  for (const auto& se : c->seg_emit)
+    for (const auto& e : se)
+      if (e.photo_current_density > 0.0) return true;
   return false;
 }
 
@@ -2320,7 +2383,13 @@ void post_push(scdat_ctx* c) {
     if (np == 0 || S.n == 0) continue;
     a.cand = S.mcc_cand.as<uint32_t>();
     a.n_cand = c->d_mcc_ncand.as<unsigned int>() + si;
-    a.ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si;
+    a.ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si * (c->segmented() ? (size_t)c->n_seg() : 1);
+    if (c->segmented()) {
+      a.tet_seg = c->d_tet_seg.as<int32_t>();
+      a.seg_n = c->d_seg_mcc_n.as<double>();
+      a.seg_dt = c->d_seg_dt64.as<double>();
+      a.seg_key = c->d_seg_key.as<uint32_t>();
+    }
     a.x = S.buf[S.cur][0].as<T>();
     a.y = S.buf[S.cur][1].as<T>();
     a.z = S.buf[S.cur][2].as<T>();
@@ -2387,19 +2456,39 @@ void post_push(scdat_ctx* c) {
   }
   if (c->event_cap > 0) launch_see<T>(e, c->event_cap, s);
   for (int g = 0; g < kMaxGroups; ++g) {
-    const scdat_surface_emission& q = c->emit[g];
     const int64_t f0 = c->face_off[g], f1 = c->face_off[g + 1];
-    if (!(q.photo_current_density > 0.0) || f1 <= f0) continue;
-    const double area = c->face_area[g];
-    const SpeciesState& S = c->sp[q.electron_species];
-    const double bound = q.photo_current_density * area * c->cfg.dt / (std::abs(S.q) * S.w);
-    if (bound > 4.0e9) throw Error(SCDAT_ERR_ARG, "photoemission: too many macro-electrons per step");
+    if (f1 <= f0) continue;
     EmitArgs<T> eg = e;
     eg.f0 = f0;
     eg.f1 = f1;
     eg.group = g;
     eg.is_conductor = c->gtype[g] == SCDAT_BC_CONDUCTOR && c->h.has_cond ? 1 : 0;
-    launch_photo<T>(eg, (int64_t)std::floor(bound) + 1, s);
+    double bound = -1.0;
+    if (c->segmented()) {
+      // one grid row per scenario, sized for the largest expected count
+      const int ns = c->n_seg();
+      for (int sg = 0; sg < ns; ++sg) {
+        const scdat_surface_emission& q = c->seg_emit[sg][g];
+        if (!(q.photo_current_density > 0.0)) continue;
+        const SpeciesState& S = c->sp[q.electron_species];
+        const double area = c->seg_face_area_h[(size_t)g * ns + sg];
+        bound = std::max(bound, q.photo_current_density * area * c->seg_dt[sg] /
+                                    (std::abs(S.q) * S.w * c->seg_wscale[sg]));
+      }
+      eg.seg_face_off = c->d_seg_face_off.as<int64_t>() + (size_t)g * (ns + 1);
+      eg.seg_dt = c->d_seg_dt64.as<double>();
+      eg.seg_wscale = c->d_seg_wscale.as<double>();
+      eg.vfloat_stride = (int64_t)(sizeof(PcgScalars) / sizeof(double));
+    } else {
+      const scdat_surface_emission& q = c->emit[g];
+      if (q.photo_current_density > 0.0) {
+        const SpeciesState& S = c->sp[q.electron_species];
+        bound = q.photo_current_density * c->face_area[g] * c->cfg.dt / (std::abs(S.q) * S.w);
+      }
+    }
+    if (bound < 0.0) continue;
+    if (bound > 4.0e9) throw Error(SCDAT_ERR_ARG, "photoemission: too many macro-electrons per step");
+    launch_photo<T>(eg, (int64_t)std::floor(bound) + 1, c->segmented() ? c->n_seg() : 1, s);
   }
   HIPCHK(hipGetLastError());
   // appended counts and impacts, read once the stream has synchronised
@@ -3942,6 +4031,8 @@ int scdat_mcc_configure(scdat_ctx* c, const scdat_mcc_process* procs, int32_t n)
   }
   c->mcc.assign(procs, procs + n);
   c->mcc_v2_stale = true;
+  std::fill(c->seg_mcc_n.begin(), c->seg_mcc_n.end(), NAN);
+  c->seg_mcc_dirty = true;
   return SCDAT_OK;
 }
 
@@ -3950,6 +4041,18 @@ int scdat_mcc_set_density(scdat_ctx* c, int32_t process, double density) {
   if (process < 0 || process >= (int)c->mcc.size() || !(density >= 0.0))
     return fail(c, SCDAT_ERR_ARG, "mcc_set_density: bad process or density");
   c->mcc[process].target_density = density;
+  c->seg_mcc_dirty = true;
+  return SCDAT_OK;
+}
+
+int scdat_set_scenario_mcc_density(scdat_ctx* c, int32_t scenario, int32_t process, double density) {
+  if (!c) return SCDAT_ERR_ARG;
+  if (!c->segmented() || scenario < 0 || scenario >= c->n_seg())
+    return fail(c, SCDAT_ERR_ARG, "set_scenario_mcc_density: no such scenario");
+  if (process < 0 || process >= (int)c->mcc.size() || !(density >= 0.0) || !std::isfinite(density))
+    return fail(c, SCDAT_ERR_ARG, "set_scenario_mcc_density: bad process or density");
+  c->seg_mcc_n[(size_t)scenario * kMaxProc + process] = density;
+  c->seg_mcc_dirty = true;
   return SCDAT_OK;
 }
 
@@ -4241,6 +4344,8 @@ int scdat_set_scenarios(scdat_ctx* c, int32_t n, const int64_t* node_start, cons
       c->seg_key[2 * sg + 1] = seed ? (uint32_t)(seed[sg] >> 32) : c->key[1];
     }
     c->seg_emit.assign((size_t)n, c->emit);
+    c->seg_mcc_n.assign((size_t)n * kMaxProc, NAN);
+    c->seg_mcc_dirty = true;
     const size_t m = (size_t)kMaxGroups * kMaxSpecies * n;
     for (DevBuf* b : {&c->d_ledger_count, &c->d_ledger_energy, &c->d_emit_count}) {
       b->alloc(8 * m);
@@ -4262,8 +4367,8 @@ int scdat_set_scenario_emission(scdat_ctx* c, int32_t scenario, int32_t group, c
     return fail(c, SCDAT_ERR_ARG, "set_scenario_emission: bad group, model or electron species");
   if (e->see_model == SCDAT_SEE_STERNGLASS && !(e->see_energy_max_ev > 0.0))
     return fail(c, SCDAT_ERR_ARG, "set_scenario_emission: Sternglass needs energy_max > 0");
-  if (e->photo_current_density > 0.0)
-    return fail(c, SCDAT_ERR_ARG, "set_scenario_emission: photoemission needs a single scenario");
+  if (!(e->photo_current_density >= 0.0))
+    return fail(c, SCDAT_ERR_ARG, "set_scenario_emission: photoemission current density < 0");
   return guarded(c, [&] {
     c->seg_emit[scenario][group] = *e;
     c->seg_emit_dirty = true;

@@ -160,6 +160,23 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t g) {
   return x * q + (x < r ? x : r) + idx;
 }
 
+// Atomic max of v (>= 0, FP64 bits ordered as integers) into out[s] for the
+// lanes of a wavefront, one atomic per distinct s (s < 0: no value).  Every
+// lane must call it.
+__device__ __forceinline__ void wave_seg_max(int s, double v, unsigned long long* out) {
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(s >= 0);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int s0 = __shfl(s, leader, 64);
+    double m = s == s0 ? v : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (lane == leader && m > 0.0) atomicMax(&out[s0], (unsigned long long)__double_as_longlong(m));
+    pending &= ~__ballot(s == s0);
+  }
+}
+
 // Deterministic block sum of doubles (wave64 shuffles, then LDS across waves).
 __device__ inline double block_sum(double v, double* lds) {
 #pragma unroll

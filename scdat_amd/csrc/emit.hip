@@ -115,10 +115,11 @@ __device__ bool append(const SpeciesOut<T>& o, const T x[3], const double v[3], 
   return true;
 }
 
-// Total collision frequency sum_p n_p sigma_p(E) sqrt(2E/m) of a species.
-__device__ double nu_total(const MccProcDev* proc, int np, double E, double mass) {
+// Total collision frequency sum_p n_p sigma_p(E) sqrt(2E/m) of a species;
+// dens non-null: n_p = dens[slot of p] (one scenario's densities).
+__device__ double nu_total(const MccProcDev* proc, int np, double E, double mass, const double* dens) {
   double t = 0.0;
-  for (int j = 0; j < np; ++j) t += proc[j].n * sigma_of(proc[j], E);
+  for (int j = 0; j < np; ++j) t += (dens ? dens[proc[j].slot] : proc[j].n) * sigma_of(proc[j], E);
   return t * sqrt(2.0 * E * kE / mass);
 }
 
@@ -128,7 +129,13 @@ __device__ double nu_total(const MccProcDev* proc, int np, double E, double mass
 // times kMccMajorantMargin.  Then P_null, and the push's accumulators reset.
 __global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
   __shared__ double lds[kBlock / 64];
-  const double v2max = __longlong_as_double((long long)*a.v2max);
+  // one block per scenario (one block without scenarios)
+  const int sg = blockIdx.x;
+  const double* dens = a.seg_n ? a.seg_n + (size_t)sg * kMaxProc : nullptr;
+  const double dt = a.seg_n ? a.seg_dt[sg] : a.dt;
+  unsigned long long* v2p = a.v2max + sg;
+  double* ctl = a.ctl + (size_t)sg * kMccCtl;
+  const double v2max = __longlong_as_double((long long)*v2p);
   const double v2cap = kMccCapFactor * v2max;
   const double ecap = 0.5 * a.mass * v2cap / kE;
   double f = 0.0;
@@ -136,15 +143,15 @@ __global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
     const double elo = 1e-8 * ecap, la = log(elo), lb = log(ecap);
     for (int k = threadIdx.x; k < kMccMajorantPoints; k += kBlock) {
       const double E = exp(la + (lb - la) * (double)k / (double)(kMccMajorantPoints - 1));
-      f = fmax(f, nu_total(a.proc, a.np, E, a.mass));
+      f = fmax(f, nu_total(a.proc, a.np, E, a.mass, dens));
     }
     if (threadIdx.x < a.np) {
       const double thr = a.proc[threadIdx.x].thr;
-      if (thr > elo && thr < ecap) f = fmax(f, nu_total(a.proc, a.np, thr * (1.0 + 1e-9), a.mass));
+      if (thr > elo && thr < ecap) f = fmax(f, nu_total(a.proc, a.np, thr * (1.0 + 1e-9), a.mass, dens));
     }
     if (threadIdx.x == 0) {
       double b = 0.0;
-      for (int j = 0; j < a.np; ++j) b += a.proc[j].n * a.proc[j].smax;
+      for (int j = 0; j < a.np; ++j) b += (dens ? dens[a.proc[j].slot] : a.proc[j].n) * a.proc[j].smax;
       f = fmax(f, b * sqrt(2.0 * elo * kE / a.mass));
     }
   }
@@ -156,23 +163,22 @@ __global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
     for (int w = 1; w < kBlock / 64; ++w) f = fmax(f, lds[w]);
     f = fmax(f, lds[0]);
     const double numax = kMccMajorantMargin * f;
-    a.ctl[0] = -expm1(-numax * a.dt);
-    a.ctl[1] = v2cap;
-    a.ctl[2] = numax;
-    *a.v2max = 0ull;
-    *a.n_cand = 0u;
+    ctl[0] = -expm1(-numax * dt);
+    ctl[1] = v2cap;
+    ctl[2] = numax;
+    *v2p = 0ull;
+    if (sg == 0) *a.n_cand = 0u;
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_v2max(const T* vx, const T* vy, const T* vz, const int32_t* cell,
-                                                  int64_t n, unsigned long long* out) {
+                                                  int64_t n, unsigned long long* out, const int32_t* tet_seg) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double v2 = 0.0;
-  if (i < n && cell[i] >= 0) v2 = (double)vx[i] * vx[i] + (double)vy[i] * vy[i] + (double)vz[i] * vz[i];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v2 = fmax(v2, __shfl_xor(v2, off, 64));
-  if ((threadIdx.x & 63) == 0 && v2 > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(v2));
+  const int c = i < n ? cell[i] : -1;
+  if (c >= 0) v2 = (double)vx[i] * vx[i] + (double)vy[i] * vy[i] + (double)vz[i] * vz[i];
+  wave_seg_max(c >= 0 ? (tet_seg ? tet_seg[c] : 0) : -1, v2, out);
 }
 
 // One candidate: the exact collision test given the pre-selection, process
@@ -184,6 +190,11 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
   const int cell = a.cell[i];
   if (cell < 0) return;
   atomicAdd(&a.counts[C_CAND], 1ull);
+  // one scenario's densities, time step, key and majorant
+  const int sg = a.tet_seg ? a.tet_seg[cell] : 0;
+  const double* dens = a.tet_seg ? a.seg_n + (size_t)sg * kMaxProc : nullptr;
+  const double dt = a.tet_seg ? a.seg_dt[sg] : a.dt;
+  const uint32_t key0 = a.tet_seg ? a.seg_key[2 * sg] : a.key0, key1 = a.tet_seg ? a.seg_key[2 * sg + 1] : a.key1;
   double v[3] = {(double)a.vx[i], (double)a.vy[i], (double)a.vz[i]};
   const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   const double speed = sqrt(v2);
@@ -192,15 +203,15 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
   double tot = 0.0;
 #pragma unroll
   for (int j = 0; j < kMaxProc; ++j) {
-    nu[j] = j < a.np ? a.proc[j].n * sigma_of(a.proc[j], E) * speed : 0.0;
+    nu[j] = j < a.np ? (dens ? dens[a.proc[j].slot] : a.proc[j].n) * sigma_of(a.proc[j], E) * speed : 0.0;
     tot += nu[j];
   }
   if (!(tot > 0.0)) return;
   const uint32_t id = (uint32_t)a.id[i];
-  const U4 r = philox4x32(id, a.step, (uint32_t)a.species, kTagMccA, a.key0, a.key1);
+  const U4 r = philox4x32(id, a.step, (uint32_t)a.species, kTagMccA, key0, key1);
   // accept with probability pex / P_c: overall probability pex = 1 - exp(-nu dt)
-  const double pc = forced ? 1.0 : a.ctl[0];
-  const double pex = -expm1(-tot * a.dt);
+  const double pc = forced ? 1.0 : a.ctl[(size_t)sg * kMccCtl];
+  const double pex = -expm1(-tot * dt);
   if (pex > pc) atomicAdd(&a.counts[C_MAJOR], 1ull);
   const double u = u01(r.y) * pc;
   if (!(u < pex)) return;
@@ -242,8 +253,8 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
     }
     case MCC_IONIZATION: {
       if (E <= p.thr) break;
-      const U4 b = philox4x32(id, a.step, (uint32_t)a.species, kTagMccB, a.key0, a.key1);
-      const U4 c = philox4x32(id, a.step, (uint32_t)a.species, kTagMccId, a.key0, a.key1);
+      const U4 b = philox4x32(id, a.step, (uint32_t)a.species, kTagMccB, key0, key1);
+      const U4 c = philox4x32(id, a.step, (uint32_t)a.species, kTagMccId, key0, key1);
       const double avail = E - p.thr;
       const double sec = u01(b.x) * avail;
       const double prim = avail - sec;
@@ -383,29 +394,40 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
 
 // Photoemission of one group, SurfaceMaterialModel.cpp:199-208 current
 // density at the group's potential; J A dt / (|q| w) macro-electrons per step.
+// With scenarios, grid row blockIdx.y is scenario s: its faces of the group,
+// its settings, time step, weight, key and floating potential, so a case
+// emits what it emits alone.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_photo(EmitArgs<T> a) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const SurfEmitDev& cf = a.g[a.group];
-  const double area = a.face_cum[a.f1 - 1];
-  const double V = a.is_conductor ? *a.vfloat : 0.0;
+  const int sg = blockIdx.y;
+  const bool seg = a.seg_face_off != nullptr;
+  const int64_t f0 = seg ? a.seg_face_off[sg] : a.f0, f1 = seg ? a.seg_face_off[sg + 1] : a.f1;
+  if (f1 <= f0) return;
+  const SurfEmitDev& cf = seg ? a.gseg[sg * kMaxGroups + a.group] : a.g[a.group];
+  if (!(cf.photo_j > 0.0)) return;
+  const double dt = seg ? a.seg_dt[sg] : a.dt;
+  const double qaw = seg ? a.q_abs_w[cf.e_species] * a.seg_wscale[sg] : a.q_abs_w[cf.e_species];
+  const uint32_t key0 = seg ? a.seg_key[2 * sg] : a.key0, key1 = seg ? a.seg_key[2 * sg + 1] : a.key1;
+  const double area = a.face_cum[f1 - 1];
+  const double V = a.is_conductor ? a.vfloat[seg ? sg * a.vfloat_stride : 0] : 0.0;
   const double j = fmax(0.0, cf.photo_j) * exp(-fmax(0.0, V - cf.v_ref) / fmax(1e-3, cf.t_ph));
-  const double expect = j * area * a.dt / a.q_abs_w[cf.e_species];
-  const U4 r0 = philox4x32((uint32_t)a.group, a.step, 0u, kTagPhoto, a.key0, a.key1);
+  const double expect = j * area * dt / qaw;
+  const U4 r0 = philox4x32((uint32_t)a.group, a.step, 0u, kTagPhoto, key0, key1);
   const double fl = floor(expect);
   const int64_t nem = (int64_t)fl + (u01(r0.x) < expect - fl ? 1 : 0);
   if (k >= nem) return;
-  const U4 b = philox4x32((uint32_t)k, a.step, (uint32_t)a.group, kTagPhotoK, a.key0, a.key1);
-  const U4 c = philox4x32((uint32_t)k, a.step, (uint32_t)a.group, kTagPhotoK2, a.key0, a.key1);
+  const U4 b = philox4x32((uint32_t)k, a.step, (uint32_t)a.group, kTagPhotoK, key0, key1);
+  const U4 c = philox4x32((uint32_t)k, a.step, (uint32_t)a.group, kTagPhotoK2, key0, key1);
   // face: first cumulative area above u * area
   const double rr = u01(b.x) * area;
-  int64_t lo = a.f0, hi = a.f1;  // search [lo, hi)
+  int64_t lo = f0, hi = f1;  // search [lo, hi)
   while (lo < hi) {
     const int64_t mid = lo + (hi - lo) / 2;
     if (a.face_cum[mid] > rr) hi = mid;
     else lo = mid + 1;
   }
-  const int64_t fi = lo < a.f1 ? lo : a.f1 - 1;
+  const int64_t fi = lo < f1 ? lo : f1 - 1;
   const int t = a.face_tet[fi], kf = a.face_k[fi];
   const double* p[3];
   int mm = 0;
@@ -422,7 +444,7 @@ __global__ void __launch_bounds__(kBlock) k_photo(EmitArgs<T> a) {
   emit_velocity(n, cf.t_ph, o.m, u01(b.w), u01(c.x), u01(c.y), u01(c.z), v);
   const T xt[3] = {(T)xe[0], (T)xe[1], (T)xe[2]};
   if (append(o, xt, v, t, gen_id(c.w), a.tets, a.rho, a.counts)) {
-    atomicAdd(&a.emit_count[a.group * kMaxSpecies + cf.e_species], 1ull);
+    atomicAdd(&a.emit_count[((size_t)sg * kMaxGroups + a.group) * kMaxSpecies + cf.e_species], 1ull);
     atomicAdd(&a.counts[C_PHOTO], 1ull);
   }
 }
@@ -463,9 +485,9 @@ void launch_see(const EmitArgs<T>& a, int64_t max_events, hipStream_t s) {
   hipLaunchKernelGGL(k_see<T>, dim3(blocks_for(max_events)), dim3(kBlock), 0, s, a);
 }
 template <typename T>
-void launch_photo(const EmitArgs<T>& a, int64_t max_count, hipStream_t s) {
-  if (max_count <= 0 || a.f1 <= a.f0) return;
-  hipLaunchKernelGGL(k_photo<T>, dim3(blocks_for(max_count)), dim3(kBlock), 0, s, a);
+void launch_photo(const EmitArgs<T>& a, int64_t max_count, int n_seg, hipStream_t s) {
+  if (max_count <= 0 || (!a.seg_face_off && a.f1 <= a.f0) || n_seg < 1) return;
+  hipLaunchKernelGGL(k_photo<T>, dim3(blocks_for(max_count), a.seg_face_off ? n_seg : 1), dim3(kBlock), 0, s, a);
 }
 template <typename T>
 void launch_inject(const SpeciesOut<T>& o, int64_t n, const double* x, const double* v, const int32_t* cell,
@@ -480,24 +502,24 @@ void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s) {
 }
 
 void launch_mcc_majorant(const MccMajorantArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mcc_majorant, dim3(1), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_mcc_majorant, dim3(a.seg_n ? a.n_seg : 1), dim3(kBlock), 0, s, a);
 }
 template <typename T>
 void launch_v2max(const T* vx, const T* vy, const T* vz, const int32_t* cell, int64_t n, unsigned long long* v2max,
-                  hipStream_t s) {
+                  const int32_t* tet_seg, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_v2max<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, vx, vy, vz, cell, n, v2max);
+  hipLaunchKernelGGL(k_v2max<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, vx, vy, vz, cell, n, v2max, tet_seg);
 }
 
 #define SCDAT_EMIT_INST(T)                                                        \
   template void launch_v2max<T>(const T*, const T*, const T*, const int32_t*, int64_t, unsigned long long*, \
-                                hipStream_t);                                     \
+                                const int32_t*, hipStream_t);                     \
   template void launch_inject<T>(const SpeciesOut<T>&, int64_t, const double*, const double*, const int32_t*, \
                                  const int32_t*, const TetMap<T>*, unsigned long long*, unsigned long long*, \
                                  hipStream_t);                                    \
   template void launch_mcc<T>(const MccArgs<T>&, hipStream_t);                    \
   template void launch_see<T>(const EmitArgs<T>&, int64_t, hipStream_t);          \
-  template void launch_photo<T>(const EmitArgs<T>&, int64_t, hipStream_t);
+  template void launch_photo<T>(const EmitArgs<T>&, int64_t, int, hipStream_t);
 SCDAT_EMIT_INST(float)
 SCDAT_EMIT_INST(double)
 #undef SCDAT_EMIT_INST

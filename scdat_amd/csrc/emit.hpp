@@ -79,6 +79,14 @@ struct MccArgs {
   const TetMap<T>* tets;
   unsigned long long* rho;
   unsigned long long* counts;
+  // batched scenarios (null: one scenario): segment of each tet, target
+  // density of every process slot per segment [segment][kMaxProc], FP64 time
+  // step and random key (2 words) per segment; ctl then holds kMccCtl
+  // doubles per segment
+  const int32_t* tet_seg;
+  const double* seg_n;
+  const double* seg_dt;
+  const uint32_t* seg_key;
 };
 
 struct SurfEmitDev {
@@ -119,10 +127,21 @@ struct EmitArgs {
   const SurfEmitDev* gseg;
   const uint32_t* seg_key;
   uint32_t electron_mask;  // species whose impacts are secondary-emission events
+  // photoemission per scenario (grid y = scenario): faces [seg_face_off[s],
+  // seg_face_off[s + 1]) of this group, cumulative areas restarting at each
+  // scenario; FP64 time step and weight multiplier per scenario; the
+  // floating potential of scenario s at vfloat + s * vfloat_stride doubles
+  const int64_t* seg_face_off;
+  const double* seg_dt;
+  const double* seg_wscale;
+  int64_t vfloat_stride;
 };
 
 // Majorant of one species from the largest |v|^2 of the previous push; also
 // clears that maximum and the candidate count for the coming push.
+// With scenarios (seg_n non-null) one block per scenario: densities
+// seg_n[s][slot], time step seg_dt[s], maximum v2max[s], controls
+// ctl + s kMccCtl.
 struct MccMajorantArgs {
   MccProcDev proc[kMaxProc];
   int np;
@@ -130,19 +149,24 @@ struct MccMajorantArgs {
   unsigned long long* v2max;  // FP64 bits of max |v|^2 (non-negative: ordered as integers)
   unsigned int* n_cand;
   double* ctl;                // kMccCtl doubles
+  const double* seg_n;
+  const double* seg_dt;
+  int n_seg;
 };
 void launch_mcc_majorant(const MccMajorantArgs& a, hipStream_t s);
-// max |v|^2 over the alive particles of one species into *v2max (atomic max)
+// max |v|^2 over the alive particles of one species into *v2max (atomic
+// max); with tet_seg non-null into v2max[segment of the particle's cell]
 template <typename T>
 void launch_v2max(const T* vx, const T* vy, const T* vz, const int32_t* cell, int64_t n, unsigned long long* v2max,
-                  hipStream_t s);
+                  const int32_t* tet_seg, hipStream_t s);
 template <typename T>
 void launch_mcc(const MccArgs<T>& a, hipStream_t s);
 template <typename T>
 void launch_see(const EmitArgs<T>& a, int64_t max_events, hipStream_t s);
-// max_count: upper bound of the group's emissions this step (grid size)
+// max_count: upper bound of the group's emissions this step (grid size);
+// n_seg > 1: one grid row per scenario (a.seg_face_off set)
 template <typename T>
-void launch_photo(const EmitArgs<T>& a, int64_t max_count, hipStream_t s);
+void launch_photo(const EmitArgs<T>& a, int64_t max_count, int n_seg, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s);
 // n particles (x, v: 3n FP64, cell, id) appended to o and deposited
 template <typename T>

@@ -128,6 +128,8 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
     bool is_cand = false;  // collision candidate: entry = slot | kMccForced when forced
     uint32_t cand = 0;
     int32_t f[4] = {0, 0, 0, 0};
+    int mseg = -1;  // scenario of this lane's |v|^2 for the collision majorant
+    double mv2 = 0.0;
     int c = (i < a.n) ? a.cell[i] : -1;
     if (c >= 0) {
       ++started;
@@ -234,16 +236,21 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
         a.z[i] = x[2];
         a.cell[i] = c;
         if (a.mcc_ctl) {
-          // null-collision pre-selection on the post-push velocity
+          // null-collision pre-selection on the post-push velocity (with
+          // scenarios: the scenario's majorant and key)
           const double v2 = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+          const double* ctl = a.mcc_ctl + (a.tet_seg ? (size_t)sg * kMccCtl : 0);
+          const uint32_t k0 = a.tet_seg ? a.mcc_seg_key[2 * sg] : a.mcc_key0;
+          const uint32_t k1 = a.tet_seg ? a.mcc_seg_key[2 * sg + 1] : a.mcc_key1;
           v2max = fmax(v2max, v2);
-          if (v2 > a.mcc_ctl[1]) {
+          mseg = sg;
+          mv2 = v2;
+          if (v2 > ctl[1]) {
             is_cand = true;
             cand = (uint32_t)i | kMccForced;
           } else {
-            const U4 r = philox4x32((uint32_t)a.id[i], a.mcc_step, (uint32_t)a.species, kTagMccA, a.mcc_key0,
-                                    a.mcc_key1);
-            is_cand = u01(r.x) < a.mcc_ctl[0];
+            const U4 r = philox4x32((uint32_t)a.id[i], a.mcc_step, (uint32_t)a.species, kTagMccA, k0, k1);
+            is_cand = u01(r.x) < ctl[0];
             cand = (uint32_t)i;
           }
         }
@@ -252,6 +259,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
       a.vy[i] = v[1];
       a.vz[i] = v[2];
     }
+    if (a.mcc_ctl && a.tet_seg) wave_seg_max(mseg, mv2, a.mcc_v2max);
     if (a.mcc_ctl) {
       // wave-aggregated append of this chunk's candidates
       const uint64_t m = __ballot(is_cand);
@@ -266,7 +274,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
     }
     tile_runs(key, f, a.cs, a.tets, a.rho, tile);
   }
-  if (a.mcc_ctl) {
+  if (a.mcc_ctl && !a.tet_seg) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v2max = fmax(v2max, __shfl_xor(v2max, off, 64));
     if ((threadIdx.x & 63) == 0 && v2max > 0.0)

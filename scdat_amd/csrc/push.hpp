@@ -69,6 +69,10 @@ struct PushArgs {
   unsigned int* mcc_ncand;
   unsigned long long* mcc_v2max;
   uint32_t mcc_step, mcc_key0, mcc_key1;
+  // with scenarios (tet_seg): kMccCtl controls, |v|^2 maximum and random
+  // key (2 words) per scenario, at mcc_ctl + s kMccCtl, mcc_v2max + s and
+  // mcc_seg_key + 2 s
+  const uint32_t* mcc_seg_key;
 };
 
 // blocks of one push launch over n particle slots with `chunks` particles

@@ -28,20 +28,23 @@ def _by_id(x, v, cell, ids, n):
     return X, V
 
 
-@pytest.mark.parametrize("precision", [1, 0])
-def test_batched_cases_equal_standalone_runs(precision):
-    inputs = [B.case_inputs(sc) for sc in CASES]
-    bc, u = B.batch_context(CASES, inputs, precision=precision, tol=1e-10, sort_interval=3)
-    singles = [B.standalone_context(sc, st, precision=precision, tol=1e-10, sort_interval=3)
-               for sc, st in zip(CASES, inputs)]
-    for _ in range(7):
+def _compare_batch_with_standalone(cases, precision, steps=7):
+    """Runs `cases` batched and one by one; asserts bit equality of fields,
+    floating potentials, ledgers and particles (by id) for every case.
+    Returns the batch context and the stand-alone contexts."""
+    inputs = [B.case_inputs(sc) for sc in cases]
+    bc, u = B.batch_context(cases, inputs, precision=precision, tol=1e-10, sort_interval=3)
+    mcc = B._has_mcc(cases)
+    singles = [B.standalone_context(sc, st, precision=precision, tol=1e-10, sort_interval=3, mcc=mcc)
+               for sc, st in zip(cases, inputs)]
+    n_species = 3 if mcc else 2
+    for _ in range(steps):
         bc.step(1)
         for c in singles:
             c.step(1)
     rho_b, phi_b, E_b = bc.fields()
-    xb = [bc.get_particles(s) for s in (0, 1)]
-    parts = [B.split_particles(u, *xb[s]) for s in (0, 1)]
-    see_total = 0
+    xb = [bc.get_particles(s) for s in range(n_species)]
+    parts = [B.split_particles(u, *xb[s]) for s in range(n_species)]
     for k, (c, st) in enumerate(zip(singles, inputs)):
         n0 = int(u["node_start"][k])
         n1 = n0 + len(st["mesh"]["xyz"])
@@ -62,8 +65,7 @@ def test_batched_cases_equal_standalone_runs(precision):
             for key in ("absorbed_electron_particles", "absorbed_ion_particles", "emitted_electron_particles"):
                 assert lb[g][key] == l1[g][key], (k, g, key)
             assert lb[g]["absorbed_electron_charge_c"] == pytest.approx(l1[g]["absorbed_electron_charge_c"], rel=1e-14)
-        see_total += sum(l1[g]["emitted_electron_particles"] for g in range(6, 12))
-        for s in (0, 1):
+        for s in range(n_species):
             x1, v1, c1, i1 = c.get_particles(s)
             xk, vk, ck, ik = parts[s][k]
             live = c1 >= 0
@@ -75,7 +77,48 @@ def test_batched_cases_equal_standalone_runs(precision):
             Xk, Vk = _by_id(xk[ik >= 0], vk[ik >= 0], ck[ik >= 0], ik[ik >= 0], n)
             np.testing.assert_array_equal(X1, Xk)
             np.testing.assert_array_equal(V1, Vk)
+            # generated particles: same ids, and per id the same state
+            gen1 = live & (i1 < 0)
+            genk = ik < 0
+            o1 = np.argsort(i1[gen1], kind="stable")
+            ok = np.argsort(ik[genk], kind="stable")
+            assert i1[gen1][o1].tolist() == ik[genk][ok].tolist()
+            if len(np.unique(i1[gen1])) == gen1.sum():
+                np.testing.assert_array_equal(x1[gen1][o1], xk[genk][ok])
+                np.testing.assert_array_equal(v1[gen1][o1], vk[genk][ok])
+    return bc, singles
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_batched_cases_equal_standalone_runs(precision):
+    _, singles = _compare_batch_with_standalone(CASES, precision)
+    see_total = sum(c.ledger()[g]["emitted_electron_particles"] for c in singles for g in range(6, 12))
     assert see_total > 0  # secondary emission ran in some case
+
+
+# per-case oxygen background (0: none in that case) and +x-face photoemission
+MCC_PHOTO_CASES = [
+    dict(CASES[0], neutral_density=1e19, photo_j=1e-3),
+    dict(CASES[1], neutral_density=3e18, photo_j=2e-4),
+    dict(CASES[2], neutral_density=0.0, photo_j=2e-5),
+    dict(CASES[3], neutral_density=1e19, photo_j=0.0),
+]
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_batched_collisions_and_photoemission_equal_standalone_runs(precision):
+    bc, singles = _compare_batch_with_standalone(MCC_PHOTO_CASES, precision)
+    eb = bc.event_counts()
+    es = [c.event_counts() for c in singles]
+    print("batch events", eb)
+    for key in ("created", "removed", "see_emitted", "photo_emitted"):
+        assert eb[key] == sum(e[key] for e in es), key
+    for p in range(3):
+        assert eb["collisions"][p] == sum(e["collisions"][p] for e in es), p
+    assert sum(eb["collisions"][:3]) > 0      # collisions ran
+    assert eb["created"] > 0                  # ionisation made particles
+    assert eb["photo_emitted"] > 0            # photoemission ran
+    assert es[2]["collisions"][:3] == [0, 0, 0]  # the case without background
 
 
 def test_batched_scenarios_reject_single_case_features():
@@ -87,3 +130,5 @@ def test_batched_scenarios_reject_single_case_features():
         bc.set_conductor_circuit(circ, [node] * 7)
     with pytest.raises(S.ScdatError):
         bc.set_scenario_emission(5, 6, S.SEE_STERNGLASS, 0, 2.0, 400.0)
+    with pytest.raises(S.ScdatError):
+        bc.set_scenario_mcc_density(0, 0, 1e12)  # no process configured
