@@ -129,7 +129,7 @@ def box_context(case, sort_interval, row_patterns=True, precond="two-level", cel
 
 
 def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns=True, precond="two-level",
-            cell_order=0):
+            cell_order=0, graph_steps=0):
     import scdat_amd as S
     t0 = time.time()
     ctx = box_context(case, sort_interval, row_patterns, precond, cell_order)
@@ -151,8 +151,25 @@ def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns
     kt = {"push": ctx.kernel_time(S.KERNEL_PUSH), "pcg_iter": ctx.kernel_time(S.KERNEL_PCG)}
     pushes = st1["particle_pushes"] - st0["particle_pushes"]
     log(f"gpu: {steps} steps in {wall:.3f}s, pushes {pushes}, stages(ms) {stages}, kernels {kt}, stats {st1}")
+    solver = ctx.solver_info()
+    graph = None
+    if graph_steps > 0:
+        # the same context continued with the launches around the PCG
+        # iteration replayed from HIP graphs (no per-kernel events there)
+        ctx.set_kernel_timing(False)
+        ctx.step_graph = 1
+        p0 = ctx.stats()["particle_pushes"]
+        t2 = time.perf_counter()
+        ctx.step(graph_steps)
+        ctx.sync()
+        gw = time.perf_counter() - t2
+        gp = ctx.stats()["particle_pushes"] - p0
+        graph = dict(value=gp / gw, ms_per_step=1e3 * gw / graph_steps, steps=graph_steps,
+                     stage_ms_total=ctx.stage_times())
+        ctx.step_graph = 0
+        log(f"gpu, captured steps: {graph_steps} steps in {gw:.3f}s, pushes {gp}")
     return ctx, dict(wall=wall, pushes=pushes, stages=stages, kt=kt, stats=st1, stats0=st0, chunks=chunks,
-                     solver=ctx.solver_info())
+                     solver=solver, graph=graph)
 
 
 def run_cpu_baseline(case, steps, threads):
@@ -214,8 +231,14 @@ def run_sweep(args):
     bwall, bpush = timed(bc)
     stages = bc.stage_times()
     conv = [bc.scenario_stats(k) for k in range(len(sweep))]
-    del bc
     log(f"batched: {args.steps} steps {bwall:.3f}s, pushes {bpush}, stages {stages}")
+    # the same context, continued with the steps replayed from HIP graphs
+    bc.step_graph = 1
+    gwall, gpush = timed(bc)
+    gstages = bc.stage_times()
+    log(f"batched, captured steps: {args.steps} steps {gwall:.3f}s, pushes {gpush}, stages {gstages}")
+    bc.close()
+    del bc
     swall, spush = 0.0, 0
     t0 = time.time()
     for k, (sc, st) in enumerate(zip(sweep, cases)):
@@ -250,6 +273,10 @@ def run_sweep(args):
         "sequential": {"value": spush / swall, "ms_per_step": 1e3 * swall / args.steps,
                        "note": "the same cases, each in its own context, one after another (timed steps only)"},
         "speedup_vs_sequential": value / (spush / swall),
+        "step_graph": {"value": gpush / gwall, "ms_per_step": 1e3 * gwall / args.steps, "stage_ms_total": gstages,
+                       "note": "the same context continued for as many steps with the launches around the "
+                               "PCG iteration replayed from HIP graphs (SCDAT_SHAPE_STEP_GRAPH); value and "
+                               "ms_per_step above are the eager launches"},
         "stage_ms_total": stages,
         "cases_converged": int(sum(1 for q in conv if q["pcg_converged"])),
         "pcg_iterations_max": int(max(q["pcg_iterations"] for q in conv)),
@@ -582,6 +609,9 @@ def main():
                     help="Poisson preconditioner; two-level uses the tiled node order")
     ap.add_argument("--cell-order", type=int, choices=(0, 1), default=0,
                     help="order of the cells along the sorted particles: 0 tet index, 1 Morton")
+    ap.add_argument("--graph-steps", type=int, default=50,
+                    help="box: after the timed steps, this many more with the launches around the PCG "
+                         "iteration replayed from HIP graphs (reported under step_graph; 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -609,7 +639,7 @@ def main():
     name, ncu = S.device_info()
     case = build_case(args.hex, args.particles, args.seed)
     ctx, g = run_gpu(case, args.steps, args.warmup, args.sort_interval, not args.no_kernel_timing,
-                     bool(args.row_patterns), args.precond, args.cell_order)
+                     bool(args.row_patterns), args.precond, args.cell_order, args.graph_steps)
 
     m = case["mesh"]
     n_nodes, n_tets = len(m["kind"]), len(m["tets"])

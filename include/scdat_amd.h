@@ -665,6 +665,15 @@ int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
  * bit for bit (the ledger's FP64 energy sums are atomics in block order:
  * equal to rounding).                                                     */
 #define SCDAT_SHAPE_CELL_ORDER 4
+/* SCDAT_SHAPE_STEP_GRAPH: 1 records the launches of scdat_step around the
+ * Poisson iteration (push, collisions, emission, right-hand side; floating
+ * potential, phi and E, sorts) as HIP graphs, one per launch configuration
+ * met in the call, and replays them; slot counts and the step number are
+ * then kept on the device.  Same kernels, arguments and order as the eager
+ * loop, so the same results.  0 (default): eager launches.  Contexts with a
+ * surface circuit or a field-emission cathode (host-side stages) always
+ * step eagerly.  No counterpart in the reference.                          */
+#define SCDAT_SHAPE_STEP_GRAPH 5
 int scdat_set_launch_shape(scdat_ctx* ctx, int32_t what, int32_t value);
 int scdat_get_launch_shape(scdat_ctx* ctx, int32_t what, int32_t* value);
 int scdat_autotune(scdat_ctx* ctx);

@@ -131,6 +131,7 @@ SHAPE_ROW_PATTERNS = 1
 SHAPE_PCG_BLOCK = 2
 SHAPE_TILE_STAGING = 3
 SHAPE_CELL_ORDER = 4
+SHAPE_STEP_GRAPH = 5
 
 
 class ScenarioStats(C.Structure):
@@ -746,6 +747,18 @@ class Context:
     @cell_order.setter
     def cell_order(self, mode):
         self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_CELL_ORDER, int(mode)))
+
+    @property
+    def step_graph(self):
+        """1: step() replays the launches around the Poisson iteration from
+        HIP graphs recorded in the call (same results as eager launches)."""
+        v = C.c_int32()
+        self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_STEP_GRAPH, C.byref(v)))
+        return v.value
+
+    @step_graph.setter
+    def step_graph(self, mode):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_STEP_GRAPH, int(mode)))
 
     def charging_forward(self, params, steps, events=EVENTS_FREE):
         """Floating potential after `steps` steps of the differentiable charging

@@ -781,6 +781,20 @@ struct scdat_ctx {
   double kt_sum[2] = {0, 0};
   int64_t kt_n[2] = {0, 0};
   bool is_fp64() const { return cfg.precision == 1; }
+  // Captured steps (SCDAT_SHAPE_STEP_GRAPH): while a scdat_step call runs,
+  // the slot counts and the step number live on the device and the launches
+  // around the Poisson iteration are replayed from HIP graphs, one per
+  // launch configuration (density buffer parity, particle buffer parities,
+  // sorted species) of that call.
+  int step_graph = 0;
+  bool capturing = false;  // launches are being recorded: device counts, no host reads
+  DevBuf d_nslots, d_step;
+  std::map<uint64_t, hipGraphExec_t> gexec;
+  int64_t graph_replays = 0, graph_builds = 0;
+  void drop_graphs() {
+    for (auto& kv : gexec) (void)hipGraphExecDestroy(kv.second);
+    gexec.clear();
+  }
 };
 
 namespace {
@@ -1599,14 +1613,23 @@ FloatArgs float_args(scdat_ctx* c) {
   return fa;
 }
 
+// Right-hand side and initial guess of a step's solve.
 template <typename T>
-void do_solve(scdat_ctx* c, int rho_idx) {
+void solve_setup(scdat_ctx* c, int rho_idx) {
   const EllDev A = ell(c);
   FieldBufs f = fbufs(c);
   launch_rhs(A, f, c->d_rho[rho_idx].as<unsigned long long>(), c->unit, c->cfg.load_sign, c->stream);
   // initial guess: 2 x_n - x_{n-1} on free rows (x_n alone from the second solve)
   launch_extrapolate(f, c->d_xprev.as<double>(), A.n, !c->have_xprev, c->stream);
   c->have_xprev = true;
+}
+
+// The PCG solve (host-driven) and the failed-solve policy; float: then the
+// floating potential.
+template <typename T>
+void solve_run(scdat_ctx* c, bool flt = true) {
+  const EllDev A = ell(c);
+  FieldBufs f = fbufs(c);
   const bool fp64 = c->is_fp64();
   pcg_drive<T>(c, f, c->cfg.pcg_tol, fp64 ? 0.0 : c->cfg.inner_tol, fp64 ? 4 : 12, c->inner_hist);
   if (!c->last_conv) {
@@ -1620,7 +1643,13 @@ void do_solve(scdat_ctx* c, int rho_idx) {
     c->failed_solves++;
     if (c->batch_failed++ == 0) c->first_failed_step = c->steps;
   }
-  if (c->h.has_cond && c->n_patch == 0) launch_float(f, float_args(c), c->stream);
+  if (flt && c->h.has_cond && c->n_patch == 0) launch_float(f, float_args(c), c->stream);
+}
+
+template <typename T>
+void do_solve(scdat_ctx* c, int rho_idx) {
+  solve_setup<T>(c, rho_idx);
+  solve_run<T>(c);
 }
 
 // Snapshot of the cumulative ledger and emission counts (read after the next
@@ -1717,7 +1746,7 @@ int pack_procs(const scdat_ctx* c, int si, MccProcDev* out) {
 void upload_seg_mcc(scdat_ctx* c);
 
 template <typename T>
-void mcc_prepare(scdat_ctx* c) {
+void mcc_prepare(scdat_ctx* c, bool majorants = true) {
   hipStream_t s = c->stream;
   const int ns = (int)c->sp.size();
   // controls and |v|^2 maxima per (species, scenario)
@@ -1745,6 +1774,7 @@ void mcc_prepare(scdat_ctx* c) {
     if (!species_has_mcc(c, si)) continue;
     SpeciesState& S = c->sp[si];
     if (S.mcc_cand.bytes < 4 * (size_t)S.cap) S.mcc_cand.alloc(4 * (size_t)S.cap);
+    if (!majorants) continue;  // allocations and maxima only (before a stream capture)
     MccMajorantArgs m{};
     m.np = pack_procs(c, si, m.proc);
     m.mass = S.m;
@@ -1788,6 +1818,12 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   a.vz = S.buf[S.cur][5].as<T>();
   a.cell = S.buf[S.cur][6].as<int32_t>();
   a.n = S.n;
+  if (c->capturing) {
+    a.n = S.cap;
+    a.n_dev = c->d_nslots.as<int64_t>() + si;
+    a.step_dev = c->d_step.as<uint32_t>();
+    a.n_pushed = c->d_nslots.as<int64_t>() + kMaxSpecies + si;
+  }
   a.species = si;
   a.qm = (T)(S.q / S.m);
   a.cs = S.cs;
@@ -1938,7 +1974,9 @@ void autotune_push(scdat_ctx* c) {
 template <typename T>
 void zero_rho(scdat_ctx* c, int k) {
   if (!c->rho_clean[k]) {
-    HIPCHK(hipMemsetAsync(c->d_rho[k].p, 0, 8 * (size_t)c->h.nn, c->stream));
+    // a fill kernel while recording (no memset nodes in captured graphs)
+    if (c->capturing) launch_fill_u64(c->d_rho[k].as<unsigned long long>(), c->h.nn, c->stream);
+    else HIPCHK(hipMemsetAsync(c->d_rho[k].p, 0, 8 * (size_t)c->h.nn, c->stream));
     c->rho_clean[k] = true;
   }
 }
@@ -1957,7 +1995,7 @@ template <typename T>
 void do_push(scdat_ctx* c, std::vector<std::pair<size_t, size_t>>* ev, size_t* evn) {
   zero_rho<T>(c, c->rho_target);
   for (size_t si = 0; si < c->sp.size(); ++si) {
-    if (c->sp[si].n == 0) continue;
+    if ((c->capturing ? c->sp[si].cap : c->sp[si].n) == 0) continue;
     const PushArgs<T> a = push_args<T>(c, (int)si);
     c->sp[si].stat_blocks = push_blocks(a.n, a.chunks);
     if (ev) {
@@ -2044,6 +2082,14 @@ uint32_t sort_species_copy(scdat_ctx* c, int si) {
   a.perm = c->d_perm.as<uint32_t>();
   if (c->d_stage.bytes < sizeof(SortRec<T>) * (size_t)S.n) throw Error(SCDAT_ERR_STATE, "sort: staging buffer too small");
   a.stage = c->d_stage.as<SortRec<T>>();
+  if (c->capturing) {
+    // the grids cover the capacity; the count stays on the device
+    a.n = S.cap;
+    a.n_dev = c->d_nslots.as<int64_t>() + si;
+    launch_sort<T>(a, c->stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   launch_sort<T>(a, c->stream);
   HIPCHK(hipGetLastError());
   uint32_t tot = 0;
@@ -2055,10 +2101,10 @@ uint32_t sort_species_copy(scdat_ctx* c, int si) {
 template <typename T>
 void do_sort_species(scdat_ctx* c, int si) {
   SpeciesState& S = c->sp[si];
-  if (S.n == 0) return;
+  if ((c->capturing ? S.cap : S.n) == 0) return;
   const uint32_t tot = sort_species_copy<T>(c, si);
   S.cur = 1 - S.cur;
-  S.n = tot;
+  if (!c->capturing) S.n = tot;
 }
 
 template <typename T>
@@ -2234,6 +2280,7 @@ SpeciesOut<T> species_out(scdat_ctx* c, int si) {
   o.cell = S.buf[S.cur][6].as<int32_t>();
   o.id = S.buf[S.cur][7].as<int32_t>();
   o.base = S.n;
+  if (c->capturing) o.base_dev = c->d_nslots.as<int64_t>() + si;
   o.cap = S.cap;
   o.append = c->d_append.as<unsigned long long>() + si;
   o.m = S.m;
@@ -2380,7 +2427,8 @@ void post_push(scdat_ctx* c) {
     MccArgs<T> a{};
     const int np = pack_procs(c, si, a.proc);
     SpeciesState& S = c->sp[si];
-    if (np == 0 || S.n == 0) continue;
+    if (np == 0 || (c->capturing ? S.cap : S.n) == 0) continue;
+    if (c->capturing) a.step_dev = c->d_step.as<uint32_t>();
     a.cand = S.mcc_cand.as<uint32_t>();
     a.n_cand = c->d_mcc_ncand.as<unsigned int>() + si;
     a.ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si * (c->segmented() ? (size_t)c->n_seg() : 1);
@@ -2398,7 +2446,7 @@ void post_push(scdat_ctx* c) {
     a.vz = S.buf[S.cur][5].as<T>();
     a.cell = S.buf[S.cur][6].as<int32_t>();
     a.id = S.buf[S.cur][7].as<int32_t>();
-    a.n = S.n;
+    a.n = c->capturing ? S.cap : S.n;
     a.species = si;
     a.np = np;
     a.mass = S.m;
@@ -2438,6 +2486,7 @@ void post_push(scdat_ctx* c) {
   e.counts = counts;
   e.emit_count = c->d_emit_count.as<unsigned long long>();
   e.step = step;
+  if (c->capturing) e.step_dev = c->d_step.as<uint32_t>();
   e.key0 = c->key[0];
   e.key1 = c->key[1];
   e.dt = c->cfg.dt;
@@ -2491,6 +2540,14 @@ void post_push(scdat_ctx* c) {
     launch_photo<T>(eg, (int64_t)std::floor(bound) + 1, c->segmented() ? c->n_seg() : 1, s);
   }
   HIPCHK(hipGetLastError());
+  if (c->capturing) {
+    // the slot counts advance on the device
+    CountArgs ca{};
+    ca.n_species = ns;
+    for (int si = 0; si < ns; ++si) ca.cap[si] = c->sp[si].cap;
+    launch_commit_counts(c->d_nslots.as<int64_t>(), c->d_append.as<unsigned long long>(), ca, s);
+    return;
+  }
   // appended counts and impacts, read once the stream has synchronised
   HIPCHK(hipMemcpyAsync(c->h_append, c->d_append.p, 8 * kMaxSpecies, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(c->h_append + kMaxSpecies, c->d_n_events.p, 8, hipMemcpyDeviceToHost, s));
@@ -2580,6 +2637,192 @@ void do_step(scdat_ctx* c, int n) {
     }
   }
 }
+
+// Host bookkeeping the launches of a captured step change, saved before a
+// recording and restored after it, so that a replay and a recording leave
+// the same host state.
+struct HostLaunchState {
+  bool rho_clean[2];
+  int rho_last, rho_target;
+  int cur[kMaxSpecies];
+  bool push_mcc, have_xprev;
+  static HostLaunchState of(const scdat_ctx* c) {
+    HostLaunchState h{};
+    h.rho_clean[0] = c->rho_clean[0];
+    h.rho_clean[1] = c->rho_clean[1];
+    h.rho_last = c->rho_last;
+    h.rho_target = c->rho_target;
+    for (size_t k = 0; k < c->sp.size(); ++k) h.cur[k] = c->sp[k].cur;
+    h.push_mcc = c->push_mcc;
+    h.have_xprev = c->have_xprev;
+    return h;
+  }
+  void restore(scdat_ctx* c) const {
+    c->rho_clean[0] = rho_clean[0];
+    c->rho_clean[1] = rho_clean[1];
+    c->rho_last = rho_last;
+    c->rho_target = rho_target;
+    for (size_t k = 0; k < c->sp.size(); ++k) c->sp[k].cur = cur[k];
+    c->push_mcc = push_mcc;
+    c->have_xprev = have_xprev;
+  }
+};
+
+// Replays the graph recorded for `key`, recording it first (fn issues the
+// launches) when this configuration has not run in the current call.
+template <typename F>
+void replay(scdat_ctx* c, uint64_t key, F&& fn) {
+  auto it = c->gexec.find(key);
+  if (it == c->gexec.end()) {
+    const HostLaunchState saved = HostLaunchState::of(c);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    try {
+      fn();
+    } catch (...) {
+      c->capturing = false;
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      saved.restore(c);
+      throw;
+    }
+    c->capturing = false;
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(rc);
+    saved.restore(c);
+    it = c->gexec.emplace(key, e).first;
+    c->graph_builds++;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->stream));
+  c->graph_replays++;
+}
+
+// Steps with the launches around the Poisson iteration replayed from HIP
+// graphs (SCDAT_SHAPE_STEP_GRAPH): the same kernels with the same arguments
+// in the same order as do_step, so the results are the same bits; the slot
+// counts and the step number (the random streams' counter) are read by the
+// kernels from the device, and the PCG iteration stays host-driven (its
+// inner counts vary from step to step).  Per step: graph 1 = push, impacts,
+// collisions, emission, slot counts, right-hand side and initial guess;
+// the solve; graph 2 = floating potential, phi and E, the sorts due.
+template <typename T>
+void do_step_graph(scdat_ctx* c, int n) {
+  hipStream_t s = c->stream;
+  const int ns = (int)c->sp.size();
+  c->push_ms_sum = 0;
+  c->push_launches = 0;
+  c->batch_failed = 0;
+  for (int k = 0; k < 5; ++k) c->stage_ms[k] = 0;
+  // before recording: uploads, allocations and stale maxima, and the counts
+  if (c->segmented() && c->seg_emit_dirty) upload_seg_emit(c);
+  if (!c->mcc.empty()) mcc_prepare<T>(c, false);
+  if (c->cell_order == 1 && !c->cell_key_ready) build_cell_key(c);
+  // slot counts, then the counts the last push of each species covered
+  if (c->d_nslots.bytes < 16 * (size_t)kMaxSpecies) c->d_nslots.alloc(16 * (size_t)kMaxSpecies);
+  if (c->d_step.bytes < 4) c->d_step.alloc(16);
+  {
+    std::vector<int64_t> cnt(kMaxSpecies, 0);
+    for (int si = 0; si < ns; ++si) cnt[si] = c->sp[si].n;
+    cnt.resize(2 * kMaxSpecies, 0);
+    const uint32_t step = (uint32_t)c->steps;
+    HIPCHK(hipMemcpyAsync(c->d_nslots.p, cnt.data(), 16 * (size_t)kMaxSpecies, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->d_step.p, &step, 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  c->drop_graphs();
+  const bool post = has_post_push(c);
+  struct StageEv { size_t a, b, cc, e; };
+  std::vector<StageEv> sev;
+  size_t evn = 0;
+  try {
+    for (int k = 0; k < n; ++k) {
+      StageEv se;
+      se.a = evn++;
+      HIPCHK(hipEventRecord(take_event(c, se.a), s));
+      uint64_t cur = 0;
+      for (int si = 0; si < ns; ++si) cur |= (uint64_t)c->sp[si].cur << si;
+      const uint64_t kpre = 1ull | (uint64_t)c->rho_target << 1 | (uint64_t)c->rho_clean[c->rho_target] << 2 |
+                            cur << 8;
+      replay(c, kpre, [&] {
+        if (c->event_cap > 0) launch_fill_u64(c->d_n_events.as<unsigned long long>(), 1, s);
+        if (!c->mcc.empty()) {
+          mcc_prepare<T>(c);
+          c->push_mcc = true;
+        }
+        do_push<T>(c, nullptr, nullptr);
+        c->push_mcc = false;
+        if (post) post_push<T>(c);
+        solve_setup<T>(c, c->rho_last);
+      });
+      // what the recorded launches do to the host state
+      c->rho_clean[c->rho_target] = false;
+      c->rho_last = c->rho_target;
+      c->rho_target = 1 - c->rho_target;
+      c->have_xprev = true;
+      se.b = evn++;
+      HIPCHK(hipEventRecord(take_event(c, se.b), s));
+      solve_run<T>(c, false);
+      se.cc = evn++;
+      HIPCHK(hipEventRecord(take_event(c, se.cc), s));
+      uint64_t sorted = 0;
+      const int64_t next = c->steps + 1;
+      if (c->cfg.sort_interval > 0)
+        for (int si = 0; si < ns; ++si) {
+          const int every = c->sp[si].sort_every > 0 ? c->sp[si].sort_every : c->cfg.sort_interval;
+          if (next % every == 0 && c->sp[si].cap > 0) sorted |= 1ull << si;
+        }
+      const uint64_t kpost = 2ull | (uint64_t)c->rho_target << 2 | cur << 8 | sorted << 24;
+      replay(c, kpost, [&] {
+        FieldBufs f = fbufs(c);
+        if (c->h.has_cond && c->n_patch == 0) launch_float(f, float_args(c), s);
+        do_field<T>(c, c->d_rho[c->rho_target].as<unsigned long long>());
+        for (int si = 0; si < ns; ++si)
+          if ((sorted >> si) & 1ull) do_sort_species<T>(c, si);
+        launch_step_advance(c->d_step.as<uint32_t>(), s);
+      });
+      c->rho_clean[c->rho_target] = true;
+      for (int si = 0; si < ns; ++si)
+        if ((sorted >> si) & 1ull) c->sp[si].cur = 1 - c->sp[si].cur;
+      c->steps++;
+      c->sim_time += c->segmented() ? c->seg_dt[0] : c->cfg.dt;
+      se.e = evn++;
+      HIPCHK(hipEventRecord(take_event(c, se.e), s));
+      sev.push_back(se);
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    c->drop_graphs();
+    throw;
+  }
+  // the counts back on the host
+  std::vector<int64_t> cnt(2 * kMaxSpecies, 0);
+  HIPCHK(hipMemcpyAsync(cnt.data(), c->d_nslots.p, 16 * (size_t)kMaxSpecies, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int si = 0; si < ns; ++si) {
+    c->sp[si].n = cnt[si];
+    // the statistics read the partials of the blocks the last push needed
+    // (the grid covered the capacity; the others hold zeros)
+    if (c->sp[si].cap > 0) c->sp[si].stat_blocks = push_blocks(cnt[kMaxSpecies + si], c->push_chunks);
+  }
+  for (auto& se : sev) {
+    float t;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[se.a], c->ev_pool[se.b]));
+    c->stage_ms[0] += t;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[se.b], c->ev_pool[se.cc]));
+    c->stage_ms[2] += t;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[se.cc], c->ev_pool[se.e]));
+    c->stage_ms[3] += t;
+  }
+  c->drop_graphs();
+}
+
+// Whether this context's steps can be recorded: host-side stages (surface
+// circuit, field-emission cathodes) need the eager loop.
+bool graph_capable(const scdat_ctx* c) { return c->n_patch == 0 && c->fe_mask == 0; }
 
 template <typename T>
 void set_particles_t(scdat_ctx* c, int si, int64_t n, const double* x, const double* v,
@@ -3366,6 +3609,7 @@ int scdat_create(const scdat_config* cfg, scdat_ctx** out) {
 void scdat_destroy(scdat_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->drop_graphs();
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   for (auto e : c->kt_pool) (void)hipEventDestroy(e);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -3555,8 +3799,14 @@ int scdat_step(scdat_ctx* c, int32_t n) {
   if (!c->initialized) return fail(c, SCDAT_ERR_STATE, "step: call scdat_initialize first");
   if (n < 0) return fail(c, SCDAT_ERR_ARG, "step: negative count");
   const int rc = guarded(c, [&] {
-    if (c->is_fp64()) do_step<double>(c, n);
-    else do_step<float>(c, n);
+    const bool graph = c->step_graph && graph_capable(c) && n > 0;
+    if (c->is_fp64()) {
+      if (graph) do_step_graph<double>(c, n);
+      else do_step<double>(c, n);
+    } else {
+      if (graph) do_step_graph<float>(c, n);
+      else do_step<float>(c, n);
+    }
   });
   if (rc == SCDAT_OK && c->batch_failed > 0) {
     c->err = std::to_string(c->batch_failed) + " of " + std::to_string(n) +
@@ -4499,6 +4749,11 @@ int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
     c->cell_order = value;
     return SCDAT_OK;
   }
+  if (what == SCDAT_SHAPE_STEP_GRAPH) {
+    if (value != 0 && value != 1) return fail(c, SCDAT_ERR_ARG, "set_launch_shape: step graph must be 0 or 1");
+    c->step_graph = value;
+    return SCDAT_OK;
+  }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS || !(value == 1 || value == 2 || value == 4 || value == 8))
     return fail(c, SCDAT_ERR_ARG, "set_launch_shape: push chunks must be 1, 2, 4 or 8");
   c->push_chunks = value;
@@ -4521,6 +4776,10 @@ int scdat_get_launch_shape(scdat_ctx* c, int32_t what, int32_t* value) {
   }
   if (what == SCDAT_SHAPE_CELL_ORDER) {
     *value = c->cell_order;
+    return SCDAT_OK;
+  }
+  if (what == SCDAT_SHAPE_STEP_GRAPH) {
+    *value = c->step_graph;
     return SCDAT_OK;
   }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS) return SCDAT_ERR_ARG;

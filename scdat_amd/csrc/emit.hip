@@ -98,7 +98,7 @@ __device__ void deposit_one(const TetMap<T>* tets, unsigned long long* rho, int 
 template <typename T>
 __device__ bool append(const SpeciesOut<T>& o, const T x[3], const double v[3], int cell, int32_t id,
                        const TetMap<T>* tets, unsigned long long* rho, unsigned long long* counts) {
-  const int64_t slot = o.base + (int64_t)atomicAdd(o.append, 1ull);
+  const int64_t slot = (o.base_dev ? *o.base_dev : o.base) + (int64_t)atomicAdd(o.append, 1ull);
   if (slot >= o.cap) {
     atomicAdd(&counts[C_OVERFLOW], 1ull);
     return false;
@@ -208,7 +208,8 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
   }
   if (!(tot > 0.0)) return;
   const uint32_t id = (uint32_t)a.id[i];
-  const U4 r = philox4x32(id, a.step, (uint32_t)a.species, kTagMccA, key0, key1);
+  const uint32_t step = a.step_dev ? *a.step_dev : a.step;
+  const U4 r = philox4x32(id, step, (uint32_t)a.species, kTagMccA, key0, key1);
   // accept with probability pex / P_c: overall probability pex = 1 - exp(-nu dt)
   const double pc = forced ? 1.0 : a.ctl[(size_t)sg * kMccCtl];
   const double pex = -expm1(-tot * dt);
@@ -253,8 +254,8 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
     }
     case MCC_IONIZATION: {
       if (E <= p.thr) break;
-      const U4 b = philox4x32(id, a.step, (uint32_t)a.species, kTagMccB, key0, key1);
-      const U4 c = philox4x32(id, a.step, (uint32_t)a.species, kTagMccId, key0, key1);
+      const U4 b = philox4x32(id, step, (uint32_t)a.species, kTagMccB, key0, key1);
+      const U4 c = philox4x32(id, step, (uint32_t)a.species, kTagMccId, key0, key1);
       const double avail = E - p.thr;
       const double sec = u01(b.x) * avail;
       const double prim = avail - sec;
@@ -370,7 +371,8 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
   const double E = 0.5 * a.mass[ev.species] * v2 / kE;
   const double cosi = sp > 0.0 ? fabs(ev.v[0] * n[0] + ev.v[1] * n[1] + ev.v[2] * n[2]) / sp : 1.0;
   const double dl = see_yield(cf.see_model, cf.dmax, cf.emax, E, cosi);
-  const U4 r = philox4x32((uint32_t)ev.id, a.step, (uint32_t)ev.species, kTagSee, k0, k1);
+  const uint32_t step = a.step_dev ? *a.step_dev : a.step;
+  const U4 r = philox4x32((uint32_t)ev.id, step, (uint32_t)ev.species, kTagSee, k0, k1);
   const double fl = floor(dl);
   int nem = (int)fmin(fl, (double)kMaxEmit) + (u01(r.x) < dl - fl ? 1 : 0);
   nem = nem < kMaxEmit ? nem : kMaxEmit;
@@ -382,8 +384,8 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
   int made = 0;
   for (int k = 0; k < nem; ++k) {
     const uint32_t sk = ((uint32_t)ev.species << 8) | (uint32_t)k;
-    const U4 b = philox4x32((uint32_t)ev.id, a.step, sk, kTagSeeK, k0, k1);
-    const U4 c = philox4x32((uint32_t)ev.id, a.step, sk, kTagSeeId, k0, k1);
+    const U4 b = philox4x32((uint32_t)ev.id, step, sk, kTagSeeK, k0, k1);
+    const U4 c = philox4x32((uint32_t)ev.id, step, sk, kTagSeeId, k0, k1);
     double v[3];
     emit_velocity(n, cf.t_see, o.m, u01(b.x), u01(b.y), u01(b.z), u01(b.w), v);
     made += append(o, xt, v, ev.cell, gen_id(c.x), a.tets, a.rho, a.counts) ? 1 : 0;
@@ -413,12 +415,13 @@ __global__ void __launch_bounds__(kBlock) k_photo(EmitArgs<T> a) {
   const double V = a.is_conductor ? a.vfloat[seg ? sg * a.vfloat_stride : 0] : 0.0;
   const double j = fmax(0.0, cf.photo_j) * exp(-fmax(0.0, V - cf.v_ref) / fmax(1e-3, cf.t_ph));
   const double expect = j * area * dt / qaw;
-  const U4 r0 = philox4x32((uint32_t)a.group, a.step, 0u, kTagPhoto, key0, key1);
+  const uint32_t step = a.step_dev ? *a.step_dev : a.step;
+  const U4 r0 = philox4x32((uint32_t)a.group, step, 0u, kTagPhoto, key0, key1);
   const double fl = floor(expect);
   const int64_t nem = (int64_t)fl + (u01(r0.x) < expect - fl ? 1 : 0);
   if (k >= nem) return;
-  const U4 b = philox4x32((uint32_t)k, a.step, (uint32_t)a.group, kTagPhotoK, key0, key1);
-  const U4 c = philox4x32((uint32_t)k, a.step, (uint32_t)a.group, kTagPhotoK2, key0, key1);
+  const U4 b = philox4x32((uint32_t)k, step, (uint32_t)a.group, kTagPhotoK, key0, key1);
+  const U4 c = philox4x32((uint32_t)k, step, (uint32_t)a.group, kTagPhotoK2, key0, key1);
   // face: first cumulative area above u * area
   const double rr = u01(b.x) * area;
   int64_t lo = f0, hi = f1;  // search [lo, hi)
@@ -467,9 +470,26 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+__global__ void k_commit_counts(int64_t* n_dev, unsigned long long* append, CountArgs a) {
+  const int k = threadIdx.x;
+  if (k >= a.n_species) return;
+  const int64_t n = n_dev[k] + (int64_t)append[k];
+  n_dev[k] = n < a.cap[k] ? n : a.cap[k];
+  append[k] = 0ull;
+}
+
+__global__ void k_step_advance(uint32_t* step) { *step += 1u; }
+
 inline uint32_t blocks_for(int64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
+
+void launch_commit_counts(int64_t* n_dev, unsigned long long* append, const CountArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_commit_counts, dim3(1), dim3(kMaxSpecies), 0, s, n_dev, append, a);
+}
+void launch_step_advance(uint32_t* step_dev, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, s, step_dev);
+}
 
 template <typename T>
 void launch_mcc(const MccArgs<T>& a, hipStream_t s) {

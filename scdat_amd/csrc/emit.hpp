@@ -49,6 +49,7 @@ struct SpeciesOut {
   int32_t* cell;
   int32_t* id;
   int64_t base, cap;
+  const int64_t* base_dev;  // captured steps: the slot count on the device (null: base)
   unsigned long long* append;
   double m;
   int64_t cs;  // integer charge per unit weight (deposit)
@@ -87,6 +88,7 @@ struct MccArgs {
   const double* seg_n;
   const double* seg_dt;
   const uint32_t* seg_key;
+  const uint32_t* step_dev;  // captured steps: the step number on the device (null: step)
 };
 
 struct SurfEmitDev {
@@ -135,6 +137,7 @@ struct EmitArgs {
   const double* seg_dt;
   const double* seg_wscale;
   int64_t vfloat_stride;
+  const uint32_t* step_dev;  // captured steps: the step number on the device (null: step)
 };
 
 // Majorant of one species from the largest |v|^2 of the previous push; also
@@ -168,6 +171,16 @@ void launch_see(const EmitArgs<T>& a, int64_t max_events, hipStream_t s);
 template <typename T>
 void launch_photo(const EmitArgs<T>& a, int64_t max_count, int n_seg, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s);
+// Captured steps: every species' slot count n_dev[k] += append[k], capped at
+// cap[k] (the appends past the capacity were counted as overflow), for
+// k < n_species; then the append counters are cleared.
+struct CountArgs {
+  int64_t cap[kMaxSpecies];
+  int n_species;
+};
+void launch_commit_counts(int64_t* n_dev, unsigned long long* append, const CountArgs& a, hipStream_t s);
+// Captured steps: the device step number advances by one.
+void launch_step_advance(uint32_t* step_dev, hipStream_t s);
 // n particles (x, v: 3n FP64, cell, id) appended to o and deposited
 template <typename T>
 void launch_inject(const SpeciesOut<T>& o, int64_t n, const double* x, const double* v, const int32_t* cell,

@@ -109,7 +109,22 @@ __device__ __forceinline__ void tile_runs(int key, int32_t f[4], int64_t cs,
 // One block pushes CH x kBlock consecutive particles.
 template <typename T, int CH>
 __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
-  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t n = a.n_dev ? *a.n_dev : a.n;
+  // a captured step's grid covers the capacity: the blocks the count needs
+  // are remapped among themselves (so every XCD gets its share), the others
+  // only publish empty partials
+  const int64_t need = (n + (int64_t)CH * kBlock - 1) / ((int64_t)CH * kBlock);
+  const uint32_t nact = need < (int64_t)gridDim.x ? (uint32_t)need : gridDim.x;
+  if (a.n_pushed && blockIdx.x == 0 && threadIdx.x == 0) *a.n_pushed = n;
+  if (blockIdx.x >= nact) {
+    if (threadIdx.x == 0) {
+      a.ke_partial[blockIdx.x] = 0.0;
+      a.alive_partial[blockIdx.x] = 0;
+    }
+    return;
+  }
+  const uint32_t lb = xcd_remap(blockIdx.x, nact);
+  const uint32_t mcc_step = a.step_dev ? *a.step_dev : a.mcc_step;
   __shared__ DepositTile tile;
   __shared__ unsigned int led_n[kMaxGroups];
   __shared__ double led_e[kMaxGroups];
@@ -130,7 +145,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
     int32_t f[4] = {0, 0, 0, 0};
     int mseg = -1;  // scenario of this lane's |v|^2 for the collision majorant
     double mv2 = 0.0;
-    int c = (i < a.n) ? a.cell[i] : -1;
+    int c = (i < n) ? a.cell[i] : -1;
     if (c >= 0) {
       ++started;
       T x[3] = {a.x[i], a.y[i], a.z[i]};
@@ -249,7 +264,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
             is_cand = true;
             cand = (uint32_t)i | kMccForced;
           } else {
-            const U4 r = philox4x32((uint32_t)a.id[i], a.mcc_step, (uint32_t)a.species, kTagMccA, k0, k1);
+            const U4 r = philox4x32((uint32_t)a.id[i], mcc_step, (uint32_t)a.species, kTagMccA, k0, k1);
             is_cand = u01(r.x) < ctl[0];
             cand = (uint32_t)i;
           }
@@ -336,6 +351,16 @@ template <typename T>
 void launch_push(const PushArgs<T>& a, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   if (a.n <= 0) return;
   const dim3 g(push_blocks(a.n, a.chunks)), b(kBlock);
+  if (!start && !stop) {
+    // plain launches (stream capture records them as kernel nodes)
+    switch (a.chunks) {
+      case 1: hipLaunchKernelGGL((push_kernel<T, 1>), g, b, 0, s, a); break;
+      case 2: hipLaunchKernelGGL((push_kernel<T, 2>), g, b, 0, s, a); break;
+      case 8: hipLaunchKernelGGL((push_kernel<T, 8>), g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL((push_kernel<T, 4>), g, b, 0, s, a); break;
+    }
+    return;
+  }
   switch (a.chunks) {
     case 1: hipExtLaunchKernelGGL((push_kernel<T, 1>), g, b, 0, s, start, stop, 0, a); break;
     case 2: hipExtLaunchKernelGGL((push_kernel<T, 2>), g, b, 0, s, start, stop, 0, a); break;

@@ -26,6 +26,11 @@ struct PushArgs {
   T *x, *y, *z, *vx, *vy, *vz;
   int32_t* cell;
   int64_t n;
+  // captured steps: slot count and step number on the device (null: n and
+  // mcc_step); n then only sizes the grid
+  const int64_t* n_dev;
+  const uint32_t* step_dev;
+  int64_t* n_pushed;  // captured steps: receives the count this push covered
   int species;
   T qm;
   int64_t cs;           // integer charge per particle per unit weight 2^-24

@@ -43,8 +43,9 @@ __device__ __forceinline__ int cell_rank(const int32_t* key, int c) { return key
 
 // Particles per cell: one atomic per run of equal cell.
 __global__ void __launch_bounds__(kBlock) k_hist(const int32_t* cell, const int32_t* key, int64_t n,
-                                                  uint32_t* cnt) {
+                                                  const int64_t* n_dev, uint32_t* cnt) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n_dev) n = *n_dev;
   const int c = i < n ? cell[i] : -1;
   int start, len;
   wave_run(c, start, len);
@@ -121,7 +122,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(uint32_t* out, int64_t n, c
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_rank(SortArgs<T> a) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int c = i < a.n ? a.src.cell[i] : -1;
+  const int64_t n = a.n_dev ? *a.n_dev : a.n;
+  const int c = i < n ? a.src.cell[i] : -1;
   int start, len;
   wave_run(c, start, len);
   const int lane = threadIdx.x & 63;
@@ -162,6 +164,8 @@ __global__ void __launch_bounds__(kBlock) k_gather(SortArgs<T> a) {
   a.dst.id[d] = r.id;
 }
 
+__global__ void k_store_count(const uint32_t* total, int64_t* n_dev) { *n_dev = (int64_t)*total; }
+
 inline uint32_t blocks(int64_t n, int64_t per) { return (uint32_t)((n + per - 1) / per); }
 }  // namespace
 
@@ -170,7 +174,7 @@ void launch_sort(const SortArgs<T>& a, hipStream_t s) {
   // zero-fill by kernel (no memset nodes, so the sort can be graph-captured)
   hipLaunchKernelGGL(k_zero_u32, dim3(blocks(a.ncell, kBlock)), dim3(kBlock), 0, s, a.count, a.ncell);
   if (a.n > 0) hipLaunchKernelGGL(k_hist, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a.src.cell, a.key, a.n,
-                                a.count);
+                                (const int64_t*)a.n_dev, a.count);
   const uint32_t nt = blocks(a.ncell, kScanTile);
   hipLaunchKernelGGL(k_scan_tile, dim3(nt), dim3(kBlock), 0, s, a.count, a.offset, a.ncell, a.tiles);
   hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kBlock), 0, s, a.tiles, (int64_t)nt, a.total);
@@ -180,6 +184,7 @@ void launch_sort(const SortArgs<T>& a, hipStream_t s) {
     hipLaunchKernelGGL(k_rank<T>, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_gather<T>, dim3(blocks(a.n, kBlock)), dim3(kBlock), 0, s, a);
   }
+  if (a.n_dev) hipLaunchKernelGGL(k_store_count, dim3(1), dim3(1), 0, s, (const uint32_t*)a.total, a.n_dev);
 }
 
 template void launch_sort<float>(const SortArgs<float>&, hipStream_t);

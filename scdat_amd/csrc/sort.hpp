@@ -35,6 +35,9 @@ struct SortArgs {
   uint32_t* total;  // [1] alive count after the sort
   uint32_t* perm;   // [n] source slot of each destination slot
   SortRec<T>* stage;  // [n] the particles as records, in source order
+  // captured steps: the source slot count on the device (n then only sizes
+  // the grids); the alive count is stored back into it after the sort
+  int64_t* n_dev;
 };
 
 template <typename T>
