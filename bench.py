@@ -740,6 +740,10 @@ def main():
                if pcg_ms > 0 else 0.0, "note": "algorithmic bytes of the PCG iteration kernel per inner iteration over the "
                "whole Poisson stage time (FP64 residuals, launches and host polls included)"},
         "stage_ms_total": g["stages"],
+        "step_graph": None if g["graph"] is None else dict(g["graph"], note=(
+            "the same context continued for as many steps with the launches around the PCG iteration "
+            "replayed from HIP graphs (SCDAT_SHAPE_STEP_GRAPH); the particle count falls as the walls "
+            "absorb, so compare ms_per_step")),
         "launch_shape": {"push_particles_per_thread": g["chunks"], "chosen_by": "scdat_autotune",
                          "cell_order": "morton" if args.cell_order == 1 else "tet index"},
         "solver": {"rows": sv["rows"], "ell_width": width, "row_patterns": npat, "pcg_block": sv["pcg_block"],

@@ -674,6 +674,21 @@ int scdat_get_solver_info(scdat_ctx* ctx, scdat_solver_info* out);
  * surface circuit or a field-emission cathode (host-side stages) always
  * step eagerly.  No counterpart in the reference.                          */
 #define SCDAT_SHAPE_STEP_GRAPH 5
+/* SCDAT_SHAPE_LATTICE_PUSH: 1 lets the FP32 push locate the end tet of a
+ * sub-step that left its start tet on the grid, when the mesh is a
+ * rectilinear grid of Kuhn tetrahedra (the box mesher's), the end point is
+ * at least 1e-3 (barycentric) inside a tet and every hexahedron between
+ * start and end is whole with no boundary face; every other sub-step walks
+ * tet faces.  The end tet is the walk's, so results are the same bits.  0
+ * (default): every sub-step walks; scdat_autotune picks whichever pushes
+ * faster.  Reads back 1 only when the push uses it (FP32, lattice mesh).   */
+#define SCDAT_SHAPE_LATTICE_PUSH 6
+/* SCDAT_SHAPE_DEPOSIT_BUCKETS: 1 (default) sends each push block's reduced
+ * charge table to per-512-row buckets that one block per row range sums
+ * after the pushes (one atomic per block and range, plain stores for the
+ * entries); 0: each block adds its table to the charge with one 64-bit
+ * global atomic per node.  Integer sums: the same charge either way.      */
+#define SCDAT_SHAPE_DEPOSIT_BUCKETS 7
 int scdat_set_launch_shape(scdat_ctx* ctx, int32_t what, int32_t value);
 int scdat_get_launch_shape(scdat_ctx* ctx, int32_t what, int32_t* value);
 int scdat_autotune(scdat_ctx* ctx);

@@ -132,6 +132,8 @@ SHAPE_PCG_BLOCK = 2
 SHAPE_TILE_STAGING = 3
 SHAPE_CELL_ORDER = 4
 SHAPE_STEP_GRAPH = 5
+SHAPE_LATTICE_PUSH = 6
+SHAPE_DEPOSIT_BUCKETS = 7
 
 
 class ScenarioStats(C.Structure):
@@ -759,6 +761,31 @@ class Context:
     @step_graph.setter
     def step_graph(self, mode):
         self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_STEP_GRAPH, int(mode)))
+
+    @property
+    def lattice_push(self):
+        """1 when the FP32 push locates sub-steps clear of boundary faces on
+        the grid of a Kuhn-lattice mesh instead of walking tet faces (same
+        results; off by default, autotune() picks the faster)."""
+        v = C.c_int32()
+        self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_LATTICE_PUSH, C.byref(v)))
+        return v.value
+
+    @lattice_push.setter
+    def lattice_push(self, mode):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_LATTICE_PUSH, int(mode)))
+
+    @property
+    def deposit_buckets(self):
+        """1 (default): push blocks hand their reduced charge to per-row-range
+        buckets summed after the pushes; 0: one global atomic per node."""
+        v = C.c_int32()
+        self._chk(lib().scdat_get_launch_shape(self.h, SHAPE_DEPOSIT_BUCKETS, C.byref(v)))
+        return v.value
+
+    @deposit_buckets.setter
+    def deposit_buckets(self, mode):
+        self._chk(lib().scdat_set_launch_shape(self.h, SHAPE_DEPOSIT_BUCKETS, int(mode)))
 
     def charging_forward(self, params, steps, events=EVENTS_FREE):
         """Floating potential after `steps` steps of the differentiable charging

@@ -786,6 +786,16 @@ struct scdat_ctx {
   // around the Poisson iteration are replayed from HIP graphs, one per
   // launch configuration (density buffer parity, particle buffer parities,
   // sorted species) of that call.
+  // lattice location tables of the FP32 push (lat_dev.tet_hex null: the
+  // mesh is not a Kuhn lattice) and SCDAT_SHAPE_LATTICE_PUSH
+  DevBuf d_lat_tet_hex, d_lat_hex_tet;
+  LatticeDev lat_dev{};
+  int lat_mode = 0;
+  // deposit buckets (SCDAT_SHAPE_DEPOSIT_BUCKETS): entry counts and entries
+  // per 512-row range of the charge
+  DevBuf d_dep_count, d_dep_ent;
+  int dep_cap = 0;
+  int dep_mode = 1;
   int step_graph = 0;
   bool capturing = false;  // launches are being recorded: device counts, no host reads
   DevBuf d_nslots, d_step;
@@ -1080,6 +1090,130 @@ void build_face_table(scdat_ctx* c) {
   c->d_seg_face_off.upload(c->seg_face_off_h, s);
 }
 
+static bool lattice_indices(int64_t nn, const double* xyz, std::vector<int32_t>& ijk, int32_t dims[3],
+                            std::vector<double>* lines_out);
+
+// Lattice location tables of the FP32 push (LatticeDev).  Built when every
+// tet is a Kuhn tet of a hexahedron of a rectilinear grid (nodes v000,
+// v000 + e_p0, v000 + e_p0 + e_p1, v111) and no two tets take one
+// (hexahedron, permutation) slot, with at most kLatAxis hexahedra per axis
+// whose grid lines form at most kLatSeg runs of equal spacing (the device
+// maps every line of a run to its index within 1e-5 of a cell); otherwise
+// the push walks every sub-step.  A hexahedron is whole when its six tets
+// exist and none of their faces is a boundary face; its clearance is the
+// number (at most 7) of nested cubes of whole hexahedra around it: 0 (not
+// whole), 1 (whole), 2 (it and its 26 neighbours whole), ...
+void build_lattice(scdat_ctx* c) {
+  c->lat_dev = LatticeDev{};
+  const HostMesh& h = c->h;
+  if (h.nt >= ((int64_t)1 << 29) - 1) return;
+  std::vector<int32_t> ijk;
+  int32_t dims[3];
+  std::vector<double> L[3];
+  if (!lattice_indices(h.nn, h.xyz.data(), ijk, dims, L)) return;
+  int nh[3];
+  for (int r = 0; r < 3; ++r) {
+    nh[r] = dims[r] - 1;
+    if (nh[r] < 1 || nh[r] > kLatAxis) return;
+  }
+  const int64_t nhex = (int64_t)nh[0] * nh[1] * nh[2];
+  if (nhex > ((int64_t)1 << 28)) return;
+  // runs of equal spacing along each axis, checked as the device evaluates them
+  LatticeDev dv{};
+  for (int r = 0; r < 3; ++r) {
+    dv.nh[r] = nh[r];
+    int ns = 0, i0 = 0;
+    while (i0 < nh[r]) {
+      const double h0 = L[r][i0 + 1] - L[r][i0];
+      int i1 = i0 + 1;
+      while (i1 < nh[r] && std::abs((L[r][i1 + 1] - L[r][i1]) - h0) <= 1e-9 * h0) ++i1;
+      if (ns == kLatSeg) return;
+      dv.sx[r][ns] = (float)L[r][i0];
+      dv.sinv[r][ns] = (float)(1.0 / h0);
+      dv.si[r][ns] = i0;
+      for (int j = i0; j <= i1; ++j) {
+        const float t = ((float)L[r][j] - dv.sx[r][ns]) * dv.sinv[r][ns];
+        if (std::abs((double)t - (double)(j - i0)) > 1e-5) return;
+      }
+      ++ns;
+      i0 = i1;
+    }
+    dv.nseg[r] = ns;
+    dv.si[r][ns] = nh[r];
+    dv.lo[r] = (float)L[r].front();
+    dv.hi[r] = (float)L[r].back();
+  }
+  // permutation index of (p0, p1), the order of lat_locate
+  static const int kIdx[3][3] = {{-1, 0, 1}, {2, -1, 3}, {4, 5, -1}};
+  std::vector<int32_t> hex_tet((size_t)(6 * nhex), -1);
+  std::vector<int64_t> hex_of((size_t)h.nt);
+  for (int64_t t = 0; t < h.nt; ++t) {
+    int mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
+    for (int q = 0; q < 4; ++q)
+      for (int r = 0; r < 3; ++r) mn[r] = std::min(mn[r], ijk[3 * (size_t)h.tets[4 * t + q] + r]);
+    int n0 = 0, n3 = 0, p0 = -1, p2 = -1;
+    for (int q = 0; q < 4; ++q) {
+      int off[3], sum = 0;
+      for (int r = 0; r < 3; ++r) {
+        off[r] = ijk[3 * (size_t)h.tets[4 * t + q] + r] - mn[r];
+        if (off[r] < 0 || off[r] > 1) return;
+        sum += off[r];
+      }
+      if (sum == 0) ++n0;
+      else if (sum == 3) ++n3;
+      else if (sum == 1) p0 = off[0] ? 0 : (off[1] ? 1 : 2);
+      else p2 = !off[0] ? 0 : (!off[1] ? 1 : 2);
+    }
+    if (n0 != 1 || n3 != 1 || p0 < 0 || p2 < 0 || p0 == p2) return;
+    const int p1 = 3 - p0 - p2;
+    const int64_t hx = ((int64_t)mn[0] * nh[1] + mn[1]) * nh[2] + mn[2];
+    int32_t& slot = hex_tet[(size_t)(6 * hx + kIdx[p0][p1])];
+    if (slot >= 0) return;  // two tets on one slot (overlapping meshes)
+    slot = (int32_t)t;
+    hex_of[t] = hx;
+  }
+  // whole hexahedra, then erosions by the 3x3x3 cube
+  std::vector<uint8_t> e((size_t)nhex, 0), d((size_t)nhex, 0);
+  for (int64_t hx = 0; hx < nhex; ++hx) {
+    bool whole = true;
+    for (int k = 0; k < 6 && whole; ++k) {
+      const int32_t t = hex_tet[(size_t)(6 * hx + k)];
+      whole = t >= 0;
+      for (int f = 0; f < 4 && whole; ++f) whole = h.nb[4 * (size_t)t + f] >= 0;
+    }
+    e[hx] = whole ? 1 : 0;
+  }
+  const int64_t stride[3] = {(int64_t)nh[1] * nh[2], nh[2], 1};
+  std::vector<uint8_t> b((size_t)nhex);
+  for (int level = 0; level < 7; ++level) {
+    for (int64_t hx = 0; hx < nhex; ++hx) d[hx] = (uint8_t)(d[hx] + e[hx]);
+    for (int r = 0; r < 3; ++r) {
+      for (int64_t hx = 0; hx < nhex; ++hx) {
+        const int64_t ir = (hx / stride[r]) % nh[r];
+        b[hx] = e[hx] && ir > 0 && ir + 1 < nh[r] && e[hx - stride[r]] && e[hx + stride[r]];
+      }
+      e.swap(b);
+    }
+  }
+  std::vector<uint32_t> packed((size_t)(6 * nhex));
+  for (int64_t q = 0; q < 6 * nhex; ++q)
+    packed[q] = hex_tet[q] < 0 ? 0u : ((uint32_t)(hex_tet[q] + 1) | (uint32_t)d[q / 6] << 29);
+  std::vector<uint32_t> tet_hex((size_t)h.nt);
+  for (int64_t t = 0; t < h.nt; ++t) {
+    const int64_t hx = hex_of[t];
+    const uint32_t i = (uint32_t)(hx / stride[0]), j = (uint32_t)((hx / stride[1]) % nh[1]),
+                   k = (uint32_t)(hx % nh[2]);
+    tet_hex[t] = i | j << 10 | k << 20;
+  }
+  hipStream_t s = c->stream;
+  c->d_lat_tet_hex.upload(tet_hex, s);
+  c->d_lat_hex_tet.upload(packed, s);
+  HIPCHK(hipStreamSynchronize(s));  // the staging vectors are local
+  dv.tet_hex = c->d_lat_tet_hex.as<uint32_t>();
+  dv.hex_tet = c->d_lat_hex_tet.as<uint32_t>();
+  c->lat_dev = dv;
+}
+
 template <typename T>
 void upload_mesh_device(scdat_ctx* c) {
   HostMesh& h = c->h;
@@ -1107,6 +1241,9 @@ void upload_mesh_device(scdat_ctx* c) {
     }
     c->d_wmap.upload(wm, s);
     HIPCHK(hipStreamSynchronize(s));  // the staging vector is local
+    build_lattice(c);
+  } else {
+    c->lat_dev = LatticeDev{};
   }
   c->d_groups.upload(h.group, s);
   c->d_partner.upload(h.partner, s);
@@ -1865,6 +2002,9 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
     a.seg_wscale = c->d_seg_wscale.as<double>();
   }
   a.chunks = c->push_chunks;
+  if (sizeof(T) == 4 && c->lat_mode) a.lat = c->lat_dev;
+  if (c->dep_mode && c->dep_cap > 0) a.dep = DepBuckets{c->d_dep_count.as<unsigned int>(), c->d_dep_ent.as<DepEntry>(),
+                                                        c->dep_cap};
   if (c->push_mcc && species_has_mcc(c, si)) {
     const size_t nseg = c->segmented() ? (size_t)c->n_seg() : 1;
     a.mcc_ctl = c->d_mcc_ctl.as<double>() + (size_t)kMccCtl * si * nseg;
@@ -1879,7 +2019,8 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
   return a;
 }
 
-// Launch-shape tuner: for 1, 2, 4 and 8 particles per thread, pushes
+// Launch-shape tuner: for 1, 2, 4 and 8 particles per thread (each with and
+// without lattice location where the mesh allows it), pushes
 // cell-sorted scratch copies of every species for one sort cycle of that
 // species (its sort cadence, at most 20 pushes; 10 without sorting) in the
 // current field and times every push: a species' push slows as its
@@ -1918,9 +2059,13 @@ void autotune_push(scdat_ctx* c) {
   }
   const int cand[4] = {1, 2, 4, 8};
   double best = 1e300;
-  int pick = c->push_chunks;
+  int pick = c->push_chunks, pick_lat = 0;
   std::vector<int64_t> nsc((size_t)ns, 0);
-  for (int k = 0; k < 4; ++k) {
+  // lattice location is tried where the mesh has its tables (FP32)
+  const int n_lat = sizeof(T) == 4 && c->lat_dev.tet_hex ? 2 : 1;
+  for (int kk = 0; kk < 4 * n_lat; ++kk) {
+    const int k = kk % 4;
+    c->lat_mode = kk / 4;
     // scratch copies in cell order, as the stepping loop has them after a sort
     for (int si = 0; si < ns; ++si)
       if (c->sp[si].n) nsc[si] = sort_species_copy<T>(c, si);
@@ -1955,6 +2100,9 @@ void autotune_push(scdat_ctx* c) {
         launch_push<T>(a, s, e0, e1);
         HIPCHK(hipGetLastError());
       }
+      if (c->dep_mode && c->dep_cap > 0)
+        launch_dep_reduce(DepBuckets{c->d_dep_count.as<unsigned int>(), c->d_dep_ent.as<DepEntry>(), c->dep_cap},
+                          rho.as<unsigned long long>(), c->h.nn, s);
       HIPCHK(hipStreamSynchronize(s));
       for (int si = 0; si < ns; ++si) {
         if (!nsc[si] || p >= P[si]) continue;
@@ -1966,9 +2114,11 @@ void autotune_push(scdat_ctx* c) {
     if (tsum < best) {
       best = tsum;
       pick = cand[k];
+      pick_lat = c->lat_mode;
     }
   }
   c->push_chunks = pick;
+  c->lat_mode = pick_lat;
 }
 
 template <typename T>
@@ -2007,9 +2157,34 @@ void do_push(scdat_ctx* c, std::vector<std::pair<size_t, size_t>>* ev, size_t* e
     }
     HIPCHK(hipGetLastError());
   }
+  if (c->dep_mode && c->dep_cap > 0)
+    launch_dep_reduce(DepBuckets{c->d_dep_count.as<unsigned int>(), c->d_dep_ent.as<DepEntry>(), c->dep_cap},
+                      c->d_rho[c->rho_target].as<unsigned long long>(), c->h.nn, c->stream);
   c->rho_clean[c->rho_target] = false;
   c->rho_last = c->rho_target;
   c->rho_target = 1 - c->rho_target;
+}
+
+// Deposit buckets sized for the species' capacities: room for half an entry
+// per particle slot and range (a push makes about a quarter: a block's
+// table holds the nodes of its particles' tets), at most 32768 per range.
+void dep_setup(scdat_ctx* c) {
+  int64_t cap = 0;
+  for (const auto& S : c->sp) cap += S.cap;
+  const int64_t ranges = (c->h.nn + kDepRangeRows - 1) / kDepRangeRows;
+  if (ranges <= 0 || cap <= 0) {
+    c->dep_cap = 0;
+    return;
+  }
+  int64_t per = std::max<int64_t>(1024, cap / (2 * ranges));
+  int64_t p2 = 1024;
+  while (p2 < per && p2 < 32768) p2 <<= 1;
+  if (c->dep_cap == (int)p2 && c->d_dep_count.bytes >= 4 * (size_t)ranges) return;
+  c->d_dep_count.alloc(4 * (size_t)ranges);
+  HIPCHK(hipMemsetAsync(c->d_dep_count.p, 0, c->d_dep_count.bytes, c->stream));
+  c->d_dep_ent.alloc(sizeof(DepEntry) * (size_t)ranges * (size_t)p2);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->dep_cap = (int)p2;
 }
 
 static uint64_t morton3(const double* p, const double* lo, const double* hi);
@@ -2232,6 +2407,7 @@ void do_initialize(scdat_ctx* c) {
     c->d_gshift.upload(sh, c->stream);
   }
   upload_segments<T>(c);
+  dep_setup(c);
   solve_phi1<T>(c);
   if (c->n_patch > 0) {
     c->patch_q.fill(0.0);
@@ -2929,7 +3105,8 @@ static uint64_t morton3(const double* p, const double* lo, const double* hi) {
 // coordinate is one of a few grid-line values per axis, as on a structured
 // box mesh whose lines are graded to put a conductor's faces on them):
 // false when the grid lines would not be mostly occupied by nodes.
-static bool lattice_indices(int64_t nn, const double* xyz, std::vector<int32_t>& ijk, int32_t dims[3]) {
+static bool lattice_indices(int64_t nn, const double* xyz, std::vector<int32_t>& ijk, int32_t dims[3],
+                            std::vector<double>* lines_out) {
   if (nn < 8) return false;
   std::vector<double> line[3];
   for (int r = 0; r < 3; ++r) {
@@ -2953,6 +3130,8 @@ static bool lattice_indices(int64_t nn, const double* xyz, std::vector<int32_t>&
       size_t k = (size_t)(std::upper_bound(L.begin(), L.end(), x + 1e-9 * (L.back() - L.front())) - L.begin());
       ijk[3 * i + r] = (int32_t)(k - 1);
     }
+  if (lines_out)
+    for (int r = 0; r < 3; ++r) lines_out[r] = line[r];
   return true;
 }
 
@@ -2967,7 +3146,7 @@ static bool lattice_indices(int64_t nn, const double* xyz, std::vector<int32_t>&
 static bool tiled_perm(int64_t nn, const double* xyz, std::vector<int32_t>& perm, HostMesh& h) {
   std::vector<int32_t> ijk;
   int32_t dims[3];
-  if (!lattice_indices(nn, xyz, ijk, dims)) return false;
+  if (!lattice_indices(nn, xyz, ijk, dims, nullptr)) return false;
   int32_t nt[3];
   for (int r = 0; r < 3; ++r) nt[r] = (dims[r] - 1 + 7) / 8;
   const int64_t ntiles = (int64_t)nt[0] * nt[1] * nt[2];
@@ -4754,6 +4933,17 @@ int scdat_set_launch_shape(scdat_ctx* c, int32_t what, int32_t value) {
     c->step_graph = value;
     return SCDAT_OK;
   }
+  if (what == SCDAT_SHAPE_LATTICE_PUSH) {
+    if (value != 0 && value != 1) return fail(c, SCDAT_ERR_ARG, "set_launch_shape: lattice push must be 0 or 1");
+    c->lat_mode = value;
+    return SCDAT_OK;
+  }
+  if (what == SCDAT_SHAPE_DEPOSIT_BUCKETS) {
+    if (value != 0 && value != 1)
+      return fail(c, SCDAT_ERR_ARG, "set_launch_shape: deposit buckets must be 0 or 1");
+    c->dep_mode = value;
+    return SCDAT_OK;
+  }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS || !(value == 1 || value == 2 || value == 4 || value == 8))
     return fail(c, SCDAT_ERR_ARG, "set_launch_shape: push chunks must be 1, 2, 4 or 8");
   c->push_chunks = value;
@@ -4780,6 +4970,15 @@ int scdat_get_launch_shape(scdat_ctx* c, int32_t what, int32_t* value) {
   }
   if (what == SCDAT_SHAPE_STEP_GRAPH) {
     *value = c->step_graph;
+    return SCDAT_OK;
+  }
+  if (what == SCDAT_SHAPE_DEPOSIT_BUCKETS) {
+    *value = c->dep_mode;
+    return SCDAT_OK;
+  }
+  if (what == SCDAT_SHAPE_LATTICE_PUSH) {
+    // 1 only when the push locates on the lattice (FP32, lattice mesh)
+    *value = c->lat_mode && c->lat_dev.tet_hex && !c->is_fp64() ? 1 : 0;
     return SCDAT_OK;
   }
   if (what != SCDAT_SHAPE_PUSH_CHUNKS) return SCDAT_ERR_ARG;

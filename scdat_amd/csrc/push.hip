@@ -59,6 +59,56 @@ __device__ __forceinline__ void tile_flush(DepositTile& L, unsigned long long* r
   }
 }
 
+// The block's table into the deposit buckets (every thread of the block
+// calls it, after a barrier that follows the last insertion).
+__device__ void tile_flush_buckets(DepositTile& L, unsigned long long* rho, const DepBuckets& B) {
+  static_assert(kHash == 2 * kBlock, "two table slots per thread");
+  __shared__ int32_t rid[kDepRanges];
+  __shared__ unsigned int rcount[kDepRanges], rbase[kDepRanges];
+  if (threadIdx.x < kDepRanges) {
+    rid[threadIdx.x] = -1;
+    rcount[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  int slot[2] = {-1, -1};
+  unsigned int rank[2] = {0u, 0u};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int j = threadIdx.x + q * kBlock;
+    const int32_t k = L.key[j];
+    if (k < 0 || L.val[j] == 0ull) continue;
+    const int32_t r = k / kDepRangeRows;
+    for (int e = 0; e < kDepRanges; ++e) {
+      const int32_t o = atomicCAS(&rid[e], -1, r);
+      if (o == -1 || o == r) {
+        slot[q] = e;
+        break;
+      }
+    }
+    if (slot[q] >= 0) rank[q] = atomicAdd(&rcount[slot[q]], 1u);
+    else atomicAdd(&rho[k], L.val[j]);  // more ranges than the block tracks
+  }
+  __syncthreads();
+  if (threadIdx.x < kDepRanges && rid[threadIdx.x] >= 0)
+    rbase[threadIdx.x] = atomicAdd(&B.count[rid[threadIdx.x]], rcount[threadIdx.x]);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    if (slot[q] < 0) continue;
+    const int j = threadIdx.x + q * kBlock;
+    const unsigned int at = rbase[slot[q]] + rank[q];
+    if (at < (unsigned int)B.cap) {
+      DepEntry d;
+      d.node = L.key[j];
+      d.pad = 0;
+      d.val = L.val[j];
+      B.ent[(size_t)rid[slot[q]] * (size_t)B.cap + at] = d;
+    } else {
+      atomicAdd(&rho[L.key[j]], L.val[j]);  // bucket full
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void tile_runs(int key, int32_t f[4], int64_t cs,
                                           const TetMap<T>* __restrict__ tets,
@@ -106,8 +156,10 @@ __device__ __forceinline__ void tile_runs(int key, int32_t f[4], int64_t cs,
   }
 }
 
-// One block pushes CH x kBlock consecutive particles.
-template <typename T, int CH>
+// One block pushes CH x kBlock consecutive particles.  LAT (FP32, lattice
+// meshes): sub-steps that stay clear of boundary faces locate their end tet
+// on the grid instead of walking.
+template <typename T, int CH, bool LAT>
 __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
   const int64_t n = a.n_dev ? *a.n_dev : a.n;
   // a captured step's grid covers the capacity: the blocks the count needs
@@ -152,10 +204,19 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
       T v[3] = {a.vx[i], a.vy[i], a.vz[i]};
       const T v0[3] = {v[0], v[1], v[2]};
       // FP32: walk records (one dependent load per face crossing); the nodes
-      // of the sub-step's start cell come from the tet map
+      // of the sub-step's start cell come from the tet map.  LAT: the tet
+      // map of the start cell (affine map and nodes in one record) and its
+      // hexahedron; the walk record only when a sub-step walks.
       using Map = typename std::conditional<sizeof(T) == 4, WalkMap<float>, TetMap<T>>::type;
       Map m;
-      load_map(a, c, m);
+      TetMap<T> tm;
+      uint32_t th = 0;
+      if constexpr (LAT) {
+        tm = a.tets[c];
+        th = a.lat.tet_hex[c];
+      } else {
+        load_map(a, c, m);
+      }
       int sg = 0;
       T remaining = a.dt;
       double wt = a.weight;
@@ -172,13 +233,17 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
       while (remaining > T(1e-20) && sub < a.max_sub) {
         ++sub;
         T lam[4];
-        bary(m, x, lam);
         T E[3];
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (LAT) {
+          bary(tm, x, lam);
+          gather_E(a, tm.node, lam, E);
+        } else if constexpr (sizeof(T) == 4) {
+          bary(m, x, lam);
           const int4 nd4 = *reinterpret_cast<const int4*>(a.tets[c].node);
           const int32_t nd[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
           gather_E(a, nd, lam, E);
         } else {
+          bary(m, x, lam);
           gather_E(a, m, lam, E);
         }
         const T h =
@@ -186,7 +251,31 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
         boris(v, E, a.B, a.has_b != 0, a.qm, h);
         T xs[3] = {x[0], x[1], x[2]};
         T xn[3] = {fma_t(v[0], h, x[0]), fma_t(v[1], h, x[1]), fma_t(v[2], h, x[2])};
-        const int r = walk(a, c, xs, xn, v, group, hit, m, face, last);
+        int r;
+        if constexpr (LAT) {
+          // the walk's first test: still inside the start tet
+          T l1[4];
+          bary(tm, xn, l1);
+          if (l1[0] >= -a.tol && l1[1] >= -a.tol && l1[2] >= -a.tol && l1[3] >= -a.tol) {
+            r = c;
+          } else {
+            uint32_t hn = 0;
+            r = lat_locate(a.lat, th, xn, hn);
+            if (r == -3) {
+              // the walk record only here, so it shares registers with tm
+              Map wm;
+              load_map(a, c, wm);
+              r = walk(a, c, xs, xn, v, group, hit, wm, face, last);
+              if (r >= 0) hn = a.lat.tet_hex[r];
+            }
+            if (r >= 0) {
+              tm = a.tets[r];
+              th = hn;
+            }
+          }
+        } else {
+          r = walk(a, c, xs, xn, v, group, hit, m, face, last);
+        }
         if (r < 0) {
           dead = r;
           break;
@@ -240,7 +329,8 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
         a.z[i] = hit[2];
       } else {
         T lam[4];
-        bary(m, x, lam);
+        if constexpr (LAT) bary(tm, x, lam);
+        else bary(m, x, lam);
         lambda_fixed(lam, f);
         key = c;
         alive += 1;
@@ -296,7 +386,8 @@ __global__ void __launch_bounds__(kBlock) push_kernel(PushArgs<T> a) {
       atomicMax(a.mcc_v2max, (unsigned long long)__double_as_longlong(v2max));
   }
   __syncthreads();
-  tile_flush(tile, a.rho);
+  if (a.dep.count) tile_flush_buckets(tile, a.rho, a.dep);
+  else tile_flush(tile, a.rho);
   if (threadIdx.x < kMaxGroups) {
     const unsigned int n = led_n[threadIdx.x];
     if (n) {
@@ -343,29 +434,65 @@ __global__ void __launch_bounds__(kBlock) deposit_kernel(PushArgs<T> a) {
   tile_flush(tile, a.rho);
 }
 
+// One block per 512-row range: its bucket summed in LDS, added to its rows.
+__global__ void __launch_bounds__(kDepRangeRows) k_dep_reduce(DepBuckets b, unsigned long long* rho, int64_t n_rows) {
+  __shared__ unsigned long long acc[kDepRangeRows];
+  acc[threadIdx.x] = 0ull;
+  const unsigned int n = min(b.count[blockIdx.x], (unsigned int)b.cap);
+  __syncthreads();
+  const DepEntry* e = b.ent + (size_t)blockIdx.x * (size_t)b.cap;
+  for (unsigned int q = threadIdx.x; q < n; q += kDepRangeRows) {
+    const DepEntry d = e[q];
+    atomicAdd(&acc[d.node - (int32_t)blockIdx.x * kDepRangeRows], d.val);
+  }
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * kDepRangeRows + threadIdx.x;
+  if (row < n_rows && acc[threadIdx.x]) rho[row] += acc[threadIdx.x];
+  if (threadIdx.x == 0) b.count[blockIdx.x] = 0u;
+}
+
+void launch_dep_reduce(const DepBuckets& b, unsigned long long* rho, int64_t n_rows, hipStream_t s) {
+  if (!b.count || n_rows <= 0) return;
+  const int64_t nr = (n_rows + kDepRangeRows - 1) / kDepRangeRows;
+  hipLaunchKernelGGL(k_dep_reduce, dim3((uint32_t)nr), dim3(kDepRangeRows), 0, s, b, rho, n_rows);
+}
+
 int push_blocks(int64_t n, int chunks) {
   return (int)((n + (int64_t)kBlock * chunks - 1) / ((int64_t)kBlock * chunks));
+}
+
+template <typename T, bool LAT>
+void launch_push_shape(const PushArgs<T>& a, dim3 g, dim3 b, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
+  if constexpr (LAT && sizeof(T) != 4) {
+    return;  // the lattice path is FP32 only
+  } else {
+    if (!start && !stop) {
+      // plain launches (stream capture records them as kernel nodes)
+      switch (a.chunks) {
+        case 1: hipLaunchKernelGGL((push_kernel<T, 1, LAT>), g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((push_kernel<T, 2, LAT>), g, b, 0, s, a); break;
+        case 8: hipLaunchKernelGGL((push_kernel<T, 8, LAT>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((push_kernel<T, 4, LAT>), g, b, 0, s, a); break;
+      }
+      return;
+    }
+    switch (a.chunks) {
+      case 1: hipExtLaunchKernelGGL((push_kernel<T, 1, LAT>), g, b, 0, s, start, stop, 0, a); break;
+      case 2: hipExtLaunchKernelGGL((push_kernel<T, 2, LAT>), g, b, 0, s, start, stop, 0, a); break;
+      case 8: hipExtLaunchKernelGGL((push_kernel<T, 8, LAT>), g, b, 0, s, start, stop, 0, a); break;
+      default: hipExtLaunchKernelGGL((push_kernel<T, 4, LAT>), g, b, 0, s, start, stop, 0, a); break;
+    }
+  }
 }
 
 template <typename T>
 void launch_push(const PushArgs<T>& a, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   if (a.n <= 0) return;
   const dim3 g(push_blocks(a.n, a.chunks)), b(kBlock);
-  if (!start && !stop) {
-    // plain launches (stream capture records them as kernel nodes)
-    switch (a.chunks) {
-      case 1: hipLaunchKernelGGL((push_kernel<T, 1>), g, b, 0, s, a); break;
-      case 2: hipLaunchKernelGGL((push_kernel<T, 2>), g, b, 0, s, a); break;
-      case 8: hipLaunchKernelGGL((push_kernel<T, 8>), g, b, 0, s, a); break;
-      default: hipLaunchKernelGGL((push_kernel<T, 4>), g, b, 0, s, a); break;
-    }
-    return;
-  }
-  switch (a.chunks) {
-    case 1: hipExtLaunchKernelGGL((push_kernel<T, 1>), g, b, 0, s, start, stop, 0, a); break;
-    case 2: hipExtLaunchKernelGGL((push_kernel<T, 2>), g, b, 0, s, start, stop, 0, a); break;
-    case 8: hipExtLaunchKernelGGL((push_kernel<T, 8>), g, b, 0, s, start, stop, 0, a); break;
-    default: hipExtLaunchKernelGGL((push_kernel<T, 4>), g, b, 0, s, start, stop, 0, a); break;
+  if (sizeof(T) == 4 && a.lat.tet_hex) {
+    launch_push_shape<T, true>(a, g, b, s, start, stop);
+  } else {
+    launch_push_shape<T, false>(a, g, b, s, start, stop);
   }
 }
 template <typename T>

@@ -9,6 +9,57 @@
 
 namespace scdat {
 
+// Lattice location for the FP32 push.  On a mesh of Kuhn tetrahedra over a
+// rectilinear grid (the box mesher's: six tets per hexahedron around its
+// v000-v111 diagonal), the tet holding a point follows from arithmetic: the
+// hexahedron from the grid line runs of each axis, the tet from the
+// descending order of the point's fractional coordinates u in it (its
+// barycentric coordinates are 1 - u_p0, u_p0 - u_p1, u_p1 - u_p2, u_p2).
+// A sub-step whose end point left its start tet is located this way when
+// (a) every barycentric coordinate of the end point is at least kLatEps, so
+// the end tet is the one the face walk reaches (no face is within rounding
+// of the point), and (b) the end hexahedron's clearance d exceeds the
+// Chebyshev distance from the start hexahedron, so every hexahedron between
+// them is whole and the segment crosses no boundary face.  Otherwise the
+// sub-step walks.  Results are those of the walk, bit for bit.
+constexpr int kLatSeg = 4;        // uniform-spacing runs of grid lines per axis
+constexpr int kLatAxis = 1024;    // hexahedra per axis (10-bit indices)
+constexpr float kLatEps = 1e-3f;  // smallest barycentric coordinate located on the grid
+struct LatticeDev {
+  const uint32_t* tet_hex;  // per tet: its hexahedron i | j << 10 | k << 20 (null: no lattice)
+  // 6 per hexahedron (i slowest, k fastest), by Kuhn permutation:
+  // (tet + 1) | clearance << 29, 0 where the tet is missing; clearance 0 (the
+  // hexahedron is not whole) to 7 (whole hexahedra to Chebyshev distance 6)
+  const uint32_t* hex_tet;
+  int nh[3];                // hexahedra per axis
+  int nseg[3];              // uniform-spacing runs per axis
+  float sx[3][kLatSeg];     // first line of each run
+  float sinv[3][kLatSeg];   // 1 / spacing of each run
+  int si[3][kLatSeg + 1];   // index of each run's first line; si[r][nseg[r]] = nh[r]
+  float lo[3], hi[3];       // first and last line of each axis
+};
+
+// Deposit buckets.  A push block reduces its particles' charge into an LDS
+// table keyed by node; the table then goes to the nodes' 512-row ranges:
+// each range has a bucket of (node, charge) entries, a block reserves room
+// in a bucket with one atomic per range it touches and writes its entries
+// with plain stores, and after the pushes one block per range sums its
+// bucket in LDS and adds the sums to its own rows of the charge.  An entry
+// that finds its bucket full (or a block touching more than kDepRanges
+// ranges) goes to the charge with a global atomic.  Integer sums, so the
+// charge is the same bits in any case.
+constexpr int kDepRangeRows = 512;
+constexpr int kDepRanges = 16;
+struct alignas(16) DepEntry {
+  int32_t node, pad;
+  unsigned long long val;
+};
+struct DepBuckets {
+  unsigned int* count;  // [ranges] entries appended since the last reduction (null: no buckets)
+  DepEntry* ent;        // [ranges][cap]
+  int cap;
+};
+
 template <typename T>
 struct PushArgs {
   // mesh
@@ -78,6 +129,8 @@ struct PushArgs {
   // key (2 words) per scenario, at mcc_ctl + s kMccCtl, mcc_v2max + s and
   // mcc_seg_key + 2 s
   const uint32_t* mcc_seg_key;
+  LatticeDev lat;  // FP32 only (lat.tet_hex null: every sub-step walks)
+  DepBuckets dep;  // dep.count null: blocks add their tables with global atomics
 };
 
 // blocks of one push launch over n particle slots with `chunks` particles
@@ -89,5 +142,8 @@ void launch_push(const PushArgs<T>& a, hipStream_t s, hipEvent_t start = nullptr
                  hipEvent_t stop = nullptr);
 template <typename T>
 void launch_deposit(const PushArgs<T>& a, hipStream_t s);
+// Sums the deposit buckets into rho (n_ranges = rows rounded up to
+// kDepRangeRows) and empties them.
+void launch_dep_reduce(const DepBuckets& b, unsigned long long* rho, int64_t n_rows, hipStream_t s);
 
 }  // namespace scdat

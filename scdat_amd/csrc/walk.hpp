@@ -126,4 +126,54 @@ __device__ __forceinline__ int walk(const PushArgs<T>& a, int cur, T x0[3], T x1
   return -2;
 }
 
+// Grid cell of x along axis r of the lattice and the fractional coordinate
+// in it, from the run's first line and spacing; false outside the grid or
+// within kLatEps of a grid line.
+__device__ __forceinline__ bool lat_axis(const LatticeDev& L, int r, float x, int& i, float& u) {
+  if (!(x >= L.lo[r] && x < L.hi[r])) return false;
+  // the run holding x, selected with compile-time indices (no per-lane
+  // indexing of the argument tables)
+  float x0 = L.sx[r][0], inv = L.sinv[r][0];
+  int j0 = L.si[r][0], j1 = L.si[r][1];
+#pragma unroll
+  for (int k = 1; k < kLatSeg; ++k) {
+    if (k < L.nseg[r] && x >= L.sx[r][k]) {
+      x0 = L.sx[r][k];
+      inv = L.sinv[r][k];
+      j0 = L.si[r][k];
+      j1 = L.si[r][k + 1];
+    }
+  }
+  const float t = (x - x0) * inv;
+  const float ft = floorf(t);
+  i = j0 + (int)ft;
+  u = t - ft;
+  return i >= j0 && i < j1 && u >= kLatEps && u <= 1.0f - kLatEps;
+}
+
+// End tet of a sub-step that left its start tet, from the start tet's
+// hexahedron th to xn on the lattice; -3 when the sub-step must walk.
+// hx_out receives the end hexahedron (packed like th).
+__device__ __forceinline__ int lat_locate(const LatticeDev& L, uint32_t th, const float xn[3], uint32_t& hx_out) {
+  int i[3];
+  float u[3];
+  if (!lat_axis(L, 0, xn[0], i[0], u[0]) || !lat_axis(L, 1, xn[1], i[1], u[1]) ||
+      !lat_axis(L, 2, xn[2], i[2], u[2]))
+    return -3;
+  // Kuhn permutation (p0, p1, p2), u[p0] >= u[p1] >= u[p2], in the order
+  // (012, 021, 102, 120, 201, 210); near-equal coordinates walk
+  const float d01 = u[0] - u[1], d02 = u[0] - u[2], d12 = u[1] - u[2];
+  if (fabsf(d01) < kLatEps || fabsf(d02) < kLatEps || fabsf(d12) < kLatEps) return -3;
+  const bool a01 = d01 > 0.0f, a02 = d02 > 0.0f, a12 = d12 > 0.0f;
+  const int p = (a01 && a02) ? (a12 ? 0 : 1) : ((!a01 && a12) ? (a02 ? 2 : 3) : (a01 ? 4 : 5));
+  const int64_t hx = ((int64_t)i[0] * L.nh[1] + i[1]) * L.nh[2] + i[2];
+  const uint32_t e = L.hex_tet[6 * hx + p];
+  const int di = abs(i[0] - (int)(th & 1023u)), dj = abs(i[1] - (int)((th >> 10) & 1023u)),
+            dk = abs(i[2] - (int)((th >> 20) & 1023u));
+  const int dm = di > dj ? (di > dk ? di : dk) : (dj > dk ? dj : dk);
+  if (e == 0u || dm >= (int)(e >> 29)) return -3;
+  hx_out = (uint32_t)i[0] | (uint32_t)i[1] << 10 | (uint32_t)i[2] << 20;
+  return (int)(e & 0x1FFFFFFFu) - 1;
+}
+
 }  // namespace scdat

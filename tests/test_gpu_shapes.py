@@ -28,6 +28,32 @@ def test_push_shapes_give_identical_results(precision):
         assert s["active_particles"] == out[0][2]["active_particles"]
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_deposit_buckets_give_identical_results(precision):
+    """Push blocks handing their charge tables to per-range buckets give the
+    charge of one global atomic per node, bit for bit."""
+    st = plasma_setup(n=12, npart=30000, seed=7)
+    out = []
+    for mode in (1, 0):
+        c = make_ctx(st, precision=precision, tol=1e-10, sort_interval=3)
+        c.deposit_buckets = mode
+        assert c.deposit_buckets == mode
+        c.initialize()
+        c.push()
+        rho1 = c.rho_fixed()[0]
+        c.step(6)
+        out.append((rho1, c.rho_fixed()[0], c.fields(), c.get_particles_by_id(0, 30000), c.stats()))
+    a, b = out
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    for x, y in zip(a[2], b[2]):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a[3][0], b[3][0])
+    assert a[4]["floating_potential"] == b[4]["floating_potential"]
+    with pytest.raises(S.ScdatError):
+        c.deposit_buckets = 2
+
+
 def test_autotune_keeps_state():
     st = plasma_setup(n=10, npart=20000, seed=6)
     c = make_ctx(st, precision=0, tol=1e-10, sort_interval=0)
