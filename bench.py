@@ -20,7 +20,8 @@ Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 (log-uniform), Te = Ti log-uniform in 1 eV..20 keV, n log-uniform in
 1e6..1e11 m^-3, Sternglass SEE delta_max uniform in 1..3 on the conductor,
 FP32 with FP64 refinement; all cases advance together in one context, and
-the same cases are then run one after another in their own contexts.
+the same cases are then run one after another in their own contexts, and
+the batch continues with its steps replayed from HIP graphs.
 value = particle pushes of all cases / wall time of the batched steps.
 
 --workload adjoint runs the discrete adjoint of the floating potential on
@@ -37,6 +38,11 @@ null-collision MCC), Sternglass secondary emission (delta_max 2 at 400 eV)
 on the conductor and 20 uA/m^2 photoemission from its sunlit +x face, FP32
 with FP64 refinement, dt = 0.2 h / v_th,e, the configuration's 2000 steps
 timed after the warm-up (--geo-steps).  value = particle pushes / wall time.
+
+--workload fp64 runs the reference-harness case in FP64: 16^3 hexes, 10^5
+electrons + 10^5 protons (Te = Ti = 1 eV, n = 1e10), 100 collisionless steps,
+Jacobi-PCG to 1e-10, beside the FP64 oracle on the same inputs (CPU
+baseline and the potential's error after the steps).
 
 --copy-calibration MB runs only a plain device copy of MB megabytes (the
 reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
@@ -164,10 +170,17 @@ def run_gpu(case, steps, warmup, sort_interval, kernel_timing=True, row_patterns
         ctx.sync()
         gw = time.perf_counter() - t2
         gp = ctx.stats()["particle_pushes"] - p0
-        graph = dict(value=gp / gw, ms_per_step=1e3 * gw / graph_steps, steps=graph_steps,
-                     stage_ms_total=ctx.stage_times())
+        gst = ctx.stage_times()
         ctx.step_graph = 0
-        log(f"gpu, captured steps: {graph_steps} steps in {gw:.3f}s, pushes {gp}")
+        p1 = ctx.stats()["particle_pushes"]
+        t3 = time.perf_counter()
+        ctx.step(graph_steps)
+        ctx.sync()
+        ew = time.perf_counter() - t3
+        ep = ctx.stats()["particle_pushes"] - p1
+        graph = dict(value=gp / gw, ms_per_step=1e3 * gw / graph_steps, steps=graph_steps, stage_ms_total=gst,
+                     eager_after=dict(value=ep / ew, ms_per_step=1e3 * ew / graph_steps))
+        log(f"gpu, captured steps: {graph_steps} steps in {gw:.3f}s, pushes {gp}; eager after {ew:.3f}s, {ep}")
     return ctx, dict(wall=wall, pushes=pushes, stages=stages, kt=kt, stats=st1, stats0=st0, chunks=chunks,
                      solver=solver, graph=graph)
 
@@ -232,11 +245,16 @@ def run_sweep(args):
     stages = bc.stage_times()
     conv = [bc.scenario_stats(k) for k in range(len(sweep))]
     log(f"batched: {args.steps} steps {bwall:.3f}s, pushes {bpush}, stages {stages}")
-    # the same context, continued with the steps replayed from HIP graphs
+    # the same context continued with the steps replayed from HIP graphs,
+    # then eagerly again (the particle count falls as the walls absorb, so
+    # the captured window is compared with the eager windows either side)
     bc.step_graph = 1
     gwall, gpush = timed(bc)
     gstages = bc.stage_times()
-    log(f"batched, captured steps: {args.steps} steps {gwall:.3f}s, pushes {gpush}, stages {gstages}")
+    bc.step_graph = 0
+    ewall, epush = timed(bc)
+    log(f"batched, captured steps: {args.steps} steps {gwall:.3f}s, pushes {gpush}, stages {gstages}; "
+        f"eager after: {ewall:.3f}s, pushes {epush}")
     bc.close()
     del bc
     swall, spush = 0.0, 0
@@ -274,9 +292,10 @@ def run_sweep(args):
                        "note": "the same cases, each in its own context, one after another (timed steps only)"},
         "speedup_vs_sequential": value / (spush / swall),
         "step_graph": {"value": gpush / gwall, "ms_per_step": 1e3 * gwall / args.steps, "stage_ms_total": gstages,
+                       "eager_after": {"value": epush / ewall, "ms_per_step": 1e3 * ewall / args.steps},
                        "note": "the same context continued for as many steps with the launches around the "
-                               "PCG iteration replayed from HIP graphs (SCDAT_SHAPE_STEP_GRAPH); value and "
-                               "ms_per_step above are the eager launches"},
+                               "PCG iteration replayed from HIP graphs (SCDAT_SHAPE_STEP_GRAPH), then as many "
+                               "eager steps again; value and ms_per_step above are the first eager window"},
         "stage_ms_total": stages,
         "cases_converged": int(sum(1 for q in conv if q["pcg_converged"])),
         "pcg_iterations_max": int(max(q["pcg_iterations"] for q in conv)),
@@ -570,6 +589,92 @@ def run_geo(args):
     print(json.dumps(res), flush=True)
 
 
+def run_fp64(args):
+    """The reference-harness case in FP64: 16^3 hexes, 10^5 electrons + 10^5
+    protons, 100 collisionless steps, Jacobi-PCG to 1e-10; the FP64 oracle
+    runs the same steps from the same inputs (the CPU baseline, and the
+    error figures)."""
+    import scdat_amd as S
+    name, _ = S.device_info()
+    steps = args.fp64_steps
+    case = build_case(16, 200_000, 20261203)
+    m = case["mesh"]
+    ctx = S.Context(precision=1, dt=case["dt"], pcg_tol=1e-10, sort_interval=args.sort_interval)
+    ctx.set_mesh(m["xyz"], m["tets"], m["kind"], group=m["group"])
+    for g in range(6, 12):
+        ctx.set_boundary(g, S.BC_CONDUCTOR)
+    for (q, mass), (x, v, c) in zip(case["qm"], (case["e"], case["i"])):
+        sid = ctx.add_species(q, mass, case["w"], len(x))
+        ctx.set_particles(sid, x, v, c)
+    ctx.initialize()
+    chunks = ctx.autotune()
+    ctx.sync()
+    st0 = ctx.stats()
+    t1 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    wall = time.perf_counter() - t1
+    st1 = ctx.stats()
+    pushes = st1["particle_pushes"] - st0["particle_pushes"]
+    _, phi_g, _ = ctx.fields()
+    stages = ctx.stage_times()
+    log(f"fp64: {steps} steps in {wall:.3f}s, pushes {pushes}, stages {stages}")
+    res = {
+        "metric": "particle-steps/s incl. Poisson, FP64 reference-harness case 16^3 / 2x10^5 ptcl",
+        "value": pushes / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic Maxwellian plasma (Philox seed 20261203)",
+        "config": {"workload": "pic box 1m, conductor 0.2m floating, 16^3 hex -> %d tets, %d nodes, 2x10^5 "
+                               "particles (e-/H+), Te=Ti=1eV, n=1e10, dt=0.2/wpe, %d steps, Jacobi-PCG 1e-10, FP64"
+                               % (len(m["tets"]), len(m["kind"]), steps)},
+        "device": name,
+        "launch_shape": {"push_particles_per_thread": chunks},
+        "stage_ms_total": stages,
+        "floating_potential_v": st1["floating_potential"],
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        import oracle
+        o = oracle.Oracle(m["xyz"], m["tets"], m["kind"], group=m["group"], threads=args.cpu_threads)
+        for g in range(6, 12):
+            o.set_group(g, 3)
+        o.set_poisson(0, S.VACUUM_PERMITTIVITY, 1, 1e-10, 50000)
+        o.set_push(case["dt"], 1, 1e-3, 64, 0)
+        for (q, mass), (x, v, c) in zip(case["qm"], (case["e"], case["i"])):
+            k = o.add_species(q, mass, case["w"])
+            o.set_particles(k, x, v, c)
+        o.deposit()
+        o.solve()
+        o.efield()
+        cw, cp = 0.0, 0
+        for _ in range(steps):
+            alive = sum(int((o.get_particles(k)[2] >= 0).sum()) for k in range(2))
+            t1 = time.perf_counter()
+            o.step(1)
+            cw += time.perf_counter() - t1
+            cp += alive
+        _, phi_o, _ = o.fields()
+        vo = o.floating_potential()
+        res["cpu_baseline"] = {"value": cp / cw, "unit": "particle-steps/s", "cores": oracle.lib().orc_num_threads(),
+                               "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "the same %d steps from the same inputs, FP64 oracle with OpenMP" % steps}
+        res["gpu_over_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        res["error_vs_fp64_oracle"] = {
+            "steps": steps,
+            "phi_rel_max": float(np.max(np.abs(phi_g - phi_o)) / max(np.max(np.abs(phi_o)), 1e-300)),
+            "floating_potential_rel": abs(st1["floating_potential"] - vo) / max(abs(vo), 1e-300)}
+        log(f"cpu baseline: {steps} steps {cw:.2f}s")
+    print(json.dumps(res), flush=True)
+
+
 def copy_calibration(mb):
     import torch
     n = int(mb * 1e6) // 4
@@ -598,7 +703,9 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no per-launch events in the timed region (no roofline figures)")
     ap.add_argument("--copy-calibration", type=float, default=0.0)
-    ap.add_argument("--workload", choices=("box", "sweep", "adjoint", "sampler", "arc", "geo"), default="box")
+    ap.add_argument("--workload", choices=("box", "sweep", "adjoint", "sampler", "arc", "geo", "fp64"),
+                    default="box")
+    ap.add_argument("--fp64-steps", type=int, default=100)
     ap.add_argument("--geo-steps", type=int, default=2000)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--scenarios", type=int, default=256)
@@ -633,6 +740,9 @@ def main():
         return
     if args.workload == "geo":
         run_geo(args)
+        return
+    if args.workload == "fp64":
+        run_fp64(args)
         return
 
     import scdat_amd as S
@@ -742,8 +852,8 @@ def main():
         "stage_ms_total": g["stages"],
         "step_graph": None if g["graph"] is None else dict(g["graph"], note=(
             "the same context continued for as many steps with the launches around the PCG iteration "
-            "replayed from HIP graphs (SCDAT_SHAPE_STEP_GRAPH); the particle count falls as the walls "
-            "absorb, so compare ms_per_step")),
+            "replayed from HIP graphs (SCDAT_SHAPE_STEP_GRAPH), then as many eager steps (eager_after); "
+            "the particle count falls as the walls absorb, so compare with the eager windows either side")),
         "launch_shape": {"push_particles_per_thread": g["chunks"], "chosen_by": "scdat_autotune",
                          "cell_order": "morton" if args.cell_order == 1 else "tet index"},
         "solver": {"rows": sv["rows"], "ell_width": width, "row_patterns": npat, "pcg_block": sv["pcg_block"],
