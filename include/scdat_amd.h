@@ -42,6 +42,13 @@ extern "C" {
 #define SCDAT_BC_ESCAPE 4     /* no condition set (the default): the particle leaves,
                                  marked ESCAPED, no ledger entry or surface charge
                                  (PICCycle.cpp:1254-1257,1355-1377)                 */
+#define SCDAT_BC_HOST 5       /* a caller-side condition (a BoundaryConditionBase
+                                 subclass with no device form): the particle leaves
+                                 the device at the face crossing and its impact is
+                                 kept for the caller (scdat_get_host_impacts), who
+                                 runs processParticle and appends what survives or
+                                 is emitted; no ledger entry or surface charge on
+                                 the device (PICCycle.cpp:1221-1293)                */
 
 /* node kinds */
 #define SCDAT_NODE_FREE 0
@@ -141,6 +148,19 @@ int scdat_set_boundary(scdat_ctx* ctx, int32_t group, int32_t type, int32_t part
                        const double* shift);
 /* Particles that left through ESCAPE groups, per species, since the start. */
 int scdat_get_escaped(scdat_ctx* ctx, int32_t species, int64_t* n);
+/* Impacts on SCDAT_BC_HOST groups during the last step (the slow path for
+ * BoundaryConditionBase::processParticle, BoundaryConditions.h:116-121, that
+ * PICCycle::applyRuntimeBoundaryCondition calls, PICCycle.cpp:1221-1293):
+ * species, particle id, group, the crossing point (on the face) and the
+ * velocity there.  Writes at most max entries and the count of the step's
+ * impacts into *n (more than max: call again with a larger buffer).  Only
+ * the last step's impacts are kept, so a caller with such groups steps one
+ * step at a time.                                                          */
+typedef struct {
+  int32_t species, id, group, cell;
+  double x[3], v[3];
+} scdat_impact;
+int scdat_get_host_impacts(scdat_ctx* ctx, int64_t max, scdat_impact* out, int64_t* n);
 
 /* Species (ParticleManager::createElectron/createIon semantics,
  * Tools/Particle/include/ParticleManager.h:821-926): charge q (C), mass m (kg),

@@ -31,7 +31,7 @@ VACUUM_PERMITTIVITY = 8.8541878128e-12
 BOLTZMANN = 1.380649e-23
 
 OK, ERR_ARG, ERR_STATE, ERR_DEVICE, ERR_NOMEM, ERR_SOLVER = 0, -1, -2, -3, -4, -5
-BC_ABSORB, BC_PERIODIC, BC_REFLECT, BC_CONDUCTOR, BC_ESCAPE = 0, 1, 2, 3, 4
+BC_ABSORB, BC_PERIODIC, BC_REFLECT, BC_CONDUCTOR, BC_ESCAPE, BC_HOST = 0, 1, 2, 3, 4, 5
 KERNEL_PUSH, KERNEL_PCG = 0, 1
 NODE_FREE, NODE_DIRICHLET, NODE_CONDUCTOR = 0, 1, 2
 MAX_GROUPS = 16
@@ -86,6 +86,11 @@ class LedgerEntry(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Impact(C.Structure):
+    _fields_ = [("species", C.c_int32), ("id", C.c_int32), ("group", C.c_int32), ("cell", C.c_int32),
+                ("x", C.c_double * 3), ("v", C.c_double * 3)]
 
 
 MCC_ELASTIC, MCC_INELASTIC, MCC_IONIZATION, MCC_EXCITATION = 0, 1, 2, 3
@@ -231,6 +236,7 @@ SIGNATURES = {
     "scdat_set_mesh": (C.c_int, [_P, C.c_int64, _D, C.c_int64, _I32, _I32, _D, _I32]),
     "scdat_set_boundary": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _D]),
     "scdat_get_escaped": (C.c_int, [_P, C.c_int32, _I64]),
+    "scdat_get_host_impacts": (C.c_int, [_P, C.c_int64, C.POINTER(Impact), _I64]),
     "scdat_add_species": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_int64, _I32]),
     "scdat_set_particles": (C.c_int, [_P, C.c_int32, C.c_int64, _D, _D, _I32]),
     "scdat_get_particle_count": (C.c_int, [_P, C.c_int32, _I64, _I64]),
@@ -498,6 +504,18 @@ class Context:
         n = C.c_int64()
         self._chk(lib().scdat_get_escaped(self.h, species, C.byref(n)))
         return n.value
+
+    def host_impacts(self):
+        """Impacts on BC_HOST groups in the last step: a list of dicts with
+        species, id, group, cell, x (crossing point) and v (velocity)."""
+        n = C.c_int64()
+        self._chk(lib().scdat_get_host_impacts(self.h, 0, None, C.byref(n)))
+        if n.value == 0:
+            return []
+        arr = (Impact * n.value)()
+        self._chk(lib().scdat_get_host_impacts(self.h, n.value, arr, C.byref(n)))
+        return [dict(species=e.species, id=e.id, group=e.group, cell=e.cell, x=np.array(e.x[:]),
+                     v=np.array(e.v[:])) for e in arr[:n.value]]
 
     def add_species(self, q, m, w, capacity):
         sid = C.c_int32()

@@ -1993,6 +1993,9 @@ PushArgs<T> push_args(scdat_ctx* c, int si) {
       for (const auto& se : c->seg_emit) on = on || se[g].see_model != SCDAT_SEE_OFF;
       if (on && (c->gtype[g] == SCDAT_BC_CONDUCTOR || c->gtype[g] == SCDAT_BC_ABSORB)) a.see_mask |= 1u << g;
     }
+  if (c->event_cap > 0)
+    for (int g = 0; g < kMaxGroups; ++g)
+      if (c->gtype[g] == SCDAT_BC_HOST) a.see_mask |= 1u << g;  // impacts for the caller
   a.events = c->d_events.as<SeeEvent>();
   a.n_events = c->d_n_events.as<unsigned long long>();
   a.event_cap = c->event_cap;
@@ -2408,6 +2411,15 @@ void do_initialize(scdat_ctx* c) {
   }
   upload_segments<T>(c);
   dep_setup(c);
+  bool host_groups = false;
+  for (int g = 0; g < kMaxGroups; ++g) host_groups = host_groups || c->gtype[g] == SCDAT_BC_HOST;
+  if (host_groups && c->event_cap == 0) {
+    // impacts per step: a 1/64 of all species' capacity, at least 65536
+    int64_t cap = 0;
+    for (const auto& S : c->sp) cap += S.cap;
+    c->event_cap = std::max<int64_t>(65536, cap / 64);
+    c->d_events.alloc(sizeof(SeeEvent) * (size_t)c->event_cap);
+  }
   solve_phi1<T>(c);
   if (c->n_patch > 0) {
     c->patch_q.fill(0.0);
@@ -3644,7 +3656,8 @@ void fill_ledger(scdat_ctx* c, const LedgerHost& L, int s0, int s1, scdat_ledger
         E.emitted_electron_particles += (int64_t)k;
         E.emitted_electron_charge_c += (double)k * c->sp[sp].q * (c->sp[sp].w * ws);
       }
-      if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;  // escaped: not recorded
+      // escaped: not recorded; caller-side conditions record their own
+      if (c->gtype[g] == SCDAT_BC_ESCAPE || c->gtype[g] == SCDAT_BC_HOST) continue;
       for (size_t sp = 0; sp < c->sp.size(); ++sp) {
         const unsigned long long k = L.cnt[lo + g * kMaxSpecies + sp];
         const double q = (double)k * c->sp[sp].q * (c->sp[sp].w * ws);
@@ -3831,7 +3844,7 @@ int scdat_set_mesh(scdat_ctx* c, int64_t n_nodes, const double* xyz, int64_t n_t
 int scdat_set_boundary(scdat_ctx* c, int32_t group, int32_t type, int32_t partner_group,
                        const double* shift) {
   if (!c) return SCDAT_ERR_ARG;
-  if (group < 0 || group >= kMaxGroups || type < 0 || type > 4)
+  if (group < 0 || group >= kMaxGroups || type < 0 || type > SCDAT_BC_HOST)
     return fail(c, SCDAT_ERR_ARG, "set_boundary: bad group or type");
   c->gtype[group] = type;
   c->gpartner[group] = partner_group;
@@ -3839,6 +3852,48 @@ int scdat_set_boundary(scdat_ctx* c, int32_t group, int32_t type, int32_t partne
   for (int r = 0; r < 3; ++r) c->gshift[group][r] = shift ? shift[r] : 0.0;
   c->initialized = false;
   return SCDAT_OK;
+}
+
+int scdat_get_host_impacts(scdat_ctx* c, int64_t max, scdat_impact* out, int64_t* n) {
+  if (!c || !n || max < 0 || (max > 0 && !out)) return SCDAT_ERR_ARG;
+  if (!c->initialized) return fail(c, SCDAT_ERR_STATE, "get_host_impacts: not initialized");
+  return guarded(c, [&] {
+    *n = 0;
+    if (c->event_cap <= 0) return;
+    unsigned long long ne = 0;
+    HIPCHK(hipMemcpyAsync(&ne, c->d_n_events.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t m = std::min<int64_t>((int64_t)ne, c->event_cap);
+    std::vector<SeeEvent> ev((size_t)m);
+    if (m) HIPCHK(hipMemcpyAsync(ev.data(), c->d_events.p, sizeof(SeeEvent) * (size_t)m, hipMemcpyDeviceToHost,
+                                 c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // in (species, id) order: the order does not depend on the push's schedule
+    std::vector<const SeeEvent*> hits;
+    for (const auto& e : ev)
+      if (c->gtype[e.group_face & 0xFF] == SCDAT_BC_HOST) hits.push_back(&e);
+    std::sort(hits.begin(), hits.end(), [](const SeeEvent* a, const SeeEvent* b) {
+      return a->species != b->species ? a->species < b->species : a->id < b->id;
+    });
+    int64_t k = 0;
+    for (const SeeEvent* e : hits) {
+      if (k < max) {
+        scdat_impact& o = out[k];
+        o.species = e->species;
+        o.id = e->id;
+        o.group = e->group_face & 0xFF;
+        o.cell = e->cell;
+        for (int r = 0; r < 3; ++r) {
+          o.x[r] = e->x[r];
+          o.v[r] = e->v[r];
+        }
+      }
+      ++k;
+    }
+    *n = k;
+    if ((int64_t)ne > c->event_cap)
+      throw Error(SCDAT_ERR_NOMEM, "get_host_impacts: more impacts in the step than the impact buffer holds");
+  });
 }
 
 int scdat_get_escaped(scdat_ctx* c, int32_t species, int64_t* n) {
@@ -4280,7 +4335,7 @@ int scdat_get_stats(scdat_ctx* c, scdat_stats* st) {
       for (int sg = 0; sg < c->n_seg(); ++sg)
         for (int g = 0; g < kMaxGroups; ++g)
           for (size_t s = 0; s < c->sp.size(); ++s) {
-            if (c->gtype[g] == SCDAT_BC_ESCAPE) continue;
+            if (c->gtype[g] == SCDAT_BC_ESCAPE || c->gtype[g] == SCDAT_BC_HOST) continue;
             const size_t o = ((size_t)sg * kMaxGroups + g) * kMaxSpecies + s;
             const double qw = c->sp[s].q * (c->sp[s].w * seg_scale(c, sg));
             const double q = (double)L.cnt[o] * qw;
@@ -4335,7 +4390,7 @@ int scdat_get_stats_noalloc(scdat_ctx* c, scdat_stats* st) {
   a.led_cnt = c->d_ledger_count.as<unsigned long long>();
   a.led_emi = c->d_emit_count.as<unsigned long long>();
   for (int g = 0; g < kMaxGroups; ++g) {
-    if (c->gtype[g] != SCDAT_BC_ESCAPE) a.surf_mask |= 1u << g;
+    if (c->gtype[g] != SCDAT_BC_ESCAPE && c->gtype[g] != SCDAT_BC_HOST) a.surf_mask |= 1u << g;
     if (c->gtype[g] == SCDAT_BC_CONDUCTOR) a.cond_mask |= 1u << g;
   }
   a.q_init = c->q_init;
