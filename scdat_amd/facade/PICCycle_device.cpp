@@ -20,8 +20,11 @@
 //  * boundary conditions map to device kinds: AbsorbingBoundaryCondition
 //    with probability 1, PeriodicBoundaryCondition (its period vector along
 //    the face's axis), ReflectiveBoundaryCondition; no condition lets
-//    particles escape (PICCycle.cpp:1254-1257); other kinds are refused by
-//    initialize() with a message;
+//    particles escape (PICCycle.cpp:1254-1257); any other subclass runs on
+//    the host (the slow path): the device hands back each step's impacts on
+//    its face (SCDAT_BC_HOST) and applyRuntimeBoundaryCondition calls its
+//    processParticle as PICCycle.cpp:1221-1293 does, records the ledger,
+//    and the survivors and emitted particles return with the next upload;
 //  * executeTimeStep() uploads the manager's active particles (the caller
 //    may have changed them between steps, as PicMccSurfaceCurrentSampler
 //    does with its host collisions), takes one device step with the E of
@@ -48,6 +51,7 @@
 #include <iostream>
 #include <map>
 #include <memory>
+#include <optional>
 #include <tuple>
 #include <unordered_map>
 #include <vector>
@@ -77,6 +81,10 @@ struct DeviceState {
   std::vector<double> sent_potential;  // potential_ as the device last received it
   std::array<int, 6> face_kind{};      // device kind per bounding-box face
   double surface_charge_seen = 0.0;    // device surface_deposited_charge already counted
+  // what caller-side conditions (SCDAT_BC_HOST faces) recorded: added to the
+  // device ledger, which does not count those faces
+  std::array<PICCycle::SurfaceCurrentLedgerEntry, 6> host_ledger{};
+  bool host_faces = false;
   bool ready = false;
   ~DeviceState() {
     if (ctx) scdat_destroy(ctx);
@@ -330,11 +338,11 @@ bool PICCycle::initialize() {
     } else if (dynamic_cast<Particle::ReflectiveBoundaryCondition*>(bc.get())) {
       kind_f = SCDAT_BC_REFLECT;
     } else {
-      std::cerr << "PICCycle (device): boundary condition kind on face " << f
-                << " has no device form (absorbing, periodic and reflective do)" << std::endl;
-      return false;
+      // no device form: its processParticle runs on the host after each step
+      kind_f = SCDAT_BC_HOST;
     }
     st->face_kind[f] = kind_f;
+    st->host_faces = st->host_faces || kind_f == SCDAT_BC_HOST;
     rc = scdat_set_boundary(ctx, f, kind_f, partner, shift);
     if (rc != SCDAT_OK) return report("setBoundaryCondition", ctx, rc);
   }
@@ -432,7 +440,8 @@ bool upload_particles(DeviceState* st, ParticleManager& pm) {
 // state, removed ones their terminal status; particles created on the device
 // (none without collisions or emission) are appended.
 void download_particles(DeviceState* st, ParticleManager& pm, const std::array<int, 6>& face_kind,
-                        const double lo[3], const double hi[3]) {
+                        const double lo[3], const double hi[3],
+                        const std::vector<std::vector<char>>* host_hit = nullptr) {
   auto& cont = pm.getContainer();
   std::vector<Particle::Particle*> by_index;
   by_index.reserve(cont.size());
@@ -472,6 +481,8 @@ void download_particles(DeviceState* st, ParticleManager& pm, const std::array<i
             best = d;
           }
         }
+        // a particle stopped at a caller-side face: its condition decides
+        if (host_hit && (std::size_t)i < (*host_hit)[s].size() && (*host_hit)[s][(std::size_t)i]) continue;
         if (f >= 0 && face_kind[f] == SCDAT_BC_ESCAPE) p->markAsEscaped();
         else p->markAsAbsorbed();
       }
@@ -519,6 +530,14 @@ void PICCycle::updateStatistics() {
       e.absorbed_electron_particles = (std::size_t)led[f].absorbed_electron_particles;
       e.absorbed_ion_particles = (std::size_t)led[f].absorbed_ion_particles;
       e.emitted_electron_particles = (std::size_t)led[f].emitted_electron_particles;
+      const SurfaceCurrentLedgerEntry& h = st->host_ledger[f];
+      e.absorbed_electron_charge_c += h.absorbed_electron_charge_c;
+      e.absorbed_ion_charge_c += h.absorbed_ion_charge_c;
+      e.emitted_electron_charge_c += h.emitted_electron_charge_c;
+      e.incident_energy_j += h.incident_energy_j;
+      e.absorbed_electron_particles += h.absorbed_electron_particles;
+      e.absorbed_ion_particles += h.absorbed_ion_particles;
+      e.emitted_electron_particles += h.emitted_electron_particles;
     }
 }
 
@@ -563,6 +582,155 @@ int run_steps(DeviceState* st, std::vector<double>& potential,
 
 }  // namespace
 
+// ---- caller-side boundary conditions (the slow path) -------------------------
+
+namespace {
+
+Utils::Vector3D inward_normal(PICCycle::BoundaryFace face) {
+  const int f = static_cast<int>(face);
+  double c[3] = {0.0, 0.0, 0.0};
+  c[f / 2] = (f & 1) ? -1.0 : 1.0;
+  return Utils::Vector3D(c[0], c[1], c[2]);
+}
+
+}  // namespace
+
+// PICCycle.cpp:1074-1133: the face whose plane the position lies beyond by
+// more than outside_tolerance (the farthest), or within near_tolerance of it.
+std::optional<PICCycle::BoundaryFace> PICCycle::detectBoundaryFace(const Utils::Point3D& position,
+                                                                   double outside_tolerance, bool allow_near_match,
+                                                                   double near_tolerance) const {
+  const double x[3] = {position.x(), position.y(), position.z()};
+  const double lo[3] = {domain_bounds_.min_corner.x(), domain_bounds_.min_corner.y(), domain_bounds_.min_corner.z()};
+  const double hi[3] = {domain_bounds_.max_corner.x(), domain_bounds_.max_corner.y(), domain_bounds_.max_corner.z()};
+  int best_f = -1;
+  double best = outside_tolerance;
+  for (int f = 0; f < 6; ++f) {
+    const int a = f / 2;
+    const double d = (f & 1) ? x[a] - hi[a] : lo[a] - x[a];
+    if (d > best) {
+      best = d;
+      best_f = f;
+    }
+  }
+  if (best_f < 0 && allow_near_match) {
+    double nearest = near_tolerance;
+    for (int f = 0; f < 6; ++f) {
+      const int a = f / 2;
+      const double d = std::abs((f & 1) ? hi[a] - x[a] : x[a] - lo[a]);
+      if (d <= nearest) {
+        nearest = d;
+        best_f = f;
+      }
+    }
+  }
+  if (best_f < 0) return std::nullopt;
+  return static_cast<BoundaryFace>(best_f);
+}
+
+// PICCycle.cpp:1135-1176: clamp into the box, onto the face's plane, then
+// inward by the offset.
+Utils::Point3D PICCycle::projectToBoundaryFace(BoundaryFace face, const Utils::Point3D& position,
+                                               double inward_offset) const {
+  const double lo[3] = {domain_bounds_.min_corner.x(), domain_bounds_.min_corner.y(), domain_bounds_.min_corner.z()};
+  const double hi[3] = {domain_bounds_.max_corner.x(), domain_bounds_.max_corner.y(), domain_bounds_.max_corner.z()};
+  double x[3] = {position.x(), position.y(), position.z()};
+  const int f = static_cast<int>(face);
+  for (int r = 0; r < 3; ++r) x[r] = std::clamp(x[r], lo[r], hi[r]);
+  x[f / 2] = (f & 1) ? hi[f / 2] : lo[f / 2];
+  x[f / 2] += ((f & 1) ? -1.0 : 1.0) * inward_offset;
+  for (int r = 0; r < 3; ++r) x[r] = std::clamp(x[r], lo[r], hi[r]);
+  return Utils::Point3D(x[0], x[1], x[2]);
+}
+
+// PICCycle.cpp:1295-1329, into the host part of the ledger.
+void PICCycle::recordSurfaceBoundaryInteraction(BoundaryFace face, const ParticleTypeDef& particle,
+                                                const std::vector<ParticleTypeDef>& emitted_particles) {
+  DeviceState* st = state_of(cross_tetrahedron_);
+  if (!st) return;
+  SurfaceCurrentLedgerEntry& e = st->host_ledger[static_cast<std::size_t>(face)];
+  const double qw = particle.getCharge() * particle.getWeight();
+  if (particle.getStatus() == ::SCDAT::Particle::ParticleStatus::ABSORBED) {
+    e.incident_energy_j += particle.getKineticEnergy() * std::max(1.0, particle.getWeight());
+    if (particle.getCharge() < 0.0) {
+      e.absorbed_electron_charge_c += qw;
+      e.absorbed_electron_particles += 1;
+    } else {
+      e.absorbed_ion_charge_c += qw;
+      e.absorbed_ion_particles += 1;
+    }
+  }
+  for (const auto& p : emitted_particles)
+    if (p.getCharge() < 0.0) {
+      e.emitted_electron_charge_c += p.getCharge() * p.getWeight();
+      e.emitted_electron_particles += 1;
+    }
+}
+
+// PICCycle.cpp:1221-1293 on a particle that stopped at a caller-side face:
+// up to six passes of processParticle; an active particle's element is left
+// unknown (the next upload locates it).
+bool PICCycle::applyRuntimeBoundaryCondition(ParticlePtr particle, ElementId hint_element,
+                                             std::vector<ParticleTypeDef>& emitted_particles,
+                                             ElementId& resolved_element) {
+  (void)hint_element;
+  resolved_element = static_cast<ElementId>(-1);
+  if (!particle || !domain_bounds_.valid) return false;
+  const double span = std::max({domain_bounds_.max_corner.x() - domain_bounds_.min_corner.x(),
+                                domain_bounds_.max_corner.y() - domain_bounds_.min_corner.y(),
+                                domain_bounds_.max_corner.z() - domain_bounds_.min_corner.z(), 1.0e-12});
+  const double outside_tol = std::max(1.0e-10, 1.0e-8 * span);
+  const double near_tol = std::max(5.0 * outside_tol, 1.0e-9 * span);
+  for (int pass = 0; pass < 6; ++pass) {
+    if (!particle->isActive()) return true;
+    const auto face = detectBoundaryFace(particle->getPosition(), outside_tol, pass == 0, near_tol);
+    if (!face) return true;
+    const auto& condition = boundary_conditions_[static_cast<std::size_t>(*face)];
+    if (!condition) return false;
+    auto generated = condition->processParticle(*particle, projectToBoundaryFace(*face, particle->getPosition()),
+                                                inward_normal(*face), params_.time_step);
+    emitted_particles.insert(emitted_particles.end(), generated.begin(), generated.end());
+    recordSurfaceBoundaryInteraction(*face, *particle, generated);
+    if (particle->getStatus() == ::SCDAT::Particle::ParticleStatus::ABSORBED) {
+      cycle_stats_.surface_deposited_charge += particle->getCharge() * particle->getWeight();
+      return true;
+    }
+    if (particle->getStatus() == ::SCDAT::Particle::ParticleStatus::ESCAPED) return true;
+  }
+  return true;
+}
+
+// PICCycle.cpp:1331-1379 over the particles that stopped at caller-side
+// faces in the last step: each goes through applyRuntimeBoundaryCondition;
+// one that stays active is moved a hair inside the box (its crossing point
+// lies on the face) and located by the next upload; a failed condition
+// marks it escaped.
+void PICCycle::processRuntimeBoundaryEvents(const std::vector<RuntimeBoundaryEvent>& events,
+                                            std::vector<ParticleTypeDef>& emitted_particles) {
+  if (!particle_manager_ || events.empty()) return;
+  auto& container = particle_manager_->getContainer();
+  auto it_begin = container.begin();
+  const double lo[3] = {domain_bounds_.min_corner.x(), domain_bounds_.min_corner.y(), domain_bounds_.min_corner.z()};
+  const double hi[3] = {domain_bounds_.max_corner.x(), domain_bounds_.max_corner.y(), domain_bounds_.max_corner.z()};
+  const double hair = 1e-12 * std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+  for (const auto& event : events) {
+    if (event.particle_index >= container.size()) continue;
+    auto& particle = *(it_begin + event.particle_index);
+    if (!particle.isActive()) continue;
+    ElementId resolved = static_cast<ElementId>(-1);
+    if (!applyRuntimeBoundaryCondition(&particle, event.hint_element, emitted_particles, resolved)) {
+      particle.markAsEscaped();
+      continue;
+    }
+    if (particle.isActive()) {
+      const auto pos = particle.getPosition();
+      double x[3] = {pos.x(), pos.y(), pos.z()};
+      for (int r = 0; r < 3; ++r) x[r] = std::clamp(x[r], lo[r] + hair, hi[r] - hair);
+      particle.setPosition(Utils::Point3D(x[0], x[1], x[2]));
+    }
+  }
+}
+
 bool PICCycle::executeTimeStep() {
   DeviceState* st = state_of(cross_tetrahedron_);
   if (!st || !st->ready) {
@@ -573,7 +741,38 @@ bool PICCycle::executeTimeStep() {
   if (rc != SCDAT_OK && rc != SCDAT_ERR_SOLVER) return report("executeTimeStep", st->ctx, rc);
   const double lo[3] = {domain_bounds_.min_corner.x(), domain_bounds_.min_corner.y(), domain_bounds_.min_corner.z()};
   const double hi[3] = {domain_bounds_.max_corner.x(), domain_bounds_.max_corner.y(), domain_bounds_.max_corner.z()};
-  download_particles(st, *particle_manager_, st->face_kind, lo, hi);
+  if (!st->host_faces) {
+    download_particles(st, *particle_manager_, st->face_kind, lo, hi);
+  } else {
+    // the step's impacts on caller-side faces, in (species, id) order
+    int64_t n = 0;
+    if (scdat_get_host_impacts(st->ctx, 0, nullptr, &n) != SCDAT_OK) return report("host impacts", st->ctx, 1);
+    std::vector<scdat_impact> imp((std::size_t)n);
+    if (n && scdat_get_host_impacts(st->ctx, n, imp.data(), &n) != SCDAT_OK)
+      return report("host impacts", st->ctx, 1);
+    std::vector<std::vector<char>> hit(st->species.size());
+    for (std::size_t q = 0; q < st->species.size(); ++q) hit[q].assign(st->species[q].members.size(), 0);
+    for (const scdat_impact& e : imp)
+      if (e.species >= 0 && (std::size_t)e.species < hit.size() && e.id >= 0 &&
+          (std::size_t)e.id < hit[(std::size_t)e.species].size())
+        hit[(std::size_t)e.species][(std::size_t)e.id] = 1;
+    download_particles(st, *particle_manager_, st->face_kind, lo, hi, &hit);
+    std::vector<RuntimeBoundaryEvent> events;
+    for (const scdat_impact& e : imp) {
+      const auto& members = st->species[(std::size_t)e.species].members;
+      if (e.id < 0 || (std::size_t)e.id >= members.size()) continue;
+      RuntimeBoundaryEvent ev;
+      ev.particle_index = members[(std::size_t)e.id];
+      ev.hint_element = static_cast<ElementId>(e.cell);
+      st->cell_of[ev.particle_index] = -1;
+      events.push_back(ev);
+    }
+    std::vector<ParticleTypeDef> emitted;
+    processRuntimeBoundaryEvents(events, emitted);
+    // emitted particles join the manager; they need one of the cycle's
+    // (charge, mass, weight) species to reach the device
+    for (const auto& p : emitted) particle_manager_->getContainer().addParticle(p);
+  }
   updateStatistics();
   return rc == SCDAT_OK;
 }
@@ -583,6 +782,12 @@ bool PICCycle::executeSimulation(double total_time) {
   DeviceState* st = state_of(cross_tetrahedron_);
   const int n = (int)std::llround(total_time / params_.time_step);
   if (n <= 0) return true;
+  if (st->host_faces) {
+    // caller-side conditions see every step's impacts
+    bool ok = true;
+    for (int k = 0; k < n; ++k) ok = executeTimeStep() && ok;
+    return ok;
+  }
   const int rc = run_steps(st, potential_, electric_field_, *particle_manager_, n);
   if (rc != SCDAT_OK && rc != SCDAT_ERR_SOLVER) return report("executeSimulation", st->ctx, rc);
   const double lo[3] = {domain_bounds_.min_corner.x(), domain_bounds_.min_corner.y(), domain_bounds_.min_corner.z()};
