@@ -96,9 +96,9 @@ __device__ void deposit_one(const TetMap<T>* tets, unsigned long long* rho, int 
 
 // Appends one particle; returns false (and counts an overflow) when full.
 template <typename T>
-__device__ bool append(const SpeciesOut<T>& o, const T x[3], const double v[3], int cell, int32_t id,
-                       const TetMap<T>* tets, unsigned long long* rho, unsigned long long* counts) {
-  const int64_t slot = (o.base_dev ? *o.base_dev : o.base) + (int64_t)atomicAdd(o.append, 1ull);
+__device__ bool append_at(const SpeciesOut<T>& o, int64_t r, const T x[3], const double v[3], int cell, int32_t id,
+                          const TetMap<T>* tets, unsigned long long* rho, unsigned long long* counts) {
+  const int64_t slot = (o.base_dev ? *o.base_dev : o.base) + r;
   if (slot >= o.cap) {
     atomicAdd(&counts[C_OVERFLOW], 1ull);
     return false;
@@ -114,6 +114,13 @@ __device__ bool append(const SpeciesOut<T>& o, const T x[3], const double v[3], 
   deposit_one(tets, rho, cell, x, o.cs);
   return true;
 }
+// Appends one particle at the next free slot (r: index r among this step's
+// appends to the species, reserved from o.append).
+template <typename T>
+__device__ bool append(const SpeciesOut<T>& o, const T x[3], const double v[3], int cell, int32_t id,
+                       const TetMap<T>* tets, unsigned long long* rho, unsigned long long* counts) {
+  return append_at(o, (int64_t)atomicAdd(o.append, 1ull), x, v, cell, id, tets, rho, counts);
+}
 
 // Total collision frequency sum_p n_p sigma_p(E) sqrt(2E/m) of a species;
 // dens non-null: n_p = dens[slot of p] (one scenario's densities).
@@ -123,12 +130,61 @@ __device__ double nu_total(const MccProcDev* proc, int np, double E, double mass
   return t * sqrt(2.0 * E * kE / mass);
 }
 
+// The tables of the kernel arguments that a lane indexes with its own value
+// (process, species, group) are copied to LDS first, by one lane with
+// compile-time indices: indexing the argument block per lane, or passing it
+// by reference to a call, copies the whole block to scratch memory in every
+// thread.
+template <typename E, int N>
+__device__ __forceinline__ void stage(const E (&src)[N], E* dst) {
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) dst[q] = src[q];
+  }
+}
+
+// out[s] for an index s uniform over the wavefront, with compile-time indices.
+template <typename T>
+__device__ __forceinline__ SpeciesOut<T> out_at(const SpeciesOut<T> (&out)[kMaxSpecies], int s) {
+  SpeciesOut<T> o = out[0];
+#pragma unroll
+  for (int q = 1; q < kMaxSpecies; ++q)
+    if (q == s) o = out[q];
+  return o;
+}
+
+// Slots for the lanes of a wavefront that want one (the same counter):
+// one atomic per wavefront; every lane taking part must call it.
+__device__ __forceinline__ int64_t wave_reserve(unsigned long long* counter, bool want) {
+  const uint64_t m = __ballot(want);
+  if (!m) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned long long base = 0ull;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, 64);
+  return want ? (int64_t)(base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull))) : -1;
+}
+
+// Adds the wavefront's sum of v to *p with one atomic; every lane of the
+// wavefront must call it.
+__device__ __forceinline__ void wave_add(unsigned long long* p, unsigned long long v) {
+  unsigned long long t = v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  const uint64_t m = __ballot(true);
+  if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1 && t) atomicAdd(p, t);
+}
+
 // One block per species: nu_max over log-spaced energies up to the cap (and
 // just above every threshold, where a cross section may switch on), plus
 // the bound sum n sigma_max v over the energies below the lowest sample;
 // times kMccMajorantMargin.  Then P_null, and the push's accumulators reset.
 __global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
   __shared__ double lds[kBlock / 64];
+  __shared__ MccProcDev sp[kMaxProc];
+  stage(a.proc, sp);
+  __syncthreads();
   // one block per scenario (one block without scenarios)
   const int sg = blockIdx.x;
   const double* dens = a.seg_n ? a.seg_n + (size_t)sg * kMaxProc : nullptr;
@@ -143,15 +199,15 @@ __global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
     const double elo = 1e-8 * ecap, la = log(elo), lb = log(ecap);
     for (int k = threadIdx.x; k < kMccMajorantPoints; k += kBlock) {
       const double E = exp(la + (lb - la) * (double)k / (double)(kMccMajorantPoints - 1));
-      f = fmax(f, nu_total(a.proc, a.np, E, a.mass, dens));
+      f = fmax(f, nu_total(sp, a.np, E, a.mass, dens));
     }
-    if (threadIdx.x < a.np) {
-      const double thr = a.proc[threadIdx.x].thr;
-      if (thr > elo && thr < ecap) f = fmax(f, nu_total(a.proc, a.np, thr * (1.0 + 1e-9), a.mass, dens));
+    if ((int)threadIdx.x < a.np) {
+      const double thr = sp[threadIdx.x].thr;
+      if (thr > elo && thr < ecap) f = fmax(f, nu_total(sp, a.np, thr * (1.0 + 1e-9), a.mass, dens));
     }
     if (threadIdx.x == 0) {
       double b = 0.0;
-      for (int j = 0; j < a.np; ++j) b += (dens ? dens[a.proc[j].slot] : a.proc[j].n) * a.proc[j].smax;
+      for (int j = 0; j < a.np; ++j) b += (dens ? dens[sp[j].slot] : sp[j].n) * sp[j].smax;
       f = fmax(f, b * sqrt(2.0 * elo * kE / a.mass));
     }
   }
@@ -182,9 +238,10 @@ __global__ void __launch_bounds__(kBlock) k_v2max(const T* vx, const T* vy, cons
 }
 
 // One candidate: the exact collision test given the pre-selection, process
-// choice, outcome.
+// choice, outcome.  sp, so: the process and output tables in LDS.
 template <typename T>
-__device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
+__device__ __forceinline__ void mcc_candidate(const MccArgs<T>& a, const MccProcDev* sp, const SpeciesOut<T>* so,
+                                              uint32_t entry) {
   const bool forced = (entry & kMccForced) != 0u;
   const int64_t i = (int64_t)(entry & ~kMccForced);
   const int cell = a.cell[i];
@@ -203,7 +260,7 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
   double tot = 0.0;
 #pragma unroll
   for (int j = 0; j < kMaxProc; ++j) {
-    nu[j] = j < a.np ? (dens ? dens[a.proc[j].slot] : a.proc[j].n) * sigma_of(a.proc[j], E) * speed : 0.0;
+    nu[j] = j < a.np ? (dens ? dens[sp[j].slot] : sp[j].n) * sigma_of(sp[j], E) * speed : 0.0;
     tot += nu[j];
   }
   if (!(tot > 0.0)) return;
@@ -230,7 +287,7 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
       if (R < cum) stop = true;
     }
   }
-  const MccProcDev& p = a.proc[j];
+  const MccProcDev& p = sp[j];
   atomicAdd(&a.counts[p.slot], 1ull);
   double d[3];
   switch (p.type) {
@@ -265,13 +322,13 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
       a.vy[i] = (T)(sp * d[1]);
       a.vz[i] = (T)(sp * d[2]);
       const T xp[3] = {a.x[i], a.y[i], a.z[i]};
-      const SpeciesOut<T>& oe = a.out[p.e_species];
+      const SpeciesOut<T>& oe = so[p.e_species];
       const double se = sqrt(2.0 * sec * kE / oe.m);
       iso_dir(u01(b.y), u01(b.z), d);
       const double ve[3] = {se * d[0], se * d[1], se * d[2]};
       int made = append(oe, xp, ve, cell, gen_id(c.x), a.tets, a.rho, a.counts) ? 1 : 0;
       const double vi[3] = {0.0, 0.0, 0.0};
-      made += append(a.out[p.i_species], xp, vi, cell, gen_id(c.y), a.tets, a.rho, a.counts) ? 1 : 0;
+      made += append(so[p.i_species], xp, vi, cell, gen_id(c.y), a.tets, a.rho, a.counts) ? 1 : 0;
       atomicAdd(&a.counts[C_CREATED], (unsigned long long)made);
       break;
     }
@@ -285,9 +342,14 @@ __device__ void mcc_candidate(const MccArgs<T>& a, uint32_t entry) {
 // Candidates of one species (grid-stride over the count the push left).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_mcc(MccArgs<T> a) {
+  __shared__ MccProcDev sp[kMaxProc];
+  __shared__ SpeciesOut<T> so[kMaxSpecies];
+  stage(a.proc, sp);
+  stage(a.out, so);
+  __syncthreads();
   const unsigned int nc = *a.n_cand;
   for (unsigned int q = blockIdx.x * kBlock + threadIdx.x; q < nc; q += gridDim.x * kBlock)
-    mcc_candidate(a, a.cand[q]);
+    mcc_candidate(a, sp, so, a.cand[q]);
 }
 
 // ---- surface emission ------------------------------------------------------
@@ -346,12 +408,11 @@ __device__ void emit_velocity(const double n[3], double T, double m, double ua, 
   for (int r = 0; r < 3; ++r) v[r] = sp * (st * cp * t1[r] + st * sn * t2[r] + ct * n[r]);
 }
 
+// Secondary emission of one recorded impact.  sg, so, sm: the group,
+// output and mass tables in LDS.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const unsigned long long ne = *a.n_events;
-  const int64_t nev = (int64_t)ne < a.event_cap ? (int64_t)ne : a.event_cap;
-  if (e >= nev) return;
+__device__ __forceinline__ void see_event(const EmitArgs<T>& a, const SurfEmitDev* sg, const SpeciesOut<T>* so,
+                                          const double* sm, int64_t e) {
   const SeeEvent ev = a.events[e];
   if (!((a.electron_mask >> ev.species) & 1u)) return;  // ion impacts: field emission, on the host
   const int g = ev.group_face & 0xFF, face = ev.group_face >> 8;
@@ -362,13 +423,13 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
     k0 = a.seg_key[2 * sgm];
     k1 = a.seg_key[2 * sgm + 1];
   }
-  const SurfEmitDev& cf = a.tet_seg ? a.gseg[sgm * kMaxGroups + g] : a.g[g];
+  const SurfEmitDev& cf = a.tet_seg ? a.gseg[sgm * kMaxGroups + g] : sg[g];
   const TetMap<T> m = a.tets[ev.cell];
   double n[3];
   inward_normal(m, face, n);
   const double v2 = ev.v[0] * ev.v[0] + ev.v[1] * ev.v[1] + ev.v[2] * ev.v[2];
   const double sp = sqrt(v2);
-  const double E = 0.5 * a.mass[ev.species] * v2 / kE;
+  const double E = 0.5 * sm[ev.species] * v2 / kE;
   const double cosi = sp > 0.0 ? fabs(ev.v[0] * n[0] + ev.v[1] * n[1] + ev.v[2] * n[2]) / sp : 1.0;
   const double dl = see_yield(cf.see_model, cf.dmax, cf.emax, E, cosi);
   const uint32_t step = a.step_dev ? *a.step_dev : a.step;
@@ -380,19 +441,42 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
   double xe[3];
   nudge(a.xyz, a.tet4, ev.cell, ev.x, xe);
   const T xt[3] = {(T)xe[0], (T)xe[1], (T)xe[2]};
-  const SpeciesOut<T>& o = a.out[cf.e_species];
+  const SpeciesOut<T>& o = so[cf.e_species];
   int made = 0;
+  // the impact's secondaries take consecutive slots (one atomic per impact)
+  const int64_t r0 = (int64_t)atomicAdd(o.append, (unsigned long long)nem);
   for (int k = 0; k < nem; ++k) {
     const uint32_t sk = ((uint32_t)ev.species << 8) | (uint32_t)k;
     const U4 b = philox4x32((uint32_t)ev.id, step, sk, kTagSeeK, k0, k1);
     const U4 c = philox4x32((uint32_t)ev.id, step, sk, kTagSeeId, k0, k1);
     double v[3];
     emit_velocity(n, cf.t_see, o.m, u01(b.x), u01(b.y), u01(b.z), u01(b.w), v);
-    made += append(o, xt, v, ev.cell, gen_id(c.x), a.tets, a.rho, a.counts) ? 1 : 0;
+    made += append_at(o, r0 + k, xt, v, ev.cell, gen_id(c.x), a.tets, a.rho, a.counts) ? 1 : 0;
   }
   atomicAdd(&a.emit_count[((size_t)sgm * kMaxGroups + g) * kMaxSpecies + cf.e_species], (unsigned long long)made);
   atomicAdd(&a.counts[C_SEE], (unsigned long long)made);
 }
+
+// Impacts of the step, grid-stride over however many the push recorded.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
+  __shared__ SurfEmitDev sg[kMaxGroups];
+  __shared__ SpeciesOut<T> so[kMaxSpecies];
+  __shared__ double sm[kMaxSpecies];
+  stage(a.g, sg);
+  stage(a.out, so);
+  stage(a.mass, sm);
+  __syncthreads();
+  const unsigned long long ne = *a.n_events;
+  const int64_t nev = (int64_t)ne < a.event_cap ? (int64_t)ne : a.event_cap;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nev; e += (int64_t)gridDim.x * kBlock)
+    see_event(a, sg, so, sm, e);
+}
+
+template <typename T>
+__device__ bool photo_one(const EmitArgs<T>& a, const SpeciesOut<T>& o, int64_t r, int64_t k, int sg, int64_t f0,
+                          int64_t f1, double area, const SurfEmitDev& cf, uint32_t step, uint32_t key0,
+                          uint32_t key1);
 
 // Photoemission of one group, SurfaceMaterialModel.cpp:199-208 current
 // density at the group's potential; J A dt / (|q| w) macro-electrons per step.
@@ -419,7 +503,23 @@ __global__ void __launch_bounds__(kBlock) k_photo(EmitArgs<T> a) {
   const U4 r0 = philox4x32((uint32_t)a.group, step, 0u, kTagPhoto, key0, key1);
   const double fl = floor(expect);
   const int64_t nem = (int64_t)fl + (u01(r0.x) < expect - fl ? 1 : 0);
-  if (k >= nem) return;
+  if ((int64_t)blockIdx.x * kBlock >= nem) return;  // a block past the count (uniform)
+  const bool emit = k < nem;
+  // one slot per emitting lane, one atomic per wavefront
+  const SpeciesOut<T> o = out_at(a.out, cf.e_species);
+  const int64_t r = wave_reserve(o.append, emit);
+  bool made = false;
+  if (emit) made = photo_one(a, o, r, k, sg, f0, f1, area, cf, step, key0, key1);
+  wave_add(&a.emit_count[((size_t)sg * kMaxGroups + a.group) * kMaxSpecies + cf.e_species], made ? 1ull : 0ull);
+  wave_add(&a.counts[C_PHOTO], made ? 1ull : 0ull);
+}
+
+// One photoelectron of group a.group (scenario sg): its face, the point on
+// it, the flux-weighted velocity, appended at slot index r.
+template <typename T>
+__device__ bool photo_one(const EmitArgs<T>& a, const SpeciesOut<T>& o, int64_t r, int64_t k, int sg, int64_t f0,
+                          int64_t f1, double area, const SurfEmitDev& cf, uint32_t step, uint32_t key0,
+                          uint32_t key1) {
   const U4 b = philox4x32((uint32_t)k, step, (uint32_t)a.group, kTagPhotoK, key0, key1);
   const U4 c = philox4x32((uint32_t)k, step, (uint32_t)a.group, kTagPhotoK2, key0, key1);
   // face: first cumulative area above u * area
@@ -443,13 +543,9 @@ __global__ void __launch_bounds__(kBlock) k_photo(EmitArgs<T> a) {
   for (int q = 0; q < 3; ++q) pt[q] = (1.0 - sq) * p[0][q] + sq * (1.0 - u2) * p[1][q] + sq * u2 * p[2][q];
   nudge(a.xyz, a.tet4, t, pt, xe);
   inward_normal(a.tets[t], kf, n);
-  const SpeciesOut<T>& o = a.out[cf.e_species];
   emit_velocity(n, cf.t_ph, o.m, u01(b.w), u01(c.x), u01(c.y), u01(c.z), v);
   const T xt[3] = {(T)xe[0], (T)xe[1], (T)xe[2]};
-  if (append(o, xt, v, t, gen_id(c.w), a.tets, a.rho, a.counts)) {
-    atomicAdd(&a.emit_count[((size_t)sg * kMaxGroups + a.group) * kMaxSpecies + cf.e_species], 1ull);
-    atomicAdd(&a.counts[C_PHOTO], 1ull);
-  }
+  return append_at(o, r, xt, v, t, gen_id(c.w), a.tets, a.rho, a.counts);
 }
 
 // Appends host-made particles (field emission) to a species and deposits
@@ -502,7 +598,8 @@ void launch_mcc(const MccArgs<T>& a, hipStream_t s) {
 template <typename T>
 void launch_see(const EmitArgs<T>& a, int64_t max_events, hipStream_t s) {
   if (max_events <= 0) return;
-  hipLaunchKernelGGL(k_see<T>, dim3(blocks_for(max_events)), dim3(kBlock), 0, s, a);
+  // impacts are few: a small grid that loops over however many there are
+  hipLaunchKernelGGL(k_see<T>, dim3((uint32_t)std::min<int64_t>(64, blocks_for(max_events))), dim3(kBlock), 0, s, a);
 }
 template <typename T>
 void launch_photo(const EmitArgs<T>& a, int64_t max_count, int n_seg, hipStream_t s) {
