@@ -955,6 +955,40 @@ __global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_box(EllDev A, FieldBuf
       lds[2 * NW + wv] = c;
     }
   }
+  static_assert(2 * PB >= 1000, "two box points per thread");
+  // this iteration's operands do not depend on the sums: r, w, s at this
+  // thread's two box points and the row's own entries are loaded now, so
+  // that they arrive while the sums, the step lengths and the coarse step
+  // are formed (the neighbour block of a point from the global table;
+  // unconditional loads from clamped rows, since a load under a branch is
+  // waited for where the branch joins)
+  const float* __restrict__ rc = ib.r[cur];
+  const float* __restrict__ wc = ib.w[cur];
+  const float* __restrict__ sp = ib.s[nxt];
+  int nr[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int q = min((int)threadIdx.x + u * PB, 999);
+    const int bx = q % 10 - 1, by = (q / 10) % 10 - 1, bz = q / 100 - 1;
+    nr[u] = A.nbb[(size_t)bt * 27 + ((bx >> 3) + 1) + 3 * ((by >> 3) + 1) + 9 * ((bz >> 3) + 1)];
+  }
+  const int64_t is = i < n ? i : n - 1;
+  const float ri = rc[is], wi = wc[is], so = sp[is], pold = ib.p[is], xold = ib.x[is];
+  float br[2], bw[2], bs[2];
+  bool bv[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int q = (int)threadIdx.x + u * PB;
+    const int bx = q % 10 - 1, by = (q / 10) % 10 - 1, bz = q / 100 - 1;
+    const int code = ((bx >> 3) + 1) + 3 * ((by >> 3) + 1) + 9 * ((bz >> 3) + 1);
+    const int nb = nr[u] >= 0 ? nr[u] : bt;
+    const int64_t row = (int64_t)nb * PB + (bx & 7) + 8 * (by & 7) + 64 * (bz & 7);
+    bv[u] = q < 1000 && (code == 13 || nb != bt) && row < n;
+    const int64_t rs = bv[u] ? row : is;
+    br[u] = rc[rs];
+    bw[u] = wc[rs];
+    bs[u] = sp[rs];
+  }
   if (threadIdx.x < 27) nbl[threadIdx.x] = mynb >= 0 ? mynb : bt;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1044,27 +1078,11 @@ __global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_box(EllDev A, FieldBuf
       }
     }
   }
-  const float* __restrict__ rc = ib.r[cur];
-  const float* __restrict__ wc = ib.w[cur];
-  const float* __restrict__ sp = ib.s[nxt];
   // stage r_{i+1} over the box
-  for (int q = threadIdx.x; q < 1000; q += PB) {
-    const int bx = q % 10 - 1, by = (q / 10) % 10 - 1, bz = q / 100 - 1;
-    const int code = ((bx >> 3) + 1) + 3 * ((by >> 3) + 1) + 9 * ((bz >> 3) + 1);
-    const int nb = nbl[code];
-    const int64_t row = (int64_t)nb * PB + (bx & 7) + 8 * (by & 7) + 64 * (bz & 7);
-    float v = 0.0f;
-    if ((code == 13 || nb != bt) && row < n) v = r_next(rc[row], wc[row], sp[row], a, b);
-    box[q] = v;
-  }
-  // the row's own entries, in flight across the barrier
-  float ri = 0.0f, wi = 0.0f, so = 0.0f, pold = 0.0f, xold = 0.0f;
-  if (i < n) {
-    ri = rc[i];
-    wi = wc[i];
-    so = sp[i];
-    pold = ib.p[i];
-    xold = ib.x[i];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int q = (int)threadIdx.x + u * PB;
+    if (q < 1000) box[q] = bv[u] ? r_next(br[u], bw[u], bs[u], a, b) : 0.0f;
   }
   __syncthreads();  // box, zl, zown
   double g1 = 0.0, d1 = 0.0, rr1 = 0.0, wrest = 0.0;
