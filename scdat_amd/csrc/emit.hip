@@ -20,9 +20,12 @@
 // All arithmetic of these rare events is FP64 whatever the particle storage
 // precision; random numbers come from Philox keyed by (seed, particle id,
 // step, species), so results do not depend on thread order.  New particles
-// are appended with one atomic per particle and deposited into the step's
-// charge at once (fixed point, so the order of deposits does not matter).
+// take their slots from the species' append counter (one atomic per
+// ionisation product, per impact for its secondaries, per wavefront for
+// photoelectrons) and are deposited into the step's charge at once (fixed
+// point, so the order of deposits does not matter).
 #include <algorithm>
+#include <cstddef>
 
 #include "emit.hpp"
 #include "rng.hpp"
@@ -131,17 +134,32 @@ __device__ double nu_total(const MccProcDev* proc, int np, double E, double mass
 }
 
 // The tables of the kernel arguments that a lane indexes with its own value
-// (process, species, group) are copied to LDS first, by one lane with
-// compile-time indices: indexing the argument block per lane, or passing it
-// by reference to a call, copies the whole block to scratch memory in every
-// thread.
-template <typename E, int N>
-__device__ __forceinline__ void stage(const E (&src)[N], E* dst) {
-  if (threadIdx.x == 0) {
+// (process, species, group) are copied to LDS first: indexing the argument
+// block per lane, or passing it by reference to a call, copies the whole
+// block to scratch memory in every thread.  The block reads bytes [off, off +
+// bytes) of its argument segment one dword per lane (the kernels take one
+// argument, which starts the segment with the host struct's layout).
+template <int NT, size_t OFF, size_t BYTES>
+__device__ __forceinline__ void stage_args(void* dst) {
+  static_assert(OFF % 4 == 0 && BYTES % 4 == 0, "whole dwords");
+  const uint32_t* src =
+      reinterpret_cast<const uint32_t*>((const char*)__builtin_amdgcn_kernarg_segment_ptr() + OFF);
+  uint32_t* d = static_cast<uint32_t*>(dst);
+  constexpr int kWords = (int)(BYTES / 4);
+  uint32_t v[(kWords + NT - 1) / NT];
 #pragma unroll
-    for (int q = 0; q < N; ++q) dst[q] = src[q];
+  for (int k = 0; k * NT < kWords; ++k) {
+    const int t = k * NT + (int)threadIdx.x;
+    v[k] = src[t < kWords ? t : 0];
+  }
+#pragma unroll
+  for (int k = 0; k * NT < kWords; ++k) {
+    const int t = k * NT + (int)threadIdx.x;
+    if (t < kWords) d[t] = v[k];
   }
 }
+// field of the kernel's argument struct Args to LDS, by a block of NT threads
+#define SCDAT_STAGE(NT, Args, field, dst) stage_args<NT, offsetof(Args, field), sizeof(Args::field)>(dst)
 
 // out[s] for an index s uniform over the wavefront, with compile-time indices.
 template <typename T>
@@ -180,10 +198,14 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, unsigned long lo
 // just above every threshold, where a cross section may switch on), plus
 // the bound sum n sigma_max v over the energies below the lowest sample;
 // times kMccMajorantMargin.  Then P_null, and the push's accumulators reset.
-__global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
-  __shared__ double lds[kBlock / 64];
+// One energy per thread: the FP64 cross sections are a long dependent chain
+// per energy, and the block is alone on its CU.
+constexpr int kMajBlock = kMccMajorantPoints;
+static_assert(kMajBlock <= 1024 && kMajBlock % 64 == 0, "one energy per thread of one block");
+__global__ void __launch_bounds__(kMajBlock) k_mcc_majorant(MccMajorantArgs a) {
+  __shared__ double lds[kMajBlock / 64];
   __shared__ MccProcDev sp[kMaxProc];
-  stage(a.proc, sp);
+  SCDAT_STAGE(kMajBlock, MccMajorantArgs, proc, sp);
   __syncthreads();
   // one block per scenario (one block without scenarios)
   const int sg = blockIdx.x;
@@ -197,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
   double f = 0.0;
   if (ecap > 0.0) {
     const double elo = 1e-8 * ecap, la = log(elo), lb = log(ecap);
-    for (int k = threadIdx.x; k < kMccMajorantPoints; k += kBlock) {
+    for (int k = threadIdx.x; k < kMccMajorantPoints; k += kMajBlock) {
       const double E = exp(la + (lb - la) * (double)k / (double)(kMccMajorantPoints - 1));
       f = fmax(f, nu_total(sp, a.np, E, a.mass, dens));
     }
@@ -216,7 +238,7 @@ __global__ void __launch_bounds__(kBlock) k_mcc_majorant(MccMajorantArgs a) {
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = f;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < kBlock / 64; ++w) f = fmax(f, lds[w]);
+    for (int w = 1; w < kMajBlock / 64; ++w) f = fmax(f, lds[w]);
     f = fmax(f, lds[0]);
     const double numax = kMccMajorantMargin * f;
     ctl[0] = -expm1(-numax * dt);
@@ -344,8 +366,8 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_mcc(MccArgs<T> a) {
   __shared__ MccProcDev sp[kMaxProc];
   __shared__ SpeciesOut<T> so[kMaxSpecies];
-  stage(a.proc, sp);
-  stage(a.out, so);
+  SCDAT_STAGE(kBlock, MccArgs<T>, proc, sp);
+  SCDAT_STAGE(kBlock, MccArgs<T>, out, so);
   __syncthreads();
   const unsigned int nc = *a.n_cand;
   for (unsigned int q = blockIdx.x * kBlock + threadIdx.x; q < nc; q += gridDim.x * kBlock)
@@ -463,9 +485,9 @@ __global__ void __launch_bounds__(kBlock) k_see(EmitArgs<T> a) {
   __shared__ SurfEmitDev sg[kMaxGroups];
   __shared__ SpeciesOut<T> so[kMaxSpecies];
   __shared__ double sm[kMaxSpecies];
-  stage(a.g, sg);
-  stage(a.out, so);
-  stage(a.mass, sm);
+  SCDAT_STAGE(kBlock, EmitArgs<T>, g, sg);
+  SCDAT_STAGE(kBlock, EmitArgs<T>, out, so);
+  SCDAT_STAGE(kBlock, EmitArgs<T>, mass, sm);
   __syncthreads();
   const unsigned long long ne = *a.n_events;
   const int64_t nev = (int64_t)ne < a.event_cap ? (int64_t)ne : a.event_cap;
@@ -619,7 +641,7 @@ void launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s) {
 }
 
 void launch_mcc_majorant(const MccMajorantArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mcc_majorant, dim3(a.seg_n ? a.n_seg : 1), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_mcc_majorant, dim3(a.seg_n ? a.n_seg : 1), dim3(kMajBlock), 0, s, a);
 }
 template <typename T>
 void launch_v2max(const T* vx, const T* vy, const T* vz, const int32_t* cell, int64_t n, unsigned long long* v2max,
