@@ -225,6 +225,9 @@ int scdat_set_pcg_controls(scdat_ctx* ctx, double tol, int32_t max_iter);
 
 int scdat_set_efield(scdat_ctx* ctx, const double* E);
 int scdat_set_potential(scdat_ctx* ctx, const double* phi);
+/* scdat_set_rho: charge (C) per node, held in the fixed point of the
+ * species' unit; in a context with no species (a field solve alone) the
+ * unit is 2^-60 of the call's largest |rho|, a power of two.               */
 int scdat_set_rho(scdat_ctx* ctx, const double* rho);
 
 /* Replaces PICCycle::getChargeDensity / getPotential / getElectricField

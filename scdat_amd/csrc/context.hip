@@ -4202,6 +4202,19 @@ int scdat_set_rho(scdat_ctx* c, const double* rho) {
   if (!c->has_mesh) return fail(c, SCDAT_ERR_STATE, "set_rho: no mesh");
   return guarded(c, [&] {
     const size_t nn = (size_t)c->h.nn;
+    if (c->sp.empty() && !c->segmented()) {
+      // no species to fix the unit: 2^-60 of the largest |rho| of this call,
+      // a power of two (a field solve without particles)
+      double mx = 0.0;
+      for (size_t i = 0; i < nn; ++i) {
+        if (!std::isfinite(rho[i])) throw Error(SCDAT_ERR_ARG, "set_rho: non-finite value");
+        mx = std::max(mx, std::abs(rho[i]));
+      }
+      int e = 0;
+      std::frexp(mx, &e);
+      c->unit = mx > 0.0 ? std::ldexp(1.0, e - 60) : 1.0;
+      c->w_ref = c->unit * 4294967296.0 / kElem;
+    }
     std::vector<long long> fx(nn);
     for (size_t i = 0; i < nn; ++i) fx[i] = std::llround(rho[to_user(c, (int64_t)i)] / node_unit(c, (int64_t)i));
     const int k = c->rho_target;

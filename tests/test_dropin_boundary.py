@@ -12,7 +12,13 @@ the symbol tables show that the replacement defines every PICCycle member
 those callers need and nothing else a reference link could clash with.
 Compile only: the reference's ParticleManager needs <format>, which this
 image's standard library lacks, so no reference program is linked or run.
-Skipped where the reference sources are absent."""
+scdat_amd/facade/PoissonSolver_device.cpp, the device's
+SCDAT::FieldSolver::PoissonSolver, replaces Tools/FieldSolver/src/
+PoissonSolver.cpp the same way: it defines what the reference's file defines
+and nothing else, and, linked with the reference's mesh sources, it refuses
+to solve where no GPU is visible (no host fallback); its GPU parity is in
+tests/test_gpu_poisson_dropin.py.  Skipped where the reference sources are
+absent."""
 import os
 import re
 import subprocess
@@ -39,10 +45,13 @@ def _symbols(obj, flags):
 
 @pytest.fixture(scope="module")
 def objects():
-    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref"), "dropin"],
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref"), "all", "dropin"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    return {k: os.path.join(DROP, k + ".o") for k in ("PICCycle_device", "main_PICcore", "PicMccSurfaceCurrentSampler")}
+    objs = {k: os.path.join(DROP, k + ".o")
+            for k in ("PICCycle_device", "main_PICcore", "PicMccSurfaceCurrentSampler", "PoissonSolver_device")}
+    objs["PoissonSolver_reference"] = os.path.join(ROOT, "oracle", "_ref", "obj", "FieldSolver", "src", "PoissonSolver.o")
+    return objs
 
 
 def test_replacement_defines_every_member_the_reference_callers_use(objects):
@@ -68,3 +77,43 @@ def test_replacement_defines_only_picycle_members(objects):
     for member in ("initialize()", "executeTimeStep()", "executeSimulation(double)", "updateElectricField()",
                    "clearBoundaryConditions()", "findParticleElement(SCDAT::Particle::Particle*) const"):
         assert any(n == "SCDAT::PICcore::PICCycle::" + member for n in strong), member
+
+
+POISSON = ("SCDAT::FieldSolver::PoissonSolver::", "SCDAT::FieldSolver::BoundaryCondition::")
+
+
+def test_poisson_replacement_defines_what_the_reference_file_defines(objects):
+    ref = {name for kind, name in _symbols(objects["PoissonSolver_reference"], ["--defined-only", "--extern-only"])
+           if kind == "T" and name.startswith(POISSON)}
+    assert len(ref) >= 30, ref
+    mine = {name for kind, name in _symbols(objects["PoissonSolver_device"], ["--defined-only", "--extern-only"])
+            if kind == "T"}
+    # the host least-squares helper is private and unused here: the device
+    # computes the nodal gradient
+    missing = sorted(n for n in ref - mine if "computeLeastSquaresGradient" not in n)
+    assert not missing, missing
+    strong = [name for kind, name in _symbols(objects["PoissonSolver_device"], ["--defined-only", "--extern-only"])
+              if kind in "TDBR"]
+    others = [n for n in strong if not n.startswith(POISSON)]
+    assert not others, others
+
+
+def test_poisson_replacement_refuses_without_a_device(objects):
+    import ctypes as C
+
+    import numpy as np
+
+    import scdat_amd as S
+    try:
+        S.device_info()
+        pytest.skip("a GPU is visible: the device solve is checked in tests/test_gpu_poisson_dropin.py")
+    except S.ScdatError:
+        pass
+    L = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libscdat_poisson_dropin.so"))
+    cap = 1000
+    n = C.c_int(0)
+    z, phi, E, tot = np.zeros(cap), np.zeros(cap), np.zeros(3 * cap), np.zeros(3)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    rc = L.ref_poisson_plate(cap, C.byref(n), P(z), P(phi), P(E), P(tot))
+    assert rc == 1                      # solve() returned false
+    assert n.value > 0 and not phi[:n.value].any()  # and nothing was computed on the host
