@@ -626,6 +626,13 @@ __global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_tl(EllDev A, FieldBufs
   const double g_prev = sc->hist[1 - cur], a_prev = sc->hist[2 + (1 - cur)];
   const double tol = sc->inner_tol_eff;
   const bool stopped = sc->inner_done || sc->total_it + it >= sc->max_it;
+  // a launch past the end of this pass (or of the solve) returns before its
+  // loads: it would only sum the partials to find that it has nothing to do
+  if (idle_now) return;
+  if (stopped) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->inner_done = 1;
+    return;
+  }
   const int myc = (int)threadIdx.x < K ? A.cnb[q0 + threadIdx.x] : 0;  // in flight with the partials
   {
     // partials two per thread (16-byte loads; slots are 256-byte aligned, so
@@ -883,6 +890,13 @@ __global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_box(EllDev A, FieldBuf
   const double g_prev = sc->hist[1 - cur], a_prev = sc->hist[2 + (1 - cur)];
   const double tol = sc->inner_tol_eff;
   const bool stopped = sc->inner_done || sc->total_it + it >= sc->max_it;
+  // a launch past the end of this pass (or of the solve) returns before its
+  // loads: it would only sum the partials to find that it has nothing to do
+  if (idle_now) return;
+  if (stopped) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->inner_done = 1;
+    return;
+  }
   int K = 0;
   if constexpr (TL) {
     const int nc = A.nc;
