@@ -817,8 +817,8 @@ def main():
                 continue  # measured on another form of the iteration
             traffic[k] = v
             traffic_src[k] = ("profiles/pmc_traffic_64cube.json: FETCH_SIZE + WRITE_SIZE per launch of %s, "
-                              "scaled by a device copy (commands in profiles/pmc_traffic_64cube_16M.md)"
-                              % kn.get(k, k))
+                              "scaled by a device copy, over the first 20 steps (the whole plasma; commands in "
+                              "profiles/pmc_traffic_64cube_16M.md)" % kn.get(k, k))
     # launches that did work in the timed region (PCG: one per inner iteration;
     # every 16th carries timing events)
     iters = g["stats"]["pcg_iterations_total"] - g["stats0"]["pcg_iterations_total"]
