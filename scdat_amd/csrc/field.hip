@@ -1274,9 +1274,12 @@ __global__ void __launch_bounds__(kPB) k_seg_summary(FieldBufs f) {
   }
 }
 
-// Floating potential from Gauss's law on the conductor (one block per segment).
-__global__ void __launch_bounds__(kBlock) k_float(FieldBufs f, FloatArgs fa) {
-  __shared__ double lds[kBlock / 64];
+// Floating potential from Gauss's law on the conductor (one block per
+// segment; 1024 threads: the conductor's coupling entries are a few per
+// thread, each a dependent gather).
+constexpr int kFloatBlock = 1024;
+__global__ void __launch_bounds__(kFloatBlock) k_float(FieldBufs f, FloatArgs fa) {
+  __shared__ double lds[kFloatBlock / 64];
   const int sg = blockIdx.x;
   const int64_t j0 = f.kc_off ? f.kc_off[sg] : 0, j1 = f.kc_off ? f.kc_off[sg + 1] : f.n_kc;
   double s = 0.0;
@@ -1511,7 +1514,7 @@ void launch_clear_stall(const FieldBufs& f, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_stall, dim3(1), dim3(1), 0, s, f.sc);
 }
 void launch_float(const FieldBufs& f, const FloatArgs& fa, hipStream_t s) {
-  hipLaunchKernelGGL(k_float, dim3(f.blk_seg ? f.n_seg : 1), dim3(kBlock), 0, s, f, fa);
+  hipLaunchKernelGGL(k_float, dim3(f.blk_seg ? f.n_seg : 1), dim3(kFloatBlock), 0, s, f, fa);
 }
 __global__ void __launch_bounds__(kBlock) k_extrapolate(FieldBufs f, double* prev, int64_t n, int first) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
