@@ -408,6 +408,68 @@ __global__ void __launch_bounds__(PB) k_cg_start(EllDev A, FieldBufs f, InnerBuf
   if (A.nc) publish_restrict(wv, slot(f, S_WC, G), lds);
 }
 
+// k_cg_start in the tiled order: u_j = r_j (+ D_j zr_0 of j's block,
+// two-level) staged over the tile's 10^3 box in LDS, each row reading its
+// neighbours by direction code from the box pattern table.  The values, the
+// slot order and the fused operations are k_cg_start's, so both give the
+// same bits.
+template <int W, bool TL>
+__global__ void __launch_bounds__(kPcgBlock) k_cg_start_box(EllDev A, FieldBufs f, InnerBufs<float> ib) {
+  constexpr int PB = kPcgBlock;
+  static_assert(2 * PB >= 1000, "two box points per thread");
+  __shared__ double lds[3 * (PB / 64)];
+  __shared__ float box[1000];  // u_j at (x, y, z) in [-1, 8]^3: (x+1) + 10 (y+1) + 100 (z+1)
+  const int G = gridDim.x;
+  if (idle(f.sc)) return;
+  const int64_t n = A.n;
+  const int bt = (int)blockIdx.x;
+  const int64_t i = (int64_t)bt * PB + threadIdx.x;
+  const float* __restrict__ r = ib.r[0];
+  const float* __restrict__ dc = A.dco32;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int q = (int)threadIdx.x + u * PB;
+    if (q >= 1000) continue;
+    const int bx = q % 10 - 1, by = (q / 10) % 10 - 1, bz = q / 100 - 1;
+    const int code = ((bx >> 3) + 1) + 3 * ((by >> 3) + 1) + 9 * ((bz >> 3) + 1);
+    const int nr = A.nbb[(size_t)bt * 27 + code];
+    const int nb = nr >= 0 ? nr : bt;
+    const int64_t row = (int64_t)nb * PB + (bx & 7) + 8 * (by & 7) + 64 * (bz & 7);
+    float v = 0.0f;
+    if ((code == 13 || nb != bt) && row < n) v = TL ? fmaf(dc[row], ib.zr[0][nb], r[row]) : r[row];
+    box[q] = v;
+  }
+  __syncthreads();
+  double g = 0.0, d = 0.0, rr = 0.0, wv = 0.0;
+  if (i < n) {
+    const int pj = (int)A.bpat[i];
+    const int x = threadIdx.x & 7, y = (threadIdx.x >> 3) & 7, z = threadIdx.x >> 6;
+    const int cpos = (x + 1) + 10 * (y + 1) + 100 * (z + 1);
+    const uint4 dq = *reinterpret_cast<const uint4*>(A.bd + (size_t)pj * 16);
+    const uint32_t dw[4] = {dq.x, dq.y, dq.z, dq.w};
+    const float* pv = A.bv + (size_t)pj * W;
+    float acc = 0.0f;
+#pragma unroll 5
+    for (int k = 0; k < W; ++k) {
+      const int dcode = (int)((dw[k >> 2] >> (8 * (k & 3))) & 0xffu);
+      const int dx = (dcode & 3) - 1, dy = ((dcode >> 2) & 3) - 1, dz = (dcode >> 4) - 1;
+      acc = fmaf(pv[k], box[cpos + dx + 10 * dy + 100 * dz], acc);
+    }
+    const float ri = r[i];
+    float u = A.dinv32[i] * ri;
+    if (TL && dc[i] != 0.0f) {
+      u += ib.zr[0][bt];
+      wv += (double)acc;
+    }
+    ib.w[0][i] = acc;
+    g += (double)ri * (double)u;
+    d += (double)acc * (double)u;
+    rr += (double)ri * (double)ri;
+  }
+  publish3<PB>(g, d, rr, slot(f, S_G0, G), slot(f, S_D0, G), slot(f, S_RR0, G), lds);
+  if (TL) publish_restrict(wv, slot(f, S_WC, G), lds);
+}
+
 // One iteration.  W > 0: compile-time ELL width, fully unrolled, with the 2W
 // coalesced slot loads issued before the scalar prologue; W = 0: runtime width.
 // TL: two-level preconditioner (coarse terms from the coarse vectors of
@@ -1407,7 +1469,27 @@ void launch_inner_init(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& 
   const int G = grid_rows(A.n, A.pb);
   SCDAT_PB(hipLaunchKernelGGL((k_inner_init<T, PB>), dim3(G), dim3(PB), 0, s, A, f, ib));
   if (A.nc) hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 0, 0, G);
-  SCDAT_PB(hipLaunchKernelGGL((k_cg_start<T, PB>), dim3(G), dim3(PB), 0, s, A, f, ib));
+  bool started = false;
+  if constexpr (sizeof(T) == 4) {
+    if (A.box && (A.nc == 0 || A.fused)) {  // tile-staged, as the iterations
+      const dim3 g(grid_rows(A.n, kPcgBlock)), b(kPcgBlock);
+      started = true;
+      switch (A.w * 2 + (A.nc ? 1 : 0)) {
+#define SCDAT_W(w)                                                                  \
+  case 2 * w:                                                                       \
+    hipLaunchKernelGGL((k_cg_start_box<w, false>), g, b, 0, s, A, f, ib);           \
+    break;                                                                          \
+  case 2 * w + 1:                                                                   \
+    hipLaunchKernelGGL((k_cg_start_box<w, true>), g, b, 0, s, A, f, ib);            \
+    break;
+        SCDAT_W(13) SCDAT_W(14) SCDAT_W(15) SCDAT_W(16)
+#undef SCDAT_W
+        default:
+          started = false;  // ell() only stages these widths
+      }
+    }
+  }
+  if (!started) SCDAT_PB(hipLaunchKernelGGL((k_cg_start<T, PB>), dim3(G), dim3(PB), 0, s, A, f, ib));
   // fused: iteration 0 forms zw_0 from the restriction k_cg_start published
   if (A.nc && !A.fused)
     hipLaunchKernelGGL(k_coarse<T>, dim3(coarse_blocks(A)), dim3(kPB), 0, s, A, f, ib, 1, 0, G);
