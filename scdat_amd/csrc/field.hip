@@ -1256,7 +1256,8 @@ __global__ void __launch_bounds__(kPcgBlock, 6) k_cg_iter_box(EllDev A, FieldBuf
 // x += |r| d (skipped when the outer loop has already converged).
 template <typename T, int PB>
 __global__ void __launch_bounds__(PB) k_apply(EllDev A, FieldBufs f, InnerBufs<T> ib) {
-  PcgScalars* sc = f.sc + seg_of_block(f, blockIdx.x, gridDim.x).s;
+  const SegRange sg = seg_of_block(f, blockIdx.x, gridDim.x);
+  PcgScalars* sc = f.sc + sg.s;
   if (idle(sc)) return;
   if (!sc->inner_done) {
     // a pass launched without a host poll ran out of its batch: wait for
@@ -1264,21 +1265,17 @@ __global__ void __launch_bounds__(PB) k_apply(EllDev A, FieldBufs f, InnerBufs<T
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->stall = 1;
     return;
   }
+  if ((int)blockIdx.x == sg.b0 && threadIdx.x == 0) {
+    // the pass is counted by the segment's first block (no block of this
+    // launch reads the counts)
+    if (sc->outer_it < 16) sc->pass_it[sc->outer_it] = sc->inner_it;
+    sc->outer_it += 1;
+    sc->total_it += sc->inner_it;
+  }
   const double rn = sc->rnorm;
   const int64_t r0 = (int64_t)blockIdx.x * PB;
   for (int64_t i = r0 + threadIdx.x; i < r0 + PB && i < A.n; i += PB)
     f.x64[i] = fma(rn, (double)ib.x[i], f.x64[i]);
-}
-
-// one thread per segment
-__global__ void k_count_outer(FieldBufs f, int n_seg) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_seg) return;
-  PcgScalars* sc = f.sc + s;
-  if (idle(sc)) return;
-  if (sc->outer_it < 16) sc->pass_it[sc->outer_it] = sc->inner_it;
-  sc->outer_it += 1;
-  sc->total_it += sc->inner_it;
 }
 
 __global__ void k_clear_stall(PcgScalars* sc) { sc->stall = 0; }
@@ -1581,8 +1578,6 @@ void launch_inner_iter(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& 
 template <typename T>
 void launch_apply(const EllDev& A, const FieldBufs& f, const InnerBufs<T>& ib, hipStream_t s) {
   SCDAT_PB(hipLaunchKernelGGL((k_apply<T, PB>), dim3(grid_rows(A.n, A.pb)), dim3(PB), 0, s, A, f, ib));
-  const int ns = f.blk_seg ? f.n_seg : 1;
-  hipLaunchKernelGGL(k_count_outer, dim3((ns + kPB - 1) / kPB), dim3(kPB), 0, s, f, ns);
 }
 size_t partial_doubles(int64_t n) {  // smallest block, slots 256-byte aligned
   return (size_t)S_COUNT * (size_t)((grid_rows(n, 256) + 31) & ~31) + 16;
