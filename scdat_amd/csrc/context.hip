@@ -1527,13 +1527,13 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
     int pass, it;
   };
   std::vector<Sample> samples;
-  size_t judged = 0;
-  auto judge = [&](size_t upto, int pass, int count) {
-    for (; judged < upto; ++judged) {
-      const Sample& q = samples[judged];
+  size_t tallied = 0;
+  auto tally = [&](size_t upto, int pass, int count) {
+    for (; tallied < upto; ++tallied) {
+      const Sample& q = samples[tallied];
       if (q.pass != pass || q.it >= count) continue;
       float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, c->kt_pool[2 * judged], c->kt_pool[2 * judged + 1]));
+      HIPCHK(hipEventElapsedTime(&ms, c->kt_pool[2 * tallied], c->kt_pool[2 * tallied + 1]));
       c->kt_sum[1] += ms;
       c->kt_n[1] += 1;
     }
@@ -1617,7 +1617,7 @@ void pcg_drive(scdat_ctx* c, const FieldBufs& f, double tol, double inner_tol, i
       if (hist) hist[outer] = 0;  // no speculation at this index next time
       break;
     }
-    if (!chain) judge(samples.size(), outer, c->h_sc->inner_it);
+    if (!chain) tally(samples.size(), outer, c->h_sc->inner_it);
     ++passes;
     if (hist) hist[outer] = c->h_sc->inner_it;
     launch_apply<T>(A, f, ib, s);

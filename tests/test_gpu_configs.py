@@ -72,7 +72,7 @@ def test_config0_fp64_100_steps_matches_oracle():
     n_abs_o = sum(int(led_o[g, 4] + led_o[g, 5]) for g in range(12))
     n_abs_c = sum(led_c[g]["absorbed_electron_particles"] + led_c[g]["absorbed_ion_particles"] for g in range(12))
     print("absorbed: oracle", n_abs_o, "device", n_abs_c)
-    # north_star bounds: potential 1e-6 relative (FP64), trajectories 1e-5
+    # the configuration's bounds: potential 1e-6 relative (FP64), trajectories 1e-5
     # (relative to the 1 m domain) over the 100 steps
     assert max(r["phi"] for r in curve) < 1e-6
     assert max(r["E"] for r in curve) < 1e-6
